@@ -1,0 +1,143 @@
+"""Benchmark of the HMC molecular-dynamics step on one MI355X.
+
+Workload (default): 4D SU(3) + sextet scalar (Young rows {2}, d=6), L=32
+(1,048,576 sites), beta=5.7, kappa=0.1, lambda=0.5, eps=tau/nmd=1/20, FP32 MD
+(links, momenta, phi, pi, forces in FP32). A "step" is one fused MD step of
+the whole lattice (kick then drift of every site), i.e. V site-updates; the
+timed region runs K steps back to back with the fields resident in HBM.
+
+Prints one JSON line: metric/value (site-updates/s), roofline of the MD-step
+kernel (algorithmic bytes per launch / HIP-event launch time against 8 TB/s),
+and the CPU baseline (the reference compiled from its sources, or the FP64
+oracle restatement, timed on this host on a 2-step sample of the same lattice).
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # name: (D, N, rep, L, beta, kappa, lambda, precision, d)
+    "su3_sextet_L32_fp32": (4, 3, "2", 32, 5.7, 0.1, 0.5, "fp32", 6),
+    "su2_fund_L16_fp32": (4, 2, "fund", 16, 2.3, 0.265, 0.5, "fp32", 2),
+    "su2_fund_L16_fp64": (4, 2, "fund", 16, 2.3, 0.265, 0.5, "fp64", 2),
+    "su2_pure_L8_fp64": (4, 2, "", 8, 2.3, 0.0, 0.0, "fp64", 0),
+}
+
+
+def algorithmic_bytes_per_site(D, N, d, w):
+    """Every state element read once and written once per MD step:
+    B = 2 [D 2N^2 w + D (N^2-1) w + 2 (2d) w] (U, P, phi, pi)."""
+    return 2 * (D * 2 * N * N * w + D * (N * N - 1) * w + 2 * (2 * d) * w)
+
+
+def load_traffic(workload):
+    path = os.path.join(ROOT, "profiles", f"md_step_{workload}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+def cpu_baseline(workload, sample_steps=2):
+    """Reference (compiled from its own sources by oracle/Makefile) on this host."""
+    D, N, rep, L, *_ = WORKLOADS[workload]
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close")
+    harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if workload == "su3_sextet_L32_fp32" and os.access(harness, os.X_OK):
+        out = subprocess.run([harness, "bench", str(L), str(sample_steps)], capture_output=True, text=True, env=env, timeout=600)
+        r = json.loads(out.stdout.strip().splitlines()[-1])
+        return {"value": r["site_updates_per_s"], "unit": "site-updates/s", "cores": threads, "kind": "reference",
+                "sample": f"reference GaugeHiggsHMC<4,3> + GeneralRep{{2}} FP64, {L}^4, {sample_steps} drift+kick MD steps "
+                          f"({r['seconds']:.1f} s), OMP_NUM_THREADS={threads}"}
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    o = O.Oracle(D, N, L, rep, seed=1)
+    o.couplings(beta=WORKLOADS[workload][4], kappa=WORKLOADS[workload][5], lam=0.5, nmd=20)
+    o.hot(0.8)
+    if rep:
+        o.phi_gaussian(12345, 1.0)
+    o.refresh()
+    sec = O.lib().orc_time_kick_drift(o.h, sample_steps, 0.05)
+    return {"value": o.V * sample_steps / sec, "unit": "site-updates/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle restatement, {L}^4, {sample_steps} kick+drift MD steps ({sec:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="su3_sextet_L32_fp32", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU path is not built.", file=sys.stderr)
+        return 2
+
+    import gauge_higgs_amd as gha
+    D, N, rep, L, beta, kappa, lam, prec, d = WORKLOADS[args.workload]
+    precision = gha.FP32MD if prec == "fp32" else gha.FP64
+    w = 4 if prec == "fp32" else 8
+    g = gha.GaugeHiggsHMC(L, rep, seed=1, D=D, N=N, precision=precision)
+    g.configure(beta=beta, kappa=kappa, lam=lam, tau=1.0, nmd=20)
+    g.hot(0.8, 12345, 1.0)
+    g.refresh_momenta()
+    V = g.lat.vols[0]
+    if args.warmup:
+        g.lat.time_md_steps(args.warmup)
+    g.lat.sync()
+    try:
+        import torch
+        torch.cuda.synchronize()
+    except Exception:
+        pass
+    t0 = time.perf_counter()
+    ms = g.lat.time_md_steps(args.steps)  # HIP events on the context stream, bracketing K launches
+    g.lat.sync()
+    wall = time.perf_counter() - t0
+    sec = ms / 1e3
+    value = V * args.steps / sec
+    bpsite = algorithmic_bytes_per_site(D, N, d, w)
+    launch_s = sec / args.steps
+    achieved = V * bpsite / launch_s / 1e9
+    traffic = load_traffic(args.workload)
+    line = {
+        "metric": "MD site-updates/s",
+        "value": value,
+        "unit": "site-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if prec == "fp32" else "f64",
+        "data": "synthetic (reference hot start sigma=0.8, Gaussian phi sigma=1, Gaussian momenta; seed 1)",
+        "config": {"workload": args.workload, "D": D, "N": N, "rep": rep or "none", "L": L, "sites": V, "beta": beta,
+                   "kappa": kappa, "lambda": lam, "eps": 0.05, "wall_s": wall},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "md_step_kernel", "algorithmic_bytes_per_site": bpsite},
+    }
+    if not args.no_cpu:
+        try:
+            line["cpu_baseline"] = cpu_baseline(args.workload)
+        except Exception as e:  # the GPU figure stands on its own
+            line["cpu_baseline"] = {"value": None, "error": str(e)[:200]}
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

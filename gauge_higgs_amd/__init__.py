@@ -1,0 +1,7 @@
+"""MI355X-native gauge+Higgs HMC molecular dynamics (drop-in for the gh_hmc /
+hmc_higgs path of hlammiv/gauge_higgs). The compute lives in libghx.so (HIP,
+gfx950); this package is the host-side mirror of the reference interface."""
+from .hmc import (COLD, FP32MD, FP64, HOT, LEAPFROG, OMELYAN, GaugeHiggsHMC, GaugeHMC, GhxError, Lattices)
+from .rep import rep_query
+
+__all__ = ["GaugeHMC", "GaugeHiggsHMC", "Lattices", "GhxError", "rep_query", "FP64", "FP32MD", "LEAPFROG", "OMELYAN", "HOT", "COLD"]

@@ -1,0 +1,728 @@
+// C-ABI implementation: context, device buffers, trajectory driver.
+// The trajectory (refresh, save, H_i, MD, reunitarize, H_f, Metropolis,
+// restore) runs entirely on the context's stream with no host round trip, so
+// it can be captured once into a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/ghx.h"
+#include "kernels.hpp"
+#include "ops.hpp"
+#include "rep_build.hpp"
+
+using namespace ghx;
+
+namespace {
+constexpr double kOmelyanLambda = 0.1931833275037836;  // config.hpp:25
+constexpr unsigned long long kTagP_pure = 0xA1, kTagMC_pure = 0xB2;        // hmc.hpp:74,86
+constexpr unsigned long long kTagP = 0xC3, kTagPi = 0xD4, kTagMC = 0xE5;   // gauge_higgs_hmc.hpp:167-185
+}  // namespace
+
+struct ghx_ctx {
+  int D = 4, N = 2, nrep = 1, prec = GHX_FP64;
+  size_t tsize = 8;
+  std::vector<std::array<int, 4>> L;
+  std::vector<RepDesc> hreps;
+  std::vector<int> hblk_rep, hred_rep;
+  std::vector<unsigned> hred_first;
+  unsigned nblk = 0, nred = 0, Vp = 0;
+  // representation
+  RepInfo rep;
+  bool ops_ok = false;
+  Ops ops{};
+  // integrator
+  int nmd = 20, integ = GHX_OMELYAN;
+  bool reunit = true;
+  // device state
+  RepDesc* dreps = nullptr;
+  int *dblk_rep = nullptr, *dred_rep = nullptr;
+  unsigned* dred_first = nullptr;
+  void *U[2] = {nullptr, nullptr}, *Usave = nullptr, *P = nullptr;
+  void *phi[2] = {nullptr, nullptr}, *phisave = nullptr, *pi = nullptr;
+  void *F = nullptr, *Fphi = nullptr;
+  double *partials = nullptr, *sums = nullptr, *Hi = nullptr, *Hf = nullptr, *Hwork = nullptr;
+  McState* mc = nullptr;
+  int cur = 0;
+  hipStream_t stream = nullptr;
+  hipGraphExec_t graph = nullptr;
+  int graph_cur = -1;
+  std::string err;
+
+  size_t nU() const { return size_t(D) * 2 * N * N; }
+  size_t nP() const { return size_t(D) * (N * N - 1); }
+  size_t nphi() const { return size_t(2) * rep.d; }
+  LaunchCtx lc() const {
+    LaunchCtx c;
+    c.stream = stream; c.nblk = nblk; c.nred = nred; c.Vp = Vp;
+    c.reps = dreps; c.blk_rep = dblk_rep; c.red_rep = dred_rep; c.red_first = dred_first;
+    c.partials = partials; c.W = rep.W.data(); c.d = rep.d; c.real_rep = rep.real ? 1 : 0;
+    return c;
+  }
+};
+
+namespace {
+
+int fail(ghx_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+#define HIPCHK(call)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess) return fail(ctx, GHX_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+void free_fields(ghx_ctx* c) {
+  for (void** p : {&c->U[0], &c->U[1], &c->Usave, &c->P, &c->phi[0], &c->phi[1], &c->phisave, &c->pi, &c->F, &c->Fphi})
+    if (*p) { (void)hipFree(*p); *p = nullptr; }
+}
+
+int alloc_zero(ghx_ctx* ctx, void** p, size_t bytes) {
+  if (bytes == 0) { *p = nullptr; return GHX_OK; }
+  HIPCHK(hipMalloc(p, bytes));
+  HIPCHK(hipMemsetAsync(*p, 0, bytes, ctx->stream));
+  return GHX_OK;
+}
+
+int alloc_fields(ghx_ctx* ctx) {
+  free_fields(ctx);
+  const size_t V = ctx->Vp, ts = ctx->tsize;
+  int rc;
+  for (int b = 0; b < 2; ++b) if ((rc = alloc_zero(ctx, &ctx->U[b], ctx->nU() * V * ts))) return rc;
+  if ((rc = alloc_zero(ctx, &ctx->Usave, ctx->nU() * V * ts))) return rc;
+  if ((rc = alloc_zero(ctx, &ctx->P, ctx->nP() * V * ts))) return rc;
+  if ((rc = alloc_zero(ctx, &ctx->F, ctx->nP() * V * ts))) return rc;
+  if (ctx->rep.d > 0) {
+    for (int b = 0; b < 2; ++b) if ((rc = alloc_zero(ctx, &ctx->phi[b], ctx->nphi() * V * ts))) return rc;
+    if ((rc = alloc_zero(ctx, &ctx->phisave, ctx->nphi() * V * ts))) return rc;
+    if ((rc = alloc_zero(ctx, &ctx->pi, ctx->nphi() * V * ts))) return rc;
+    if ((rc = alloc_zero(ctx, &ctx->Fphi, ctx->nphi() * V * ts))) return rc;
+  }
+  ctx->cur = 0;
+  return GHX_OK;
+}
+
+int upload_reps(ghx_ctx* ctx) {
+  HIPCHK(hipMemcpyAsync(ctx->dreps, ctx->hreps.data(), sizeof(RepDesc) * ctx->nrep, hipMemcpyHostToDevice, ctx->stream));
+  return GHX_OK;
+}
+
+int select_ops(ghx_ctx* ctx) {
+  const int p = ctx->prec == GHX_FP32MD ? 1 : 0;
+  const RepInfo& r = ctx->rep;
+  bool ok = false;
+  switch (r.kind) {
+    case REP_NONE: ok = ops_none(p, ctx->N, ctx->D, &ctx->ops); break;
+    case REP_FUND: ok = ops_fund(p, ctx->N, ctx->D, &ctx->ops); break;
+    case REP_ADJ: ok = ops_adj(p, ctx->N, ctx->D, &ctx->ops); break;
+    case REP_TENSOR: ok = ops_tensor(p, ctx->N, ctx->D, r.nbox, r.d, &ctx->ops); break;
+  }
+  ctx->ops_ok = ok;
+  if (!ok)
+    return fail(ctx, GHX_E_UNSUPPORTED, "no compiled kernels for N=" + std::to_string(ctx->N) + " D=" + std::to_string(ctx->D) +
+                                            " rep=" + r.name);
+  return GHX_OK;
+}
+
+int check_ready(ghx_ctx* ctx) {
+  if (!ctx) return GHX_E_INVALID;
+  if (!ctx->ops_ok) return fail(ctx, GHX_E_INVALID, "representation not set (call ghx_set_rep)");
+  return GHX_OK;
+}
+
+bool has_phi(const ghx_ctx* c) { return c->rep.d > 0; }
+
+// Sum per-site quantities into ctx->sums [nrep][kNQ] for buffer `b`.
+void launch_action(ghx_ctx* c, int b) {
+  const LaunchCtx l = c->lc();
+  c->ops.action(l, c->U[b], c->P, c->phi[b], c->pi);
+  hipLaunchKernelGGL(finalize_kernel, dim3(c->nrep), dim3(kRedBlock), 0, c->stream, c->dreps, c->partials, kNQ, c->sums);
+}
+void launch_hamiltonian(ghx_ctx* c, int b, double* H) {
+  launch_action(c, b);
+  hipLaunchKernelGGL(hamiltonian_kernel, dim3((c->nrep + 63) / 64), dim3(64), 0, c->stream, c->dreps, c->nrep, c->N, c->D,
+                     has_phi(c) ? 1 : 0, c->sums, H);
+}
+
+// One fused kick+drift (or kick-only when cd == 0 and !drift) on buffer cur.
+void launch_step(ghx_ctx* c, double ck, double cd, bool drift) {
+  const LaunchCtx l = c->lc();
+  const int in = c->cur, out = c->cur ^ 1;
+  int flags = F_KICK | (drift ? F_DRIFT : 0);
+  c->ops.md_step(l, c->U[in], c->U[out], c->P, c->phi[in], c->phi[out], c->pi, c->F, c->Fphi, ck, cd, flags);
+  if (drift) c->cur = out;
+}
+
+// md_evolve as a sequence of fused launches (gauge_higgs_hmc.hpp:149-165):
+// leapfrog  K(e/2)D(e) [K(e)D(e)]^{nmd-1} K(e/2)
+// Omelyan   K(le)D(e/2)K((1-2l)e)D(e/2)K(2le)D(e/2)...K((1-2l)e)D(e/2)K(le)
+void launch_md(ghx_ctx* c) {
+  if (c->integ == GHX_LEAPFROG) {
+    launch_step(c, 0.5, 1.0, true);
+    for (int i = 1; i < c->nmd; ++i) launch_step(c, 1.0, 1.0, true);
+    launch_step(c, 0.5, 0.0, false);
+  } else {
+    const double lam = kOmelyanLambda;
+    launch_step(c, lam, 0.5, true);
+    for (int i = 0; i < c->nmd; ++i) {
+      launch_step(c, 1.0 - 2.0 * lam, 0.5, true);
+      if (i != c->nmd - 1) launch_step(c, 2.0 * lam, 0.5, true);
+    }
+    launch_step(c, lam, 0.0, false);
+  }
+}
+
+// Bring the current buffer back to buffer 0 (reunitarizing if requested).
+void launch_settle(ghx_ctx* c, bool reunit) {
+  const LaunchCtx l = c->lc();
+  if (reunit) {
+    c->ops.reunit(l, c->U[c->cur], c->U[0]);
+  } else if (c->cur != 0) {
+    (void)hipMemcpyAsync(c->U[0], c->U[1], c->nU() * c->Vp * c->tsize, hipMemcpyDeviceToDevice, c->stream);
+  }
+  if (c->cur != 0 && has_phi(c))
+    (void)hipMemcpyAsync(c->phi[0], c->phi[1], c->nphi() * c->Vp * c->tsize, hipMemcpyDeviceToDevice, c->stream);
+  c->cur = 0;
+}
+
+// One trajectory of every replica, all on the stream.
+void launch_trajectory(ghx_ctx* c) {
+  const LaunchCtx l = c->lc();
+  const bool hp = has_phi(c);
+  c->ops.refresh(l, c->P, c->pi, c->mc, hp ? kTagP : kTagP_pure, kTagPi);
+  (void)hipMemcpyAsync(c->Usave, c->U[0], c->nU() * c->Vp * c->tsize, hipMemcpyDeviceToDevice, c->stream);
+  if (hp) (void)hipMemcpyAsync(c->phisave, c->phi[0], c->nphi() * c->Vp * c->tsize, hipMemcpyDeviceToDevice, c->stream);
+  launch_hamiltonian(c, 0, c->Hi);
+  launch_md(c);
+  launch_settle(c, c->reunit);
+  launch_hamiltonian(c, 0, c->Hf);
+  hipLaunchKernelGGL(metropolis_kernel, dim3((c->nrep + 63) / 64), dim3(64), 0, c->stream, c->dreps, c->nrep, c->Hi, c->Hf, c->mc,
+                     hp ? kTagMC : kTagMC_pure);
+  if (c->tsize == 8)
+    hipLaunchKernelGGL(restore_kernel<double>, dim3(c->nblk), dim3(kBlock), 0, c->stream, c->dreps, c->dblk_rep, c->mc, c->Vp,
+                       (double*)c->U[0], (const double*)c->Usave, int(c->nU()), (double*)c->phi[0], (const double*)c->phisave,
+                       hp ? int(c->nphi()) : 0);
+  else
+    hipLaunchKernelGGL(restore_kernel<float>, dim3(c->nblk), dim3(kBlock), 0, c->stream, c->dreps, c->dblk_rep, c->mc, c->Vp,
+                       (float*)c->U[0], (const float*)c->Usave, int(c->nU()), (float*)c->phi[0], (const float*)c->phisave,
+                       hp ? int(c->nphi()) : 0);
+}
+
+template <typename T>
+void soa_to_aos(const std::vector<T>& soa, size_t Vp, unsigned off, unsigned vol, int nplane_per_site, double* out) {
+  for (unsigned s = 0; s < vol; ++s)
+    for (int c = 0; c < nplane_per_site; ++c) out[size_t(s) * nplane_per_site + c] = double(soa[size_t(c) * Vp + off + s]);
+}
+template <typename T>
+void aos_to_soa(std::vector<T>& soa, size_t Vp, unsigned off, unsigned vol, int nplane_per_site, const double* in) {
+  for (unsigned s = 0; s < vol; ++s)
+    for (int c = 0; c < nplane_per_site; ++c) soa[size_t(c) * Vp + off + s] = T(in[size_t(s) * nplane_per_site + c]);
+}
+
+// The AoS site record [mu][...] maps to planes (mu*per_mu + c): identical ordering.
+int get_planes(ghx_ctx* ctx, const void* dev, int nplanes, int replica, double* out) {
+  const RepDesc& rd = ctx->hreps[replica];
+  if (ctx->tsize == 8) {
+    std::vector<double> h(size_t(nplanes) * ctx->Vp);
+    HIPCHK(hipMemcpyAsync(h.data(), dev, h.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    soa_to_aos(h, ctx->Vp, rd.offset, rd.vol, nplanes, out);
+  } else {
+    std::vector<float> h(size_t(nplanes) * ctx->Vp);
+    HIPCHK(hipMemcpyAsync(h.data(), dev, h.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    soa_to_aos(h, ctx->Vp, rd.offset, rd.vol, nplanes, out);
+  }
+  return GHX_OK;
+}
+int set_planes(ghx_ctx* ctx, void* dev, int nplanes, int replica, const double* in) {
+  const RepDesc& rd = ctx->hreps[replica];
+  if (ctx->tsize == 8) {
+    std::vector<double> h(size_t(nplanes) * ctx->Vp);
+    HIPCHK(hipMemcpyAsync(h.data(), dev, h.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    aos_to_soa(h, ctx->Vp, rd.offset, rd.vol, nplanes, in);
+    HIPCHK(hipMemcpyAsync(dev, h.data(), h.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  } else {
+    std::vector<float> h(size_t(nplanes) * ctx->Vp);
+    HIPCHK(hipMemcpyAsync(h.data(), dev, h.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    aos_to_soa(h, ctx->Vp, rd.offset, rd.vol, nplanes, in);
+    HIPCHK(hipMemcpyAsync(dev, h.data(), h.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  }
+  return GHX_OK;
+}
+
+void drop_graph(ghx_ctx* c) {
+  if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ghx_create(ghx_ctx** out, int D, int N, int n_replicas, const int* L, int precision) {
+  if (!out) return GHX_E_INVALID;
+  *out = nullptr;
+  if ((D != 3 && D != 4) || (N != 2 && N != 3) || n_replicas < 1 || !L || (precision != GHX_FP64 && precision != GHX_FP32MD))
+    return GHX_E_INVALID;
+  std::unique_ptr<ghx_ctx> c(new ghx_ctx);
+  ghx_ctx* ctx = c.get();
+  ctx->D = D; ctx->N = N; ctx->nrep = n_replicas; ctx->prec = precision;
+  ctx->tsize = precision == GHX_FP64 ? 8 : 4;
+  unsigned off = 0, red = 0;
+  for (int r = 0; r < n_replicas; ++r) {
+    RepDesc rd{};
+    unsigned long long vol = 1;
+    for (int m = 0; m < 4; ++m) rd.L[m] = 1;
+    for (int m = 0; m < D; ++m) {
+      const int e = L[r * D + m];
+      if (e < 2) return GHX_E_INVALID;
+      rd.L[m] = e;
+      rd.stride[m] = unsigned(vol);
+      vol *= unsigned(e);
+    }
+    if (vol > (1ull << 30)) return GHX_E_INVALID;
+    rd.vol = unsigned(vol);
+    rd.offset = off;
+    rd.beta = 2.3; rd.kappa = 0.1; rd.lambda = 0.5; rd.tau = 1.0; rd.eps = 1.0 / 20;
+    rd.seed = 1;
+    const unsigned nb = (rd.vol + kBlock - 1) / kBlock;
+    for (unsigned b = 0; b < nb; ++b) ctx->hblk_rep.push_back(r);
+    off += nb * kBlock;
+    const unsigned nr = (rd.vol + kRedBlock - 1) / kRedBlock;
+    rd.blk_begin = red;
+    for (unsigned b = 0; b < nr; ++b) { ctx->hred_rep.push_back(r); ctx->hred_first.push_back(rd.offset + b * kRedBlock); }
+    red += nr;
+    rd.blk_end = red;
+    ctx->hreps.push_back(rd);
+  }
+  ctx->Vp = off;
+  ctx->nblk = unsigned(ctx->hblk_rep.size());
+  ctx->nred = red;
+  HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  HIPCHK(hipMalloc((void**)&ctx->dreps, sizeof(RepDesc) * n_replicas));
+  HIPCHK(hipMalloc((void**)&ctx->dblk_rep, sizeof(int) * ctx->nblk));
+  HIPCHK(hipMalloc((void**)&ctx->dred_rep, sizeof(int) * ctx->nred));
+  HIPCHK(hipMalloc((void**)&ctx->dred_first, sizeof(unsigned) * ctx->nred));
+  HIPCHK(hipMalloc((void**)&ctx->partials, sizeof(double) * kNQ * ctx->nred));
+  HIPCHK(hipMalloc((void**)&ctx->sums, sizeof(double) * kNQ * n_replicas));
+  HIPCHK(hipMalloc((void**)&ctx->Hi, sizeof(double) * 5 * n_replicas));
+  HIPCHK(hipMalloc((void**)&ctx->Hf, sizeof(double) * 5 * n_replicas));
+  HIPCHK(hipMalloc((void**)&ctx->Hwork, sizeof(double) * 5 * n_replicas));
+  HIPCHK(hipMalloc((void**)&ctx->mc, sizeof(McState) * n_replicas));
+  HIPCHK(hipMemsetAsync(ctx->mc, 0, sizeof(McState) * n_replicas, ctx->stream));
+  HIPCHK(hipMemcpyAsync(ctx->dblk_rep, ctx->hblk_rep.data(), sizeof(int) * ctx->nblk, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(ctx->dred_rep, ctx->hred_rep.data(), sizeof(int) * ctx->nred, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(ctx->dred_first, ctx->hred_first.data(), sizeof(unsigned) * ctx->nred, hipMemcpyHostToDevice, ctx->stream));
+  int rc = upload_reps(ctx);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  *out = c.release();
+  return GHX_OK;
+}
+
+int ghx_destroy(ghx_ctx* ctx) {
+  if (!ctx) return GHX_E_INVALID;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  drop_graph(ctx);
+  free_fields(ctx);
+  for (void* p : {(void*)ctx->dreps, (void*)ctx->dblk_rep, (void*)ctx->dred_rep, (void*)ctx->dred_first, (void*)ctx->partials,
+                  (void*)ctx->sums, (void*)ctx->Hi, (void*)ctx->Hf, (void*)ctx->Hwork, (void*)ctx->mc})
+    if (p) (void)hipFree(p);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return GHX_OK;
+}
+
+const char* ghx_last_error(const ghx_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int ghx_set_rep(ghx_ctx* ctx, const char* spec, int* d_out) {
+  if (!ctx) return GHX_E_INVALID;
+  try {
+    ctx->rep = build_rep(ctx->N, spec ? spec : "");
+  } catch (const std::exception& e) {
+    ctx->ops_ok = false;
+    return fail(ctx, GHX_E_REP, e.what());
+  }
+  drop_graph(ctx);
+  int rc = select_ops(ctx);
+  if (rc) return rc;
+  if ((rc = alloc_fields(ctx))) return rc;
+  if (d_out) *d_out = ctx->rep.d;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return GHX_OK;
+}
+
+int ghx_rep_info(const ghx_ctx* ctx, int* d, int* nality, int* real_rep, long* dimT) {
+  if (!ctx) return GHX_E_INVALID;
+  if (d) *d = ctx->rep.d;
+  if (nality) *nality = ctx->rep.nality;
+  if (real_rep) *real_rep = ctx->rep.real ? 1 : 0;
+  if (dimT) *dimT = ctx->rep.dimT;
+  return GHX_OK;
+}
+
+int ghx_rep_generators(const ghx_ctx* ctx, double* T) {
+  if (!ctx || !T) return GHX_E_INVALID;
+  for (size_t i = 0; i < ctx->rep.T.size(); ++i) { T[2 * i] = ctx->rep.T[i].real(); T[2 * i + 1] = ctx->rep.T[i].imag(); }
+  return GHX_OK;
+}
+
+int ghx_rep_basis(const ghx_ctx* ctx, double* W) {
+  if (!ctx || !W) return GHX_E_INVALID;
+  std::copy(ctx->rep.W.begin(), ctx->rep.W.end(), W);
+  return GHX_OK;
+}
+
+int ghx_set_couplings(ghx_ctx* ctx, int replica, double beta, double kappa, double lambda, double tau) {
+  if (!ctx || replica < 0 || replica >= ctx->nrep) return GHX_E_INVALID;
+  RepDesc& rd = ctx->hreps[replica];
+  rd.beta = beta; rd.kappa = kappa; rd.lambda = lambda; rd.tau = tau; rd.eps = tau / ctx->nmd;
+  return upload_reps(ctx);
+}
+
+int ghx_set_integrator(ghx_ctx* ctx, int nmd, int integrator, int reunit_each_traj) {
+  if (!ctx || nmd < 1 || (integrator != GHX_LEAPFROG && integrator != GHX_OMELYAN)) return GHX_E_INVALID;
+  ctx->nmd = nmd; ctx->integ = integrator; ctx->reunit = reunit_each_traj != 0;
+  for (auto& rd : ctx->hreps) rd.eps = rd.tau / nmd;
+  drop_graph(ctx);
+  return upload_reps(ctx);
+}
+
+int ghx_set_seed(ghx_ctx* ctx, int replica, uint64_t seed) {
+  if (!ctx || replica < 0 || replica >= ctx->nrep) return GHX_E_INVALID;
+  ctx->hreps[replica].seed = seed;
+  return upload_reps(ctx);
+}
+
+int ghx_init_fields(ghx_ctx* ctx, int replica, int link_mode, double sigma_U, int phi_mode, uint64_t phi_stream, double phi_sigma) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (replica < 0 || replica >= ctx->nrep) return fail(ctx, GHX_E_INVALID, "replica out of range");
+  const LaunchCtx l = ctx->lc();
+  if (ctx->cur != 0) launch_settle(ctx, false);
+  ctx->ops.init_links(l, ctx->U[0], replica, link_mode == GHX_HOT ? 1 : 0, sigma_U);
+  if (has_phi(ctx)) {
+    if (ctx->tsize == 8)
+      hipLaunchKernelGGL(init_phi_kernel<double>, dim3(ctx->nblk), dim3(kBlock), 0, ctx->stream, ctx->dreps, ctx->dblk_rep, ctx->Vp,
+                         (double*)ctx->phi[0], ctx->rep.d, ctx->rep.real ? 1 : 0, replica, phi_mode ? 1 : 0,
+                         (unsigned long long)phi_stream, phi_sigma);
+    else
+      hipLaunchKernelGGL(init_phi_kernel<float>, dim3(ctx->nblk), dim3(kBlock), 0, ctx->stream, ctx->dreps, ctx->dblk_rep, ctx->Vp,
+                         (float*)ctx->phi[0], ctx->rep.d, ctx->rep.real ? 1 : 0, replica, phi_mode ? 1 : 0,
+                         (unsigned long long)phi_stream, phi_sigma);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return GHX_OK;
+}
+
+int ghx_set_counters(ghx_ctx* ctx, int replica, uint64_t traj_count, uint64_t accept_count) {
+  if (!ctx || replica < 0 || replica >= ctx->nrep) return GHX_E_INVALID;
+  McState m{};
+  HIPCHK(hipMemcpyAsync(&m, ctx->mc + replica, sizeof m, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  m.traj_count = traj_count; m.accept_count = accept_count;
+  HIPCHK(hipMemcpyAsync(ctx->mc + replica, &m, sizeof m, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return GHX_OK;
+}
+
+int ghx_get_counters(const ghx_ctx* cctx, int replica, uint64_t* traj_count, uint64_t* accept_count, double* last_dH) {
+  ghx_ctx* ctx = const_cast<ghx_ctx*>(cctx);
+  if (!ctx || replica < 0 || replica >= ctx->nrep) return GHX_E_INVALID;
+  McState m{};
+  HIPCHK(hipMemcpyAsync(&m, ctx->mc + replica, sizeof m, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (traj_count) *traj_count = m.traj_count;
+  if (accept_count) *accept_count = m.accept_count;
+  if (last_dH) *last_dH = m.last_dH;
+  return GHX_OK;
+}
+
+int ghx_refresh_momenta(ghx_ctx* ctx) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  ctx->ops.refresh(ctx->lc(), ctx->P, ctx->pi, ctx->mc, has_phi(ctx) ? kTagP : kTagP_pure, kTagPi);
+  HIPCHK(hipGetLastError());
+  return GHX_OK;
+}
+
+int ghx_kick(ghx_ctx* ctx, double eps) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  const LaunchCtx l = ctx->lc();
+  const int b = ctx->cur;
+  ctx->ops.md_step(l, ctx->U[b], ctx->U[b ^ 1], ctx->P, ctx->phi[b], ctx->phi[b ^ 1], ctx->pi, ctx->F, ctx->Fphi, eps, 0.0,
+                   F_KICK | F_ABS_EPS);
+  HIPCHK(hipGetLastError());
+  return GHX_OK;
+}
+
+int ghx_drift(ghx_ctx* ctx, double eps) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  const LaunchCtx l = ctx->lc();
+  const int b = ctx->cur;
+  ctx->ops.drift(l, ctx->U[b], ctx->U[b ^ 1], ctx->P, ctx->phi[b], ctx->phi[b ^ 1], ctx->pi, eps, F_DRIFT | F_ABS_EPS);
+  ctx->cur ^= 1;
+  HIPCHK(hipGetLastError());
+  return GHX_OK;
+}
+
+int ghx_md_evolve(ghx_ctx* ctx) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  launch_md(ctx);
+  HIPCHK(hipGetLastError());
+  return GHX_OK;
+}
+
+int ghx_hamiltonian(ghx_ctx* ctx, double* H_parts) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  launch_hamiltonian(ctx, ctx->cur, ctx->Hwork);
+  HIPCHK(hipGetLastError());
+  if (H_parts) {
+    HIPCHK(hipMemcpyAsync(H_parts, ctx->Hwork, sizeof(double) * 5 * ctx->nrep, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  }
+  return GHX_OK;
+}
+
+int ghx_trajectory(ghx_ctx* ctx, int* accepted, double* dH) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (ctx->cur != 0) launch_settle(ctx, false);
+  launch_trajectory(ctx);
+  HIPCHK(hipGetLastError());
+  if (accepted || dH) {
+    std::vector<McState> m(ctx->nrep);
+    HIPCHK(hipMemcpyAsync(m.data(), ctx->mc, sizeof(McState) * ctx->nrep, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (int r = 0; r < ctx->nrep; ++r) {
+      if (accepted) accepted[r] = m[r].accepted;
+      if (dH) dH[r] = m[r].last_dH;
+    }
+    for (int r = 0; r < ctx->nrep; ++r)
+      if (!std::isfinite(m[r].last_dH)) return fail(ctx, GHX_E_NAN, "non-finite dH in replica " + std::to_string(r));
+  }
+  return GHX_OK;
+}
+
+int ghx_trajectories(ghx_ctx* ctx, int n, int use_graph) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (ctx->cur != 0) launch_settle(ctx, false);
+  if (!use_graph) {
+    for (int i = 0; i < n; ++i) launch_trajectory(ctx);
+    HIPCHK(hipGetLastError());
+    return GHX_OK;
+  }
+  if (!ctx->graph) {
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    launch_trajectory(ctx);
+    HIPCHK(hipStreamEndCapture(ctx->stream, &g));
+    HIPCHK(hipGraphInstantiate(&ctx->graph, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+  }
+  for (int i = 0; i < n; ++i) HIPCHK(hipGraphLaunch(ctx->graph, ctx->stream));
+  return GHX_OK;
+}
+
+int ghx_observables(ghx_ctx* ctx, double* out) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (!out) return GHX_E_INVALID;
+  const int b = ctx->cur;
+  launch_action(ctx, b);
+  std::vector<double> s(size_t(kNQ) * ctx->nrep), pl(ctx->nrep);
+  HIPCHK(hipMemcpyAsync(s.data(), ctx->sums, sizeof(double) * s.size(), hipMemcpyDeviceToHost, ctx->stream));
+  const LaunchCtx l = ctx->lc();
+  ctx->ops.polyakov(l, ctx->U[b]);
+  hipLaunchKernelGGL(finalize_kernel, dim3(ctx->nrep), dim3(kRedBlock), 0, ctx->stream, ctx->dreps, ctx->partials, 1, ctx->Hwork);
+  HIPCHK(hipMemcpyAsync(pl.data(), ctx->Hwork, sizeof(double) * ctx->nrep, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (int r = 0; r < ctx->nrep; ++r) {
+    const RepDesc& rd = ctx->hreps[r];
+    const double* q = &s[size_t(r) * kNQ];
+    const double V = rd.vol, nplaq = V * (ctx->D * (ctx->D - 1) / 2);
+    double* o = out + size_t(r) * 6;
+    o[0] = q[0] / (ctx->N * nplaq);
+    o[1] = (rd.beta / ctx->N) * (ctx->N * nplaq - q[0]);
+    o[2] = has_phi(ctx) ? q[2] - 2.0 * rd.kappa * q[1] : 0.0;
+    o[3] = has_phi(ctx) ? q[1] / (V * ctx->D) : 0.0;
+    o[4] = has_phi(ctx) ? q[3] / V : 0.0;
+    o[5] = pl[r] / ((V / rd.L[ctx->D - 1]) * ctx->N);
+  }
+  return GHX_OK;
+}
+
+int ghx_max_unitarity(ghx_ctx* ctx, int replica, double* out) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (replica < 0 || replica >= ctx->nrep || !out) return GHX_E_INVALID;
+  ctx->ops.unitarity(ctx->lc(), ctx->U[ctx->cur]);
+  const RepDesc& rd = ctx->hreps[replica];
+  std::vector<double> p(rd.blk_end - rd.blk_begin);
+  HIPCHK(hipMemcpyAsync(p.data(), ctx->partials + rd.blk_begin, sizeof(double) * p.size(), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  *out = p.empty() ? 0.0 : *std::max_element(p.begin(), p.end());
+  return GHX_OK;
+}
+
+int ghx_reunitarize(ghx_ctx* ctx) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  launch_settle(ctx, true);
+  HIPCHK(hipGetLastError());
+  return GHX_OK;
+}
+
+int ghx_get_fields(ghx_ctx* ctx, int replica, double* U, double* P, double* phi, double* pi) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (replica < 0 || replica >= ctx->nrep) return GHX_E_INVALID;
+  if (U && (rc = get_planes(ctx, ctx->U[ctx->cur], int(ctx->nU()), replica, U))) return rc;
+  if (P && (rc = get_planes(ctx, ctx->P, int(ctx->nP()), replica, P))) return rc;
+  if (has_phi(ctx)) {
+    if (phi && (rc = get_planes(ctx, ctx->phi[ctx->cur], int(ctx->nphi()), replica, phi))) return rc;
+    if (pi && (rc = get_planes(ctx, ctx->pi, int(ctx->nphi()), replica, pi))) return rc;
+  }
+  return GHX_OK;
+}
+
+int ghx_set_fields(ghx_ctx* ctx, int replica, const double* U, const double* P, const double* phi, const double* pi) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (replica < 0 || replica >= ctx->nrep) return GHX_E_INVALID;
+  if (U && (rc = set_planes(ctx, ctx->U[ctx->cur], int(ctx->nU()), replica, U))) return rc;
+  if (P && (rc = set_planes(ctx, ctx->P, int(ctx->nP()), replica, P))) return rc;
+  if (has_phi(ctx)) {
+    if (phi && (rc = set_planes(ctx, ctx->phi[ctx->cur], int(ctx->nphi()), replica, phi))) return rc;
+    if (pi && (rc = set_planes(ctx, ctx->pi, int(ctx->nphi()), replica, pi))) return rc;
+  }
+  return GHX_OK;
+}
+
+int ghx_forces(ghx_ctx* ctx, int replica, double* F_link, double* F_phi) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (replica < 0 || replica >= ctx->nrep) return GHX_E_INVALID;
+  const LaunchCtx l = ctx->lc();
+  const int b = ctx->cur;
+  ctx->ops.md_step(l, ctx->U[b], ctx->U[b ^ 1], ctx->P, ctx->phi[b], ctx->phi[b ^ 1], ctx->pi, ctx->F, ctx->Fphi, 0.0, 0.0,
+                   F_FORCE | F_ABS_EPS);
+  HIPCHK(hipGetLastError());
+  if (F_link && (rc = get_planes(ctx, ctx->F, int(ctx->nP()), replica, F_link))) return rc;
+  if (F_phi && has_phi(ctx) && (rc = get_planes(ctx, ctx->Fphi, int(ctx->nphi()), replica, F_phi))) return rc;
+  return GHX_OK;
+}
+
+int ghx_replica_swap(ghx_ctx* ctx, int i, int j, double u, int* accepted, double* delta) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (i < 0 || j < 0 || i >= ctx->nrep || j >= ctx->nrep || i == j) return GHX_E_INVALID;
+  const RepDesc &ri = ctx->hreps[i], &rj = ctx->hreps[j];
+  for (int m = 0; m < ctx->D; ++m)
+    if (ri.L[m] != rj.L[m]) return fail(ctx, GHX_E_INVALID, "replica swap needs equal extents");
+  if (ctx->cur != 0) launch_settle(ctx, false);
+  launch_action(ctx, 0);
+  std::vector<double> s(size_t(kNQ) * ctx->nrep);
+  HIPCHK(hipMemcpyAsync(s.data(), ctx->sums, sizeof(double) * s.size(), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  // S_r(C) = (beta_r/N)(N nplaq - sumReTr(C)) + sum V(C) - 2 kappa_r B(C); only the
+  // coupling-dependent terms survive in S_i(C_j)+S_j(C_i)-S_i(C_i)-S_j(C_j).
+  const double *qi = &s[size_t(i) * kNQ], *qj = &s[size_t(j) * kNQ];
+  if (has_phi(ctx) && ri.lambda != rj.lambda) return fail(ctx, GHX_E_INVALID, "replica swap needs equal lambda");
+  const double dgauge = (ri.beta - rj.beta) / ctx->N * (qi[0] - qj[0]);
+  const double dkappa = has_phi(ctx) ? (ri.kappa - rj.kappa) * 2.0 * (qi[1] - qj[1]) : 0.0;
+  const double Delta = dgauge + dkappa;
+  const bool acc = (Delta <= 0.0) || (u < std::exp(-Delta));
+  if (delta) *delta = Delta;
+  if (accepted) *accepted = acc ? 1 : 0;
+  if (!acc) return GHX_OK;
+  // exchange U and phi site ranges (same volume): planes are strided by Vp
+  const size_t ts = ctx->tsize;
+  void* tmp = nullptr;
+  HIPCHK(hipMalloc(&tmp, size_t(ri.vol) * ts));
+  auto swap_planes = [&](void* base, size_t nplanes) -> int {
+    for (size_t c = 0; c < nplanes; ++c) {
+      char* pi_ = (char*)base + (c * ctx->Vp + ri.offset) * ts;
+      char* pj_ = (char*)base + (c * ctx->Vp + rj.offset) * ts;
+      HIPCHK(hipMemcpyAsync(tmp, pi_, size_t(ri.vol) * ts, hipMemcpyDeviceToDevice, ctx->stream));
+      HIPCHK(hipMemcpyAsync(pi_, pj_, size_t(ri.vol) * ts, hipMemcpyDeviceToDevice, ctx->stream));
+      HIPCHK(hipMemcpyAsync(pj_, tmp, size_t(ri.vol) * ts, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    return GHX_OK;
+  };
+  if ((rc = swap_planes(ctx->U[0], ctx->nU()))) return rc;
+  if (has_phi(ctx) && (rc = swap_planes(ctx->phi[0], ctx->nphi()))) return rc;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipFree(tmp));
+  return GHX_OK;
+}
+
+int ghx_sync(ghx_ctx* ctx) {
+  if (!ctx) return GHX_E_INVALID;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return GHX_OK;
+}
+
+int ghx_time_md_steps(ghx_ctx* ctx, int steps, double* ms_total) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (steps < 1 || !ms_total) return GHX_E_INVALID;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, ctx->stream));
+  for (int i = 0; i < steps; ++i) launch_step(ctx, 1.0, 1.0, true);
+  HIPCHK(hipEventRecord(e1, ctx->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms_total = ms;
+  return GHX_OK;
+}
+
+int ghx_rep_query(int N, const char* spec, int* d, int* nality, int* real_rep, long* dimT, double* W, double* T) {
+  if (N != 2 && N != 3) return GHX_E_INVALID;
+  RepInfo r;
+  try {
+    r = build_rep(N, spec ? spec : "");
+  } catch (const std::exception&) {
+    return GHX_E_REP;
+  }
+  if (d) *d = r.d;
+  if (nality) *nality = r.nality;
+  if (real_rep) *real_rep = r.real ? 1 : 0;
+  if (dimT) *dimT = r.dimT;
+  if (W) std::copy(r.W.begin(), r.W.end(), W);
+  if (T)
+    for (size_t i = 0; i < r.T.size(); ++i) { T[2 * i] = r.T[i].real(); T[2 * i + 1] = r.T[i].imag(); }
+  return GHX_OK;
+}
+
+int64_t ghx_volume(const ghx_ctx* ctx) {
+  if (!ctx) return 0;
+  int64_t v = 0;
+  for (const auto& rd : ctx->hreps) v += rd.vol;
+  return v;
+}
+
+}  // extern "C"

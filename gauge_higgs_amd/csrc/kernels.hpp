@@ -1,0 +1,544 @@
+// gfx950 kernels of the MD step and its reductions, templated on storage type T,
+// colours N, dimension D and representation policy REP.
+//
+// HBM layout (structure of arrays, one plane per real component, each plane
+// holds every site of every replica, stride Vp):
+//   U   plane (mu*2N^2 + 2(iN+j) + re/im)      links, row-major N x N
+//   P   plane (mu*(N^2-1) + a)                  link momenta (generator order)
+//   phi plane (2k + re/im)                      scalar field, W-basis components
+// Replicas are laid back to back; each starts on a block boundary so a block
+// never straddles two lattices (blk_rep maps block -> replica).
+#pragma once
+#include "reps.hpp"
+
+namespace ghx {
+
+constexpr int kBlock = 128;     // threads per block of the per-site kernels
+constexpr int kRedBlock = 256;  // threads per block of the reduction kernels
+constexpr int kNQ = 6;          // per-site reduction quantities
+
+struct RepDesc {
+  int L[4];
+  unsigned stride[4];
+  unsigned offset;  // first global site of this replica (multiple of kBlock)
+  unsigned vol;
+  double beta, kappa, lambda, tau, eps;
+  unsigned long long seed;
+  unsigned blk_begin, blk_end;  // reduction blocks of this replica
+};
+
+enum StepFlags : int { F_KICK = 1, F_DRIFT = 2, F_FORCE = 4, F_ABS_EPS = 8 };
+
+template <typename T>
+struct StepArgs {
+  const RepDesc* reps;
+  const int* blk_rep;
+  unsigned Vp;
+  const T* U_in;
+  T* U_out;
+  T* P;
+  const T* phi_in;
+  T* phi_out;
+  T* pi;
+  T* F_out;
+  T* Fphi_out;
+  double ck, cd;  // kick / drift coefficient (times eps of the replica unless F_ABS_EPS)
+  int flags;
+};
+
+template <typename T, typename R, int N>
+GHX_DEV Mat<R, N> load_link(const T* __restrict__ U, unsigned Vp, int mu, unsigned g) {
+  Mat<R, N> m;
+  const T* p = U + size_t(mu) * (2 * N * N) * Vp + g;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      m.e[i][j].r = R(p[size_t(2 * (i * N + j)) * Vp]);
+      m.e[i][j].i = R(p[size_t(2 * (i * N + j) + 1) * Vp]);
+    }
+  return m;
+}
+template <typename T, typename R, int N>
+GHX_DEV void store_link(T* __restrict__ U, unsigned Vp, int mu, unsigned g, const Mat<R, N>& m) {
+  T* p = U + size_t(mu) * (2 * N * N) * Vp + g;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      p[size_t(2 * (i * N + j)) * Vp] = T(m.e[i][j].r);
+      p[size_t(2 * (i * N + j) + 1) * Vp] = T(m.e[i][j].i);
+    }
+}
+template <typename T, typename R, int DR>
+GHX_DEV void load_vec(const T* __restrict__ f, unsigned Vp, unsigned g, cx<R>* v) {
+#pragma unroll
+  for (int k = 0; k < DR; ++k) { v[k].r = R(f[size_t(2 * k) * Vp + g]); v[k].i = R(f[size_t(2 * k + 1) * Vp + g]); }
+}
+template <typename T, typename R, int DR>
+GHX_DEV void store_vec(T* __restrict__ f, unsigned Vp, unsigned g, const cx<R>* v) {
+#pragma unroll
+  for (int k = 0; k < DR; ++k) { f[size_t(2 * k) * Vp + g] = T(v[k].r); f[size_t(2 * k + 1) * Vp + g] = T(v[k].i); }
+}
+
+// Site -> neighbour offsets for a periodic lattice (geometry.hpp:20-38 order:
+// direction 0 fastest). up[m]/dn[m] are signed offsets to add to the site.
+template <int D>
+struct Nbr {
+  int up[D], dn[D];
+  int x[D];
+  GHX_DEV Nbr(const RepDesc& r, unsigned s) {
+#pragma unroll
+    for (int m = 0; m < D; ++m) {
+      const unsigned xm = (s / r.stride[m]) % unsigned(r.L[m]);
+      x[m] = int(xm);
+      const int st = int(r.stride[m]);
+      up[m] = (int(xm) == r.L[m] - 1) ? -(r.L[m] - 1) * st : st;
+      dn[m] = (xm == 0) ? (r.L[m] - 1) * st : -st;
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Fused MD step: kick (P -= eps F_link, pi += eps F_phi) then drift
+// (U' = exp(i eps P) U, phi' = phi + eps pi) for one site's D links and phi.
+// With F_FORCE the forces are written out instead (parity / FD tests).
+// Reads U_in/phi_in (all neighbours), writes U_out/phi_out (double buffer),
+// updates P and pi in place. Gauge force: gauge_wilson.hpp:12-27,71-83;
+// matter forces: scalar_higgs.hpp:156-213; drift: gauge_higgs_hmc.hpp:136-147.
+// ---------------------------------------------------------------------------
+template <typename T, int N, int D, class REP>
+__global__ __launch_bounds__(kBlock) void md_step_kernel(StepArgs<T> a, typename REP::template Params<T> rp) {
+  using R = T;
+  constexpr int NC = 2 * N * N;
+  constexpr int NGN = N * N - 1;
+  constexpr int DR = REP::DR;
+  const int r = a.blk_rep[blockIdx.x];
+  const RepDesc& rd = a.reps[r];
+  const unsigned s = blockIdx.x * kBlock + threadIdx.x - rd.offset;
+  if (s >= rd.vol) return;
+  const unsigned g = rd.offset + s;
+  const unsigned Vp = a.Vp;
+  const Nbr<D> nb(rd, s);
+  const bool abs_eps = a.flags & F_ABS_EPS;
+  const R ek = R(abs_eps ? a.ck : a.ck * rd.eps);
+  const R ed = R(abs_eps ? a.cd : a.cd * rd.eps);
+  const R bN = R(rd.beta / N);
+  (void)NC;
+
+  cx<R> phx[DR > 0 ? DR : 1], fph[DR > 0 ? DR : 1];
+  R k2 = R(0);
+  if constexpr (DR > 0) {
+    k2 = R(2.0 * rd.kappa);
+    load_vec<T, R, DR>(a.phi_in, Vp, g, phx);
+    R n2 = R(0);
+#pragma unroll
+    for (int k = 0; k < DR; ++k) n2 += phx[k].r * phx[k].r + phx[k].i * phx[k].i;
+    const R c = R(-2) * (R(1) + R(2.0 * rd.lambda) * (n2 - R(1)));  // -2 dV/dphi^*
+#pragma unroll
+    for (int k = 0; k < DR; ++k) fph[k] = cscale(phx[k], c);
+  }
+
+#pragma unroll 1
+  for (int mu = 0; mu < D; ++mu) {
+    const Mat<R, N> Ux = load_link<T, R, N>(a.U_in, Vp, mu, g);
+    Mat<R, N> Sig = mzero<R, N>();
+    const unsigned gpm = g + nb.up[mu];
+#pragma unroll 1
+    for (int nu = 0; nu < D; ++nu) {
+      if (nu == mu) continue;
+      // forward staple U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag
+      {
+        const Mat<R, N> A = load_link<T, R, N>(a.U_in, Vp, nu, gpm);
+        const Mat<R, N> B = load_link<T, R, N>(a.U_in, Vp, mu, g + nb.up[nu]);
+        const Mat<R, N> C = load_link<T, R, N>(a.U_in, Vp, nu, g);
+        mac_nd(Sig, mul_nd(A, B), C);
+      }
+      // backward staple U_nu(x+mu-nu)^dag U_mu(x-nu)^dag U_nu(x-nu)
+      {
+        const unsigned gmn = g + nb.dn[nu];
+        const Mat<R, N> A = load_link<T, R, N>(a.U_in, Vp, nu, gpm + nb.dn[nu]);
+        const Mat<R, N> B = load_link<T, R, N>(a.U_in, Vp, mu, gmn);
+        const Mat<R, N> C = load_link<T, R, N>(a.U_in, Vp, nu, gmn);
+        mac_nn(Sig, mul_dd(A, B), C);
+      }
+    }
+    Mat<R, N> X = mul_nn(Ux, Sig);
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int j = 0; j < N; ++j) X.e[i][j] = cscale(X.e[i][j], bN);
+    if constexpr (DR > 0) {
+      cx<R> phy[DR], psi[DR];
+      load_vec<T, R, DR>(a.phi_in, Vp, gpm, phy);
+      Mat<R, N> M;
+      REP::fwd_M(Ux, phy, phx, psi, M, rp);
+#pragma unroll
+      for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+          X.e[i][j].r = fma(k2, M.e[i][j].r, X.e[i][j].r);
+          X.e[i][j].i = fma(k2, M.e[i][j].i, X.e[i][j].i);
+        }
+      const unsigned gmm = g + nb.dn[mu];
+      cx<R> phb[DR], chi[DR];
+      load_vec<T, R, DR>(a.phi_in, Vp, gmm, phb);
+      const Mat<R, N> Ub = load_link<T, R, N>(a.U_in, Vp, mu, gmm);
+      REP::apply_dag(Ub, phb, chi, rp);
+#pragma unroll
+      for (int k = 0; k < DR; ++k) {
+        fph[k].r = fma(k2, psi[k].r + chi[k].r, fph[k].r);
+        fph[k].i = fma(k2, psi[k].i + chi[k].i, fph[k].i);
+      }
+    }
+    R f[NGN];
+    im_tr_gen<R, N>(X, f);
+    T* Pm = a.P + size_t(mu) * NGN * Vp + g;
+    if (a.flags & F_FORCE) {
+      T* Fm = a.F_out + size_t(mu) * NGN * Vp + g;
+#pragma unroll
+      for (int c = 0; c < NGN; ++c) Fm[size_t(c) * Vp] = T(f[c]);
+      continue;
+    }
+    R p[NGN];
+#pragma unroll
+    for (int c = 0; c < NGN; ++c) p[c] = R(Pm[size_t(c) * Vp]);
+    if (a.flags & F_KICK) {
+#pragma unroll
+      for (int c = 0; c < NGN; ++c) { p[c] = fma(-ek, f[c], p[c]); Pm[size_t(c) * Vp] = T(p[c]); }
+    }
+    if (a.flags & F_DRIFT) {
+      const Mat<R, N> H = alg_to_mat<R, N>(p, ed);
+      const Mat<R, N> E = expi<R, N>(H);
+      store_link<T, R, N>(a.U_out, Vp, mu, g, mul_nn(E, Ux));
+    }
+  }
+  if constexpr (DR > 0) {
+    if (a.flags & F_FORCE) { store_vec<T, R, DR>(a.Fphi_out, Vp, g, fph); return; }
+    cx<R> pv[DR];
+    load_vec<T, R, DR>(a.pi, Vp, g, pv);
+    if (a.flags & F_KICK) {
+#pragma unroll
+      for (int k = 0; k < DR; ++k) { pv[k].r = fma(ek, fph[k].r, pv[k].r); pv[k].i = fma(ek, fph[k].i, pv[k].i); }
+      store_vec<T, R, DR>(a.pi, Vp, g, pv);
+    }
+    if (a.flags & F_DRIFT) {
+#pragma unroll
+      for (int k = 0; k < DR; ++k) { phx[k].r = fma(ed, pv[k].r, phx[k].r); phx[k].i = fma(ed, pv[k].i, phx[k].i); }
+      store_vec<T, R, DR>(a.phi_out, Vp, g, phx);
+    }
+  }
+}
+
+// Drift alone (U' = exp(i eps P) U, phi' = phi + eps pi) — for the kick/drift API.
+template <typename T, int N, int D, int DR>
+__global__ __launch_bounds__(kBlock) void drift_kernel(StepArgs<T> a) {
+  using R = T;
+  constexpr int NGN = N * N - 1;
+  const int r = a.blk_rep[blockIdx.x];
+  const RepDesc& rd = a.reps[r];
+  const unsigned s = blockIdx.x * kBlock + threadIdx.x - rd.offset;
+  if (s >= rd.vol) return;
+  const unsigned g = rd.offset + s, Vp = a.Vp;
+  const R ed = R((a.flags & F_ABS_EPS) ? a.cd : a.cd * rd.eps);
+#pragma unroll 1
+  for (int mu = 0; mu < D; ++mu) {
+    R p[NGN];
+    const T* Pm = a.P + size_t(mu) * NGN * Vp + g;
+#pragma unroll
+    for (int c = 0; c < NGN; ++c) p[c] = R(Pm[size_t(c) * Vp]);
+    const Mat<R, N> Ux = load_link<T, R, N>(a.U_in, Vp, mu, g);
+    store_link<T, R, N>(a.U_out, Vp, mu, g, mul_nn(expi<R, N>(alg_to_mat<R, N>(p, ed)), Ux));
+  }
+  if constexpr (DR > 0) {
+    cx<R> ph[DR], pv[DR];
+    load_vec<T, R, DR>(a.phi_in, Vp, g, ph);
+    load_vec<T, R, DR>(a.pi, Vp, g, pv);
+#pragma unroll
+    for (int k = 0; k < DR; ++k) { ph[k].r = fma(ed, pv[k].r, ph[k].r); ph[k].i = fma(ed, pv[k].i, ph[k].i); }
+    store_vec<T, R, DR>(a.phi_out, Vp, g, ph);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic FP64 block reduction of kNQ quantities per thread.
+// ---------------------------------------------------------------------------
+template <int NQ>
+GHX_DEV void block_reduce_store(double (&q)[NQ], double* __restrict__ out) {
+  __shared__ double sh[NQ][kRedBlock / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) {
+    double v = q[k];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if (lane == 0) sh[k][wid] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NQ) {
+    double v = 0.0;
+#pragma unroll
+    for (int w = 0; w < kRedBlock / 64; ++w) v += sh[threadIdx.x][w];
+    out[threadIdx.x] = v;
+  }
+}
+
+struct ActArgs {
+  const RepDesc* reps;
+  const int* red_rep;  // reduction block -> replica
+  const unsigned* red_first;  // reduction block -> first global site
+  unsigned Vp;
+  double* partials;  // [nblocks][kNQ]
+};
+
+// Per-site FP64 contributions to H and the observables:
+//  q0 sum_{mu<nu} Re Tr U_pl (gauge_wilson.hpp:32-65)
+//  q1 sum_mu Re phi_x^dag D(U_mu(x)) phi_{x+mu} (scalar_higgs.hpp:128-143, 217-229)
+//  q2 V(phi_x), q3 |phi_x|^2 (scalar_higgs.hpp:111-125, 232-239)
+//  q4 sum P^2 (fields.hpp:69-74), q5 sum |pi|^2 (scalar_field.hpp:53-57)
+template <typename T, int N, int D, class REPD>
+__global__ __launch_bounds__(kRedBlock) void action_kernel(ActArgs a, const T* __restrict__ U, const T* __restrict__ P,
+                                                          const T* __restrict__ phi, const T* __restrict__ pi,
+                                                          typename REPD::template Params<double> rp) {
+  using R = double;
+  constexpr int NGN = N * N - 1;
+  constexpr int DR = REPD::DR;
+  const int r = a.red_rep[blockIdx.x];
+  const RepDesc& rd = a.reps[r];
+  const unsigned s = a.red_first[blockIdx.x] - rd.offset + threadIdx.x;
+  double q[kNQ] = {0, 0, 0, 0, 0, 0};
+  if (s < rd.vol) {
+    const unsigned g = rd.offset + s, Vp = a.Vp;
+    const Nbr<D> nb(rd, s);
+#pragma unroll 1
+    for (int mu = 0; mu < D; ++mu) {
+      const Mat<R, N> Ux = load_link<T, R, N>(U, Vp, mu, g);
+      const unsigned gpm = g + nb.up[mu];
+#pragma unroll 1
+      for (int nu = mu + 1; nu < D; ++nu) {
+        const Mat<R, N> A = mul_nn(Ux, load_link<T, R, N>(U, Vp, nu, gpm));
+        const Mat<R, N> B = mul_nn(load_link<T, R, N>(U, Vp, nu, g), load_link<T, R, N>(U, Vp, mu, g + nb.up[nu]));
+        q[0] += retr_nd(A, B);  // Re Tr(U_mu U_nu(x+mu) (U_nu U_mu(x+nu))^dag)
+      }
+      const T* Pm = P + size_t(mu) * NGN * Vp + g;
+#pragma unroll
+      for (int c = 0; c < NGN; ++c) { const double v = double(Pm[size_t(c) * Vp]); q[4] += v * v; }
+      if constexpr (DR > 0) {
+        cx<R> px[DR], py[DR], dy[DR];
+        load_vec<T, R, DR>(phi, Vp, g, px);
+        load_vec<T, R, DR>(phi, Vp, gpm, py);
+        REPD::apply(Ux, py, dy, rp);
+#pragma unroll
+        for (int k = 0; k < DR; ++k) q[1] += px[k].r * dy[k].r + px[k].i * dy[k].i;
+      }
+    }
+    if constexpr (DR > 0) {
+      cx<R> px[DR], pv[DR];
+      load_vec<T, R, DR>(phi, Vp, g, px);
+      load_vec<T, R, DR>(pi, Vp, g, pv);
+      double n2 = 0.0, p2 = 0.0;
+#pragma unroll
+      for (int k = 0; k < DR; ++k) { n2 += px[k].r * px[k].r + px[k].i * px[k].i; p2 += pv[k].r * pv[k].r + pv[k].i * pv[k].i; }
+      q[2] = n2 + rd.lambda * (n2 - 1.0) * (n2 - 1.0);
+      q[3] = n2;
+      q[5] = p2;
+    }
+  }
+  block_reduce_store<kNQ>(q, a.partials + size_t(blockIdx.x) * kNQ);
+}
+
+// Sum the partials of each replica in a fixed order: one block per replica.
+static __global__ __launch_bounds__(kRedBlock) void finalize_kernel(const RepDesc* reps, const double* partials, int nq, double* sums) {
+  const RepDesc& rd = reps[blockIdx.x];
+  __shared__ double sh[kRedBlock];
+  for (int k = 0; k < nq; ++k) {
+    double v = 0.0;
+    for (unsigned b = rd.blk_begin + threadIdx.x; b < rd.blk_end; b += kRedBlock) v += partials[size_t(b) * nq + k];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = kRedBlock / 2; o > 0; o >>= 1) {
+      if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) sums[size_t(blockIdx.x) * nq + k] = sh[0];
+    __syncthreads();
+  }
+}
+
+// Hamiltonian parts from the sums: [KE_P, KE_pi, S_g, S_H, H] (gauge_higgs_hmc.hpp:105-110, hmc.hpp:32)
+static __global__ void hamiltonian_kernel(const RepDesc* reps, int nrep, int N, int D, int has_phi, const double* sums, double* H) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrep) return;
+  const RepDesc& rd = reps[r];
+  const double* q = sums + size_t(r) * kNQ;
+  const double nplaq = double(rd.vol) * (D * (D - 1) / 2);
+  const double KP = 0.5 * q[4];
+  const double Kpi = has_phi ? 0.5 * q[5] : 0.0;
+  const double Sg = (rd.beta / N) * (double(N) * nplaq - q[0]);
+  const double SH = has_phi ? (q[2] - 2.0 * rd.kappa * q[1]) : 0.0;
+  double* h = H + size_t(r) * 5;
+  h[0] = KP; h[1] = Kpi; h[2] = Sg; h[3] = SH; h[4] = KP + Kpi + Sg + SH;
+}
+
+// Metropolis accept/reject per replica, FP64 (gauge_higgs_hmc.hpp:183-191, hmc.hpp:83-90).
+struct McState {
+  unsigned long long traj_count, accept_count;
+  double last_dH;
+  int accepted;
+  int pad;
+};
+static __global__ void metropolis_kernel(const RepDesc* reps, int nrep, const double* Hi, const double* Hf, McState* st,
+                                  unsigned long long tag) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrep) return;
+  McState& m = st[r];
+  const double dH = Hf[size_t(r) * 5 + 4] - Hi[size_t(r) * 5 + 4];
+  m.last_dH = dH;
+  m.traj_count += 1;
+  const double u = draw_uniform(reps[r].seed, stream_key(tag, m.traj_count));
+  const bool acc = (dH <= 0.0) || (u < exp(-dH));
+  m.accepted = acc ? 1 : 0;
+  if (acc) m.accept_count += 1;
+}
+
+// Restore U, phi from the saved copies for rejected replicas.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void restore_kernel(const RepDesc* reps, const int* blk_rep, const McState* st, unsigned Vp,
+                                                        T* U, const T* Us, int nU, T* phi, const T* phis, int nphi) {
+  const int r = blk_rep[blockIdx.x];
+  if (st[r].accepted) return;
+  const RepDesc& rd = reps[r];
+  const unsigned s = blockIdx.x * kBlock + threadIdx.x - rd.offset;
+  if (s >= rd.vol) return;
+  const unsigned g = rd.offset + s;
+  for (int c = 0; c < nU; ++c) U[size_t(c) * Vp + g] = Us[size_t(c) * Vp + g];
+  for (int c = 0; c < nphi; ++c) phi[size_t(c) * Vp + g] = phis[size_t(c) * Vp + g];
+}
+
+// Momentum heatbath (fields.hpp:59-66, scalar_field.hpp:40-51): P^a from
+// key(stream, s, mu, a), pi_k from key(stream', s, 2k[+1]); FP64 Gaussians.
+template <typename T, int N, int D>
+__global__ __launch_bounds__(kBlock) void refresh_kernel(const RepDesc* reps, const int* blk_rep, unsigned Vp, T* P, T* pi, int d,
+                                                        int real_rep, const McState* st, unsigned long long tagP,
+                                                        unsigned long long tagPi) {
+  constexpr int NGN = N * N - 1;
+  const int r = blk_rep[blockIdx.x];
+  const RepDesc& rd = reps[r];
+  const unsigned s = blockIdx.x * kBlock + threadIdx.x - rd.offset;
+  if (s >= rd.vol) return;
+  const unsigned g = rd.offset + s;
+  const unsigned long long t = st[r].traj_count;
+  const unsigned long long sP = stream_key(tagP, t);
+  for (int mu = 0; mu < D; ++mu)
+    for (int c = 0; c < NGN; ++c)
+      P[size_t(mu * NGN + c) * Vp + g] = T(draw_gauss(rd.seed, stream_key(sP, s, mu, c)));
+  if (d > 0) {
+    const unsigned long long sPi = stream_key(tagPi, t);
+    for (int k = 0; k < d; ++k) {
+      pi[size_t(2 * k) * Vp + g] = T(draw_gauss(rd.seed, stream_key(sPi, s, 2 * k)));
+      pi[size_t(2 * k + 1) * Vp + g] = real_rep ? T(0) : T(draw_gauss(rd.seed, stream_key(sPi, s, 2 * k + 1)));
+    }
+  }
+}
+
+// Start configurations: mode 0 cold (U = 1), 1 reference hot start
+// U = exp(i sigma g_a T^a), g_a = gauss(key(1, s, mu, a)) (fields.hpp:25-34).
+template <typename T, int N, int D>
+__global__ __launch_bounds__(kBlock) void init_links_kernel(const RepDesc* reps, const int* blk_rep, unsigned Vp, T* U, int replica,
+                                                           int mode, double sigma) {
+  constexpr int NGN = N * N - 1;
+  const int r = blk_rep[blockIdx.x];
+  if (r != replica) return;
+  const RepDesc& rd = reps[r];
+  const unsigned s = blockIdx.x * kBlock + threadIdx.x - rd.offset;
+  if (s >= rd.vol) return;
+  const unsigned g = rd.offset + s;
+  for (int mu = 0; mu < D; ++mu) {
+    Mat<double, N> m;
+    if (mode == 0) m = meye<double, N>();
+    else {
+      double v[NGN];
+      for (int c = 0; c < NGN; ++c) v[c] = sigma * draw_gauss(rd.seed, stream_key(1, s, mu, c));
+      m = expi<double, N>(alg_to_mat<double, N>(v, 1.0));
+    }
+    store_link<T, double, N>(U, Vp, mu, g, m);
+  }
+}
+// phi: mode 0 cold (phi_0 = c), 1 Gaussian with sigma from key(stream, s, 2k[+1]) (scalar_field.hpp:33-51)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void init_phi_kernel(const RepDesc* reps, const int* blk_rep, unsigned Vp, T* phi, int d, int real_rep,
+                                                         int replica, int mode, unsigned long long stream, double sigma) {
+  const int r = blk_rep[blockIdx.x];
+  if (r != replica) return;
+  const RepDesc& rd = reps[r];
+  const unsigned s = blockIdx.x * kBlock + threadIdx.x - rd.offset;
+  if (s >= rd.vol) return;
+  const unsigned g = rd.offset + s;
+  for (int k = 0; k < d; ++k) {
+    double re, im;
+    if (mode == 0) { re = (k == 0) ? sigma : 0.0; im = 0.0; }
+    else {
+      re = sigma * draw_gauss(rd.seed, stream_key(stream, s, 2 * k));
+      im = real_rep ? 0.0 : sigma * draw_gauss(rd.seed, stream_key(stream, s, 2 * k + 1));
+    }
+    phi[size_t(2 * k) * Vp + g] = T(re);
+    phi[size_t(2 * k + 1) * Vp + g] = T(im);
+  }
+}
+
+// Reunitarize in FP64 (sun.hpp:49-72), reading U_in and writing U_out.
+template <typename T, int N, int D>
+__global__ __launch_bounds__(kBlock) void reunit_kernel(const RepDesc* reps, const int* blk_rep, unsigned Vp, const T* U_in, T* U_out) {
+  const int r = blk_rep[blockIdx.x];
+  const RepDesc& rd = reps[r];
+  const unsigned s = blockIdx.x * kBlock + threadIdx.x - rd.offset;
+  if (s >= rd.vol) return;
+  const unsigned g = rd.offset + s;
+  for (int mu = 0; mu < D; ++mu) {
+    Mat<double, N> m = load_link<T, double, N>(U_in, Vp, mu, g);
+    reunitarize<double, N>(m);
+    store_link<T, double, N>(U_out, Vp, mu, g, m);
+  }
+}
+
+// Max ||U^dag U - 1|| per reduction block (fields.hpp:38-42).
+template <typename T, int N, int D>
+__global__ __launch_bounds__(kRedBlock) void unitarity_kernel(ActArgs a, const T* U) {
+  const int r = a.red_rep[blockIdx.x];
+  const RepDesc& rd = a.reps[r];
+  const unsigned s = a.red_first[blockIdx.x] - rd.offset + threadIdx.x;
+  double w = 0.0;
+  if (s < rd.vol) {
+    const unsigned g = rd.offset + s;
+    for (int mu = 0; mu < D; ++mu) w = fmax(w, unitarity_violation<double, N>(load_link<T, double, N>(U, a.Vp, mu, g)));
+  }
+  __shared__ double sh[kRedBlock];
+  sh[threadIdx.x] = w;
+  __syncthreads();
+  for (int o = kRedBlock / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) a.partials[blockIdx.x] = sh[0];
+}
+
+// Polyakov loop in direction D-1 (observables.hpp:18-34): one thread per
+// transverse site; per-reduction-block partial sums of Re Tr.
+template <typename T, int N, int D>
+__global__ __launch_bounds__(kRedBlock) void polyakov_kernel(ActArgs a, const T* U) {
+  const int r = a.red_rep[blockIdx.x];
+  const RepDesc& rd = a.reps[r];
+  const unsigned s = a.red_first[blockIdx.x] - rd.offset + threadIdx.x;
+  double q[1] = {0.0};
+  const unsigned Lt = unsigned(rd.L[D - 1]);
+  const unsigned nt = rd.vol / Lt;  // transverse sites are s < nt (x_{D-1} = 0)
+  if (s < nt) {
+    const unsigned st = rd.stride[D - 1];
+    Mat<double, N> Pl = load_link<T, double, N>(U, a.Vp, D - 1, rd.offset + s);
+    for (unsigned t = 1; t < Lt; ++t) Pl = mul_nn(Pl, load_link<T, double, N>(U, a.Vp, D - 1, rd.offset + s + t * st));
+    q[0] = retrace(Pl);
+  }
+  block_reduce_store<1>(q, a.partials + blockIdx.x);
+}
+
+}  // namespace ghx

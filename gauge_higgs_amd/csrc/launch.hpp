@@ -1,0 +1,75 @@
+// Templated launchers that fill an Ops table for one (T, N, D, rep) combination.
+#pragma once
+#include "kernels.hpp"
+#include "ops.hpp"
+
+namespace ghx {
+
+template <class REP, typename P>
+struct ParamFill {
+  static typename REP::template Params<P> make(const LaunchCtx&) { return {}; }
+};
+template <typename R, int N, int NB, int DR, typename P>
+struct ParamFill<RepTensor<R, N, NB, DR>, P> {
+  using Rep = RepTensor<R, N, NB, DR>;
+  static typename Rep::template Params<P> make(const LaunchCtx& c) {
+    typename Rep::template Params<P> p;
+    for (int l = 0; l < DR; ++l)
+      for (int t = 0; t < Rep::DIMT; ++t) p.W[l][t] = P(c.W[l * Rep::DIMT + t]);
+    return p;
+  }
+};
+
+template <typename T, int N, int D, template <typename, int> class REPT>
+struct Launch {
+  using REP = REPT<T, N>;
+  using REPD = REPT<double, N>;
+  static void md_step(const LaunchCtx& c, const void* U_in, void* U_out, void* P, const void* phi_in, void* phi_out, void* pi,
+                      void* F, void* Fphi, double ck, double cd, int flags) {
+    StepArgs<T> a{c.reps, c.blk_rep, c.Vp, (const T*)U_in, (T*)U_out, (T*)P, (const T*)phi_in, (T*)phi_out, (T*)pi, (T*)F, (T*)Fphi,
+                  ck, cd, flags};
+    hipLaunchKernelGGL((md_step_kernel<T, N, D, REP>), dim3(c.nblk), dim3(kBlock), 0, c.stream, a, ParamFill<REP, T>::make(c));
+  }
+  static void drift(const LaunchCtx& c, const void* U_in, void* U_out, const void* P, const void* phi_in, void* phi_out,
+                    const void* pi, double cd, int flags) {
+    StepArgs<T> a{c.reps, c.blk_rep, c.Vp, (const T*)U_in, (T*)U_out, (T*)P, (const T*)phi_in, (T*)phi_out, (T*)pi, nullptr, nullptr,
+                  0.0, cd, flags};
+    hipLaunchKernelGGL((drift_kernel<T, N, D, REP::DR>), dim3(c.nblk), dim3(kBlock), 0, c.stream, a);
+  }
+  static void action(const LaunchCtx& c, const void* U, const void* P, const void* phi, const void* pi) {
+    ActArgs a{c.reps, c.red_rep, c.red_first, c.Vp, c.partials};
+    hipLaunchKernelGGL((action_kernel<T, N, D, REPD>), dim3(c.nred), dim3(kRedBlock), 0, c.stream, a, (const T*)U, (const T*)P,
+                       (const T*)phi, (const T*)pi, ParamFill<REPD, double>::make(c));
+  }
+  static void refresh(const LaunchCtx& c, void* P, void* pi, const McState* st, unsigned long long tagP, unsigned long long tagPi) {
+    hipLaunchKernelGGL((refresh_kernel<T, N, D>), dim3(c.nblk), dim3(kBlock), 0, c.stream, c.reps, c.blk_rep, c.Vp, (T*)P, (T*)pi,
+                       REP::DR, c.real_rep, st, tagP, tagPi);
+  }
+  static void init_links(const LaunchCtx& c, void* U, int replica, int mode, double sigma) {
+    hipLaunchKernelGGL((init_links_kernel<T, N, D>), dim3(c.nblk), dim3(kBlock), 0, c.stream, c.reps, c.blk_rep, c.Vp, (T*)U, replica,
+                       mode, sigma);
+  }
+  static void reunit(const LaunchCtx& c, const void* U_in, void* U_out) {
+    hipLaunchKernelGGL((reunit_kernel<T, N, D>), dim3(c.nblk), dim3(kBlock), 0, c.stream, c.reps, c.blk_rep, c.Vp, (const T*)U_in,
+                       (T*)U_out);
+  }
+  static void unitarity(const LaunchCtx& c, const void* U) {
+    ActArgs a{c.reps, c.red_rep, c.red_first, c.Vp, c.partials};
+    hipLaunchKernelGGL((unitarity_kernel<T, N, D>), dim3(c.nred), dim3(kRedBlock), 0, c.stream, a, (const T*)U);
+  }
+  static void polyakov(const LaunchCtx& c, const void* U) {
+    ActArgs a{c.reps, c.red_rep, c.red_first, c.Vp, c.partials};
+    hipLaunchKernelGGL((polyakov_kernel<T, N, D>), dim3(c.nred), dim3(kRedBlock), 0, c.stream, a, (const T*)U);
+  }
+  static Ops ops() { return Ops{&md_step, &drift, &action, &refresh, &init_links, &reunit, &unitarity, &polyakov}; }
+};
+
+// Dispatch helper over precision and D in {3, 4}.
+template <int N, template <typename, int> class REPT>
+bool pick(int prec, int D, Ops* o) {
+  if (D == 4) { *o = prec ? Launch<float, N, 4, REPT>::ops() : Launch<double, N, 4, REPT>::ops(); return true; }
+  if (D == 3) { *o = prec ? Launch<float, N, 3, REPT>::ops() : Launch<double, N, 3, REPT>::ops(); return true; }
+  return false;
+}
+
+}  // namespace ghx

@@ -1,0 +1,43 @@
+// Launch table: one set of launchers per compiled (precision, N, D, rep)
+// combination. Instantiations are spread over the inst_*.hip translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace ghx {
+
+struct RepDesc;
+struct McState;
+
+struct LaunchCtx {
+  hipStream_t stream;
+  unsigned nblk, nred, Vp;
+  const RepDesc* reps;
+  const int* blk_rep;
+  const int* red_rep;
+  const unsigned* red_first;
+  double* partials;
+  const double* W;  // tensor basis on the host, d x dimT (tensor reps)
+  int d, real_rep;
+};
+
+struct Ops {
+  void (*md_step)(const LaunchCtx&, const void* U_in, void* U_out, void* P, const void* phi_in, void* phi_out, void* pi,
+                  void* F, void* Fphi, double ck, double cd, int flags);
+  void (*drift)(const LaunchCtx&, const void* U_in, void* U_out, const void* P, const void* phi_in, void* phi_out,
+                const void* pi, double cd, int flags);
+  void (*action)(const LaunchCtx&, const void* U, const void* P, const void* phi, const void* pi);
+  void (*refresh)(const LaunchCtx&, void* P, void* pi, const McState* st, unsigned long long tagP, unsigned long long tagPi);
+  void (*init_links)(const LaunchCtx&, void* U, int replica, int mode, double sigma);
+  void (*reunit)(const LaunchCtx&, const void* U_in, void* U_out);
+  void (*unitarity)(const LaunchCtx&, const void* U);
+  void (*polyakov)(const LaunchCtx&, const void* U);
+};
+
+// Returns false when the combination is not compiled. kind: RepKind; for
+// tensor reps (nbox, d) select the instantiation.
+bool ops_none(int prec, int N, int D, Ops* o);
+bool ops_fund(int prec, int N, int D, Ops* o);
+bool ops_adj(int prec, int N, int D, Ops* o);
+bool ops_tensor(int prec, int N, int D, int nbox, int d, Ops* o);
+
+}  // namespace ghx

@@ -1,0 +1,263 @@
+"""Python mirror of the reference HMC interface over the ghx C ABI.
+
+`GaugeHMC` mirrors hmc.hpp:13-96 and `GaugeHiggsHMC` mirrors
+gauge_higgs_hmc.hpp:113-293: the same public couplings (beta, kappa, lambda,
+tau, nmd, integ, reunit_each_traj), counters (traj_count, accept_count,
+last_dH) and methods (trajectory, md_evolve, kick, drift, hamiltonian,
+refresh_momenta). Every call goes to the HIP kernels in libghx.so; there is no
+CPU fallback. Fields are exchanged in the reference AoS layout as numpy arrays:
+U (V*D, N, N) complex128, P (V*D, N*N-1), phi/pi (V, d) complex128.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _lib
+
+LEAPFROG, OMELYAN = 0, 1
+FP64, FP32MD = 0, 1
+COLD, HOT = 0, 1
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+
+
+class GhxError(RuntimeError):
+    pass
+
+
+class Lattices:
+    """A ghx context: one or more replicas of the same (D, N, rep, precision)."""
+
+    def __init__(self, D, N, extents, rep="", precision=FP64):
+        lib = _lib.load()
+        self._lib = lib
+        ext = [list(e) if not np.isscalar(e) else [e] * D for e in extents]
+        self.D, self.N, self.nrep = D, N, len(ext)
+        self.extents = ext
+        flat = (C.c_int * (D * self.nrep))(*[x for e in ext for x in e])
+        h = C.c_void_p()
+        rc = lib.ghx_create(C.byref(h), D, N, self.nrep, flat, precision)
+        if rc != 0:
+            raise GhxError(f"ghx_create failed ({rc})")
+        self.h = h
+        self.precision = precision
+        d = C.c_int()
+        self._check(lib.ghx_set_rep(self.h, rep.encode(), C.byref(d)), GhxError)
+        nal, real, dimT = C.c_int(), C.c_int(), C.c_long()
+        self._check(lib.ghx_rep_info(self.h, C.byref(d), C.byref(nal), C.byref(real), C.byref(dimT)))
+        self.d, self.nality, self.real, self.dimT = d.value, nal.value, bool(real.value), dimT.value
+        self.ngen = N * N - 1
+        self.vols = [int(np.prod(e)) for e in ext]
+        self.nmd, self.integ, self.reunit = 20, OMELYAN, True
+
+    def _check(self, rc, exc=GhxError):
+        if rc != 0:
+            raise exc(f"ghx error {rc}: {self._lib.ghx_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.ghx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- configuration -------------------------------------------------------
+    def set_couplings(self, replica, beta, kappa=0.0, lam=0.0, tau=1.0):
+        self._check(self._lib.ghx_set_couplings(self.h, replica, beta, kappa, lam, tau))
+
+    def set_integrator(self, nmd, integ=OMELYAN, reunit=True):
+        self.nmd, self.integ, self.reunit = nmd, integ, reunit
+        self._check(self._lib.ghx_set_integrator(self.h, nmd, integ, int(reunit)))
+
+    def set_seed(self, replica, seed):
+        self._check(self._lib.ghx_set_seed(self.h, replica, seed))
+
+    def init_fields(self, replica, link_mode=HOT, sigma_U=0.8, phi_mode=1, phi_stream=12345, phi_sigma=0.3):
+        self._check(self._lib.ghx_init_fields(self.h, replica, link_mode, sigma_U, phi_mode, phi_stream, phi_sigma))
+
+    def set_counters(self, replica, traj, acc):
+        self._check(self._lib.ghx_set_counters(self.h, replica, traj, acc))
+
+    def counters(self, replica=0):
+        t, a, dh = C.c_uint64(), C.c_uint64(), C.c_double()
+        self._check(self._lib.ghx_get_counters(self.h, replica, C.byref(t), C.byref(a), C.byref(dh)))
+        return t.value, a.value, dh.value
+
+    # --- MD ----------------------------------------------------------------------
+    def refresh_momenta(self):
+        self._check(self._lib.ghx_refresh_momenta(self.h))
+
+    def kick(self, eps):
+        self._check(self._lib.ghx_kick(self.h, eps))
+
+    def drift(self, eps):
+        self._check(self._lib.ghx_drift(self.h, eps))
+
+    def md_evolve(self):
+        self._check(self._lib.ghx_md_evolve(self.h))
+
+    def hamiltonian_parts(self):
+        out = np.zeros(5 * self.nrep)
+        self._check(self._lib.ghx_hamiltonian(self.h, _dp(out)))
+        return out.reshape(self.nrep, 5)
+
+    def trajectory_all(self):
+        acc = (C.c_int * self.nrep)()
+        dH = np.zeros(self.nrep)
+        self._check(self._lib.ghx_trajectory(self.h, acc, _dp(dH)))
+        return np.array(list(acc), dtype=bool), dH
+
+    def trajectories(self, n, use_graph=True):
+        self._check(self._lib.ghx_trajectories(self.h, n, int(use_graph)))
+
+    def observables_all(self):
+        out = np.zeros(6 * self.nrep)
+        self._check(self._lib.ghx_observables(self.h, _dp(out)))
+        return out.reshape(self.nrep, 6)
+
+    def max_unitarity(self, replica=0):
+        v = C.c_double()
+        self._check(self._lib.ghx_max_unitarity(self.h, replica, C.byref(v)))
+        return v.value
+
+    def reunitarize(self):
+        self._check(self._lib.ghx_reunitarize(self.h))
+
+    def sync(self):
+        self._check(self._lib.ghx_sync(self.h))
+
+    def time_md_steps(self, steps):
+        ms = C.c_double()
+        self._check(self._lib.ghx_time_md_steps(self.h, steps, C.byref(ms)))
+        return ms.value
+
+    def replica_swap(self, i, j, u):
+        acc, delta = C.c_int(), C.c_double()
+        self._check(self._lib.ghx_replica_swap(self.h, i, j, u, C.byref(acc), C.byref(delta)))
+        return bool(acc.value), delta.value
+
+    # --- field exchange (reference AoS layout) ------------------------------------
+    def get(self, name, replica=0):
+        V, D, N, d = self.vols[replica], self.D, self.N, self.d
+        if name == "U":
+            buf = np.zeros(V * D * N * N * 2)
+            self._check(self._lib.ghx_get_fields(self.h, replica, _dp(buf), None, None, None))
+            return buf.view(np.complex128).reshape(V * D, N, N)
+        if name == "P":
+            buf = np.zeros(V * D * self.ngen)
+            self._check(self._lib.ghx_get_fields(self.h, replica, None, _dp(buf), None, None))
+            return buf.reshape(V * D, self.ngen)
+        buf = np.zeros(V * d * 2)
+        args = [None, None, _dp(buf), None] if name == "phi" else [None, None, None, _dp(buf)]
+        self._check(self._lib.ghx_get_fields(self.h, replica, *args))
+        return buf.view(np.complex128).reshape(V, d)
+
+    def set(self, name, arr, replica=0):
+        arr = np.ascontiguousarray(arr)
+        buf = np.ascontiguousarray(arr.view(np.float64) if np.iscomplexobj(arr) else arr, dtype=np.float64).ravel()
+        args = [None, None, None, None]
+        args[{"U": 0, "P": 1, "phi": 2, "pi": 3}[name]] = _dp(buf)
+        self._check(self._lib.ghx_set_fields(self.h, replica, *args))
+
+    def forces(self, replica=0):
+        V, D = self.vols[replica], self.D
+        F = np.zeros(V * D * self.ngen)
+        Fp = np.zeros(V * max(self.d, 1) * 2)
+        self._check(self._lib.ghx_forces(self.h, replica, _dp(F), _dp(Fp) if self.d else None))
+        return F.reshape(V * D, self.ngen), (Fp.view(np.complex128).reshape(V, self.d) if self.d else None)
+
+
+class GaugeHiggsHMC:
+    """Single-lattice gauge+Higgs HMC (gauge_higgs_hmc.hpp:113-293). rep='' gives GaugeHMC."""
+
+    def __init__(self, extents, rep, seed, D=4, N=2, precision=FP64):
+        if np.isscalar(extents):
+            extents = [extents] * D
+        self.lat = Lattices(D, N, [list(extents)], rep=rep, precision=precision)
+        self.lat.set_seed(0, seed)
+        self.beta, self.kappa, self.lam, self.tau = 2.3, 0.1, 0.5, 1.0
+        self.nmd, self.integ, self.reunit_each_traj = 20, OMELYAN, True
+        self._push()
+
+    def _push(self):
+        self.lat.set_integrator(self.nmd, self.integ, self.reunit_each_traj)
+        self.lat.set_couplings(0, self.beta, self.kappa, self.lam, self.tau)
+
+    def configure(self, **kw):
+        for k, v in kw.items():
+            setattr(self, k, v)
+        self._push()
+        return self
+
+    @property
+    def d(self):
+        return self.lat.d
+
+    @property
+    def traj_count(self):
+        return self.lat.counters()[0]
+
+    @property
+    def accept_count(self):
+        return self.lat.counters()[1]
+
+    @property
+    def last_dH(self):
+        return self.lat.counters()[2]
+
+    def acceptance(self):
+        t, a, _ = self.lat.counters()
+        return a / t if t else 0.0
+
+    def hot(self, sigma=0.8, phi_stream=12345, phi_sigma=0.3, phi_mode=1):
+        self.lat.init_fields(0, HOT, sigma, phi_mode, phi_stream, phi_sigma)
+
+    def cold(self, phi_c=1.0):
+        self.lat.init_fields(0, COLD, 0.0, 0, 0, phi_c)
+
+    def refresh_momenta(self):
+        self.lat.refresh_momenta()
+
+    def kick(self, eps):
+        self.lat.kick(eps)
+
+    def drift(self, eps):
+        self.lat.drift(eps)
+
+    def md_evolve(self):
+        self.lat.md_evolve()
+
+    def hamiltonian(self):
+        return self.lat.hamiltonian_parts()[0, 4]
+
+    def hamiltonian_parts(self):
+        return self.lat.hamiltonian_parts()[0]
+
+    def trajectory(self):
+        acc, dH = self.lat.trajectory_all()
+        return bool(acc[0])
+
+    def observables(self):
+        o = self.lat.observables_all()[0]
+        return dict(zip(["plaq", "Sg", "SH", "Llink", "Lphi", "poly"], o))
+
+    def get(self, name):
+        return self.lat.get(name)
+
+    def set(self, name, arr):
+        self.lat.set(name, arr)
+
+    def forces(self):
+        return self.lat.forces()
+
+
+def GaugeHMC(extents, seed, D=4, N=2, precision=FP64):
+    """Pure-gauge HMC (hmc.hpp:13-96): keys 0xA1 / 0xB2, H = KE_P + S_g."""
+    return GaugeHiggsHMC(extents, "", seed, D=D, N=N, precision=precision)

@@ -1,0 +1,127 @@
+/* ghx — C ABI of the MI355X-native gauge+Higgs HMC molecular-dynamics path.
+ *
+ * One context holds one or more lattices ("replicas") of the same gauge group
+ * SU(N), dimension D, representation and precision, resident in HBM on the
+ * current HIP device. All calls return 0 (GHX_OK) or a negative GHX_E_* code;
+ * ghx_last_error() gives the message. No exceptions cross this boundary.
+ * A context is not re-entrant; separate contexts are independent.
+ *
+ * Field buffers exchanged with the host use the reference's AoS layouts:
+ *   U   [(s*D+mu)*N*N + i*N + j] complex (re,im interleaved)  fields.hpp:13-16
+ *   P   [(s*D+mu)*(N*N-1) + a]    real, generator order a       fields.hpp:47-50
+ *   phi [s*d + k]                 complex                       scalar_field.hpp:12-18
+ * with s the site index of that replica (direction 0 fastest, geometry.hpp:12-53).
+ */
+#ifndef GHX_H
+#define GHX_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ghx_ctx ghx_ctx;
+
+enum {
+  GHX_OK = 0,
+  GHX_E_INVALID = -1,     /* bad argument or state */
+  GHX_E_HIP = -2,         /* HIP runtime failure */
+  GHX_E_NAN = -3,         /* non-finite Hamiltonian */
+  GHX_E_UNSUPPORTED = -4, /* (N, D, representation) combination not compiled */
+  GHX_E_REP = -5          /* invalid representation spec (GeneralRep ctor errors) */
+};
+enum { GHX_FP64 = 0, GHX_FP32MD = 1 };        /* FP32MD: U,P,phi,pi,forces in FP32; H, dH in FP64 */
+enum { GHX_LEAPFROG = 0, GHX_OMELYAN = 1 };   /* hmc.hpp:10 Integrator */
+enum { GHX_COLD = 0, GHX_HOT = 1 };           /* start modes, fields.hpp:21-34 */
+
+/* Create a context on the current HIP device. L holds n_replicas*D extents.
+ * Replaces the construction of GaugeHMC / GaugeHiggsHMC (hmc.hpp:29-30,
+ * gauge_higgs_hmc.hpp:143-148) and of ReplicaTempering's replica set
+ * (replica_tempering.hpp:97-118). */
+int ghx_create(ghx_ctx** out, int D, int N, int n_replicas, const int* L, int precision);
+int ghx_destroy(ghx_ctx* ctx);
+const char* ghx_last_error(const ghx_ctx* ctx);
+
+/* Representation of the scalar: "" or "none" (pure gauge, GaugeHMC semantics),
+ * "fund", "adj", or Young rows "r1,r2,..." with optional ":real"
+ * (hmc_higgs.cpp:23-34 make_rep; rep_general.hpp:54-84 GeneralRep ctor). */
+int ghx_set_rep(ghx_ctx* ctx, const char* spec, int* d_out);
+/* d, N-ality, real flag, tensor dimension N^n (0 for fund/adj/none)
+ * (representation.hpp:14-17). */
+int ghx_rep_info(const ghx_ctx* ctx, int* d, int* nality, int* real_rep, long* dimT);
+/* T^a_R, (N*N-1)*d*d complex (representation.hpp:17). */
+int ghx_rep_generators(const ghx_ctx* ctx, double* T);
+/* Orthonormal real basis W, d*dimT (rep_general.hpp:40). */
+int ghx_rep_basis(const ghx_ctx* ctx, double* W);
+
+/* Couplings of one replica (GaugeHiggsHMC public fields, gauge_higgs_hmc.hpp:24-30). */
+int ghx_set_couplings(ghx_ctx* ctx, int replica, double beta, double kappa, double lambda, double tau);
+/* Integrator settings shared by all replicas (hmc.hpp:21-24). */
+int ghx_set_integrator(ghx_ctx* ctx, int nmd, int integrator, int reunit_each_traj);
+/* RNG seed of one replica (Rng ctor, rng.hpp:27). */
+int ghx_set_seed(ghx_ctx* ctx, int replica, uint64_t seed);
+
+/* Start configuration of one replica: links cold or reference hot start with
+ * sigma_U (fields.hpp:21-34); phi cold(c = phi_sigma) or Gaussian from
+ * phi_stream with phi_sigma (scalar_field.hpp:33-51). */
+int ghx_init_fields(ghx_ctx* ctx, int replica, int link_mode, double sigma_U, int phi_mode, uint64_t phi_stream,
+                    double phi_sigma);
+
+/* Trajectory/accept counters (hmc.hpp:26, gauge_higgs_hmc.hpp:140-141). */
+int ghx_set_counters(ghx_ctx* ctx, int replica, uint64_t traj_count, uint64_t accept_count);
+int ghx_get_counters(const ghx_ctx* ctx, int replica, uint64_t* traj_count, uint64_t* accept_count, double* last_dH);
+
+/* Momentum heatbath of every replica with its current traj_count
+ * (gauge_higgs_hmc.hpp:167-173, hmc.hpp:74-75). */
+int ghx_refresh_momenta(ghx_ctx* ctx);
+/* P -= eps F, pi += eps F_phi (gauge_higgs_hmc.hpp:112-134, hmc.hpp:35-42). */
+int ghx_kick(ghx_ctx* ctx, double eps);
+/* U <- exp(i eps P) U, phi += eps pi (gauge_higgs_hmc.hpp:136-147, hmc.hpp:44-52). */
+int ghx_drift(ghx_ctx* ctx, double eps);
+/* One MD trajectory with fused kick+drift launches (gauge_higgs_hmc.hpp:149-165). */
+int ghx_md_evolve(ghx_ctx* ctx);
+/* H parts per replica: [KE_P, KE_pi, S_g, S_H, H] x n_replicas (gauge_higgs_hmc.hpp:105-110). */
+int ghx_hamiltonian(ghx_ctx* ctx, double* H_parts);
+/* One full HMC trajectory of every replica with the Metropolis test on the GPU
+ * (gauge_higgs_hmc.hpp:175-192, hmc.hpp:73-91). accepted/dH: n_replicas each, may be NULL. */
+int ghx_trajectory(ghx_ctx* ctx, int* accepted, double* dH);
+/* n trajectories; when use_graph, the first is captured into a hipGraph and replayed. */
+int ghx_trajectories(ghx_ctx* ctx, int n, int use_graph);
+/* Observables per replica: [plaquette, S_g, S_H, L_link, L_phi, Polyakov] x n_replicas
+ * (gauge_wilson.hpp:51-65, scalar_higgs.hpp:217-239, observables.hpp:18-34). */
+int ghx_observables(ghx_ctx* ctx, double* out);
+/* max ||U^dag U - 1|| of one replica (fields.hpp:38-42). */
+int ghx_max_unitarity(ghx_ctx* ctx, int replica, double* out);
+/* Reunitarize all links (fields.hpp:36, sun.hpp:49-72). */
+int ghx_reunitarize(ghx_ctx* ctx);
+
+/* Copy fields of one replica to / from host AoS buffers (any pointer may be NULL). */
+int ghx_get_fields(ghx_ctx* ctx, int replica, double* U, double* P, double* phi, double* pi);
+int ghx_set_fields(ghx_ctx* ctx, int replica, const double* U, const double* P, const double* phi, const double* pi);
+/* Link force F (gauge + matter, add_gauge_force + add_matter_link_force) and
+ * scalar force F_phi (scalar_force) of one replica for the current fields
+ * (gauge_wilson.hpp:71-83, scalar_higgs.hpp:156-213). */
+int ghx_forces(ghx_ctx* ctx, int replica, double* F_link, double* F_phi);
+
+/* Replica swap proposal between replicas i and j (same extents): exchanges U
+ * and phi with probability min(1, exp(-Delta)) where Delta is the change of
+ * the summed actions of both replicas (replica_tempering.hpp:171-194 for the
+ * beta ladder; the kappa ladder uses Delta = (kappa_i-kappa_j)(B_i-B_j),
+ * B = sum 2 Re phi^dag D phi'). u is the caller's uniform in (0,1). */
+int ghx_replica_swap(ghx_ctx* ctx, int i, int j, double u, int* accepted, double* delta);
+
+int ghx_sync(ghx_ctx* ctx);
+/* Time `steps` fused MD steps (kick+drift launches) with HIP events on the
+ * context's stream: total ms and the number of launches timed. */
+int ghx_time_md_steps(ghx_ctx* ctx, int steps, double* ms_total);
+/* Host-only representation query (no device needed): d, N-ality, real flag,
+ * tensor dimension, and optionally W (d*dimT) and T^a_R ((N*N-1)*d*d complex);
+ * the same construction ghx_set_rep uses (rep_general.hpp:54-84, 338-411). */
+int ghx_rep_query(int N, const char* spec, int* d, int* nality, int* real_rep, long* dimT, double* W, double* T);
+/* Total sites over all replicas. */
+int64_t ghx_volume(const ghx_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GHX_H */

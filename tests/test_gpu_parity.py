@@ -1,0 +1,197 @@
+"""GPU (HIP, gfx950) vs FP64 oracle parity through the C ABI.
+
+Same seeds, same fields: the oracle restates the reference (pinned by
+test_oracle_golden.py), so agreement here is agreement with the reference.
+Tolerances (FP64 mode): per-site forces, actions and single kick/drift steps
+to 1e-12 relative; fields after one MD trajectory to 1e-10 absolute; dH to
+1e-9 absolute (|H| ~ 1e3-1e4, so this is ~1e-13 of H). Integer RNG keys are
+exact; Gaussians use the device log/cos, compared at 1 ulp.
+"""
+import numpy as np
+import pytest
+
+import gauge_higgs_amd as gha
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+# (id, D, N, rep, L, beta, kappa)
+CASES = [
+    ("su2_pure", 4, 2, "", 4, 2.3, 0.0),
+    ("su3_pure", 4, 3, "", 4, 5.7, 0.0),
+    ("su2_fund", 4, 2, "fund", 4, 2.3, 0.265),
+    ("su2_fund_aniso", 4, 2, "fund", (4, 3, 5, 6), 2.3, 0.265),
+    ("su2_adj_d3", 3, 2, "adj", 4, 2.3, 0.3),
+    ("su3_fund_d3", 3, 3, "fund", 4, 5.7, 0.2),
+    ("su3_sym2", 4, 3, "2", 4, 5.7, 0.1),
+    ("su3_adj_d3", 3, 3, "adj", 3, 5.7, 0.15),
+    ("su3_asym_11", 4, 3, "1,1", 3, 5.7, 0.1),
+    ("su2_spin1_real", 3, 2, "2:real", 4, 2.3, 0.2),
+    ("su2_spin32", 3, 2, "3", 4, 2.3, 0.2),
+]
+IDS = [c[0] for c in CASES]
+
+
+def make_pair(case, nmd=8, omelyan=True, precision=gha.FP64, seed=1):
+    _, D, N, rep, L, beta, kappa = case
+    ext = list(L) if not np.isscalar(L) else [L] * D
+    g = gha.GaugeHiggsHMC(ext, rep, seed=seed, D=D, N=N, precision=precision)
+    g.configure(beta=beta, kappa=kappa, lam=0.5, tau=1.0, nmd=nmd, integ=gha.OMELYAN if omelyan else gha.LEAPFROG)
+    o = O.Oracle(D, N, ext, rep, seed=seed)
+    o.couplings(beta=beta, kappa=kappa, lam=0.5, tau=1.0, nmd=nmd, omelyan=omelyan)
+    g.hot(0.8, 12345, 0.3)
+    o.hot(0.8)
+    if rep:
+        o.phi_gaussian(12345, 0.3)
+    return g, o
+
+
+def relmax(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+@pytest.mark.parametrize("case", CASES, ids=IDS)
+def test_start_fields(case):
+    g, o = make_pair(case)
+    assert np.max(np.abs(g.get("U") - o.get("U"))) < 1e-14
+    if case[3]:
+        assert np.max(np.abs(g.get("phi") - o.get("phi"))) < 1e-15
+
+
+@pytest.mark.parametrize("case", CASES, ids=IDS)
+def test_momentum_refresh(case):
+    g, o = make_pair(case)
+    g.refresh_momenta()
+    o.refresh()
+    P, Po = g.get("P"), o.get("P")
+    # integer keys/uniforms exact; Box-Muller through the device log/cos: <= 2 ulp
+    assert np.max(np.abs(P - Po) / np.maximum(np.abs(Po), 1e-300)) < 5e-16
+    if case[3]:
+        assert np.max(np.abs(g.get("pi") - o.get("pi"))) < 1e-15 * max(1, np.max(np.abs(o.get("pi"))))
+
+
+@pytest.mark.parametrize("case", CASES, ids=IDS)
+def test_forces(case):
+    g, o = make_pair(case)
+    F, Fp = g.forces()
+    Fo, Fpo = o.forces()
+    assert relmax(F, Fo) < 1e-12
+    if case[3]:
+        assert relmax(Fp, Fpo) < 1e-12
+
+
+@pytest.mark.parametrize("case", CASES, ids=IDS)
+def test_observables_and_hamiltonian(case):
+    g, o = make_pair(case)
+    g.refresh_momenta()
+    o.refresh()
+    ob, obo = g.observables(), o.observables()
+    for k in ("plaq", "Sg", "SH", "Llink", "Lphi", "poly"):
+        assert abs(ob[k] - obo[k]) <= 1e-12 * max(1.0, abs(obo[k])), k
+    hp, hpo = g.hamiltonian_parts(), o.ham_parts()
+    assert np.all(np.abs(hp - hpo) <= 1e-12 * np.maximum(1.0, np.abs(hpo)))
+
+
+@pytest.mark.parametrize("case", CASES, ids=IDS)
+def test_kick_then_drift(case):
+    g, o = make_pair(case)
+    g.refresh_momenta()
+    o.refresh()
+    g.kick(0.05)
+    o.kick(0.05)
+    assert relmax(g.get("P"), o.get("P")) < 1e-13
+    if case[3]:
+        assert relmax(g.get("pi"), o.get("pi")) < 1e-13
+    g.drift(0.05)
+    o.drift(0.05)
+    assert np.max(np.abs(g.get("U") - o.get("U"))) < 1e-13
+    if case[3]:
+        assert np.max(np.abs(g.get("phi") - o.get("phi"))) < 1e-13
+
+
+@pytest.mark.parametrize("omelyan", [False, True])
+@pytest.mark.parametrize("case", CASES, ids=IDS)
+def test_md_evolve(case, omelyan):
+    g, o = make_pair(case, nmd=6, omelyan=omelyan)
+    g.refresh_momenta()
+    o.refresh()
+    g.md_evolve()
+    o.md_evolve()
+    assert np.max(np.abs(g.get("U") - o.get("U"))) < 1e-10
+    assert relmax(g.get("P"), o.get("P")) < 1e-10
+    if case[3]:
+        assert np.max(np.abs(g.get("phi") - o.get("phi"))) < 1e-10
+
+
+@pytest.mark.parametrize("case", CASES, ids=IDS)
+def test_trajectories(case):
+    g, o = make_pair(case, nmd=10)
+    for _ in range(2):
+        acc = g.trajectory()
+        acc_o, dH_o, Hi, _ = o.trajectory()
+        assert abs(g.last_dH - dH_o) < 1e-9 * max(1.0, abs(Hi) / 1e3)
+        assert acc == acc_o
+    assert g.traj_count == o.traj_count and g.accept_count == o.accept_count
+    assert np.max(np.abs(g.get("U") - o.get("U"))) < 1e-10
+    if case[3]:
+        assert np.max(np.abs(g.get("phi") - o.get("phi"))) < 1e-10
+
+
+GOLDEN_CASES = [("su2_pure", 4, 2, "", 4, 2.3, 0.0, 20), ("su3_pure", 4, 3, "", 4, 5.7, 0.0, 12),
+                ("su2_fund", 4, 2, "fund", 4, 2.3, 0.265, 20), ("su2_adj", 3, 2, "adj", 4, 2.3, 0.3, 12),
+                ("su3_fund", 3, 3, "fund", 4, 5.7, 0.2, 12), ("su3_sym2", 4, 3, "2", 4, 5.7, 0.1, 20)]
+
+
+@pytest.mark.parametrize("gc", GOLDEN_CASES, ids=[c[0] for c in GOLDEN_CASES])
+def test_against_reference_goldens(goldens, gc):
+    """The GPU against the reference's own dump (no oracle in between)."""
+    arr, sc = goldens
+    tag, D, N, rep, L, beta, kappa, nmd = gc
+    for integ in ("lf", "omel"):
+        g = gha.GaugeHiggsHMC(L, rep, seed=1, D=D, N=N)
+        g.configure(beta=beta, kappa=kappa, lam=0.5, tau=1.0, nmd=nmd, integ=gha.OMELYAN if integ == "omel" else gha.LEAPFROG)
+        g.hot(0.8, 12345, 0.3)
+        if integ == "lf":
+            F, Fp = g.forces()
+            ref = arr[tag + ("_Flink" if rep else "_F")].reshape(F.shape)
+            assert relmax(F, ref) < 1e-12
+            if rep:
+                assert relmax(Fp.view(np.float64).ravel(), arr[tag + "_Fphi"]) < 1e-12
+            ob = g.observables()
+            assert abs(ob["Sg"] - sc[tag + "_Sg"]) < 1e-12 * abs(sc[tag + "_Sg"])
+            assert abs(ob["plaq"] - sc[tag + "_plaq"]) < 1e-13
+        ntr = 2 if rep else 3
+        dH = []
+        acc = []
+        for _ in range(ntr):
+            acc.append(float(g.trajectory()))
+            dH.append(g.last_dH)
+        assert np.max(np.abs(np.array(dH) - arr[f"{tag}_{integ}_dH"])) < 1e-9
+        assert np.array_equal(np.array(acc), arr[f"{tag}_{integ}_acc"])
+        ref_U = arr[f"{tag}_{integ}_U{ntr}"].view(np.complex128).reshape(-1, N, N)
+        assert np.max(np.abs(g.get("U") - ref_U)) < 1e-10
+
+
+FP32_CASES = [c for c in CASES if c[0] in ("su2_fund", "su3_sym2", "su3_pure", "su2_adj_d3")]
+
+
+@pytest.mark.parametrize("case", FP32_CASES, ids=[c[0] for c in FP32_CASES])
+def test_fp32_md_against_fp64(case):
+    """FP32 MD: forces to ~1e-6 relative on the same (float-rounded) input;
+    dH of a trajectory against the FP64 oracle within the stated 5e-3 absolute."""
+    g, o = make_pair(case, nmd=10, precision=gha.FP32MD)
+    U = g.get("U")
+    o.set("U", U)
+    if case[3]:
+        o.set("phi", g.get("phi"))
+    F, Fp = g.forces()
+    Fo, Fpo = o.forces()
+    assert relmax(F, Fo) < 2e-5
+    if case[3]:
+        assert relmax(Fp, Fpo) < 2e-5
+    ob, obo = g.observables(), o.observables()
+    assert abs(ob["Sg"] - obo["Sg"]) < 1e-12 * abs(obo["Sg"])  # H is reduced in FP64
+    acc = g.trajectory()
+    acc_o, dH_o, _, _ = o.trajectory()
+    assert abs(g.last_dH - dH_o) < 5e-3
