@@ -7,7 +7,7 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libghx.so")
+LIB_PATH = os.environ.get("GHX_LIB") or os.path.join(HERE, "libghx.so")
 _lib = None
 
 
@@ -48,6 +48,10 @@ def load():
         "ghx_replica_swap": [vp, i, i, d, C.POINTER(C.c_int), dp],
         "ghx_sync": [vp],
         "ghx_time_md_steps": [vp, i, dp],
+        "ghx_set_ladders": [vp, i, C.POINTER(C.c_int), C.POINTER(C.c_int), u64],
+        "ghx_set_tempering": [vp, i],
+        "ghx_swap_step": [vp],
+        "ghx_swap_stats": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_int)],
         "ghx_rep_query": [i, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_long), dp, dp],
     }
     for name, args in sig.items():

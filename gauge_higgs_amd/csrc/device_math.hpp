@@ -311,10 +311,74 @@ template <typename R, int N, int ORDER> GHX_DEV Mat<R, N> expi_taylor(const Mat<
   for (int i = 0; i < s; ++i) res = mul_nn(res, res);
   return res;
 }
+// SU(3), FP32 drift: Cayley-Hamilton reduction of the exponential series.
+// For traceless Hermitian Q, Q^3 = c1 Q + c0 1 with c1 = Tr(Q^2)/2, c0 = det Q,
+// so Q^n = a_n 1 + b_n Q + c_n Q^2 with (a,b,c)_{n+1} = (c0 c_n, a_n + c1 c_n, b_n)
+// and exp(iQ) = f0 1 + f1 Q + f2 Q^2, f_j = sum_n i^n/n! (a,b,c)_n. After scaling
+// to ||Q||_F <= 1 the series is cut at n = 12 (1/13! < float rounding); the
+// result is squared back. Costs one Hermitian Q^2 instead of a Taylor chain.
+template <typename R> GHX_DEV Mat<R, 3> expi_su3_ch(const Mat<R, 3>& H) {
+  R n2 = fnorm2(H), sc = R(1);
+  int s = 0;
+  while (n2 * sc * sc > R(1)) { sc *= R(0.5); ++s; }
+  Mat<R, 3> Q;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Q.e[i][j] = cscale(H.e[i][j], sc);
+  Mat<R, 3> Q2;  // Hermitian: fill the upper triangle, mirror the rest
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = i; j < 3; ++j) {
+      cx<R> z = cmul(Q.e[i][0], Q.e[0][j]);
+      cfma(z, Q.e[i][1], Q.e[1][j]);
+      cfma(z, Q.e[i][2], Q.e[2][j]);
+      Q2.e[i][j] = z;
+      if (j > i) Q2.e[j][i] = cconj(z);
+    }
+  Q2.e[0][0].i = Q2.e[1][1].i = Q2.e[2][2].i = R(0);
+  const R c1 = R(0.5) * (Q2.e[0][0].r + Q2.e[1][1].r + Q2.e[2][2].r);
+  R c0 = R(0);  // det Q = Tr(Q^3)/3 = Re Tr(Q Q^2)/3
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) c0 = fma(Q.e[i][k].r, Q2.e[k][i].r, fma(-Q.e[i][k].i, Q2.e[k][i].i, c0));
+  c0 *= R(1.0 / 3.0);
+  R a = R(1), b = R(0), c = R(0);
+  R f0r = R(1), f0i = R(0), f1r = R(0), f1i = R(0), f2r = R(0), f2i = R(0);
+  R coef = R(1);
+#pragma unroll
+  for (int n = 1; n <= 12; ++n) {
+    const R an = c0 * c, bn = fma(c1, c, a), cn = b;
+    a = an; b = bn; c = cn;
+    coef *= R(1.0 / n);
+    // i^n: n%4 == 0 -> 1, 1 -> i, 2 -> -1, 3 -> -i
+    if (n % 4 == 0) { f0r += coef * a; f1r += coef * b; f2r += coef * c; }
+    else if (n % 4 == 1) { f0i += coef * a; f1i += coef * b; f2i += coef * c; }
+    else if (n % 4 == 2) { f0r -= coef * a; f1r -= coef * b; f2r -= coef * c; }
+    else { f0i -= coef * a; f1i -= coef * b; f2i -= coef * c; }
+  }
+  const cx<R> f1 = {f1r, f1i}, f2 = {f2r, f2i};
+  Mat<R, 3> E;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      cx<R> z = cmul(f2, Q2.e[i][j]);
+      cfma(z, f1, Q.e[i][j]);
+      E.e[i][j] = z;
+    }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) { E.e[i][i].r += f0r; E.e[i][i].i += f0i; }
+#pragma unroll 1
+  for (int i = 0; i < s; ++i) E = mul_nn(E, E);
+  return E;
+}
 template <typename R, int N> GHX_DEV Mat<R, N> expi(const Mat<R, N>& H) {
   if constexpr (N == 2) return expi_su2<R>(H);
   else if constexpr (sizeof(R) == 8) return expi_taylor<R, N, 18>(H);
-  else return expi_taylor<R, N, 9>(H);
+  else return expi_su3_ch<R>(H);
 }
 
 // Modified Gram-Schmidt on rows + det phase on row 0 (sun.hpp:49-72), in R.

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -55,6 +56,16 @@ struct ghx_ctx {
   hipGraphExec_t graph = nullptr;
   int graph_cur = -1;
   std::string err;
+  const char* kernel_path = "";
+  // replica exchange
+  bool temper = false;
+  std::vector<int> hpairs;  // (i,j) list: even pass pairs then odd pass pairs
+  int neven = 0, npairs = 0;
+  int *dpairs = nullptr, *dflags = nullptr, *dcfg = nullptr;
+  double *dq0 = nullptr, *dq1 = nullptr;
+  unsigned long long *dattempts = nullptr, *daccepts = nullptr;
+  SwapState* dswap = nullptr;
+  unsigned maxvol = 0;
 
   size_t nU() const { return size_t(D) * 2 * N * N; }
   size_t nP() const { return size_t(D) * (N * N - 1); }
@@ -115,6 +126,24 @@ int upload_reps(ghx_ctx* ctx) {
   return GHX_OK;
 }
 
+// True when W is the single-row n=2 basis in odometer order (RepSym2's layout).
+bool sym2_canonical(const RepInfo& r) {
+  if (r.kind != REP_TENSOR || r.rows.size() != 1 || r.rows[0] != 2) return false;
+  const int N = r.N;
+  if (r.d != N * (N + 1) / 2 || r.dimT != long(N) * N) return false;
+  int k = 0;
+  for (int i = 0; i < N; ++i)
+    for (int j = i; j < N; ++j, ++k)
+      for (long t = 0; t < r.dimT; ++t) {
+        const int a = int(t / N), b = int(t % N);
+        double want = 0.0;
+        if (i == j && a == i && b == i) want = 1.0;
+        else if (i != j && ((a == i && b == j) || (a == j && b == i))) want = std::sqrt(0.5);
+        if (std::fabs(r.W[size_t(k) * r.dimT + t] - want) > 1e-15) return false;
+      }
+  return true;
+}
+
 int select_ops(ghx_ctx* ctx) {
   const int p = ctx->prec == GHX_FP32MD ? 1 : 0;
   const RepInfo& r = ctx->rep;
@@ -123,7 +152,10 @@ int select_ops(ghx_ctx* ctx) {
     case REP_NONE: ok = ops_none(p, ctx->N, ctx->D, &ctx->ops); break;
     case REP_FUND: ok = ops_fund(p, ctx->N, ctx->D, &ctx->ops); break;
     case REP_ADJ: ok = ops_adj(p, ctx->N, ctx->D, &ctx->ops); break;
-    case REP_TENSOR: ok = ops_tensor(p, ctx->N, ctx->D, r.nbox, r.d, &ctx->ops); break;
+    case REP_TENSOR:
+      if (sym2_canonical(r) && !std::getenv("GHX_GENERIC_TENSOR")) { ok = ops_sym2(p, ctx->N, ctx->D, &ctx->ops); ctx->kernel_path = "sym2"; }
+      else { ok = ops_tensor(p, ctx->N, ctx->D, r.nbox, r.d, &ctx->ops); ctx->kernel_path = "tensor"; }
+      break;
   }
   ctx->ops_ok = ok;
   if (!ok)
@@ -191,6 +223,26 @@ void launch_settle(ghx_ctx* c, bool reunit) {
   if (c->cur != 0 && has_phi(c))
     (void)hipMemcpyAsync(c->phi[0], c->phi[1], c->nphi() * c->Vp * c->tsize, hipMemcpyDeviceToDevice, c->stream);
   c->cur = 0;
+}
+
+// Replica exchange step on buffer 0 (device-side decisions, then field swaps).
+void launch_swaps(ghx_ctx* c) {
+  if (!c->temper || c->npairs == 0) return;
+  launch_action(c, 0);
+  hipLaunchKernelGGL(swap_decide_kernel, dim3(1), dim3(1), 0, c->stream, c->dreps, c->N, c->sums, c->nrep, c->dpairs, c->npairs,
+                     c->neven, c->dcfg, c->dq0, c->dq1, c->dswap, c->dattempts, c->daccepts, c->dflags);
+  const unsigned nb = (c->maxvol + kBlock - 1) / kBlock;
+  const int nphi = has_phi(c) ? int(c->nphi()) : 0;
+  for (int pass = 0; pass < 2; ++pass) {
+    const int k0 = pass ? c->neven : 0, k1 = pass ? c->npairs : c->neven;
+    if (k1 <= k0) continue;
+    if (c->tsize == 8)
+      hipLaunchKernelGGL(swap_apply_kernel<double>, dim3(nb, k1 - k0), dim3(kBlock), 0, c->stream, c->dreps, c->dpairs, c->dflags, k0,
+                         c->Vp, (double*)c->U[0], int(c->nU()), (double*)c->phi[0], nphi);
+    else
+      hipLaunchKernelGGL(swap_apply_kernel<float>, dim3(nb, k1 - k0), dim3(kBlock), 0, c->stream, c->dreps, c->dpairs, c->dflags, k0,
+                         c->Vp, (float*)c->U[0], int(c->nU()), (float*)c->phi[0], nphi);
+  }
 }
 
 // One trajectory of every replica, all on the stream.
@@ -338,7 +390,9 @@ int ghx_destroy(ghx_ctx* ctx) {
   drop_graph(ctx);
   free_fields(ctx);
   for (void* p : {(void*)ctx->dreps, (void*)ctx->dblk_rep, (void*)ctx->dred_rep, (void*)ctx->dred_first, (void*)ctx->partials,
-                  (void*)ctx->sums, (void*)ctx->Hi, (void*)ctx->Hf, (void*)ctx->Hwork, (void*)ctx->mc})
+                  (void*)ctx->sums, (void*)ctx->Hi, (void*)ctx->Hf, (void*)ctx->Hwork, (void*)ctx->mc, (void*)ctx->dpairs,
+                  (void*)ctx->dflags, (void*)ctx->dcfg, (void*)ctx->dq0, (void*)ctx->dq1, (void*)ctx->dattempts,
+                  (void*)ctx->daccepts, (void*)ctx->dswap})
     if (p) (void)hipFree(p);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -526,7 +580,7 @@ int ghx_trajectories(ghx_ctx* ctx, int n, int use_graph) {
   if (rc) return rc;
   if (ctx->cur != 0) launch_settle(ctx, false);
   if (!use_graph) {
-    for (int i = 0; i < n; ++i) launch_trajectory(ctx);
+    for (int i = 0; i < n; ++i) { launch_trajectory(ctx); launch_swaps(ctx); }
     HIPCHK(hipGetLastError());
     return GHX_OK;
   }
@@ -534,6 +588,7 @@ int ghx_trajectories(ghx_ctx* ctx, int n, int use_graph) {
     hipGraph_t g;
     HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     launch_trajectory(ctx);
+    launch_swaps(ctx);
     HIPCHK(hipStreamEndCapture(ctx->stream, &g));
     HIPCHK(hipGraphInstantiate(&ctx->graph, g, nullptr, nullptr, 0));
     (void)hipGraphDestroy(g);
@@ -672,6 +727,88 @@ int ghx_replica_swap(ghx_ctx* ctx, int i, int j, double u, int* accepted, double
   if (has_phi(ctx) && (rc = swap_planes(ctx->phi[0], ctx->nphi()))) return rc;
   HIPCHK(hipStreamSynchronize(ctx->stream));
   HIPCHK(hipFree(tmp));
+  return GHX_OK;
+}
+
+int ghx_set_ladders(ghx_ctx* ctx, int n_ladders, const int* lengths, const int* members, uint64_t swap_seed) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (n_ladders < 0 || (n_ladders > 0 && (!lengths || !members))) return fail(ctx, GHX_E_INVALID, "bad ladder arguments");
+  std::vector<std::vector<int>> lad;
+  std::vector<int> seen(ctx->nrep, 0);
+  int off = 0;
+  for (int l = 0; l < n_ladders; ++l) {
+    std::vector<int> m(members + off, members + off + lengths[l]);
+    off += lengths[l];
+    for (int r : m) {
+      if (r < 0 || r >= ctx->nrep || seen[r]++) return fail(ctx, GHX_E_INVALID, "ladder members must be distinct replicas");
+    }
+    for (size_t k = 1; k < m.size(); ++k) {
+      const RepDesc &a = ctx->hreps[m[k - 1]], &b = ctx->hreps[m[k]];
+      for (int d = 0; d < ctx->D; ++d)
+        if (a.L[d] != b.L[d]) return fail(ctx, GHX_E_INVALID, "replicas of one ladder need equal extents");
+      if (has_phi(ctx) && a.lambda != b.lambda) return fail(ctx, GHX_E_INVALID, "replicas of one ladder need equal lambda");
+    }
+    lad.push_back(m);
+  }
+  ctx->hpairs.clear();
+  for (int pass = 0; pass < 2; ++pass) {
+    for (const auto& m : lad)
+      for (size_t p = size_t(pass); p + 1 < m.size(); p += 2) { ctx->hpairs.push_back(m[p]); ctx->hpairs.push_back(m[p + 1]); }
+    if (pass == 0) ctx->neven = int(ctx->hpairs.size() / 2);
+  }
+  ctx->npairs = int(ctx->hpairs.size() / 2);
+  for (void** p : {(void**)&ctx->dpairs, (void**)&ctx->dflags, (void**)&ctx->dcfg, (void**)&ctx->dq0, (void**)&ctx->dq1,
+                   (void**)&ctx->dattempts, (void**)&ctx->daccepts, (void**)&ctx->dswap})
+    if (*p) { (void)hipFree(*p); *p = nullptr; }
+  const int np = std::max(ctx->npairs, 1);
+  HIPCHK(hipMalloc((void**)&ctx->dpairs, sizeof(int) * 2 * np));
+  HIPCHK(hipMalloc((void**)&ctx->dflags, sizeof(int) * np));
+  HIPCHK(hipMalloc((void**)&ctx->dcfg, sizeof(int) * ctx->nrep));
+  HIPCHK(hipMalloc((void**)&ctx->dq0, sizeof(double) * ctx->nrep));
+  HIPCHK(hipMalloc((void**)&ctx->dq1, sizeof(double) * ctx->nrep));
+  HIPCHK(hipMalloc((void**)&ctx->dattempts, sizeof(unsigned long long) * np));
+  HIPCHK(hipMalloc((void**)&ctx->daccepts, sizeof(unsigned long long) * np));
+  HIPCHK(hipMalloc((void**)&ctx->dswap, sizeof(SwapState)));
+  if (ctx->npairs) HIPCHK(hipMemcpyAsync(ctx->dpairs, ctx->hpairs.data(), sizeof(int) * ctx->hpairs.size(), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemsetAsync(ctx->dflags, 0, sizeof(int) * np, ctx->stream));
+  HIPCHK(hipMemsetAsync(ctx->dattempts, 0, sizeof(unsigned long long) * np, ctx->stream));
+  HIPCHK(hipMemsetAsync(ctx->daccepts, 0, sizeof(unsigned long long) * np, ctx->stream));
+  const SwapState st{swap_seed, 0};
+  HIPCHK(hipMemcpyAsync(ctx->dswap, &st, sizeof st, hipMemcpyHostToDevice, ctx->stream));
+  ctx->maxvol = 0;
+  for (const auto& rd : ctx->hreps) ctx->maxvol = std::max(ctx->maxvol, rd.vol);
+  ctx->temper = ctx->npairs > 0;
+  drop_graph(ctx);
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return GHX_OK;
+}
+
+int ghx_set_tempering(ghx_ctx* ctx, int enabled) {
+  if (!ctx) return GHX_E_INVALID;
+  ctx->temper = enabled && ctx->npairs > 0;
+  drop_graph(ctx);
+  return GHX_OK;
+}
+
+int ghx_swap_step(ghx_ctx* ctx) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (ctx->cur != 0) launch_settle(ctx, false);
+  launch_swaps(ctx);
+  HIPCHK(hipGetLastError());
+  return GHX_OK;
+}
+
+int ghx_swap_stats(ghx_ctx* ctx, int* npairs, int* pairs, uint64_t* attempts, uint64_t* accepts, int* last_flags) {
+  if (!ctx) return GHX_E_INVALID;
+  if (npairs) *npairs = ctx->npairs;
+  if (ctx->npairs == 0) return GHX_OK;
+  if (pairs) std::copy(ctx->hpairs.begin(), ctx->hpairs.end(), pairs);
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (attempts) HIPCHK(hipMemcpy(attempts, ctx->dattempts, sizeof(uint64_t) * ctx->npairs, hipMemcpyDeviceToHost));
+  if (accepts) HIPCHK(hipMemcpy(accepts, ctx->daccepts, sizeof(uint64_t) * ctx->npairs, hipMemcpyDeviceToHost));
+  if (last_flags) HIPCHK(hipMemcpy(last_flags, ctx->dflags, sizeof(int) * ctx->npairs, hipMemcpyDeviceToHost));
   return GHX_OK;
 }
 

@@ -16,6 +16,9 @@ namespace ghx {
 constexpr int kBlock = 128;     // threads per block of the per-site kernels
 constexpr int kRedBlock = 256;  // threads per block of the reduction kernels
 constexpr int kNQ = 6;          // per-site reduction quantities
+#ifndef GHX_MD_WAVES
+#define GHX_MD_WAVES 3  // minimum waves per SIMD requested for the MD-step kernel
+#endif
 
 struct RepDesc {
   int L[4];
@@ -81,6 +84,70 @@ GHX_DEV void store_vec(T* __restrict__ f, unsigned Vp, unsigned g, const cx<R>* 
   for (int k = 0; k < DR; ++k) { f[size_t(2 * k) * Vp + g] = T(v[k].r); f[size_t(2 * k + 1) * Vp + g] = T(v[k].i); }
 }
 
+// ---- buffer-resource access for the MD-step kernel ------------------------
+// Every field is a set of planes; a load is (vector offset = site, scalar
+// offset = plane * Vp), so one VGPR addresses all planes of a site and the
+// hardware bounds check turns any stray offset into a zero read.
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+GHX_DEV rsrc_t make_rsrc(const void* p, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+template <typename T> GHX_DEV T bload(rsrc_t r, unsigned voff, unsigned soff);
+template <> GHX_DEV float bload<float>(rsrc_t r, unsigned voff, unsigned soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+template <> GHX_DEV double bload<double>(rsrc_t r, unsigned voff, unsigned soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+}
+GHX_DEV void bstore(float v, rsrc_t r, unsigned voff, unsigned soff) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff, 0);
+}
+GHX_DEV void bstore(double v, rsrc_t r, unsigned voff, unsigned soff) {
+  using v2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2, v), r, (int)voff, (int)soff, 0);
+}
+// Planes of one field: plane c of site g at byte offset (c*Vp + g)*sizeof(T).
+template <typename T>
+struct Planes {
+  rsrc_t r;
+  unsigned pstride;  // Vp * sizeof(T)
+  GHX_DEV Planes(const T* p, unsigned Vp, unsigned nplanes) : r(make_rsrc(p, nplanes * Vp * unsigned(sizeof(T)))), pstride(Vp * unsigned(sizeof(T))) {}
+  GHX_DEV T ld(unsigned g, unsigned c) const { return bload<T>(r, g * unsigned(sizeof(T)), c * pstride); }
+  GHX_DEV void st(unsigned g, unsigned c, T v) const { bstore(v, r, g * unsigned(sizeof(T)), c * pstride); }
+};
+template <typename T, typename R, int N>
+GHX_DEV Mat<R, N> bload_link(const Planes<T>& U, int mu, unsigned g) {
+  Mat<R, N> m;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      m.e[i][j].r = R(U.ld(g, unsigned(mu * 2 * N * N + 2 * (i * N + j))));
+      m.e[i][j].i = R(U.ld(g, unsigned(mu * 2 * N * N + 2 * (i * N + j) + 1)));
+    }
+  return m;
+}
+template <typename T, typename R, int N>
+GHX_DEV void bstore_link(const Planes<T>& U, int mu, unsigned g, const Mat<R, N>& m) {
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      U.st(g, unsigned(mu * 2 * N * N + 2 * (i * N + j)), T(m.e[i][j].r));
+      U.st(g, unsigned(mu * 2 * N * N + 2 * (i * N + j) + 1), T(m.e[i][j].i));
+    }
+}
+template <typename T, typename R, int DR>
+GHX_DEV void bload_vec(const Planes<T>& f, unsigned g, cx<R>* v) {
+#pragma unroll
+  for (int k = 0; k < DR; ++k) { v[k].r = R(f.ld(g, 2 * k)); v[k].i = R(f.ld(g, 2 * k + 1)); }
+}
+template <typename T, typename R, int DR>
+GHX_DEV void bstore_vec(const Planes<T>& f, unsigned g, const cx<R>* v) {
+#pragma unroll
+  for (int k = 0; k < DR; ++k) { f.st(g, 2 * k, T(v[k].r)); f.st(g, 2 * k + 1, T(v[k].i)); }
+}
+
 // Site -> neighbour offsets for a periodic lattice (geometry.hpp:20-38 order:
 // direction 0 fastest). up[m]/dn[m] are signed offsets to add to the site.
 template <int D>
@@ -108,7 +175,7 @@ struct Nbr {
 // matter forces: scalar_higgs.hpp:156-213; drift: gauge_higgs_hmc.hpp:136-147.
 // ---------------------------------------------------------------------------
 template <typename T, int N, int D, class REP>
-__global__ __launch_bounds__(kBlock) void md_step_kernel(StepArgs<T> a, typename REP::template Params<T> rp) {
+__global__ __launch_bounds__(kBlock, GHX_MD_WAVES) void md_step_kernel(StepArgs<T> a, typename REP::template Params<T> rp) {
   using R = T;
   constexpr int NC = 2 * N * N;
   constexpr int NGN = N * N - 1;
@@ -124,13 +191,14 @@ __global__ __launch_bounds__(kBlock) void md_step_kernel(StepArgs<T> a, typename
   const R ek = R(abs_eps ? a.ck : a.ck * rd.eps);
   const R ed = R(abs_eps ? a.cd : a.cd * rd.eps);
   const R bN = R(rd.beta / N);
-  (void)NC;
+  const Planes<T> Uin(a.U_in, Vp, D * NC), Uout(a.U_out, Vp, D * NC), Pp(a.P, Vp, D * NGN);
 
   cx<R> phx[DR > 0 ? DR : 1], fph[DR > 0 ? DR : 1];
   R k2 = R(0);
   if constexpr (DR > 0) {
     k2 = R(2.0 * rd.kappa);
-    load_vec<T, R, DR>(a.phi_in, Vp, g, phx);
+    const Planes<T> Ph(a.phi_in, Vp, 2 * DR);
+    bload_vec<T, R, DR>(Ph, g, phx);
     R n2 = R(0);
 #pragma unroll
     for (int k = 0; k < DR; ++k) n2 += phx[k].r * phx[k].r + phx[k].i * phx[k].i;
@@ -141,91 +209,91 @@ __global__ __launch_bounds__(kBlock) void md_step_kernel(StepArgs<T> a, typename
 
 #pragma unroll 1
   for (int mu = 0; mu < D; ++mu) {
-    const Mat<R, N> Ux = load_link<T, R, N>(a.U_in, Vp, mu, g);
-    Mat<R, N> Sig = mzero<R, N>();
+    const Mat<R, N> Ux = bload_link<T, R, N>(Uin, mu, g);
     const unsigned gpm = g + nb.up[mu];
+    // matter first (its temporaries die before the staple sum is built)
+    Mat<R, N> M;
+    if constexpr (DR > 0) {
+      const Planes<T> Ph(a.phi_in, Vp, 2 * DR);
+      {
+        cx<R> phy[DR], psi[DR];
+        bload_vec<T, R, DR>(Ph, gpm, phy);
+        REP::fwd_M(Ux, phy, phx, psi, M, rp);
+#pragma unroll
+        for (int k = 0; k < DR; ++k) { fph[k].r = fma(k2, psi[k].r, fph[k].r); fph[k].i = fma(k2, psi[k].i, fph[k].i); }
+      }
+      {
+        const unsigned gmm = g + nb.dn[mu];
+        cx<R> phb[DR], chi[DR];
+        bload_vec<T, R, DR>(Ph, gmm, phb);
+        const Mat<R, N> Ub = bload_link<T, R, N>(Uin, mu, gmm);
+        REP::apply_dag(Ub, phb, chi, rp);
+#pragma unroll
+        for (int k = 0; k < DR; ++k) { fph[k].r = fma(k2, chi[k].r, fph[k].r); fph[k].i = fma(k2, chi[k].i, fph[k].i); }
+      }
+    }
+    Mat<R, N> Sig = mzero<R, N>();
 #pragma unroll 1
     for (int nu = 0; nu < D; ++nu) {
       if (nu == mu) continue;
       // forward staple U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag
       {
-        const Mat<R, N> A = load_link<T, R, N>(a.U_in, Vp, nu, gpm);
-        const Mat<R, N> B = load_link<T, R, N>(a.U_in, Vp, mu, g + nb.up[nu]);
-        const Mat<R, N> C = load_link<T, R, N>(a.U_in, Vp, nu, g);
-        mac_nd(Sig, mul_nd(A, B), C);
+        const Mat<R, N> AB = mul_nd(bload_link<T, R, N>(Uin, nu, gpm), bload_link<T, R, N>(Uin, mu, g + nb.up[nu]));
+        mac_nd(Sig, AB, bload_link<T, R, N>(Uin, nu, g));
       }
       // backward staple U_nu(x+mu-nu)^dag U_mu(x-nu)^dag U_nu(x-nu)
       {
         const unsigned gmn = g + nb.dn[nu];
-        const Mat<R, N> A = load_link<T, R, N>(a.U_in, Vp, nu, gpm + nb.dn[nu]);
-        const Mat<R, N> B = load_link<T, R, N>(a.U_in, Vp, mu, gmn);
-        const Mat<R, N> C = load_link<T, R, N>(a.U_in, Vp, nu, gmn);
-        mac_nn(Sig, mul_dd(A, B), C);
+        const Mat<R, N> AB = mul_dd(bload_link<T, R, N>(Uin, nu, gpm + nb.dn[nu]), bload_link<T, R, N>(Uin, mu, gmn));
+        mac_nn(Sig, AB, bload_link<T, R, N>(Uin, nu, gmn));
       }
     }
     Mat<R, N> X = mul_nn(Ux, Sig);
 #pragma unroll
     for (int i = 0; i < N; ++i)
 #pragma unroll
-      for (int j = 0; j < N; ++j) X.e[i][j] = cscale(X.e[i][j], bN);
-    if constexpr (DR > 0) {
-      cx<R> phy[DR], psi[DR];
-      load_vec<T, R, DR>(a.phi_in, Vp, gpm, phy);
-      Mat<R, N> M;
-      REP::fwd_M(Ux, phy, phx, psi, M, rp);
-#pragma unroll
-      for (int i = 0; i < N; ++i)
-#pragma unroll
-        for (int j = 0; j < N; ++j) {
+      for (int j = 0; j < N; ++j) {
+        X.e[i][j] = cscale(X.e[i][j], bN);
+        if constexpr (DR > 0) {
           X.e[i][j].r = fma(k2, M.e[i][j].r, X.e[i][j].r);
           X.e[i][j].i = fma(k2, M.e[i][j].i, X.e[i][j].i);
         }
-      const unsigned gmm = g + nb.dn[mu];
-      cx<R> phb[DR], chi[DR];
-      load_vec<T, R, DR>(a.phi_in, Vp, gmm, phb);
-      const Mat<R, N> Ub = load_link<T, R, N>(a.U_in, Vp, mu, gmm);
-      REP::apply_dag(Ub, phb, chi, rp);
-#pragma unroll
-      for (int k = 0; k < DR; ++k) {
-        fph[k].r = fma(k2, psi[k].r + chi[k].r, fph[k].r);
-        fph[k].i = fma(k2, psi[k].i + chi[k].i, fph[k].i);
       }
-    }
     R f[NGN];
     im_tr_gen<R, N>(X, f);
-    T* Pm = a.P + size_t(mu) * NGN * Vp + g;
     if (a.flags & F_FORCE) {
-      T* Fm = a.F_out + size_t(mu) * NGN * Vp + g;
+      const Planes<T> Fp(a.F_out, Vp, D * NGN);
 #pragma unroll
-      for (int c = 0; c < NGN; ++c) Fm[size_t(c) * Vp] = T(f[c]);
+      for (int c = 0; c < NGN; ++c) Fp.st(g, unsigned(mu * NGN + c), T(f[c]));
       continue;
     }
     R p[NGN];
 #pragma unroll
-    for (int c = 0; c < NGN; ++c) p[c] = R(Pm[size_t(c) * Vp]);
+    for (int c = 0; c < NGN; ++c) p[c] = R(Pp.ld(g, unsigned(mu * NGN + c)));
     if (a.flags & F_KICK) {
 #pragma unroll
-      for (int c = 0; c < NGN; ++c) { p[c] = fma(-ek, f[c], p[c]); Pm[size_t(c) * Vp] = T(p[c]); }
+      for (int c = 0; c < NGN; ++c) { p[c] = fma(-ek, f[c], p[c]); Pp.st(g, unsigned(mu * NGN + c), T(p[c])); }
     }
     if (a.flags & F_DRIFT) {
       const Mat<R, N> H = alg_to_mat<R, N>(p, ed);
       const Mat<R, N> E = expi<R, N>(H);
-      store_link<T, R, N>(a.U_out, Vp, mu, g, mul_nn(E, Ux));
+      bstore_link<T, R, N>(Uout, mu, g, mul_nn(E, Ux));
     }
   }
   if constexpr (DR > 0) {
-    if (a.flags & F_FORCE) { store_vec<T, R, DR>(a.Fphi_out, Vp, g, fph); return; }
+    if (a.flags & F_FORCE) { bstore_vec<T, R, DR>(Planes<T>(a.Fphi_out, Vp, 2 * DR), g, fph); return; }
+    const Planes<T> Pi(a.pi, Vp, 2 * DR);
     cx<R> pv[DR];
-    load_vec<T, R, DR>(a.pi, Vp, g, pv);
+    bload_vec<T, R, DR>(Pi, g, pv);
     if (a.flags & F_KICK) {
 #pragma unroll
       for (int k = 0; k < DR; ++k) { pv[k].r = fma(ek, fph[k].r, pv[k].r); pv[k].i = fma(ek, fph[k].i, pv[k].i); }
-      store_vec<T, R, DR>(a.pi, Vp, g, pv);
+      bstore_vec<T, R, DR>(Pi, g, pv);
     }
     if (a.flags & F_DRIFT) {
 #pragma unroll
       for (int k = 0; k < DR; ++k) { phx[k].r = fma(ed, pv[k].r, phx[k].r); phx[k].i = fma(ed, pv[k].i, phx[k].i); }
-      store_vec<T, R, DR>(a.phi_out, Vp, g, phx);
+      bstore_vec<T, R, DR>(Planes<T>(a.phi_out, Vp, 2 * DR), g, phx);
     }
   }
 }
@@ -399,6 +467,52 @@ static __global__ void metropolis_kernel(const RepDesc* reps, int nrep, const do
   const bool acc = (dH <= 0.0) || (u < exp(-dH));
   m.accepted = acc ? 1 : 0;
   if (acc) m.accept_count += 1;
+}
+
+// Replica-exchange decisions for one step (replica_tempering.hpp:157-194),
+// all on one thread in the reference order: even pass then odd pass, pairs in
+// list order. Delta = S_i(C_j)+S_j(C_i)-S_i(C_i)-S_j(C_j) from the per-replica
+// sums q0 = sum Re Tr U_pl and q1 = sum Re phi^dag D phi' (beta and kappa
+// ladders; lambda equal): (beta_i-beta_j)/N (q0_i-q0_j) + 2(kappa_i-kappa_j)(q1_i-q1_j).
+// Accept if Delta <= 0 (the coin counter still advances), else with
+// uniform(seed, key(0x57A9, ++counter)) < exp(-Delta).
+struct SwapState {
+  unsigned long long seed, counter;
+};
+static __global__ void swap_decide_kernel(const RepDesc* reps, int N, const double* sums, int nrep, const int* pairs, int npairs,
+                                          int nfirst, int* cfg, double* q0, double* q1, SwapState* st,
+                                          unsigned long long* attempts, unsigned long long* accepts, int* flags) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int r = 0; r < nrep; ++r) { cfg[r] = r; q0[r] = sums[size_t(r) * kNQ + 0]; q1[r] = sums[size_t(r) * kNQ + 1]; }
+  for (int k = 0; k < npairs; ++k) {
+    const int i = pairs[2 * k], j = pairs[2 * k + 1];
+    const int ci = cfg[i], cj = cfg[j];
+    const double Delta = (reps[i].beta - reps[j].beta) / N * (q0[ci] - q0[cj]) + 2.0 * (reps[i].kappa - reps[j].kappa) * (q1[ci] - q1[cj]);
+    attempts[k] += 1;
+    bool acc = Delta <= 0.0;
+    if (!acc) {
+      st->counter += 1;
+      acc = draw_uniform(st->seed, stream_key(0x57A9, st->counter)) < exp(-Delta);
+    } else {
+      st->counter += 1;
+    }
+    flags[k] = acc ? 1 : 0;
+    if (acc) { accepts[k] += 1; cfg[i] = cj; cfg[j] = ci; }
+    (void)nfirst;
+  }
+}
+// Exchange U and phi site ranges of the accepted pairs in [k0, k1) of the pair list.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void swap_apply_kernel(const RepDesc* reps, const int* pairs, const int* flags, int k0, unsigned Vp,
+                                                           T* U, int nU, T* phi, int nphi) {
+  const int k = k0 + blockIdx.y;
+  if (!flags[k]) return;
+  const RepDesc &ri = reps[pairs[2 * k]], &rj = reps[pairs[2 * k + 1]];
+  const unsigned s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= ri.vol) return;
+  const unsigned gi = ri.offset + s, gj = rj.offset + s;
+  for (int c = 0; c < nU; ++c) { const T t = U[size_t(c) * Vp + gi]; U[size_t(c) * Vp + gi] = U[size_t(c) * Vp + gj]; U[size_t(c) * Vp + gj] = t; }
+  for (int c = 0; c < nphi; ++c) { const T t = phi[size_t(c) * Vp + gi]; phi[size_t(c) * Vp + gi] = phi[size_t(c) * Vp + gj]; phi[size_t(c) * Vp + gj] = t; }
 }
 
 // Restore U, phi from the saved copies for rejected replicas.

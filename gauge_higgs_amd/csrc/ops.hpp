@@ -39,5 +39,6 @@ bool ops_none(int prec, int N, int D, Ops* o);
 bool ops_fund(int prec, int N, int D, Ops* o);
 bool ops_adj(int prec, int N, int D, Ops* o);
 bool ops_tensor(int prec, int N, int D, int nbox, int d, Ops* o);
+bool ops_sym2(int prec, int N, int D, Ops* o);
 
 }  // namespace ghx

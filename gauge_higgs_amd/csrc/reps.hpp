@@ -213,4 +213,86 @@ struct RepTensor {
   }
 };
 
+// ---------------------------------------------------------------- two-index symmetric {2}
+// The single-row n=2 basis of rep_general.hpp:338-362 in odometer order over
+// i <= j: W = e_i(x)e_i, or (e_i(x)e_j + e_j(x)e_i)/sqrt 2. With Phi the
+// symmetric matrix Phi_ii = phi_k, Phi_ij = Phi_ji = phi_k/sqrt 2:
+//   D(U) phi  <->  U Phi U^T,   D(U)^dag phi  <->  U^dag Phi conj(U),
+// and the back-reaction matrix of both slots is M = 2 Psi Phx^dag.
+// Selected by the host only after checking W has exactly this form.
+template <typename R, int N>
+struct RepSym2 {
+  static constexpr int DR = N * (N + 1) / 2;
+  static constexpr int KIND = REP_TENSOR;
+  template <typename P> using Params = NoParams;
+  static GHX_DEV Mat<R, N> lift(const cx<R>* x) {
+    Mat<R, N> F;
+    int k = 0;
+    const R h = R(0.70710678118654752440);
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int j = i; j < N; ++j, ++k) {
+        if (i == j) F.e[i][i] = x[k];
+        else { F.e[i][j] = cscale(x[k], h); F.e[j][i] = F.e[i][j]; }
+      }
+    return F;
+  }
+  static GHX_DEV void project(const Mat<R, N>& S, cx<R>* y) {
+    int k = 0;
+    const R r2 = R(1.41421356237309504880);
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int j = i; j < N; ++j, ++k) y[k] = (i == j) ? S.e[i][i] : cscale(S.e[i][j], r2);
+  }
+  // S = A F B^T with F, S symmetric (upper triangle of S computed, then mirrored)
+  template <bool DAG>
+  static GHX_DEV Mat<R, N> sandwich(const Mat<R, N>& U, const Mat<R, N>& F) {
+    Mat<R, N> G;  // G = U F  (or U^dag F)
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        cx<R> z = {R(0), R(0)};
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+          if (DAG) cfmac(z, U.e[k][i], F.e[k][j]);
+          else cfma(z, U.e[i][k], F.e[k][j]);
+        }
+        G.e[i][j] = z;
+      }
+    Mat<R, N> S;  // S = G U^T (or G conj(U))
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int j = i; j < N; ++j) {
+        cx<R> z = {R(0), R(0)};
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+          if (DAG) cfmcc(z, G.e[i][k], U.e[k][j]);
+          else cfma(z, G.e[i][k], U.e[j][k]);
+        }
+        S.e[i][j] = z;
+        S.e[j][i] = z;
+      }
+    return S;
+  }
+  template <class PR>
+  static GHX_DEV void apply(const Mat<R, N>& U, const cx<R>* x, cx<R>* y, const PR&) { project(sandwich<false>(U, lift(x)), y); }
+  template <class PR>
+  static GHX_DEV void apply_dag(const Mat<R, N>& U, const cx<R>* x, cx<R>* y, const PR&) { project(sandwich<true>(U, lift(x)), y); }
+  template <class PR>
+  static GHX_DEV void fwd_M(const Mat<R, N>& U, const cx<R>* py, const cx<R>* px, cx<R>* psi, Mat<R, N>& M, const PR&) {
+    const Mat<R, N> S = sandwich<false>(U, lift(py));
+    project(S, psi);
+    const Mat<R, N> X = lift(px);
+    M = mul_nd(S, X);  // Psi Phx^dag (Phx symmetric)
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int j = 0; j < N; ++j) M.e[i][j] = cscale(M.e[i][j], R(2));
+  }
+};
+
 }  // namespace ghx

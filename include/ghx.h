@@ -110,6 +110,20 @@ int ghx_forces(ghx_ctx* ctx, int replica, double* F_link, double* F_phi);
  * B = sum 2 Re phi^dag D phi'). u is the caller's uniform in (0,1). */
 int ghx_replica_swap(ghx_ctx* ctx, int i, int j, double u, int* accepted, double* delta);
 
+/* Replica exchange ladders (replica_tempering.hpp:78-205 generalised to beta
+ * and kappa ladders): each ladder is an ordered list of distinct replicas with
+ * equal extents and lambda; adjacent members are swap partners. A swap step
+ * runs the even pass over all ladders then the odd pass, with the coin
+ * uniform(swap_seed, key(0x57A9, ++counter)) (replica_tempering.hpp:171-194).
+ * With tempering enabled, ghx_trajectories() runs one swap step after every
+ * trajectory (inside the captured hipGraph). */
+int ghx_set_ladders(ghx_ctx* ctx, int n_ladders, const int* lengths, const int* members, uint64_t swap_seed);
+int ghx_set_tempering(ghx_ctx* ctx, int enabled);
+int ghx_swap_step(ghx_ctx* ctx);
+/* Pair list (even pass then odd pass, 2 ints per pair), cumulative attempts /
+ * accepts per pair and the flags of the last swap step. Any pointer may be NULL. */
+int ghx_swap_stats(ghx_ctx* ctx, int* npairs, int* pairs, uint64_t* attempts, uint64_t* accepts, int* last_flags);
+
 int ghx_sync(ghx_ctx* ctx);
 /* Time `steps` fused MD steps (kick+drift launches) with HIP events on the
  * context's stream: total ms and the number of launches timed. */

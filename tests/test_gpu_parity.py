@@ -195,3 +195,41 @@ def test_fp32_md_against_fp64(case):
     acc = g.trajectory()
     acc_o, dH_o, _, _ = o.trajectory()
     assert abs(g.last_dH - dH_o) < 5e-3
+
+
+@pytest.mark.parametrize("N,spec", [(3, "2"), (2, "2")])
+def test_sym2_path_equals_generic_tensor_path(N, spec, monkeypatch):
+    """The direct U Phi U^T kernels for {2} against the generic lift/slot/project kernels."""
+    res = []
+    for generic in (False, True):
+        if generic:
+            monkeypatch.setenv("GHX_GENERIC_TENSOR", "1")
+        else:
+            monkeypatch.delenv("GHX_GENERIC_TENSOR", raising=False)
+        g = gha.GaugeHiggsHMC(4, spec, seed=5, D=4, N=N).configure(beta=5.7, kappa=0.12, lam=0.5, nmd=6)
+        g.hot(0.8, 12345, 0.5)
+        F, Fp = g.forces()
+        g.trajectory()
+        res.append((F, Fp, g.last_dH, g.get("U")))
+    monkeypatch.delenv("GHX_GENERIC_TENSOR", raising=False)
+    assert relmax(res[0][0], res[1][0]) < 1e-13 and relmax(res[0][1], res[1][1]) < 1e-13
+    assert abs(res[0][2] - res[1][2]) < 1e-10
+    assert np.max(np.abs(res[0][3] - res[1][3])) < 1e-12
+
+
+@pytest.mark.parametrize("eps", [0.05, 0.5, 3.0])
+def test_fp32_su3_drift_exponential(eps):
+    """FP32 SU(3) drift (Cayley-Hamilton series with scaling) against the FP64
+    Taylor-18 oracle on identical float inputs; eps = 3 exercises the squaring."""
+    g = gha.GaugeHiggsHMC(4, "", seed=3, D=4, N=3, precision=gha.FP32MD).configure(beta=5.7, nmd=4)
+    g.hot(0.8)
+    g.refresh_momenta()
+    o = O.Oracle(4, 3, 4, "", seed=3)
+    o.set("U", g.get("U"))
+    o.set("P", g.get("P"))
+    g.drift(eps)
+    o.drift(eps)
+    U = g.get("U")
+    assert np.max(np.abs(U - o.get("U"))) < 2e-6 * max(1.0, eps)
+    # stays on the group to float rounding
+    assert np.max(np.abs(np.einsum("lji,ljk->lik", U.conj(), U) - np.eye(3))) < 2e-6 * max(1.0, eps)
