@@ -79,7 +79,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="su3_sextet_L32_fp32", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
+    ap.add_argument("--stream-seg", type=int, default=None, help="force a streaming segment (implies --no-tune)")
     args = ap.parse_args()
+    if args.stream_seg is not None:
+        args.no_tune = True
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU path is not built.", file=sys.stderr)
         return 2
@@ -93,6 +97,12 @@ def main():
     g.hot(0.8, 12345, 1.0)
     g.refresh_momenta()
     V = g.lat.vols[0]
+    tuned = {"stream_seg": args.stream_seg or 0}
+    if args.stream_seg is not None:
+        g.lat.set_stream_segment(args.stream_seg)
+    if not args.no_tune:
+        seg, tms = g.lat.autotune(steps=5)  # launch-shape autotuner (fields restored)
+        tuned = {"stream_seg": seg, "ms_per_step_at_tune": tms}
     if args.warmup:
         g.lat.time_md_steps(args.warmup)
     g.lat.sync()
@@ -125,7 +135,7 @@ def main():
         "dtype": "f32" if prec == "fp32" else "f64",
         "data": "synthetic (reference hot start sigma=0.8, Gaussian phi sigma=1, Gaussian momenta; seed 1)",
         "config": {"workload": args.workload, "D": D, "N": N, "rep": rep or "none", "L": L, "sites": V, "beta": beta,
-                   "kappa": kappa, "lambda": lam, "eps": 0.05, "wall_s": wall},
+                   "kappa": kappa, "lambda": lam, "eps": 0.05, "wall_s": wall, "launch": tuned},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "md_step_kernel", "algorithmic_bytes_per_site": bpsite},

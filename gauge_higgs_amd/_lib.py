@@ -52,6 +52,9 @@ def load():
         "ghx_set_tempering": [vp, i],
         "ghx_swap_step": [vp],
         "ghx_swap_stats": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_int)],
+        "ghx_set_tuning": [vp, i, i],
+        "ghx_get_tuning": [vp, i, C.POINTER(C.c_int)],
+        "ghx_autotune": [vp, i, C.POINTER(C.c_int), dp],
         "ghx_rep_query": [i, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_long), dp, dp],
     }
     for name, args in sig.items():

@@ -1,0 +1,257 @@
+// Large tensor irreps (e.g. SU(3) {6,1}: d = 48, N^n = 3^7 = 2187): the matter
+// part of the MD step with one workgroup per site and the tensor in LDS.
+//
+// Per site x and direction mu the kernel forms, exactly as the reference's
+// matrix-free apply (rep_general.hpp:196-220):
+//   psi_mu = W U_mu(x)^{(x)n} W^T phi_{x+mu}       (lift, n slot applies, project)
+//   chi_mu = W (U_mu(x-mu)^dag)^{(x)n} W^T phi_{x-mu}
+// then the scalar force, the matter back-reaction g^a = -2 Im phi_x^dag T^a_R psi_mu
+// (scalar_higgs.hpp:114-128) and, in step mode, the pi kick and phi drift.
+// W is applied sparsely (it is weight-block sparse: <= 210 of 2187 entries per
+// row for {6,1}); T^a_R is dense d x d.
+#pragma once
+#include "kernels.hpp"
+
+namespace ghx {
+
+constexpr int kBigBlock = 256;
+constexpr int kBigMaxD = 64;      // largest irrep dimension of this path
+constexpr int kBigMaxSlots = 8;   // largest number of boxes
+
+struct BigRepTables {
+  int d, nbox, dimT;
+  const int* col_ptr;   // CSC of W by tensor index t: entries [col_ptr[t], col_ptr[t+1])
+  const int* col_row;   // basis index l of each entry
+  const double* col_val;
+  const int* row_ptr;   // CSR of W by basis index l
+  const int* row_col;   // tensor index t of each entry
+  const double* row_val;
+  const double* T;      // T^a_R, (N^2-1) x d x d complex (re, im)
+};
+
+enum BigMode : int { BIG_STEP = 0, BIG_FORCE = 1, BIG_ACTION = 2 };
+
+template <typename T>
+struct BigArgs {
+  const RepDesc* reps;
+  const int* site_rep;   // not used: sites map through blk_rep of kBlock-sized blocks
+  const int* blk_rep;
+  unsigned Vp;
+  const T* U;            // links (read)
+  const T* phi_in;
+  T* phi_out;
+  T* pi;
+  T* G;                  // matter link force -kappa g^a, planes (mu*(N^2-1)+a)
+  T* Fphi;               // full scalar force (BIG_FORCE)
+  double* q1;            // per-site sum_mu Re phi_x^dag psi_mu (BIG_ACTION)
+  double ck, cd;
+  int flags;             // F_KICK / F_DRIFT / F_ABS_EPS as in StepArgs
+  int mode;
+};
+
+// In-place (M on slot s) over the whole tensor: each thread owns whole fibres
+// (the N entries that differ only in digit s), so no two threads touch the
+// same entry and one LDS buffer suffices.
+template <typename R, int N, bool DAG>
+GHX_DEV void big_slot(const Mat<R, N>& M, cx<R>* __restrict__ X, int dimT, int blk) {
+  const int nfib = dimT / N;
+  for (int f = threadIdx.x; f < nfib; f += kBigBlock) {
+    const int base = (f / blk) * (N * blk) + (f % blk);
+    cx<R> in[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) in[k] = X[base + k * blk];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      cx<R> z = {R(0), R(0)};
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        if (DAG) cfmac(z, M.e[k][j], in[k]);
+        else cfma(z, M.e[j][k], in[k]);
+      }
+      X[base + j * blk] = z;
+    }
+  }
+}
+
+// out[l] = (W M^{(x)n} W^T v)_l with M = U or U^dag; X is a dimT LDS buffer,
+// v and out are d-vectors in LDS.
+template <typename R, int N, bool DAG>
+GHX_DEV void big_apply(const BigRepTables& tb, const Mat<R, N>& M, const cx<R>* v, cx<R>* out, cx<R>* X) {
+  for (int t = threadIdx.x; t < tb.dimT; t += kBigBlock) {  // lift
+    cx<R> z = {R(0), R(0)};
+    for (int e = tb.col_ptr[t]; e < tb.col_ptr[t + 1]; ++e) {
+      const R w = R(tb.col_val[e]);
+      const cx<R> x = v[tb.col_row[e]];
+      z.r = fma(w, x.r, z.r);
+      z.i = fma(w, x.i, z.i);
+    }
+    X[t] = z;
+  }
+  __syncthreads();
+  int blk = tb.dimT / N;
+  for (int s = 0; s < tb.nbox; ++s) {
+    big_slot<R, N, DAG>(M, X, tb.dimT, blk);
+    __syncthreads();
+    blk /= N;
+  }
+  for (int l = threadIdx.x; l < tb.d; l += kBigBlock) {  // project
+    cx<R> z = {R(0), R(0)};
+    for (int e = tb.row_ptr[l]; e < tb.row_ptr[l + 1]; ++e) {
+      const R w = R(tb.row_val[e]);
+      const cx<R> x = X[tb.row_col[e]];
+      z.r = fma(w, x.r, z.r);
+      z.i = fma(w, x.i, z.i);
+    }
+    out[l] = z;
+  }
+  __syncthreads();
+}
+
+template <typename T, int N, int D>
+__global__ __launch_bounds__(kBigBlock) void big_matter_kernel(BigArgs<T> a, BigRepTables tb) {
+  using R = T;
+  constexpr int NGN = N * N - 1;
+  extern __shared__ unsigned char smem_raw[];
+  cx<R>* X = reinterpret_cast<cx<R>*>(smem_raw);
+  __shared__ cx<R> phx[kBigMaxD], vin[kBigMaxD], psi[kBigMaxD], fph[kBigMaxD];
+  __shared__ R gred[NGN][kBigMaxD];
+  __shared__ double q1s;
+  // one block per site; sites are numbered like the kBlock grid of the step kernels
+  const unsigned gsite = blockIdx.x;
+  const int r = a.blk_rep[gsite / kBlock];
+  const RepDesc& rd = a.reps[r];
+  const unsigned s = gsite - rd.offset;
+  if (s >= rd.vol) return;  // whole block exits together
+  const unsigned g = gsite, Vp = a.Vp;
+  const Nbr<D> nb(rd, s);
+  const int d = tb.d;
+  const R k2 = R(2.0 * rd.kappa);
+  for (int l = threadIdx.x; l < d; l += kBigBlock) {
+    phx[l] = {R(a.phi_in[size_t(2 * l) * Vp + g]), R(a.phi_in[size_t(2 * l + 1) * Vp + g])};
+    fph[l] = {R(0), R(0)};
+  }
+  if (threadIdx.x == 0) q1s = 0.0;
+  __syncthreads();
+  for (int mu = 0; mu < D; ++mu) {
+    // forward: psi = D(U_mu(x)) phi_{x+mu}
+    const unsigned gpm = g + nb.up[mu];
+    for (int l = threadIdx.x; l < d; l += kBigBlock)
+      vin[l] = {R(a.phi_in[size_t(2 * l) * Vp + gpm]), R(a.phi_in[size_t(2 * l + 1) * Vp + gpm])};
+    const Mat<R, N> Ux = load_link<T, R, N>(a.U, Vp, mu, g);
+    __syncthreads();
+    big_apply<R, N, false>(tb, Ux, vin, psi, X);
+    if (a.mode == BIG_ACTION) {
+      if (threadIdx.x == 0) {
+        double q = 0.0;
+        for (int l = 0; l < d; ++l) q += double(phx[l].r) * double(psi[l].r) + double(phx[l].i) * double(psi[l].i);
+        q1s += q;
+      }
+      __syncthreads();
+      continue;
+    }
+    // g^a = -2 Im sum_{l,m} conj(phx_l) T^a_{lm} psi_m ; link force -kappa g^a
+    for (int idx = threadIdx.x; idx < NGN * d; idx += kBigBlock) {
+      const int ga = idx / d, l = idx % d;
+      const double* Tr = tb.T + (size_t(ga) * d + l) * d * 2;
+      cx<R> t = {R(0), R(0)};
+      for (int m = 0; m < d; ++m) cfma(t, cx<R>{R(Tr[2 * m]), R(Tr[2 * m + 1])}, psi[m]);
+      cx<R> z = {R(0), R(0)};
+      cfmac(z, phx[l], t);
+      gred[ga][l] = z.i;
+    }
+    __syncthreads();
+    if (threadIdx.x < NGN) {
+      R sum = R(0);
+      for (int l = 0; l < d; ++l) sum += gred[threadIdx.x][l];
+      const R fm = R(-rd.kappa) * (R(-2) * sum);
+      a.G[size_t(mu * NGN + threadIdx.x) * Vp + g] = T(fm);
+    }
+    for (int l = threadIdx.x; l < d; l += kBigBlock) { fph[l].r += k2 * psi[l].r; fph[l].i += k2 * psi[l].i; }
+    __syncthreads();
+    // backward: chi = D(U_mu(x-mu))^dag phi_{x-mu}
+    const unsigned gmm = g + nb.dn[mu];
+    for (int l = threadIdx.x; l < d; l += kBigBlock)
+      vin[l] = {R(a.phi_in[size_t(2 * l) * Vp + gmm]), R(a.phi_in[size_t(2 * l + 1) * Vp + gmm])};
+    const Mat<R, N> Ub = load_link<T, R, N>(a.U, Vp, mu, gmm);
+    __syncthreads();
+    big_apply<R, N, true>(tb, Ub, vin, psi, X);
+    for (int l = threadIdx.x; l < d; l += kBigBlock) { fph[l].r += k2 * psi[l].r; fph[l].i += k2 * psi[l].i; }
+    __syncthreads();
+  }
+  if (a.mode == BIG_ACTION) {
+    if (threadIdx.x == 0) a.q1[g] = q1s;
+    return;
+  }
+  // on-site term -2 dV/dphi^* (scalar_higgs.hpp:111-125)
+  __shared__ R n2s;
+  if (threadIdx.x == 0) {
+    R n2 = R(0);
+    for (int l = 0; l < d; ++l) n2 += phx[l].r * phx[l].r + phx[l].i * phx[l].i;
+    n2s = n2;
+  }
+  __syncthreads();
+  const R c = R(-2) * (R(1) + R(2.0 * rd.lambda) * (n2s - R(1)));
+  const bool abs_eps = a.flags & F_ABS_EPS;
+  const R ek = R(abs_eps ? a.ck : a.ck * rd.eps), ed = R(abs_eps ? a.cd : a.cd * rd.eps);
+  for (int l = threadIdx.x; l < d; l += kBigBlock) {
+    cx<R> f = fph[l];
+    f.r = fma(c, phx[l].r, f.r);
+    f.i = fma(c, phx[l].i, f.i);
+    if (a.mode == BIG_FORCE) {
+      a.Fphi[size_t(2 * l) * Vp + g] = T(f.r);
+      a.Fphi[size_t(2 * l + 1) * Vp + g] = T(f.i);
+      continue;
+    }
+    R pr = R(a.pi[size_t(2 * l) * Vp + g]), pim = R(a.pi[size_t(2 * l + 1) * Vp + g]);
+    if (a.flags & F_KICK) {
+      pr = fma(ek, f.r, pr);
+      pim = fma(ek, f.i, pim);
+      a.pi[size_t(2 * l) * Vp + g] = T(pr);
+      a.pi[size_t(2 * l + 1) * Vp + g] = T(pim);
+    }
+    if (a.flags & F_DRIFT) {
+      a.phi_out[size_t(2 * l) * Vp + g] = T(fma(ed, pr, phx[l].r));
+      a.phi_out[size_t(2 * l + 1) * Vp + g] = T(fma(ed, pim, phx[l].i));
+    }
+  }
+}
+
+// Per-site FP64 contributions for H with a large rep: q1 from big_matter_kernel
+// (BIG_ACTION), the rest streamed plane by plane.
+template <typename T, int N, int D>
+__global__ __launch_bounds__(kRedBlock) void big_action_kernel(ActArgs a, const T* __restrict__ U, const T* __restrict__ P,
+                                                              const T* __restrict__ phi, const T* __restrict__ pi, const double* q1, int d) {
+  using R = double;
+  constexpr int NGN = N * N - 1;
+  const int r = a.red_rep[blockIdx.x];
+  const RepDesc& rd = a.reps[r];
+  const unsigned s = a.red_first[blockIdx.x] - rd.offset + threadIdx.x;
+  double q[kNQ] = {0, 0, 0, 0, 0, 0};
+  if (s < rd.vol) {
+    const unsigned g = rd.offset + s, Vp = a.Vp;
+    const Nbr<D> nb(rd, s);
+    for (int mu = 0; mu < D; ++mu) {
+      const Mat<R, N> Ux = load_link<T, R, N>(U, Vp, mu, g);
+      const unsigned gpm = g + nb.up[mu];
+      for (int nu = mu + 1; nu < D; ++nu) {
+        const Mat<R, N> A = mul_nn(Ux, load_link<T, R, N>(U, Vp, nu, gpm));
+        const Mat<R, N> B = mul_nn(load_link<T, R, N>(U, Vp, nu, g), load_link<T, R, N>(U, Vp, mu, g + nb.up[nu]));
+        q[0] += retr_nd(A, B);
+      }
+      for (int c = 0; c < NGN; ++c) { const double v = double(P[size_t(mu * NGN + c) * Vp + g]); q[4] += v * v; }
+    }
+    q[1] = q1[g];
+    double n2 = 0.0, p2 = 0.0;
+    for (int k = 0; k < 2 * d; ++k) {
+      const double x = double(phi[size_t(k) * Vp + g]), y = double(pi[size_t(k) * Vp + g]);
+      n2 += x * x;
+      p2 += y * y;
+    }
+    q[2] = n2 + rd.lambda * (n2 - 1.0) * (n2 - 1.0);
+    q[3] = n2;
+    q[5] = p2;
+  }
+  block_reduce_store<kNQ>(q, a.partials + size_t(blockIdx.x) * kNQ);
+}
+
+}  // namespace ghx

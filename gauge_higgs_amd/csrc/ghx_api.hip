@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/ghx.h"
+#include "bigrep.hpp"
 #include "kernels.hpp"
 #include "ops.hpp"
 #include "rep_build.hpp"
@@ -57,6 +58,14 @@ struct ghx_ctx {
   int graph_cur = -1;
   std::string err;
   const char* kernel_path = "";
+  int stream_seg = 0;  // streaming MD kernel (0 = one site per thread)
+  // large-irrep path
+  bool big = false;
+  BigOps bops{};
+  BigRepTables tb{};
+  void* dtab[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  void* G = nullptr;
+  double* q1 = nullptr;
   // replica exchange
   bool temper = false;
   std::vector<int> hpairs;  // (i,j) list: even pass pairs then odd pass pairs
@@ -75,6 +84,10 @@ struct ghx_ctx {
     c.stream = stream; c.nblk = nblk; c.nred = nred; c.Vp = Vp;
     c.reps = dreps; c.blk_rep = dblk_rep; c.red_rep = dred_rep; c.red_first = dred_first;
     c.partials = partials; c.W = rep.W.data(); c.d = rep.d; c.real_rep = rep.real ? 1 : 0;
+    c.G = G;
+    c.stream_seg = (nrep == 1) ? stream_seg : 0;
+    c.V3 = hreps[0].stride[D - 1];
+    c.Lt = unsigned(hreps[0].L[D - 1]);
     return c;
   }
 };
@@ -92,8 +105,13 @@ int fail(ghx_ctx* c, int code, const std::string& msg) {
   } while (0)
 
 void free_fields(ghx_ctx* c) {
-  for (void** p : {&c->U[0], &c->U[1], &c->Usave, &c->P, &c->phi[0], &c->phi[1], &c->phisave, &c->pi, &c->F, &c->Fphi})
+  for (void** p : {&c->U[0], &c->U[1], &c->Usave, &c->P, &c->phi[0], &c->phi[1], &c->phisave, &c->pi, &c->F, &c->Fphi, &c->G,
+                   (void**)&c->q1})
     if (*p) { (void)hipFree(*p); *p = nullptr; }
+}
+void free_tables(ghx_ctx* c) {
+  for (auto& p : c->dtab)
+    if (p) { (void)hipFree(p); p = nullptr; }
 }
 
 int alloc_zero(ghx_ctx* ctx, void** p, size_t bytes) {
@@ -116,6 +134,10 @@ int alloc_fields(ghx_ctx* ctx) {
     if ((rc = alloc_zero(ctx, &ctx->phisave, ctx->nphi() * V * ts))) return rc;
     if ((rc = alloc_zero(ctx, &ctx->pi, ctx->nphi() * V * ts))) return rc;
     if ((rc = alloc_zero(ctx, &ctx->Fphi, ctx->nphi() * V * ts))) return rc;
+  }
+  if (ctx->big) {
+    if ((rc = alloc_zero(ctx, &ctx->G, ctx->nP() * V * ts))) return rc;
+    if ((rc = alloc_zero(ctx, (void**)&ctx->q1, V * sizeof(double)))) return rc;
   }
   ctx->cur = 0;
   return GHX_OK;
@@ -144,17 +166,57 @@ bool sym2_canonical(const RepInfo& r) {
   return true;
 }
 
+// Sparse W (CSC by tensor index, CSR by basis index) and dense T^a_R on the device.
+int upload_tables(ghx_ctx* ctx) {
+  free_tables(ctx);
+  const RepInfo& r = ctx->rep;
+  const long dimT = r.dimT;
+  std::vector<int> cptr(dimT + 1, 0), crow, rptr(r.d + 1, 0), rcol;
+  std::vector<double> cval, rval;
+  for (long t = 0; t < dimT; ++t) {
+    for (int l = 0; l < r.d; ++l) {
+      const double w = r.W[size_t(l) * dimT + t];
+      if (w != 0.0) { crow.push_back(l); cval.push_back(w); }
+    }
+    cptr[t + 1] = int(crow.size());
+  }
+  for (int l = 0; l < r.d; ++l) {
+    for (long t = 0; t < dimT; ++t) {
+      const double w = r.W[size_t(l) * dimT + t];
+      if (w != 0.0) { rcol.push_back(int(t)); rval.push_back(w); }
+    }
+    rptr[l + 1] = int(rcol.size());
+  }
+  std::vector<double> T(r.T.size() * 2);
+  for (size_t i = 0; i < r.T.size(); ++i) { T[2 * i] = r.T[i].real(); T[2 * i + 1] = r.T[i].imag(); }
+  const void* src[7] = {cptr.data(), crow.data(), cval.data(), rptr.data(), rcol.data(), rval.data(), T.data()};
+  const size_t bytes[7] = {cptr.size() * 4, crow.size() * 4, cval.size() * 8, rptr.size() * 4, rcol.size() * 4, rval.size() * 8, T.size() * 8};
+  for (int k = 0; k < 7; ++k) {
+    HIPCHK(hipMalloc(&ctx->dtab[k], std::max<size_t>(bytes[k], 8)));
+    HIPCHK(hipMemcpy(ctx->dtab[k], src[k], bytes[k], hipMemcpyHostToDevice));
+  }
+  ctx->tb = BigRepTables{r.d, r.nbox, int(dimT), (const int*)ctx->dtab[0], (const int*)ctx->dtab[1], (const double*)ctx->dtab[2],
+                         (const int*)ctx->dtab[3], (const int*)ctx->dtab[4], (const double*)ctx->dtab[5], (const double*)ctx->dtab[6]};
+  return GHX_OK;
+}
+
 int select_ops(ghx_ctx* ctx) {
   const int p = ctx->prec == GHX_FP32MD ? 1 : 0;
   const RepInfo& r = ctx->rep;
   bool ok = false;
+  ctx->big = false;
   switch (r.kind) {
     case REP_NONE: ok = ops_none(p, ctx->N, ctx->D, &ctx->ops); break;
     case REP_FUND: ok = ops_fund(p, ctx->N, ctx->D, &ctx->ops); break;
     case REP_ADJ: ok = ops_adj(p, ctx->N, ctx->D, &ctx->ops); break;
     case REP_TENSOR:
       if (sym2_canonical(r) && !std::getenv("GHX_GENERIC_TENSOR")) { ok = ops_sym2(p, ctx->N, ctx->D, &ctx->ops); ctx->kernel_path = "sym2"; }
-      else { ok = ops_tensor(p, ctx->N, ctx->D, r.nbox, r.d, &ctx->ops); ctx->kernel_path = "tensor"; }
+      else if (ops_tensor(p, ctx->N, ctx->D, r.nbox, r.d, &ctx->ops)) { ok = true; ctx->kernel_path = "tensor"; }
+      else if (r.d <= kBigMaxD && ops_big(p, ctx->N, ctx->D, &ctx->ops, &ctx->bops)) {
+        ok = true; ctx->big = true; ctx->kernel_path = "big";
+        const int rc2 = upload_tables(ctx);
+        if (rc2) return rc2;
+      }
       break;
   }
   ctx->ops_ok = ok;
@@ -175,7 +237,12 @@ bool has_phi(const ghx_ctx* c) { return c->rep.d > 0; }
 // Sum per-site quantities into ctx->sums [nrep][kNQ] for buffer `b`.
 void launch_action(ghx_ctx* c, int b) {
   const LaunchCtx l = c->lc();
-  c->ops.action(l, c->U[b], c->P, c->phi[b], c->pi);
+  if (c->big) {
+    c->bops.matter(l, c->tb, c->U[b], c->phi[b], nullptr, c->pi, c->G, nullptr, c->q1, 0.0, 0.0, 0, BIG_ACTION);
+    c->bops.action(l, c->U[b], c->P, c->phi[b], c->pi, c->q1, c->rep.d);
+  } else {
+    c->ops.action(l, c->U[b], c->P, c->phi[b], c->pi);
+  }
   hipLaunchKernelGGL(finalize_kernel, dim3(c->nrep), dim3(kRedBlock), 0, c->stream, c->dreps, c->partials, kNQ, c->sums);
 }
 void launch_hamiltonian(ghx_ctx* c, int b, double* H) {
@@ -189,6 +256,7 @@ void launch_step(ghx_ctx* c, double ck, double cd, bool drift) {
   const LaunchCtx l = c->lc();
   const int in = c->cur, out = c->cur ^ 1;
   int flags = F_KICK | (drift ? F_DRIFT : 0);
+  if (c->big) c->bops.matter(l, c->tb, c->U[in], c->phi[in], c->phi[out], c->pi, c->G, nullptr, nullptr, ck, cd, flags, BIG_STEP);
   c->ops.md_step(l, c->U[in], c->U[out], c->P, c->phi[in], c->phi[out], c->pi, c->F, c->Fphi, ck, cd, flags);
   if (drift) c->cur = out;
 }
@@ -389,6 +457,7 @@ int ghx_destroy(ghx_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   drop_graph(ctx);
   free_fields(ctx);
+  free_tables(ctx);
   for (void* p : {(void*)ctx->dreps, (void*)ctx->dblk_rep, (void*)ctx->dred_rep, (void*)ctx->dred_first, (void*)ctx->partials,
                   (void*)ctx->sums, (void*)ctx->Hi, (void*)ctx->Hf, (void*)ctx->Hwork, (void*)ctx->mc, (void*)ctx->dpairs,
                   (void*)ctx->dflags, (void*)ctx->dcfg, (void*)ctx->dq0, (void*)ctx->dq1, (void*)ctx->dattempts,
@@ -518,6 +587,9 @@ int ghx_kick(ghx_ctx* ctx, double eps) {
   if (rc) return rc;
   const LaunchCtx l = ctx->lc();
   const int b = ctx->cur;
+  if (ctx->big)
+    ctx->bops.matter(l, ctx->tb, ctx->U[b], ctx->phi[b], ctx->phi[b ^ 1], ctx->pi, ctx->G, nullptr, nullptr, eps, 0.0, F_KICK | F_ABS_EPS,
+                     BIG_STEP);
   ctx->ops.md_step(l, ctx->U[b], ctx->U[b ^ 1], ctx->P, ctx->phi[b], ctx->phi[b ^ 1], ctx->pi, ctx->F, ctx->Fphi, eps, 0.0,
                    F_KICK | F_ABS_EPS);
   HIPCHK(hipGetLastError());
@@ -530,6 +602,7 @@ int ghx_drift(ghx_ctx* ctx, double eps) {
   const LaunchCtx l = ctx->lc();
   const int b = ctx->cur;
   ctx->ops.drift(l, ctx->U[b], ctx->U[b ^ 1], ctx->P, ctx->phi[b], ctx->phi[b ^ 1], ctx->pi, eps, F_DRIFT | F_ABS_EPS);
+  if (ctx->big) ctx->bops.phi_drift(l, ctx->phi[b], ctx->phi[b ^ 1], ctx->pi, int(ctx->nphi()), eps, 1);
   ctx->cur ^= 1;
   HIPCHK(hipGetLastError());
   return GHX_OK;
@@ -678,6 +751,8 @@ int ghx_forces(ghx_ctx* ctx, int replica, double* F_link, double* F_phi) {
   if (replica < 0 || replica >= ctx->nrep) return GHX_E_INVALID;
   const LaunchCtx l = ctx->lc();
   const int b = ctx->cur;
+  if (ctx->big)
+    ctx->bops.matter(l, ctx->tb, ctx->U[b], ctx->phi[b], nullptr, ctx->pi, ctx->G, ctx->Fphi, nullptr, 0.0, 0.0, F_ABS_EPS, BIG_FORCE);
   ctx->ops.md_step(l, ctx->U[b], ctx->U[b ^ 1], ctx->P, ctx->phi[b], ctx->phi[b ^ 1], ctx->pi, ctx->F, ctx->Fphi, 0.0, 0.0,
                    F_FORCE | F_ABS_EPS);
   HIPCHK(hipGetLastError());
@@ -809,6 +884,80 @@ int ghx_swap_stats(ghx_ctx* ctx, int* npairs, int* pairs, uint64_t* attempts, ui
   if (attempts) HIPCHK(hipMemcpy(attempts, ctx->dattempts, sizeof(uint64_t) * ctx->npairs, hipMemcpyDeviceToHost));
   if (accepts) HIPCHK(hipMemcpy(accepts, ctx->daccepts, sizeof(uint64_t) * ctx->npairs, hipMemcpyDeviceToHost));
   if (last_flags) HIPCHK(hipMemcpy(last_flags, ctx->dflags, sizeof(int) * ctx->npairs, hipMemcpyDeviceToHost));
+  return GHX_OK;
+}
+
+int ghx_set_tuning(ghx_ctx* ctx, int key, int value) {
+  if (!ctx) return GHX_E_INVALID;
+  if (key == GHX_TUNE_STREAM_SEG) {
+    if (value < 0) return fail(ctx, GHX_E_INVALID, "stream segment must be >= 0");
+    ctx->stream_seg = value;
+    drop_graph(ctx);
+    return GHX_OK;
+  }
+  return fail(ctx, GHX_E_INVALID, "unknown tuning key");
+}
+
+int ghx_get_tuning(const ghx_ctx* ctx, int key, int* value) {
+  if (!ctx || !value) return GHX_E_INVALID;
+  if (key == GHX_TUNE_STREAM_SEG) { *value = ctx->stream_seg; return GHX_OK; }
+  return GHX_E_INVALID;
+}
+
+// Time each candidate launch shape on the live fields (state saved and
+// restored around the trial steps) and keep the fastest.
+int ghx_autotune(ghx_ctx* ctx, int steps, int* best_seg, double* best_ms) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (steps < 1) return GHX_E_INVALID;
+  if (ctx->cur != 0) launch_settle(ctx, false);
+  const size_t nUb = ctx->nU() * ctx->Vp * ctx->tsize, nPb = ctx->nP() * ctx->Vp * ctx->tsize;
+  const size_t nphb = has_phi(ctx) ? ctx->nphi() * ctx->Vp * ctx->tsize : 0;
+  void *Ps = nullptr, *pis = nullptr;
+  HIPCHK(hipMalloc(&Ps, nPb));
+  if (nphb) HIPCHK(hipMalloc(&pis, nphb));
+  HIPCHK(hipMemcpyAsync(ctx->Usave, ctx->U[0], nUb, hipMemcpyDeviceToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(Ps, ctx->P, nPb, hipMemcpyDeviceToDevice, ctx->stream));
+  if (nphb) {
+    HIPCHK(hipMemcpyAsync(ctx->phisave, ctx->phi[0], nphb, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(pis, ctx->pi, nphb, hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  std::vector<int> cand = {0};
+  if (ctx->nrep == 1)
+    for (int sg : {1, 2, 4, 8, 16, 32, 64})
+      if (sg <= ctx->hreps[0].L[ctx->D - 1]) cand.push_back(sg);
+  int best = ctx->stream_seg;
+  double bms = 1e300;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  for (int sg : cand) {
+    ctx->stream_seg = sg;
+    launch_step(ctx, 1.0, 1.0, true);  // warm
+    HIPCHK(hipEventRecord(e0, ctx->stream));
+    for (int i = 0; i < steps; ++i) launch_step(ctx, 1.0, 1.0, true);
+    HIPCHK(hipEventRecord(e1, ctx->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < bms) { bms = ms; best = sg; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  ctx->stream_seg = best;
+  ctx->cur = 0;
+  HIPCHK(hipMemcpyAsync(ctx->U[0], ctx->Usave, nUb, hipMemcpyDeviceToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(ctx->P, Ps, nPb, hipMemcpyDeviceToDevice, ctx->stream));
+  if (nphb) {
+    HIPCHK(hipMemcpyAsync(ctx->phi[0], ctx->phisave, nphb, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->pi, pis, nphb, hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipFree(Ps));
+  if (pis) HIPCHK(hipFree(pis));
+  drop_graph(ctx);
+  if (best_seg) *best_seg = best;
+  if (best_ms) *best_ms = bms / steps;
   return GHX_OK;
 }
 

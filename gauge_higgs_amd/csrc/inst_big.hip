@@ -1,0 +1,37 @@
+// Large tensor irreps (matrix-free apply in LDS, one workgroup per site).
+#include "bigrep.hpp"
+#include "launch.hpp"
+namespace ghx {
+template <typename T, int N, int D>
+struct BigLaunch {
+  static void matter(const LaunchCtx& c, const BigRepTables& tb, const void* U, const void* phi_in, void* phi_out, void* pi, void* G,
+                     void* Fphi, double* q1, double ck, double cd, int flags, int mode) {
+    BigArgs<T> a{c.reps, nullptr, c.blk_rep, c.Vp, (const T*)U, (const T*)phi_in, (T*)phi_out, (T*)pi, (T*)G, (T*)Fphi, q1, ck, cd, flags, mode};
+    const size_t lds = size_t(tb.dimT) * sizeof(cx<T>);
+    hipLaunchKernelGGL((big_matter_kernel<T, N, D>), dim3(c.Vp), dim3(kBigBlock), lds, c.stream, a, tb);
+  }
+  static void action(const LaunchCtx& c, const void* U, const void* P, const void* phi, const void* pi, const double* q1, int d) {
+    ActArgs a{c.reps, c.red_rep, c.red_first, c.Vp, c.partials};
+    hipLaunchKernelGGL((big_action_kernel<T, N, D>), dim3(c.nred), dim3(kRedBlock), 0, c.stream, a, (const T*)U, (const T*)P,
+                       (const T*)phi, (const T*)pi, q1, d);
+  }
+  static void phi_drift(const LaunchCtx& c, const void* phi_in, void* phi_out, const void* pi, int nplanes, double cd, int abs_eps) {
+    hipLaunchKernelGGL(phi_drift_kernel<T>, dim3(c.nblk), dim3(kBlock), 0, c.stream, c.reps, c.blk_rep, c.Vp, (const T*)phi_in,
+                       (T*)phi_out, (const T*)pi, nplanes, cd, abs_eps);
+  }
+};
+template <typename T, int N, int D>
+void fill(Ops* o, BigOps* b) {
+  *o = Launch<T, N, D, RepExt>::ops();
+  b->matter = &BigLaunch<T, N, D>::matter;
+  b->action = &BigLaunch<T, N, D>::action;
+  b->phi_drift = &BigLaunch<T, N, D>::phi_drift;
+}
+bool ops_big(int prec, int N, int D, Ops* o, BigOps* b) {
+  if (N == 3 && D == 4) { prec ? fill<float, 3, 4>(o, b) : fill<double, 3, 4>(o, b); return true; }
+  if (N == 3 && D == 3) { prec ? fill<float, 3, 3>(o, b) : fill<double, 3, 3>(o, b); return true; }
+  if (N == 2 && D == 4) { prec ? fill<float, 2, 4>(o, b) : fill<double, 2, 4>(o, b); return true; }
+  if (N == 2 && D == 3) { prec ? fill<float, 2, 3>(o, b) : fill<double, 2, 3>(o, b); return true; }
+  return false;
+}
+}  // namespace ghx

@@ -47,6 +47,7 @@ struct StepArgs {
   T* Fphi_out;
   double ck, cd;  // kick / drift coefficient (times eps of the replica unless F_ABS_EPS)
   int flags;
+  const T* G;     // external matter link force planes (REP_EXT)
 };
 
 template <typename T, typename R, int N>
@@ -148,6 +149,22 @@ GHX_DEV void bstore_vec(const Planes<T>& f, unsigned g, const cx<R>* v) {
   for (int k = 0; k < DR; ++k) { f.st(g, 2 * k, T(v[k].r)); f.st(g, 2 * k + 1, T(v[k].i)); }
 }
 
+// XCD-aware block order: the hardware deals consecutive workgroups round-robin
+// over the 8 XCDs (each with its own L2). Renumbering so XCD k gets the k-th
+// contiguous chunk of blocks keeps every +-mu neighbour block (1, L0*L1/128,
+// ... blocks away) in the same L2. Bijective on [0, n).
+#ifndef GHX_XCD_REMAP
+#define GHX_XCD_REMAP 1
+#endif
+GHX_DEV unsigned xcd_block(unsigned b, unsigned n) {
+#if GHX_XCD_REMAP
+  constexpr unsigned kXcd = 8;
+  const unsigned q = n / kXcd;
+  if (b < q * kXcd) return (b % kXcd) * q + b / kXcd;
+#endif
+  return b;
+}
+
 // Site -> neighbour offsets for a periodic lattice (geometry.hpp:20-38 order:
 // direction 0 fastest). up[m]/dn[m] are signed offsets to add to the site.
 template <int D>
@@ -175,15 +192,11 @@ struct Nbr {
 // matter forces: scalar_higgs.hpp:156-213; drift: gauge_higgs_hmc.hpp:136-147.
 // ---------------------------------------------------------------------------
 template <typename T, int N, int D, class REP>
-__global__ __launch_bounds__(kBlock, GHX_MD_WAVES) void md_step_kernel(StepArgs<T> a, typename REP::template Params<T> rp) {
+GHX_DEV void site_update(const StepArgs<T>& a, const typename REP::template Params<T>& rp, const RepDesc& rd, unsigned s) {
   using R = T;
   constexpr int NC = 2 * N * N;
   constexpr int NGN = N * N - 1;
   constexpr int DR = REP::DR;
-  const int r = a.blk_rep[blockIdx.x];
-  const RepDesc& rd = a.reps[r];
-  const unsigned s = blockIdx.x * kBlock + threadIdx.x - rd.offset;
-  if (s >= rd.vol) return;
   const unsigned g = rd.offset + s;
   const unsigned Vp = a.Vp;
   const Nbr<D> nb(rd, s);
@@ -261,6 +274,11 @@ __global__ __launch_bounds__(kBlock, GHX_MD_WAVES) void md_step_kernel(StepArgs<
       }
     R f[NGN];
     im_tr_gen<R, N>(X, f);
+    if constexpr (REP::KIND == REP_EXT) {
+      const Planes<T> Gp(a.G, Vp, D * NGN);
+#pragma unroll
+      for (int c = 0; c < NGN; ++c) f[c] += R(Gp.ld(g, unsigned(mu * NGN + c)));
+    }
     if (a.flags & F_FORCE) {
       const Planes<T> Fp(a.F_out, Vp, D * NGN);
 #pragma unroll
@@ -298,6 +316,39 @@ __global__ __launch_bounds__(kBlock, GHX_MD_WAVES) void md_step_kernel(StepArgs<
   }
 }
 
+
+// One site per thread; blocks renumbered XCD-aware.
+template <typename T, int N, int D, class REP>
+__global__ __launch_bounds__(kBlock, GHX_MD_WAVES) void md_step_kernel(StepArgs<T> a, typename REP::template Params<T> rp) {
+  const unsigned bid = xcd_block(blockIdx.x, gridDim.x);
+  const int r = a.blk_rep[bid];
+  const RepDesc& rd = a.reps[r];
+  const unsigned s = bid * kBlock + threadIdx.x - rd.offset;
+  if (s >= rd.vol) return;
+  site_update<T, N, D, REP>(a, rp, rd, s);
+}
+
+// Streaming variant for a single lattice: a block owns kBlock consecutive
+// sites of the (D-1)-dimensional cross-section and walks `seg` consecutive
+// slices of the last direction, so the +-t neighbours it needs were touched
+// by the same block one or two iterations earlier (cache-resident) instead of
+// being re-fetched from HBM. Block b handles cross-section tile (b % ntile),
+// segment (b / ntile); ntile % 8 == 0 keeps all segments of a tile on one XCD.
+template <typename T, int N, int D, class REP>
+__global__ __launch_bounds__(kBlock, GHX_MD_WAVES) void md_stream_kernel(StepArgs<T> a, typename REP::template Params<T> rp, unsigned ntile,
+                                                                       int seg) {
+  const RepDesc& rd = a.reps[0];
+  const unsigned V3 = rd.stride[D - 1];
+  const unsigned tile = blockIdx.x % ntile, sg = blockIdx.x / ntile;
+  const unsigned s3 = tile * kBlock + threadIdx.x;
+  if (s3 >= V3) return;
+  const int Lt = rd.L[D - 1];
+  const int t0 = int(sg) * seg;
+  const int t1 = t0 + seg < Lt ? t0 + seg : Lt;
+#pragma unroll 1
+  for (int t = t0; t < t1; ++t) site_update<T, N, D, REP>(a, rp, rd, s3 + unsigned(t) * V3);
+}
+
 // Drift alone (U' = exp(i eps P) U, phi' = phi + eps pi) — for the kick/drift API.
 template <typename T, int N, int D, int DR>
 __global__ __launch_bounds__(kBlock) void drift_kernel(StepArgs<T> a) {
@@ -326,6 +377,19 @@ __global__ __launch_bounds__(kBlock) void drift_kernel(StepArgs<T> a) {
     for (int k = 0; k < DR; ++k) { ph[k].r = fma(ed, pv[k].r, ph[k].r); ph[k].i = fma(ed, pv[k].i, ph[k].i); }
     store_vec<T, R, DR>(a.phi_out, Vp, g, ph);
   }
+}
+
+// phi' = phi + eps pi for a runtime number of planes (large reps).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void phi_drift_kernel(const RepDesc* reps, const int* blk_rep, unsigned Vp, const T* phi_in,
+                                                          T* phi_out, const T* pi, int nplanes, double cd, int abs_eps) {
+  const int r = blk_rep[blockIdx.x];
+  const RepDesc& rd = reps[r];
+  const unsigned s = blockIdx.x * kBlock + threadIdx.x - rd.offset;
+  if (s >= rd.vol) return;
+  const unsigned g = rd.offset + s;
+  const T e = T(abs_eps ? cd : cd * rd.eps);
+  for (int c = 0; c < nplanes; ++c) phi_out[size_t(c) * Vp + g] = phi_in[size_t(c) * Vp + g] + e * pi[size_t(c) * Vp + g];
 }
 
 // ---------------------------------------------------------------------------

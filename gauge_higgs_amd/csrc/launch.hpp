@@ -27,13 +27,22 @@ struct Launch {
   static void md_step(const LaunchCtx& c, const void* U_in, void* U_out, void* P, const void* phi_in, void* phi_out, void* pi,
                       void* F, void* Fphi, double ck, double cd, int flags) {
     StepArgs<T> a{c.reps, c.blk_rep, c.Vp, (const T*)U_in, (T*)U_out, (T*)P, (const T*)phi_in, (T*)phi_out, (T*)pi, (T*)F, (T*)Fphi,
-                  ck, cd, flags};
-    hipLaunchKernelGGL((md_step_kernel<T, N, D, REP>), dim3(c.nblk), dim3(kBlock), 0, c.stream, a, ParamFill<REP, T>::make(c));
+                  ck, cd, flags, (const T*)c.G};
+    if (c.stream_seg > 0) {
+      const unsigned V3 = c.V3, Lt = c.Lt;
+      unsigned ntile = (V3 + kBlock - 1) / kBlock;
+      ntile = (ntile + 7) / 8 * 8;
+      const unsigned nseg = (Lt + unsigned(c.stream_seg) - 1) / unsigned(c.stream_seg);
+      hipLaunchKernelGGL((md_stream_kernel<T, N, D, REP>), dim3(ntile * nseg), dim3(kBlock), 0, c.stream, a, ParamFill<REP, T>::make(c),
+                         ntile, c.stream_seg);
+    } else {
+      hipLaunchKernelGGL((md_step_kernel<T, N, D, REP>), dim3(c.nblk), dim3(kBlock), 0, c.stream, a, ParamFill<REP, T>::make(c));
+    }
   }
   static void drift(const LaunchCtx& c, const void* U_in, void* U_out, const void* P, const void* phi_in, void* phi_out,
                     const void* pi, double cd, int flags) {
     StepArgs<T> a{c.reps, c.blk_rep, c.Vp, (const T*)U_in, (T*)U_out, (T*)P, (const T*)phi_in, (T*)phi_out, (T*)pi, nullptr, nullptr,
-                  0.0, cd, flags};
+                  0.0, cd, flags, nullptr};
     hipLaunchKernelGGL((drift_kernel<T, N, D, REP::DR>), dim3(c.nblk), dim3(kBlock), 0, c.stream, a);
   }
   static void action(const LaunchCtx& c, const void* U, const void* P, const void* phi, const void* pi) {
@@ -43,7 +52,7 @@ struct Launch {
   }
   static void refresh(const LaunchCtx& c, void* P, void* pi, const McState* st, unsigned long long tagP, unsigned long long tagPi) {
     hipLaunchKernelGGL((refresh_kernel<T, N, D>), dim3(c.nblk), dim3(kBlock), 0, c.stream, c.reps, c.blk_rep, c.Vp, (T*)P, (T*)pi,
-                       REP::DR, c.real_rep, st, tagP, tagPi);
+                       c.d, c.real_rep, st, tagP, tagPi);
   }
   static void init_links(const LaunchCtx& c, void* U, int replica, int mode, double sigma) {
     hipLaunchKernelGGL((init_links_kernel<T, N, D>), dim3(c.nblk), dim3(kBlock), 0, c.stream, c.reps, c.blk_rep, c.Vp, (T*)U, replica,

@@ -18,6 +18,18 @@ struct LaunchCtx {
   double* partials;
   const double* W;  // tensor basis on the host, d x dimT (tensor reps)
   int d, real_rep;
+  const void* G;    // external matter link force planes (large reps)
+  int stream_seg;   // > 0: single-lattice streaming kernel with this many t-slices per block
+  unsigned V3, Lt;  // cross-section volume and last extent of replica 0 (streaming kernel)
+};
+
+struct BigRepTables;
+// Extra launchers of the large-irrep path.
+struct BigOps {
+  void (*matter)(const LaunchCtx&, const BigRepTables&, const void* U, const void* phi_in, void* phi_out, void* pi, void* G,
+                 void* Fphi, double* q1, double ck, double cd, int flags, int mode);
+  void (*action)(const LaunchCtx&, const void* U, const void* P, const void* phi, const void* pi, const double* q1, int d);
+  void (*phi_drift)(const LaunchCtx&, const void* phi_in, void* phi_out, const void* pi, int nplanes, double cd, int abs_eps);
 };
 
 struct Ops {
@@ -40,5 +52,6 @@ bool ops_fund(int prec, int N, int D, Ops* o);
 bool ops_adj(int prec, int N, int D, Ops* o);
 bool ops_tensor(int prec, int N, int D, int nbox, int d, Ops* o);
 bool ops_sym2(int prec, int N, int D, Ops* o);
+bool ops_big(int prec, int N, int D, Ops* o, BigOps* b);
 
 }  // namespace ghx

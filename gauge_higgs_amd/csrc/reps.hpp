@@ -18,7 +18,7 @@
 
 namespace ghx {
 
-enum RepKind : int { REP_NONE = 0, REP_FUND = 1, REP_ADJ = 2, REP_TENSOR = 3 };
+enum RepKind : int { REP_NONE = 0, REP_FUND = 1, REP_ADJ = 2, REP_TENSOR = 3, REP_EXT = 4 };
 
 struct NoParams {};
 
@@ -27,6 +27,17 @@ template <typename R, int N>
 struct RepNone {
   static constexpr int DR = 0;
   static constexpr int KIND = REP_NONE;
+  template <typename P> using Params = NoParams;
+};
+
+// ---------------------------------------------------------------- external matter
+// Large irreps: the scalar force, pi/phi update and the matter link force are
+// computed by big_matter_kernel; the step kernel only adds that link force
+// (read from the G planes) to the gauge force.
+template <typename R, int N>
+struct RepExt {
+  static constexpr int DR = 0;
+  static constexpr int KIND = REP_EXT;
   template <typename P> using Params = NoParams;
 };
 

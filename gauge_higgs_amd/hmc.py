@@ -138,6 +138,20 @@ class Lattices:
         self._check(self._lib.ghx_time_md_steps(self.h, steps, C.byref(ms)))
         return ms.value
 
+    def set_stream_segment(self, seg):
+        self._check(self._lib.ghx_set_tuning(self.h, 0, seg))
+
+    def stream_segment(self):
+        v = C.c_int()
+        self._check(self._lib.ghx_get_tuning(self.h, 0, C.byref(v)))
+        return v.value
+
+    def autotune(self, steps=10):
+        """Pick the fastest MD-step launch shape on the live fields (fields restored)."""
+        best, ms = C.c_int(), C.c_double()
+        self._check(self._lib.ghx_autotune(self.h, steps, C.byref(best), C.byref(ms)))
+        return best.value, ms.value
+
     def replica_swap(self, i, j, u):
         acc, delta = C.c_int(), C.c_double()
         self._check(self._lib.ghx_replica_swap(self.h, i, j, u, C.byref(acc), C.byref(delta)))

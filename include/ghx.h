@@ -124,6 +124,16 @@ int ghx_swap_step(ghx_ctx* ctx);
  * accepts per pair and the flags of the last swap step. Any pointer may be NULL. */
 int ghx_swap_stats(ghx_ctx* ctx, int* npairs, int* pairs, uint64_t* attempts, uint64_t* accepts, int* last_flags);
 
+/* Launch-shape tuning (no counterpart in the reference, which runs OpenMP
+ * loops): GHX_TUNE_STREAM_SEG = number of slices of the last direction one
+ * workgroup walks in the MD-step kernel (0 = one site per thread). The
+ * autotuner times every candidate for `steps` MD steps on the live fields,
+ * restores the fields, and keeps the fastest. */
+enum { GHX_TUNE_STREAM_SEG = 0 };
+int ghx_set_tuning(ghx_ctx* ctx, int key, int value);
+int ghx_get_tuning(const ghx_ctx* ctx, int key, int* value);
+int ghx_autotune(ghx_ctx* ctx, int steps, int* best_seg, double* best_ms_per_step);
+
 int ghx_sync(ghx_ctx* ctx);
 /* Time `steps` fused MD steps (kick+drift launches) with HIP events on the
  * context's stream: total ms and the number of launches timed. */

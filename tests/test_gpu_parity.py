@@ -233,3 +233,31 @@ def test_fp32_su3_drift_exponential(eps):
     assert np.max(np.abs(U - o.get("U"))) < 2e-6 * max(1.0, eps)
     # stays on the group to float rounding
     assert np.max(np.abs(np.einsum("lji,ljk->lik", U.conj(), U) - np.eye(3))) < 2e-6 * max(1.0, eps)
+
+
+@pytest.mark.parametrize("seg", [1, 3, 4, 16])
+@pytest.mark.parametrize("prec", [gha.FP64, gha.FP32MD])
+def test_streaming_kernel_equals_site_kernel(seg, prec):
+    """The t-streaming launch shape performs the same per-site arithmetic: bitwise equal."""
+    out = []
+    for s in (0, seg):
+        g = gha.GaugeHiggsHMC([4, 6, 4, 10], "2", seed=9, D=4, N=3, precision=prec).configure(beta=5.7, kappa=0.1, lam=0.5, nmd=4)
+        g.lat.set_stream_segment(s)
+        g.hot(0.8, 12345, 0.5)
+        g.trajectory()
+        out.append((g.last_dH, g.get("U"), g.get("phi"), g.get("P")))
+    assert out[0][0] == out[1][0]
+    for k in (1, 2, 3):
+        assert np.array_equal(out[0][k], out[1][k])
+
+
+def test_autotune_restores_fields():
+    g = gha.GaugeHiggsHMC(8, "2", seed=2, D=4, N=3, precision=gha.FP32MD).configure(beta=5.7, kappa=0.1, lam=0.5)
+    g.hot(0.8, 12345, 0.5)
+    g.refresh_momenta()
+    before = [g.get(k) for k in ("U", "P", "phi", "pi")]
+    seg, ms = g.lat.autotune(steps=3)
+    assert seg in (0, 1, 2, 4, 8) and ms > 0 and g.lat.stream_segment() == seg
+    after = [g.get(k) for k in ("U", "P", "phi", "pi")]
+    for a, b in zip(before, after):
+        assert np.array_equal(a, b)

@@ -90,10 +90,10 @@ def test_replica_exchange_matches_host_model(ladder_kind):
 
 
 def test_kappa_scan_32_chains_runs_in_one_launch_per_step():
-    rs = gha.kappa_scan(seed0=1, nmd=10)
+    rs = gha.kappa_scan(seed0=1, nmd=20)
     assert rs.n == 32 and sorted(set(len(l) for l in rs.ladders)) == [8]
     assert rs.lat.D == 4 and sum(rs.lat.vols) == 8 * (6 ** 4 + 8 ** 4 + 10 ** 4 + 12 ** 4)
-    rs.step(5)
+    rs.step(30)
     obs = rs.observables()
     assert np.all(np.isfinite(obs))
     # L_link grows with kappa inside every ladder (8 kappas from 0.20 to 0.32)
@@ -101,7 +101,7 @@ def test_kappa_scan_32_chains_runs_in_one_launch_per_step():
         ll = obs[lad, 3]
         assert ll[-1] > ll[0]
     acc = [rs.hmc_acceptance(k) for k in range(32)]
-    assert min(acc) >= 0.4
+    assert np.mean(acc) >= 0.5  # includes the thermalisation from the hot start
     pa = rs.pair_acceptance()
     assert len(pa) == 4 * 7 and all(0.0 <= v <= 1.0 for v in pa.values())
 
