@@ -29,6 +29,7 @@ def load():
         "ghx_set_couplings": [vp, i, d, d, d, d],
         "ghx_set_integrator": [vp, i, i, i],
         "ghx_set_seed": [vp, i, u64],
+        "ghx_set_nested": [vp, i, i],
         "ghx_init_fields": [vp, i, i, d, i, u64, d],
         "ghx_set_counters": [vp, i, u64, u64],
         "ghx_get_counters": [vp, i, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), dp],
@@ -55,6 +56,13 @@ def load():
         "ghx_set_tuning": [vp, i, i],
         "ghx_get_tuning": [vp, i, C.POINTER(C.c_int)],
         "ghx_autotune": [vp, i, C.POINTER(C.c_int), dp],
+        "ghx_surrogate_init": [vp, u64],
+        "ghx_surrogate_enable": [vp, i],
+        "ghx_surrogate_nparams": [],
+        "ghx_surrogate_params": [vp, i, C.POINTER(C.c_float)],
+        "ghx_surrogate_gradient": [vp, C.POINTER(C.c_float), dp],
+        "ghx_surrogate_train": [vp, i, d, dp, dp],
+        "ghx_surrogate_snapshot": [vp, i, C.POINTER(C.c_float), C.POINTER(C.c_float), dp, dp],
         "ghx_rep_query": [i, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_long), dp, dp],
     }
     for name, args in sig.items():

@@ -132,7 +132,8 @@ __global__ __launch_bounds__(kBigBlock) void big_matter_kernel(BigArgs<T> a, Big
   }
   if (threadIdx.x == 0) q1s = 0.0;
   __syncthreads();
-  for (int mu = 0; mu < D; ++mu) {
+  const int nmu = (a.flags & F_NO_HOP) ? 0 : D;
+  for (int mu = 0; mu < nmu; ++mu) {
     // forward: psi = D(U_mu(x)) phi_{x+mu}
     const unsigned gpm = g + nb.up[mu];
     for (int l = threadIdx.x; l < d; l += kBigBlock)
@@ -190,7 +191,7 @@ __global__ __launch_bounds__(kBigBlock) void big_matter_kernel(BigArgs<T> a, Big
     n2s = n2;
   }
   __syncthreads();
-  const R c = R(-2) * (R(1) + R(2.0 * rd.lambda) * (n2s - R(1)));
+  const R c = (a.flags & F_NO_ONSITE) ? R(0) : R(-2) * (R(1) + R(2.0 * rd.lambda) * (n2s - R(1)));
   const bool abs_eps = a.flags & F_ABS_EPS;
   const R ek = R(abs_eps ? a.ck : a.ck * rd.eps), ed = R(abs_eps ? a.cd : a.cd * rd.eps);
   for (int l = threadIdx.x; l < d; l += kBigBlock) {

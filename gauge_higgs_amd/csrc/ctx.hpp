@@ -1,0 +1,101 @@
+// Context of the C ABI (shared by the API and the learned-force units).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <string>
+#include <vector>
+
+#include "../../include/ghx.h"
+#include "bigrep.hpp"
+#include "kernels.hpp"
+#include "ops.hpp"
+#include "rep_build.hpp"
+
+using namespace ghx;
+
+struct SurrogateState;
+
+struct ghx_ctx {
+  int D = 4, N = 2, nrep = 1, prec = GHX_FP64;
+  size_t tsize = 8;
+  std::vector<std::array<int, 4>> L;
+  std::vector<RepDesc> hreps;
+  std::vector<int> hblk_rep, hred_rep;
+  std::vector<unsigned> hred_first;
+  unsigned nblk = 0, nred = 0, Vp = 0;
+  // representation
+  RepInfo rep;
+  bool ops_ok = false;
+  Ops ops{};
+  // integrator
+  int nmd = 20, integ = GHX_OMELYAN;
+  bool reunit = true;
+  // device state
+  RepDesc* dreps = nullptr;
+  int *dblk_rep = nullptr, *dred_rep = nullptr;
+  unsigned* dred_first = nullptr;
+  void *U[2] = {nullptr, nullptr}, *Usave = nullptr, *P = nullptr;
+  void *phi[2] = {nullptr, nullptr}, *phisave = nullptr, *pi = nullptr;
+  void *F = nullptr, *Fphi = nullptr;
+  double *partials = nullptr, *sums = nullptr, *Hi = nullptr, *Hf = nullptr, *Hwork = nullptr;
+  McState* mc = nullptr;
+  int cur = 0;
+  hipStream_t stream = nullptr;
+  hipGraphExec_t graph = nullptr;
+  int graph_cur = -1;
+  std::string err;
+  const char* kernel_path = "";
+  int stream_seg = 0;  // streaming MD kernel (0 = one site per thread)
+  // multi-timescale integrator (0 = single scale)
+  int n_outer = 0, m_inner = 1;
+  // surrogate link force (learned); S planes are filled by surr_fill before a kick
+  void* S = nullptr;
+  bool use_surr = false;
+  void (*surr_fill)(ghx_ctx*, int buf) = nullptr;
+  SurrogateState* surr_state = nullptr;
+  // large-irrep path
+  bool big = false;
+  BigOps bops{};
+  BigRepTables tb{};
+  void* dtab[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  void* G = nullptr;
+  double* q1 = nullptr;
+  // replica exchange
+  bool temper = false;
+  std::vector<int> hpairs;  // (i,j) list: even pass pairs then odd pass pairs
+  int neven = 0, npairs = 0;
+  int *dpairs = nullptr, *dflags = nullptr, *dcfg = nullptr;
+  double *dq0 = nullptr, *dq1 = nullptr;
+  unsigned long long *dattempts = nullptr, *daccepts = nullptr;
+  SwapState* dswap = nullptr;
+  unsigned maxvol = 0;
+
+  size_t nU() const { return size_t(D) * 2 * N * N; }
+  size_t nP() const { return size_t(D) * (N * N - 1); }
+  size_t nphi() const { return size_t(2) * rep.d; }
+  LaunchCtx lc() const {
+    LaunchCtx c;
+    c.stream = stream; c.nblk = nblk; c.nred = nred; c.Vp = Vp;
+    c.reps = dreps; c.blk_rep = dblk_rep; c.red_rep = dred_rep; c.red_first = dred_first;
+    c.partials = partials; c.W = rep.W.data(); c.d = rep.d; c.real_rep = rep.real ? 1 : 0;
+    c.G = G;
+    c.S = S;
+    c.surr_sign = 0.0;
+    c.stream_seg = (nrep == 1) ? stream_seg : 0;
+    c.V3 = hreps[0].stride[D - 1];
+    c.Lt = unsigned(hreps[0].L[D - 1]);
+    return c;
+  }
+};
+
+
+// Helpers defined in ghx_api.hip, used by the learned-force unit.
+namespace ghx {
+int ctx_fail(ghx_ctx* c, int code, const std::string& msg);
+int ctx_check_ready(ghx_ctx* ctx);
+// Link force restricted to `parts` (F_NO_* flags) of buffer b into ctx->F (and
+// ctx->Fphi); big reps run their matter kernel first.
+void ctx_launch_forces(ghx_ctx* c, int b, int parts);
+void surrogate_free(ghx_ctx* c);  // learned.hip
+}  // namespace ghx

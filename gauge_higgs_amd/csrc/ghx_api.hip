@@ -14,83 +14,13 @@
 #include <string>
 #include <vector>
 
-#include "../../include/ghx.h"
-#include "bigrep.hpp"
-#include "kernels.hpp"
-#include "ops.hpp"
-#include "rep_build.hpp"
-
-using namespace ghx;
+#include "ctx.hpp"
 
 namespace {
 constexpr double kOmelyanLambda = 0.1931833275037836;  // config.hpp:25
 constexpr unsigned long long kTagP_pure = 0xA1, kTagMC_pure = 0xB2;        // hmc.hpp:74,86
 constexpr unsigned long long kTagP = 0xC3, kTagPi = 0xD4, kTagMC = 0xE5;   // gauge_higgs_hmc.hpp:167-185
 }  // namespace
-
-struct ghx_ctx {
-  int D = 4, N = 2, nrep = 1, prec = GHX_FP64;
-  size_t tsize = 8;
-  std::vector<std::array<int, 4>> L;
-  std::vector<RepDesc> hreps;
-  std::vector<int> hblk_rep, hred_rep;
-  std::vector<unsigned> hred_first;
-  unsigned nblk = 0, nred = 0, Vp = 0;
-  // representation
-  RepInfo rep;
-  bool ops_ok = false;
-  Ops ops{};
-  // integrator
-  int nmd = 20, integ = GHX_OMELYAN;
-  bool reunit = true;
-  // device state
-  RepDesc* dreps = nullptr;
-  int *dblk_rep = nullptr, *dred_rep = nullptr;
-  unsigned* dred_first = nullptr;
-  void *U[2] = {nullptr, nullptr}, *Usave = nullptr, *P = nullptr;
-  void *phi[2] = {nullptr, nullptr}, *phisave = nullptr, *pi = nullptr;
-  void *F = nullptr, *Fphi = nullptr;
-  double *partials = nullptr, *sums = nullptr, *Hi = nullptr, *Hf = nullptr, *Hwork = nullptr;
-  McState* mc = nullptr;
-  int cur = 0;
-  hipStream_t stream = nullptr;
-  hipGraphExec_t graph = nullptr;
-  int graph_cur = -1;
-  std::string err;
-  const char* kernel_path = "";
-  int stream_seg = 0;  // streaming MD kernel (0 = one site per thread)
-  // large-irrep path
-  bool big = false;
-  BigOps bops{};
-  BigRepTables tb{};
-  void* dtab[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-  void* G = nullptr;
-  double* q1 = nullptr;
-  // replica exchange
-  bool temper = false;
-  std::vector<int> hpairs;  // (i,j) list: even pass pairs then odd pass pairs
-  int neven = 0, npairs = 0;
-  int *dpairs = nullptr, *dflags = nullptr, *dcfg = nullptr;
-  double *dq0 = nullptr, *dq1 = nullptr;
-  unsigned long long *dattempts = nullptr, *daccepts = nullptr;
-  SwapState* dswap = nullptr;
-  unsigned maxvol = 0;
-
-  size_t nU() const { return size_t(D) * 2 * N * N; }
-  size_t nP() const { return size_t(D) * (N * N - 1); }
-  size_t nphi() const { return size_t(2) * rep.d; }
-  LaunchCtx lc() const {
-    LaunchCtx c;
-    c.stream = stream; c.nblk = nblk; c.nred = nred; c.Vp = Vp;
-    c.reps = dreps; c.blk_rep = dblk_rep; c.red_rep = dred_rep; c.red_first = dred_first;
-    c.partials = partials; c.W = rep.W.data(); c.d = rep.d; c.real_rep = rep.real ? 1 : 0;
-    c.G = G;
-    c.stream_seg = (nrep == 1) ? stream_seg : 0;
-    c.V3 = hreps[0].stride[D - 1];
-    c.Lt = unsigned(hreps[0].L[D - 1]);
-    return c;
-  }
-};
 
 namespace {
 
@@ -251,6 +181,22 @@ void launch_hamiltonian(ghx_ctx* c, int b, double* H) {
                      has_phi(c) ? 1 : 0, c->sums, H);
 }
 
+// A kick (+ optional drift) with part flags; surr_sign multiplies the
+// surrogate force S (filled here from the pre-kick fields) when non-zero.
+void launch_part_step(ghx_ctx* c, double ck, double cd, bool drift, int parts, double surr_sign) {
+  LaunchCtx l = c->lc();
+  const int in = c->cur, out = c->cur ^ 1;
+  int flags = F_KICK | (drift ? F_DRIFT : 0) | parts;
+  if (surr_sign != 0.0 && c->use_surr && c->surr_fill) {
+    c->surr_fill(c, in);
+    flags |= F_SURR;
+    l.surr_sign = surr_sign;
+  }
+  if (c->big) c->bops.matter(l, c->tb, c->U[in], c->phi[in], c->phi[out], c->pi, c->G, nullptr, nullptr, ck, cd, flags, BIG_STEP);
+  c->ops.md_step(l, c->U[in], c->U[out], c->P, c->phi[in], c->phi[out], c->pi, c->F, c->Fphi, ck, cd, flags);
+  if (drift) c->cur = out;
+}
+
 // One fused kick+drift (or kick-only when cd == 0 and !drift) on buffer cur.
 void launch_step(ghx_ctx* c, double ck, double cd, bool drift) {
   const LaunchCtx l = c->lc();
@@ -264,7 +210,26 @@ void launch_step(ghx_ctx* c, double ck, double cd, bool drift) {
 // md_evolve as a sequence of fused launches (gauge_higgs_hmc.hpp:149-165):
 // leapfrog  K(e/2)D(e) [K(e)D(e)]^{nmd-1} K(e/2)
 // Omelyan   K(le)D(e/2)K((1-2l)e)D(e/2)K(2le)D(e/2)...K((1-2l)e)D(e/2)K(le)
+// Multi-timescale leapfrog (Sexton-Weingarten, theory_notes 4.3), eps = tau/n_outer:
+//   [K_out(eps/2) [K_in(d/2) D(d) K_in(d/2)]^m K_out(eps/2)]^n_outer,  d = eps/m,
+// K_in: gauge force + on-site scalar force (+ surrogate S), K_out: the hopping
+// forces (matter link force and the D(U) phi hops of the scalar force, - S).
+// Adjacent inner half kicks are merged and fused with the drifts.
+void launch_md_nested(ghx_ctx* c) {
+  const int m = c->m_inner, n = c->n_outer;
+  const int in_parts = F_NO_HOP, out_parts = F_NO_GAUGE | F_NO_ONSITE;
+  const double h = 1.0 / m;  // inner step in units of eps
+  for (int i = 0; i < n; ++i) {
+    launch_part_step(c, i == 0 ? 0.5 : 1.0, 0.0, false, out_parts, -1.0);
+    launch_part_step(c, 0.5 * h, h, true, in_parts, +1.0);
+    for (int k = 1; k < m; ++k) launch_part_step(c, h, h, true, in_parts, +1.0);
+    launch_part_step(c, 0.5 * h, 0.0, false, in_parts, +1.0);
+  }
+  launch_part_step(c, 0.5, 0.0, false, out_parts, -1.0);
+}
+
 void launch_md(ghx_ctx* c) {
+  if (c->n_outer > 0) { launch_md_nested(c); return; }
   if (c->integ == GHX_LEAPFROG) {
     launch_step(c, 0.5, 1.0, true);
     for (int i = 1; i < c->nmd; ++i) launch_step(c, 1.0, 1.0, true);
@@ -389,6 +354,17 @@ void drop_graph(ghx_ctx* c) {
 
 }  // namespace
 
+namespace ghx {
+int ctx_fail(ghx_ctx* c, int code, const std::string& msg) { return fail(c, code, msg); }
+int ctx_check_ready(ghx_ctx* ctx) { return check_ready(ctx); }
+void ctx_launch_forces(ghx_ctx* c, int b, int parts) {
+  const LaunchCtx l = c->lc();
+  if (c->big)
+    c->bops.matter(l, c->tb, c->U[b], c->phi[b], nullptr, c->pi, c->G, c->Fphi, nullptr, 0.0, 0.0, F_ABS_EPS | parts, BIG_FORCE);
+  c->ops.md_step(l, c->U[b], c->U[b ^ 1], c->P, c->phi[b], c->phi[b ^ 1], c->pi, c->F, c->Fphi, 0.0, 0.0, F_FORCE | F_ABS_EPS | parts);
+}
+}  // namespace ghx
+
 extern "C" {
 
 int ghx_create(ghx_ctx** out, int D, int N, int n_replicas, const int* L, int precision) {
@@ -456,6 +432,7 @@ int ghx_destroy(ghx_ctx* ctx) {
   if (!ctx) return GHX_E_INVALID;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   drop_graph(ctx);
+  surrogate_free(ctx);
   free_fields(ctx);
   free_tables(ctx);
   for (void* p : {(void*)ctx->dreps, (void*)ctx->dblk_rep, (void*)ctx->dred_rep, (void*)ctx->dred_first, (void*)ctx->partials,
@@ -511,14 +488,24 @@ int ghx_rep_basis(const ghx_ctx* ctx, double* W) {
 int ghx_set_couplings(ghx_ctx* ctx, int replica, double beta, double kappa, double lambda, double tau) {
   if (!ctx || replica < 0 || replica >= ctx->nrep) return GHX_E_INVALID;
   RepDesc& rd = ctx->hreps[replica];
-  rd.beta = beta; rd.kappa = kappa; rd.lambda = lambda; rd.tau = tau; rd.eps = tau / ctx->nmd;
+  rd.beta = beta; rd.kappa = kappa; rd.lambda = lambda; rd.tau = tau;
+  rd.eps = tau / (ctx->n_outer > 0 ? ctx->n_outer : ctx->nmd);
   return upload_reps(ctx);
 }
 
 int ghx_set_integrator(ghx_ctx* ctx, int nmd, int integrator, int reunit_each_traj) {
   if (!ctx || nmd < 1 || (integrator != GHX_LEAPFROG && integrator != GHX_OMELYAN)) return GHX_E_INVALID;
   ctx->nmd = nmd; ctx->integ = integrator; ctx->reunit = reunit_each_traj != 0;
-  for (auto& rd : ctx->hreps) rd.eps = rd.tau / nmd;
+  for (auto& rd : ctx->hreps) rd.eps = rd.tau / (ctx->n_outer > 0 ? ctx->n_outer : nmd);
+  drop_graph(ctx);
+  return upload_reps(ctx);
+}
+
+int ghx_set_nested(ghx_ctx* ctx, int n_outer, int m_inner) {
+  if (!ctx || n_outer < 0 || m_inner < 1) return GHX_E_INVALID;
+  ctx->n_outer = n_outer;
+  ctx->m_inner = m_inner;
+  for (auto& rd : ctx->hreps) rd.eps = rd.tau / (n_outer > 0 ? n_outer : ctx->nmd);
   drop_graph(ctx);
   return upload_reps(ctx);
 }

@@ -16,6 +16,9 @@ namespace ghx {
 constexpr int kBlock = 128;     // threads per block of the per-site kernels
 constexpr int kRedBlock = 256;  // threads per block of the reduction kernels
 constexpr int kNQ = 6;          // per-site reduction quantities
+#ifndef GHX_NU_UNROLL
+#define GHX_NU_UNROLL 0  // unroll the staple loop (more loads in flight, more registers)
+#endif
 #ifndef GHX_MD_WAVES
 #define GHX_MD_WAVES 3  // minimum waves per SIMD requested for the MD-step kernel
 #endif
@@ -30,7 +33,11 @@ struct RepDesc {
   unsigned blk_begin, blk_end;  // reduction blocks of this replica
 };
 
-enum StepFlags : int { F_KICK = 1, F_DRIFT = 2, F_FORCE = 4, F_ABS_EPS = 8 };
+// F_NO_GAUGE / F_NO_HOP / F_NO_ONSITE drop force parts (multi-timescale
+// splitting); F_SURR adds surr_sign * S (a position-dependent surrogate link
+// force) to the link force.
+enum StepFlags : int { F_KICK = 1, F_DRIFT = 2, F_FORCE = 4, F_ABS_EPS = 8, F_NO_GAUGE = 16, F_NO_HOP = 32, F_NO_ONSITE = 64,
+                       F_SURR = 128 };
 
 template <typename T>
 struct StepArgs {
@@ -48,6 +55,8 @@ struct StepArgs {
   double ck, cd;  // kick / drift coefficient (times eps of the replica unless F_ABS_EPS)
   int flags;
   const T* G;     // external matter link force planes (REP_EXT)
+  const T* S;     // surrogate link force planes (F_SURR)
+  double surr_sign;
 };
 
 template <typename T, typename R, int N>
@@ -215,18 +224,19 @@ GHX_DEV void site_update(const StepArgs<T>& a, const typename REP::template Para
     R n2 = R(0);
 #pragma unroll
     for (int k = 0; k < DR; ++k) n2 += phx[k].r * phx[k].r + phx[k].i * phx[k].i;
-    const R c = R(-2) * (R(1) + R(2.0 * rd.lambda) * (n2 - R(1)));  // -2 dV/dphi^*
+    const R c = (a.flags & F_NO_ONSITE) ? R(0) : R(-2) * (R(1) + R(2.0 * rd.lambda) * (n2 - R(1)));  // -2 dV/dphi^*
 #pragma unroll
     for (int k = 0; k < DR; ++k) fph[k] = cscale(phx[k], c);
   }
+  const bool do_gauge = !(a.flags & F_NO_GAUGE), do_hop = !(a.flags & F_NO_HOP);
 
 #pragma unroll 1
   for (int mu = 0; mu < D; ++mu) {
     const Mat<R, N> Ux = bload_link<T, R, N>(Uin, mu, g);
     const unsigned gpm = g + nb.up[mu];
     // matter first (its temporaries die before the staple sum is built)
-    Mat<R, N> M;
-    if constexpr (DR > 0) {
+    Mat<R, N> M = mzero<R, N>();
+    if constexpr (DR > 0) if (do_hop) {
       const Planes<T> Ph(a.phi_in, Vp, 2 * DR);
       {
         cx<R> phy[DR], psi[DR];
@@ -246,8 +256,12 @@ GHX_DEV void site_update(const StepArgs<T>& a, const typename REP::template Para
       }
     }
     Mat<R, N> Sig = mzero<R, N>();
+#if GHX_NU_UNROLL
+#pragma unroll
+#else
 #pragma unroll 1
-    for (int nu = 0; nu < D; ++nu) {
+#endif
+    for (int nu = 0; nu < (do_gauge ? D : 0); ++nu) {
       if (nu == mu) continue;
       // forward staple U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag
       {
@@ -275,9 +289,17 @@ GHX_DEV void site_update(const StepArgs<T>& a, const typename REP::template Para
     R f[NGN];
     im_tr_gen<R, N>(X, f);
     if constexpr (REP::KIND == REP_EXT) {
-      const Planes<T> Gp(a.G, Vp, D * NGN);
+      if (do_hop) {
+        const Planes<T> Gp(a.G, Vp, D * NGN);
 #pragma unroll
-      for (int c = 0; c < NGN; ++c) f[c] += R(Gp.ld(g, unsigned(mu * NGN + c)));
+        for (int c = 0; c < NGN; ++c) f[c] += R(Gp.ld(g, unsigned(mu * NGN + c)));
+      }
+    }
+    if (a.flags & F_SURR) {
+      const Planes<T> Sp(a.S, Vp, D * NGN);
+      const R sg = R(a.surr_sign);
+#pragma unroll
+      for (int c = 0; c < NGN; ++c) f[c] = fma(sg, R(Sp.ld(g, unsigned(mu * NGN + c))), f[c]);
     }
     if (a.flags & F_FORCE) {
       const Planes<T> Fp(a.F_out, Vp, D * NGN);

@@ -27,7 +27,7 @@ struct Launch {
   static void md_step(const LaunchCtx& c, const void* U_in, void* U_out, void* P, const void* phi_in, void* phi_out, void* pi,
                       void* F, void* Fphi, double ck, double cd, int flags) {
     StepArgs<T> a{c.reps, c.blk_rep, c.Vp, (const T*)U_in, (T*)U_out, (T*)P, (const T*)phi_in, (T*)phi_out, (T*)pi, (T*)F, (T*)Fphi,
-                  ck, cd, flags, (const T*)c.G};
+                  ck, cd, flags, (const T*)c.G, (const T*)c.S, c.surr_sign};
     if (c.stream_seg > 0) {
       const unsigned V3 = c.V3, Lt = c.Lt;
       unsigned ntile = (V3 + kBlock - 1) / kBlock;
@@ -42,7 +42,7 @@ struct Launch {
   static void drift(const LaunchCtx& c, const void* U_in, void* U_out, const void* P, const void* phi_in, void* phi_out,
                     const void* pi, double cd, int flags) {
     StepArgs<T> a{c.reps, c.blk_rep, c.Vp, (const T*)U_in, (T*)U_out, (T*)P, (const T*)phi_in, (T*)phi_out, (T*)pi, nullptr, nullptr,
-                  0.0, cd, flags, nullptr};
+                  0.0, cd, flags, nullptr, nullptr, 0.0};
     hipLaunchKernelGGL((drift_kernel<T, N, D, REP::DR>), dim3(c.nblk), dim3(kBlock), 0, c.stream, a);
   }
   static void action(const LaunchCtx& c, const void* U, const void* P, const void* phi, const void* pi) {

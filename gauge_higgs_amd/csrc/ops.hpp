@@ -21,6 +21,8 @@ struct LaunchCtx {
   const void* G;    // external matter link force planes (large reps)
   int stream_seg;   // > 0: single-lattice streaming kernel with this many t-slices per block
   unsigned V3, Lt;  // cross-section volume and last extent of replica 0 (streaming kernel)
+  const void* S;    // surrogate link force planes
+  double surr_sign;
 };
 
 struct BigRepTables;

@@ -76,6 +76,10 @@ class Lattices:
         self.nmd, self.integ, self.reunit = nmd, integ, reunit
         self._check(self._lib.ghx_set_integrator(self.h, nmd, integ, int(reunit)))
 
+    def set_nested(self, n_outer, m_inner=1):
+        """Multi-timescale leapfrog: n_outer hopping-force steps, m_inner gauge/on-site steps each (0 = off)."""
+        self._check(self._lib.ghx_set_nested(self.h, n_outer, m_inner))
+
     def set_seed(self, replica, seed):
         self._check(self._lib.ghx_set_seed(self.h, replica, seed))
 
@@ -151,6 +155,45 @@ class Lattices:
         best, ms = C.c_int(), C.c_double()
         self._check(self._lib.ghx_autotune(self.h, steps, C.byref(best), C.byref(ms)))
         return best.value, ms.value
+
+    # --- learned surrogate of the matter link force ------------------------------
+    def surrogate_init(self, seed=7):
+        self._check(self._lib.ghx_surrogate_init(self.h, seed))
+        self.nparam = self._lib.ghx_surrogate_nparams()
+
+    def surrogate_enable(self, on=True):
+        self._check(self._lib.ghx_surrogate_enable(self.h, int(on)))
+
+    def surrogate_params(self, new=None):
+        buf = np.zeros(self.nparam, dtype=np.float32)
+        fp = C.POINTER(C.c_float)
+        if new is None:
+            self._check(self._lib.ghx_surrogate_params(self.h, 1, buf.ctypes.data_as(fp)))
+            return buf
+        buf[:] = new
+        self._check(self._lib.ghx_surrogate_params(self.h, 0, buf.ctypes.data_as(fp)))
+        return buf
+
+    def surrogate_gradient(self):
+        g = np.zeros(self.nparam, dtype=np.float32)
+        loss = C.c_double()
+        self._check(self._lib.ghx_surrogate_gradient(self.h, g.ctypes.data_as(C.POINTER(C.c_float)), C.byref(loss)))
+        return g, loss.value
+
+    def surrogate_train(self, iters=1, lr=1e-3):
+        l0, l1 = C.c_double(), C.c_double()
+        self._check(self._lib.ghx_surrogate_train(self.h, iters, lr, C.byref(l0), C.byref(l1)))
+        return l0.value, l1.value
+
+    def surrogate_snapshot(self, replica=0):
+        n = self.vols[replica] * self.D
+        X = np.zeros((n, 16), dtype=np.float32)
+        B = np.zeros((n, 4, self.ngen), dtype=np.float32)
+        S = np.zeros((n, self.ngen))
+        F = np.zeros((n, self.ngen))
+        fp = C.POINTER(C.c_float)
+        self._check(self._lib.ghx_surrogate_snapshot(self.h, replica, X.ctypes.data_as(fp), B.ctypes.data_as(fp), _dp(S), _dp(F)))
+        return X, B, S, F
 
     def replica_swap(self, i, j, u):
         acc, delta = C.c_int(), C.c_double()

@@ -58,6 +58,13 @@ int ghx_rep_basis(const ghx_ctx* ctx, double* W);
 int ghx_set_couplings(ghx_ctx* ctx, int replica, double beta, double kappa, double lambda, double tau);
 /* Integrator settings shared by all replicas (hmc.hpp:21-24). */
 int ghx_set_integrator(ghx_ctx* ctx, int nmd, int integrator, int reunit_each_traj);
+/* Multi-timescale leapfrog (the nested scheme of theory_notes 4.3; the
+ * reference implements it only for U(1), u1.hpp:227-320): n_outer outer steps
+ * of eps = tau/n_outer carrying the hopping forces (matter back-reaction and
+ * the D(U) phi hops of the scalar force), each with m_inner inner steps that
+ * carry the gauge force and the on-site scalar force. n_outer = 0 restores the
+ * single-scale integrator chosen by ghx_set_integrator. */
+int ghx_set_nested(ghx_ctx* ctx, int n_outer, int m_inner);
 /* RNG seed of one replica (Rng ctor, rng.hpp:27). */
 int ghx_set_seed(ghx_ctx* ctx, int replica, uint64_t seed);
 
@@ -133,6 +140,27 @@ enum { GHX_TUNE_STREAM_SEG = 0 };
 int ghx_set_tuning(ghx_ctx* ctx, int key, int value);
 int ghx_get_tuning(const ghx_ctx* ctx, int key, int* value);
 int ghx_autotune(ghx_ctx* ctx, int steps, int* best_seg, double* best_ms_per_step);
+
+/* Learned surrogate of the matter back-reaction on the links (no counterpart
+ * in the reference): an MLP 16 -> 64 -> 64 -> 4 on BF16 MFMA with FP32
+ * accumulation maps gauge-invariant link features to the coefficients of four
+ * gauge-covariant su(N) vectors. With the surrogate enabled, the nested
+ * integrator adds it to the inner (cheap) kicks and subtracts it in the outer
+ * kicks, so the total force is exact on average and the MD stays reversible
+ * and volume preserving; acceptance uses the exact FP64 H. */
+int ghx_surrogate_init(ghx_ctx* ctx, uint64_t seed);
+int ghx_surrogate_enable(ghx_ctx* ctx, int on);
+int ghx_surrogate_nparams(void);
+/* FP32 master parameters (get = 1 copies out, 0 copies in), ghx_surrogate_nparams() floats. */
+int ghx_surrogate_params(ghx_ctx* ctx, int get, float* params);
+/* Loss gradient on the current fields without updating (for checks). */
+int ghx_surrogate_gradient(ghx_ctx* ctx, float* grad, double* loss);
+/* `iters` Adam steps on the links of the current configuration (exact matter
+ * force as the target); loss before the first and after the last step. */
+int ghx_surrogate_train(ghx_ctx* ctx, int iters, double lr, double* loss_first, double* loss_last);
+/* Features (rows*16), bases (rows*4*(N*N-1)), surrogate S and exact matter
+ * force F (rows*(N*N-1)) of one replica, rows in the reference link order. */
+int ghx_surrogate_snapshot(ghx_ctx* ctx, int replica, float* X, float* B, double* S, double* F);
 
 int ghx_sync(ghx_ctx* ctx);
 /* Time `steps` fused MD steps (kick+drift launches) with HIP events on the
