@@ -8,8 +8,10 @@ timed region runs K steps back to back with the fields resident in HBM.
 
 Prints one JSON line: metric/value (site-updates/s), roofline of the MD-step
 kernel (algorithmic bytes per launch / HIP-event launch time against 8 TB/s),
-and the CPU baseline (the reference compiled from its sources, or the FP64
-oracle restatement, timed on this host on a 2-step sample of the same lattice).
+the CPU baseline (the reference compiled from its sources, or the FP64 oracle
+restatement, timed on this host on a 2-step sample of the same lattice), and
+"learned_force": GPU seconds per accepted trajectory of exact-force HMC against
+multi-timescale HMC with the learned surrogate on SU(3) + {6,1} (d = 48), L=12.
 """
 import argparse
 import json
@@ -81,6 +83,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
     ap.add_argument("--stream-seg", type=int, default=None, help="force a streaming segment (implies --no-tune)")
+    ap.add_argument("--no-learned", action="store_true", help="skip the exact vs learned-force HMC comparison")
     args = ap.parse_args()
     if args.stream_seg is not None:
         args.no_tune = True
@@ -122,7 +125,7 @@ def main():
     achieved = V * bpsite / launch_s / 1e9
     traffic = load_traffic(args.workload)
     line = {
-        "metric": "MD site-updates/s",
+        "metric": "MD site-updates/s; GPU-s per accepted traj, learned vs exact force (1 GPU)",
         "value": value,
         "unit": "site-updates/s",
         "n_gpus": 1,
@@ -140,6 +143,10 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "md_step_kernel", "algorithmic_bytes_per_site": bpsite},
     }
+    if not args.no_learned:
+        from gauge_higgs_amd import learned
+        g.lat.close()
+        line["learned_force"] = learned.compare()
     if not args.no_cpu:
         try:
             line["cpu_baseline"] = cpu_baseline(args.workload)

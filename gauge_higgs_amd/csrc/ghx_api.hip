@@ -210,22 +210,40 @@ void launch_step(ghx_ctx* c, double ck, double cd, bool drift) {
 // md_evolve as a sequence of fused launches (gauge_higgs_hmc.hpp:149-165):
 // leapfrog  K(e/2)D(e) [K(e)D(e)]^{nmd-1} K(e/2)
 // Omelyan   K(le)D(e/2)K((1-2l)e)D(e/2)K(2le)D(e/2)...K((1-2l)e)D(e/2)K(le)
-// Multi-timescale leapfrog (Sexton-Weingarten, theory_notes 4.3), eps = tau/n_outer:
+// Multi-timescale leapfrog (Sexton-Weingarten, theory_notes.md:129-134), eps = tau/n_outer:
 //   [K_out(eps/2) [K_in(d/2) D(d) K_in(d/2)]^m K_out(eps/2)]^n_outer,  d = eps/m,
 // K_in: gauge force + on-site scalar force (+ surrogate S), K_out: the hopping
 // forces (matter link force and the D(U) phi hops of the scalar force, - S).
-// Adjacent inner half kicks are merged and fused with the drifts.
+// Adjacent inner half kicks are merged and fused with the drifts. With the
+// Omelyan integrator selected the outer scale is the minimum-norm scheme
+//   [K_out(l eps) I(eps/2) K_out((1-2l) eps) I(eps/2) K_out(l eps)]^n_outer
+// (adjacent K_out(l eps) merged), I(t) = m inner leapfrog steps over time t.
+constexpr int kInParts = F_NO_HOP, kOutParts = F_NO_GAUGE | F_NO_ONSITE;
+void launch_inner(ghx_ctx* c, double t) {  // t in units of eps
+  const int m = c->m_inner;
+  const double h = t / m;
+  launch_part_step(c, 0.5 * h, h, true, kInParts, +1.0);
+  for (int k = 1; k < m; ++k) launch_part_step(c, h, h, true, kInParts, +1.0);
+  launch_part_step(c, 0.5 * h, 0.0, false, kInParts, +1.0);
+}
 void launch_md_nested(ghx_ctx* c) {
-  const int m = c->m_inner, n = c->n_outer;
-  const int in_parts = F_NO_HOP, out_parts = F_NO_GAUGE | F_NO_ONSITE;
-  const double h = 1.0 / m;  // inner step in units of eps
-  for (int i = 0; i < n; ++i) {
-    launch_part_step(c, i == 0 ? 0.5 : 1.0, 0.0, false, out_parts, -1.0);
-    launch_part_step(c, 0.5 * h, h, true, in_parts, +1.0);
-    for (int k = 1; k < m; ++k) launch_part_step(c, h, h, true, in_parts, +1.0);
-    launch_part_step(c, 0.5 * h, 0.0, false, in_parts, +1.0);
+  const int n = c->n_outer;
+  if (c->integ == GHX_OMELYAN) {
+    const double lam = kOmelyanLambda;
+    for (int i = 0; i < n; ++i) {
+      launch_part_step(c, i == 0 ? lam : 2.0 * lam, 0.0, false, kOutParts, -1.0);
+      launch_inner(c, 0.5);
+      launch_part_step(c, 1.0 - 2.0 * lam, 0.0, false, kOutParts, -1.0);
+      launch_inner(c, 0.5);
+    }
+    launch_part_step(c, lam, 0.0, false, kOutParts, -1.0);
+    return;
   }
-  launch_part_step(c, 0.5, 0.0, false, out_parts, -1.0);
+  for (int i = 0; i < n; ++i) {
+    launch_part_step(c, i == 0 ? 0.5 : 1.0, 0.0, false, kOutParts, -1.0);
+    launch_inner(c, 1.0);
+  }
+  launch_part_step(c, 0.5, 0.0, false, kOutParts, -1.0);
 }
 
 void launch_md(ghx_ctx* c) {

@@ -1,0 +1,143 @@
+"""Exact-force HMC against multi-timescale HMC with the learned surrogate force.
+
+Workload: 4D SU(3) + complex scalar in the Young-rows {6,1} irrep (d = 48),
+beta = 5.7, kappa = 0.05, lambda = 0.5, tau = 1, FP32 MD with the FP64
+Hamiltonian and Metropolis test. The exact chain is the reference's HMC
+(gauge_higgs_hmc.hpp:149-204, Omelyan, nmd = 16). The learned chain uses the
+Sexton-Weingarten split (theory_notes.md:129-134, u1.hpp:278-320 for U(1)):
+the expensive hopping forces (matter link force and the D(U) phi hops of the
+scalar force, 12 tensor applies per site in the reference formulation) on
+n_outer coarse steps, the gauge force, the on-site potential force and the
+MLP surrogate S of the matter link force on m_inner fine steps each, with S
+subtracted again on the coarse scale. Whatever S is, the integrated force is
+the exact one, so the chain samples exactly (reversible, area preserving,
+Metropolis on the exact FP64 H); S only moves error from the coarse scale to
+the fine one.
+
+The surrogate is trained on (configuration, exact matter link force) pairs
+drawn from the exact chain, by the MFMA forward/backward kernels on the GPU.
+Costs are GPU seconds per accepted trajectory (wall time of the trajectory
+launches with the device synchronised; all work runs on one stream).
+"""
+import time
+
+import numpy as np
+
+from . import hmc as _h
+
+
+def _chain(g, n, record=True):
+    dhs, acc, t = [], 0, 0.0
+    obs = []
+    for _ in range(n):
+        a0 = g.accept_count
+        g.lat.sync()
+        t0 = time.perf_counter()
+        g.lat.trajectories(1, use_graph=True)
+        g.lat.sync()
+        t += time.perf_counter() - t0
+        acc += g.accept_count - a0
+        dhs.append(g.last_dH)
+        if record:
+            o = g.observables()
+            obs.append((o["plaq"], o["Llink"]))
+    dhs = np.array(dhs)
+    out = {"trajectories": n, "gpu_s_per_traj": t / n, "acceptance": acc / n,
+           "exp_minus_dH": float(np.mean(np.exp(-np.clip(dhs, -50, 50)))),
+           "exp_minus_dH_err": float(np.std(np.exp(-np.clip(dhs, -50, 50))) / np.sqrt(n)),
+           "mean_abs_dH": float(np.mean(np.abs(dhs)))}
+    if record:
+        ob = np.array(obs)
+        out.update(plaq=float(ob[:, 0].mean()), plaq_err=float(ob[:, 0].std() / np.sqrt(n)),
+                   Llink=float(ob[:, 1].mean()), Llink_err=float(ob[:, 1].std() / np.sqrt(n)))
+    out["gpu_s_per_accepted"] = out["gpu_s_per_traj"] / max(out["acceptance"], 1e-9)
+    return out
+
+
+RESIDUAL_GATE = 0.9  # use S only if it explains > 10 % of |F|^2 on a held-out configuration
+
+
+def compare(L=12, rep="6,1", beta=5.7, kappa=0.05, lam=0.5, nmd=16, exact_nmds=(16, 48), n_therm=40, n_meas=24,
+            splits=((8, 6),), train_configs=8, train_iters=200, lr=2e-3, seed=1, precision=_h.FP32MD,
+            therm_split=(12, 6), warm=(0.4, 0.27)):
+    """Cost per accepted trajectory of exact-force HMC and of the multi-timescale
+    chain with and without the learned surrogate, from one thermalised
+    configuration.
+
+    Start: links exp(i sigma g.T) with sigma = warm[0] and Gaussian phi with
+    sigma = warm[1] (|phi|^2 near its equilibrium value), thermalised by the
+    split integrator (any exact chain samples the same distribution; the stiff
+    on-site potential of a d = 48 scalar stalls coarse single-scale chains far
+    from equilibrium). The exact chains run Omelyan at each nmd in exact_nmds
+    (the reference setting is nmd = 16); the surrogate is trained on
+    (configuration, exact matter link force) pairs from the thermalised chain;
+    each split (n_outer, m_inner) runs with the Omelyan outer scale, with the
+    surrogate forced on ("surrogate_on") and without it ("split_no_surrogate").
+
+    The learned chain keeps the exact force as fallback and judge: it uses the
+    surrogate only when its held-out residual |S - F|^2 / |F|^2 is below
+    RESIDUAL_GATE, otherwise the coarse scale carries the exact hopping force
+    alone; "learned" reports whichever of the two measured chains that is."""
+    g = _h.GaugeHiggsHMC(L, rep, seed=seed, D=4, N=3, precision=precision)
+    g.configure(beta=beta, kappa=kappa, lam=lam, nmd=nmd, integ=_h.OMELYAN)
+    g.hot(warm[0], 12345, warm[1])
+    t0 = time.perf_counter()
+    g.lat.set_nested(*therm_split)
+    therm = _chain(g, n_therm, record=True)
+    therm_s = time.perf_counter() - t0
+
+    # training pairs: (configuration, exact matter link force) along the chain
+    g.lat.surrogate_init(seed=7)
+    losses = []
+    t0 = time.perf_counter()
+    for _ in range(train_configs):
+        g.lat.trajectories(1, use_graph=True)
+        l0, l1 = g.lat.surrogate_train(iters=train_iters, lr=lr)
+        losses.append((l0, l1))
+    g.lat.sync()
+    train_s = time.perf_counter() - t0
+    g.lat.trajectories(1, use_graph=True)
+    # held-out residual: fraction of |F|^2 the surrogate leaves unexplained on a new configuration
+    _, _, S, F = g.lat.surrogate_snapshot()
+    resid = float(np.sum((S - F) ** 2) / np.sum(F ** 2))
+    U0, phi0 = g.get("U"), g.get("phi")
+
+    def restart():
+        g.set("U", U0)
+        g.set("phi", phi0)
+
+    out = {"workload": f"su3_{rep.replace(',', '')}_L{L}", "L": L, "sites": L ** 4, "beta": beta, "kappa": kappa, "lambda": lam,
+           "start": f"hot sigma_U={warm[0]}, Gaussian phi sigma={warm[1]}; {n_therm} thermalisation trajectories "
+                    f"(split {therm_split[0]}x{therm_split[1]}, acceptance {therm['acceptance']:.2f}, plaq {therm['plaq']:.4f}, {therm_s:.1f} s)",
+           "exact": {}, "surrogate_on": {}, "split_no_surrogate": {}}
+    g.lat.set_nested(0, 1)
+    g.lat.surrogate_enable(False)
+    for n in exact_nmds:
+        restart()
+        g.configure(nmd=n)
+        out["exact"][f"omelyan_nmd{n}"] = _chain(g, n_meas)
+    g.configure(nmd=nmd)
+    for (no, mi) in splits:
+        key = f"omelyan_outer{no}_inner{mi}"
+        for name, surr in (("surrogate_on", True), ("split_no_surrogate", False)):
+            restart()
+            g.lat.set_nested(no, mi)
+            g.lat.surrogate_enable(surr)
+            out[name][key] = _chain(g, n_meas)
+    g.lat.surrogate_enable(False)
+    g.lat.set_nested(0, 1)
+    g.lat.close()
+    out["training"] = {"configs": train_configs, "pairs": train_configs * 4 * L ** 4, "adam_steps": train_configs * train_iters,
+                       "seconds": train_s, "loss_first": losses[0][0], "loss_last": losses[-1][1],
+                       "heldout_residual_fraction": resid}
+    use_s = resid < RESIDUAL_GATE
+    out["learned"] = {"surrogate_used": use_s, "gate": f"held-out residual {resid:.3f} vs {RESIDUAL_GATE}",
+                      "chains": "surrogate_on" if use_s else "split_no_surrogate"}
+    learned = out["surrogate_on" if use_s else "split_no_surrogate"]
+    cost = lambda d: min(v["gpu_s_per_accepted"] for v in d.values())
+    ref = out["exact"].get(f"omelyan_nmd{nmd}", {}).get("gpu_s_per_accepted")
+    out["learned"]["gpu_s_per_accepted"] = cost(learned)
+    out["speedup_vs_reference_nmd"] = ref / cost(learned) if ref else None
+    out["speedup_vs_best_exact"] = cost(out["exact"]) / cost(learned)
+    out["surrogate_gain"] = cost(out["split_no_surrogate"]) / cost(out["surrogate_on"])
+    return out

@@ -58,12 +58,13 @@ int ghx_rep_basis(const ghx_ctx* ctx, double* W);
 int ghx_set_couplings(ghx_ctx* ctx, int replica, double beta, double kappa, double lambda, double tau);
 /* Integrator settings shared by all replicas (hmc.hpp:21-24). */
 int ghx_set_integrator(ghx_ctx* ctx, int nmd, int integrator, int reunit_each_traj);
-/* Multi-timescale leapfrog (the nested scheme of theory_notes 4.3; the
- * reference implements it only for U(1), u1.hpp:227-320): n_outer outer steps
- * of eps = tau/n_outer carrying the hopping forces (matter back-reaction and
- * the D(U) phi hops of the scalar force), each with m_inner inner steps that
- * carry the gauge force and the on-site scalar force. n_outer = 0 restores the
- * single-scale integrator chosen by ghx_set_integrator. */
+/* Multi-timescale integrator (the nested scheme of theory_notes.md:129-134;
+ * the reference implements it only for U(1), u1.hpp:227-320): n_outer outer
+ * steps of eps = tau/n_outer carrying the hopping forces (matter back-reaction
+ * and the D(U) phi hops of the scalar force), leapfrog or Omelyan as chosen by
+ * ghx_set_integrator, each inner half/whole step made of m_inner leapfrog
+ * steps that carry the gauge force and the on-site scalar force. n_outer = 0
+ * restores the single-scale integrator. */
 int ghx_set_nested(ghx_ctx* ctx, int n_outer, int m_inner);
 /* RNG seed of one replica (Rng ctor, rng.hpp:27). */
 int ghx_set_seed(ghx_ctx* ctx, int replica, uint64_t seed);
