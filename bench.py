@@ -40,15 +40,30 @@ def algorithmic_bytes_per_site(D, N, d, w):
     return 2 * (D * 2 * N * N * w + D * (N * N - 1) * w + 2 * (2 * d) * w)
 
 
-def load_traffic(workload):
+def load_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed PMC summary (FETCH_SIZE
+    and WRITE_SIZE per dispatch, scaled by the copy-kernel calibration)."""
     path = os.path.join(ROOT, "profiles", f"md_step_{workload}_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("hbm_bytes_per_launch")
+        return json.load(f).get(kernel, {}).get("hbm_bytes_per_launch")
 
 
-def cpu_baseline(workload, sample_steps=2):
+def copy_calibration(mib):
+    """A plain device copy of known size, for scaling the HBM byte counters."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 3}))
+    return 0
+
+
+def cpu_baseline(workload, sample_steps=16):
     """Reference (compiled from its own sources by oracle/Makefile) on this host."""
     D, N, rep, L, *_ = WORKLOADS[workload]
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
@@ -84,7 +99,18 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
     ap.add_argument("--stream-seg", type=int, default=None, help="force a streaming segment (implies --no-tune)")
     ap.add_argument("--no-learned", action="store_true", help="skip the exact vs learned-force HMC comparison")
+    ap.add_argument("--only-learned", action="store_true", help="run only the exact vs learned-force comparison")
+    ap.add_argument("--learned-meas", type=int, default=24, help="measured trajectories per chain in the comparison")
+    ap.add_argument("--no-graph", action="store_true", help="launch trajectories directly instead of replaying hipGraphs")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB (calibrates the HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
+    if args.only_learned:
+        from gauge_higgs_amd import learned
+        print(json.dumps({"learned_force": learned.compare(n_meas=args.learned_meas, use_graph=not args.no_graph)}))
+        return 0
     if args.stream_seg is not None:
         args.no_tune = True
     if args.gpus != 1:
@@ -123,7 +149,8 @@ def main():
     bpsite = algorithmic_bytes_per_site(D, N, d, w)
     launch_s = sec / args.steps
     achieved = V * bpsite / launch_s / 1e9
-    traffic = load_traffic(args.workload)
+    kernel = "md_stream_kernel" if tuned.get("stream_seg") else "md_step_kernel"
+    traffic = load_traffic(args.workload, kernel)
     line = {
         "metric": "MD site-updates/s; GPU-s per accepted traj, learned vs exact force (1 GPU)",
         "value": value,
@@ -141,12 +168,12 @@ def main():
                    "kappa": kappa, "lambda": lam, "eps": 0.05, "wall_s": wall, "launch": tuned},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "md_step_kernel", "algorithmic_bytes_per_site": bpsite},
+                     "kernel": kernel, "algorithmic_bytes_per_site": bpsite, "algorithmic_bytes_per_launch": V * bpsite},
     }
     if not args.no_learned:
         from gauge_higgs_amd import learned
         g.lat.close()
-        line["learned_force"] = learned.compare()
+        line["learned_force"] = learned.compare(n_meas=args.learned_meas, use_graph=not args.no_graph)
     if not args.no_cpu:
         try:
             line["cpu_baseline"] = cpu_baseline(args.workload)

@@ -26,14 +26,14 @@ import numpy as np
 from . import hmc as _h
 
 
-def _chain(g, n, record=True):
+def _chain(g, n, record=True, use_graph=True):
     dhs, acc, t = [], 0, 0.0
     obs = []
     for _ in range(n):
         a0 = g.accept_count
         g.lat.sync()
         t0 = time.perf_counter()
-        g.lat.trajectories(1, use_graph=True)
+        g.lat.trajectories(1, use_graph=use_graph)
         g.lat.sync()
         t += time.perf_counter() - t0
         acc += g.accept_count - a0
@@ -59,7 +59,7 @@ RESIDUAL_GATE = 0.9  # use S only if it explains > 10 % of |F|^2 on a held-out c
 
 def compare(L=12, rep="6,1", beta=5.7, kappa=0.05, lam=0.5, nmd=16, exact_nmds=(16, 48), n_therm=40, n_meas=24,
             splits=((8, 6),), train_configs=8, train_iters=200, lr=2e-3, seed=1, precision=_h.FP32MD,
-            therm_split=(12, 6), warm=(0.4, 0.27)):
+            therm_split=(12, 6), warm=(0.4, 0.27), use_graph=True):
     """Cost per accepted trajectory of exact-force HMC and of the multi-timescale
     chain with and without the learned surrogate, from one thermalised
     configuration.
@@ -83,7 +83,7 @@ def compare(L=12, rep="6,1", beta=5.7, kappa=0.05, lam=0.5, nmd=16, exact_nmds=(
     g.hot(warm[0], 12345, warm[1])
     t0 = time.perf_counter()
     g.lat.set_nested(*therm_split)
-    therm = _chain(g, n_therm, record=True)
+    therm = _chain(g, n_therm, record=True, use_graph=use_graph)
     therm_s = time.perf_counter() - t0
 
     # training pairs: (configuration, exact matter link force) along the chain
@@ -91,12 +91,12 @@ def compare(L=12, rep="6,1", beta=5.7, kappa=0.05, lam=0.5, nmd=16, exact_nmds=(
     losses = []
     t0 = time.perf_counter()
     for _ in range(train_configs):
-        g.lat.trajectories(1, use_graph=True)
+        g.lat.trajectories(1, use_graph=use_graph)
         l0, l1 = g.lat.surrogate_train(iters=train_iters, lr=lr)
         losses.append((l0, l1))
     g.lat.sync()
     train_s = time.perf_counter() - t0
-    g.lat.trajectories(1, use_graph=True)
+    g.lat.trajectories(1, use_graph=use_graph)
     # held-out residual: fraction of |F|^2 the surrogate leaves unexplained on a new configuration
     _, _, S, F = g.lat.surrogate_snapshot()
     resid = float(np.sum((S - F) ** 2) / np.sum(F ** 2))
@@ -115,7 +115,7 @@ def compare(L=12, rep="6,1", beta=5.7, kappa=0.05, lam=0.5, nmd=16, exact_nmds=(
     for n in exact_nmds:
         restart()
         g.configure(nmd=n)
-        out["exact"][f"omelyan_nmd{n}"] = _chain(g, n_meas)
+        out["exact"][f"omelyan_nmd{n}"] = _chain(g, n_meas, use_graph=use_graph)
     g.configure(nmd=nmd)
     for (no, mi) in splits:
         key = f"omelyan_outer{no}_inner{mi}"
@@ -123,7 +123,7 @@ def compare(L=12, rep="6,1", beta=5.7, kappa=0.05, lam=0.5, nmd=16, exact_nmds=(
             restart()
             g.lat.set_nested(no, mi)
             g.lat.surrogate_enable(surr)
-            out[name][key] = _chain(g, n_meas)
+            out[name][key] = _chain(g, n_meas, use_graph=use_graph)
     g.lat.surrogate_enable(False)
     g.lat.set_nested(0, 1)
     g.lat.close()

@@ -3,9 +3,11 @@
 // and bench.py's cpu_baseline leg may load it; the product library never does.
 //
 // The arithmetic of every routine below follows the reference expression for
-// expression (std::complex<double>, same loop orders, same zero skips), so that
-// when both are compiled by g++ with the reference's flags the results agree
-// bit for bit. Each routine cites the reference file:line it restates.
+// expression (std::complex<double>, same loop orders, same zero skips). Integer
+// RNG streams, uniforms, Gaussians and start fields then agree bit for bit with
+// the reference compiled by the same g++; forces, actions and MD fields agree to
+// a few ulps (g++ contracts FMAs per call site). Each routine cites the
+// reference file:line it restates.
 //
 // Parity status per function (how each is pinned) is listed in oracle/README.md.
 #pragma once
