@@ -40,6 +40,13 @@ def algorithmic_bytes_per_site(D, N, d, w):
     return 2 * (D * 2 * N * N * w + D * (N * N - 1) * w + 2 * (2 * d) * w)
 
 
+def stored_bytes_per_site(D, N, d, w):
+    """What this implementation moves per site-update if nothing were re-read:
+    in FP32 the links keep N-1 rows (the last is rebuilt on load)."""
+    nr = N - 1 if w == 4 else N
+    return 2 * (D * 2 * N * nr * w + D * (N * N - 1) * w + 2 * (2 * d) * w)
+
+
 def load_traffic(workload, kernel):
     """HBM bytes per launch of `kernel` from the committed PMC summary (FETCH_SIZE
     and WRITE_SIZE per dispatch, scaled by the copy-kernel calibration)."""
@@ -168,7 +175,8 @@ def main():
                    "kappa": kappa, "lambda": lam, "eps": 0.05, "wall_s": wall, "launch": tuned},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": kernel, "algorithmic_bytes_per_site": bpsite, "algorithmic_bytes_per_launch": V * bpsite},
+                     "kernel": kernel, "algorithmic_bytes_per_site": bpsite, "algorithmic_bytes_per_launch": V * bpsite,
+                     "stored_state_bytes_per_site": stored_bytes_per_site(D, N, d, w)},
     }
     if not args.no_learned:
         from gauge_higgs_amd import learned

@@ -71,7 +71,9 @@ struct ghx_ctx {
   SwapState* dswap = nullptr;
   unsigned maxvol = 0;
 
-  size_t nU() const { return size_t(D) * 2 * N * N; }
+  // link planes per site: FP32 MD stores rows 0..N-2 (the last row is rebuilt on load)
+  int link_rows() const { return tsize == 4 ? N - 1 : N; }
+  size_t nU() const { return size_t(D) * 2 * N * link_rows(); }
   size_t nP() const { return size_t(D) * (N * N - 1); }
   size_t nphi() const { return size_t(2) * rep.d; }
   LaunchCtx lc() const {

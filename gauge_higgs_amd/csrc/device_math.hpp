@@ -288,7 +288,7 @@ template <typename R, int N, int ORDER> GHX_DEV Mat<R, N> expi_taylor(const Mat<
   const R nrm = sqrt(fnorm2(H));
   int s = 0;
   R scaled = nrm, inv2s = R(1);
-  while (scaled > R(0.5)) { scaled *= R(0.5); inv2s *= R(0.5); ++s; }
+  while (scaled > R(0.5) && s < 64) { scaled *= R(0.5); inv2s *= R(0.5); ++s; }  // bounded: non-finite input cannot spin
   Mat<R, N> A;
 #pragma unroll
   for (int i = 0; i < N; ++i)
@@ -320,7 +320,7 @@ template <typename R, int N, int ORDER> GHX_DEV Mat<R, N> expi_taylor(const Mat<
 template <typename R> GHX_DEV Mat<R, 3> expi_su3_ch(const Mat<R, 3>& H) {
   R n2 = fnorm2(H), sc = R(1);
   int s = 0;
-  while (n2 * sc * sc > R(1)) { sc *= R(0.5); ++s; }
+  while (n2 * sc * sc > R(1) && s < 64) { sc *= R(0.5); ++s; }  // bounded: non-finite input cannot spin
   Mat<R, 3> Q;
 #pragma unroll
   for (int i = 0; i < 3; ++i)

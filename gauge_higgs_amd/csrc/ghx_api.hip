@@ -366,6 +366,43 @@ int set_planes(ghx_ctx* ctx, void* dev, int nplanes, int replica, const double* 
   return GHX_OK;
 }
 
+// Links in the reference AoS layout (N x N complex per link) from / to the
+// stored rows; in FP32 MD the last row is rebuilt exactly as on the device.
+void rebuild_last_row_host(int N, std::complex<double>* m) {
+  if (N == 2) {
+    m[2] = -std::conj(m[1]);
+    m[3] = std::conj(m[0]);
+  } else {
+    for (int k = 0; k < 3; ++k) {
+      const int i = (k + 1) % 3, j = (k + 2) % 3;
+      m[6 + k] = std::conj(m[i] * m[3 + j] - m[j] * m[3 + i]);
+    }
+  }
+}
+int get_links(ghx_ctx* ctx, int replica, double* out) {
+  const int nr = ctx->link_rows(), N = ctx->N;
+  if (nr == N) return get_planes(ctx, ctx->U[ctx->cur], int(ctx->nU()), replica, out);
+  const size_t nl = size_t(ctx->hreps[replica].vol) * ctx->D, per = size_t(2) * N * nr;
+  std::vector<double> c(nl * per);
+  int rc = get_planes(ctx, ctx->U[ctx->cur], int(ctx->nU()), replica, c.data());
+  if (rc) return rc;
+  for (size_t l = 0; l < nl; ++l) {
+    std::complex<double>* m = reinterpret_cast<std::complex<double>*>(out + l * 2 * N * N);
+    for (size_t q = 0; q < per / 2; ++q) m[q] = {c[l * per + 2 * q], c[l * per + 2 * q + 1]};
+    rebuild_last_row_host(N, m);
+  }
+  return GHX_OK;
+}
+int set_links(ghx_ctx* ctx, int replica, const double* in) {
+  const int nr = ctx->link_rows(), N = ctx->N;
+  if (nr == N) return set_planes(ctx, ctx->U[ctx->cur], int(ctx->nU()), replica, in);
+  const size_t nl = size_t(ctx->hreps[replica].vol) * ctx->D, per = size_t(2) * N * nr;
+  std::vector<double> c(nl * per);
+  for (size_t l = 0; l < nl; ++l)
+    for (size_t q = 0; q < per; ++q) c[l * per + q] = in[l * 2 * N * N + q];
+  return set_planes(ctx, ctx->U[ctx->cur], int(ctx->nU()), replica, c.data());
+}
+
 void drop_graph(ghx_ctx* c) {
   if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
 }
@@ -728,7 +765,7 @@ int ghx_get_fields(ghx_ctx* ctx, int replica, double* U, double* P, double* phi,
   int rc = check_ready(ctx);
   if (rc) return rc;
   if (replica < 0 || replica >= ctx->nrep) return GHX_E_INVALID;
-  if (U && (rc = get_planes(ctx, ctx->U[ctx->cur], int(ctx->nU()), replica, U))) return rc;
+  if (U && (rc = get_links(ctx, replica, U))) return rc;
   if (P && (rc = get_planes(ctx, ctx->P, int(ctx->nP()), replica, P))) return rc;
   if (has_phi(ctx)) {
     if (phi && (rc = get_planes(ctx, ctx->phi[ctx->cur], int(ctx->nphi()), replica, phi))) return rc;
@@ -741,7 +778,7 @@ int ghx_set_fields(ghx_ctx* ctx, int replica, const double* U, const double* P, 
   int rc = check_ready(ctx);
   if (rc) return rc;
   if (replica < 0 || replica >= ctx->nrep) return GHX_E_INVALID;
-  if (U && (rc = set_planes(ctx, ctx->U[ctx->cur], int(ctx->nU()), replica, U))) return rc;
+  if (U && (rc = set_links(ctx, replica, U))) return rc;
   if (P && (rc = set_planes(ctx, ctx->P, int(ctx->nP()), replica, P))) return rc;
   if (has_phi(ctx)) {
     if (phi && (rc = set_planes(ctx, ctx->phi[ctx->cur], int(ctx->nphi()), replica, phi))) return rc;

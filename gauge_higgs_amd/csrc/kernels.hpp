@@ -3,7 +3,8 @@
 //
 // HBM layout (structure of arrays, one plane per real component, each plane
 // holds every site of every replica, stride Vp):
-//   U   plane (mu*2N^2 + 2(iN+j) + re/im)      links, row-major N x N
+//   U   plane (mu*2N*NR + 2(iN+j) + re/im)    links, row-major, rows i < NR
+//                                              (NR = N in FP64, N-1 in FP32)
 //   P   plane (mu*(N^2-1) + a)                  link momenta (generator order)
 //   phi plane (2k + re/im)                      scalar field, W-basis components
 // Replicas are laid back to back; each starts on a block boundary so a block
@@ -59,24 +60,53 @@ struct StepArgs {
   double surr_sign;
 };
 
+// Link storage. FP64 (parity mode) stores all N rows. FP32 MD stores the first
+// N-1 rows only and rebuilds the last one on load, which is exact for SU(N):
+//   SU(2): row 1 = (-conj(U01), conj(U00));  SU(3): row 2 = conj(row 0 x row 1).
+// This cuts link traffic by 1/3 (SU(3)) or 1/2 (SU(2)) in the bandwidth-bound
+// MD step; the reconstructed matrix is unitary to the rounding of rows 0..N-2.
+template <typename T, int N>
+constexpr int link_rows() { return sizeof(T) == 4 ? N - 1 : N; }
+template <typename T, int N>
+constexpr int link_reals() { return 2 * N * link_rows<T, N>(); }
+
+template <typename R, int N>
+GHX_DEV void rebuild_last_row(Mat<R, N>& m) {
+  if constexpr (N == 2) {
+    m.e[1][0] = {-m.e[0][1].r, m.e[0][1].i};
+    m.e[1][1] = {m.e[0][0].r, -m.e[0][0].i};
+  } else {
+    static_assert(N == 3, "compressed links are defined for SU(2) and SU(3)");
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int i = (k + 1) % 3, j = (k + 2) % 3;
+      const cx<R> z = csub(cmul(m.e[0][i], m.e[1][j]), cmul(m.e[0][j], m.e[1][i]));
+      m.e[2][k] = cconj(z);
+    }
+  }
+}
+
 template <typename T, typename R, int N>
 GHX_DEV Mat<R, N> load_link(const T* __restrict__ U, unsigned Vp, int mu, unsigned g) {
+  constexpr int NR = link_rows<T, N>();
   Mat<R, N> m;
-  const T* p = U + size_t(mu) * (2 * N * N) * Vp + g;
+  const T* p = U + size_t(mu) * link_reals<T, N>() * Vp + g;
 #pragma unroll
-  for (int i = 0; i < N; ++i)
+  for (int i = 0; i < NR; ++i)
 #pragma unroll
     for (int j = 0; j < N; ++j) {
       m.e[i][j].r = R(p[size_t(2 * (i * N + j)) * Vp]);
       m.e[i][j].i = R(p[size_t(2 * (i * N + j) + 1) * Vp]);
     }
+  if constexpr (NR < N) rebuild_last_row<R, N>(m);
   return m;
 }
 template <typename T, typename R, int N>
 GHX_DEV void store_link(T* __restrict__ U, unsigned Vp, int mu, unsigned g, const Mat<R, N>& m) {
-  T* p = U + size_t(mu) * (2 * N * N) * Vp + g;
+  constexpr int NR = link_rows<T, N>();
+  T* p = U + size_t(mu) * link_reals<T, N>() * Vp + g;
 #pragma unroll
-  for (int i = 0; i < N; ++i)
+  for (int i = 0; i < NR; ++i)
 #pragma unroll
     for (int j = 0; j < N; ++j) {
       p[size_t(2 * (i * N + j)) * Vp] = T(m.e[i][j].r);
@@ -109,12 +139,16 @@ template <> GHX_DEV float bload<float>(rsrc_t r, unsigned voff, unsigned soff) {
 template <> GHX_DEV double bload<double>(rsrc_t r, unsigned voff, unsigned soff) {
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
 }
+// Stores of the MD step are streamed (non-temporal, cache policy bit NT = 2):
+// nothing written in a step is read again in that step, so they should not
+// evict the neighbour links that the staples re-read through L2.
+constexpr int kStoreNT = 2;
 GHX_DEV void bstore(float v, rsrc_t r, unsigned voff, unsigned soff) {
-  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff, 0);
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff, kStoreNT);
 }
 GHX_DEV void bstore(double v, rsrc_t r, unsigned voff, unsigned soff) {
   using v2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
-  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2, v), r, (int)voff, (int)soff, 0);
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2, v), r, (int)voff, (int)soff, kStoreNT);
 }
 // Planes of one field: plane c of site g at byte offset (c*Vp + g)*sizeof(T).
 template <typename T>
@@ -127,24 +161,27 @@ struct Planes {
 };
 template <typename T, typename R, int N>
 GHX_DEV Mat<R, N> bload_link(const Planes<T>& U, int mu, unsigned g) {
+  constexpr int NR = link_rows<T, N>(), LR = link_reals<T, N>();
   Mat<R, N> m;
 #pragma unroll
-  for (int i = 0; i < N; ++i)
+  for (int i = 0; i < NR; ++i)
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-      m.e[i][j].r = R(U.ld(g, unsigned(mu * 2 * N * N + 2 * (i * N + j))));
-      m.e[i][j].i = R(U.ld(g, unsigned(mu * 2 * N * N + 2 * (i * N + j) + 1)));
+      m.e[i][j].r = R(U.ld(g, unsigned(mu * LR + 2 * (i * N + j))));
+      m.e[i][j].i = R(U.ld(g, unsigned(mu * LR + 2 * (i * N + j) + 1)));
     }
+  if constexpr (NR < N) rebuild_last_row<R, N>(m);
   return m;
 }
 template <typename T, typename R, int N>
 GHX_DEV void bstore_link(const Planes<T>& U, int mu, unsigned g, const Mat<R, N>& m) {
+  constexpr int NR = link_rows<T, N>(), LR = link_reals<T, N>();
 #pragma unroll
-  for (int i = 0; i < N; ++i)
+  for (int i = 0; i < NR; ++i)
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-      U.st(g, unsigned(mu * 2 * N * N + 2 * (i * N + j)), T(m.e[i][j].r));
-      U.st(g, unsigned(mu * 2 * N * N + 2 * (i * N + j) + 1), T(m.e[i][j].i));
+      U.st(g, unsigned(mu * LR + 2 * (i * N + j)), T(m.e[i][j].r));
+      U.st(g, unsigned(mu * LR + 2 * (i * N + j) + 1), T(m.e[i][j].i));
     }
 }
 template <typename T, typename R, int DR>
@@ -203,7 +240,7 @@ struct Nbr {
 template <typename T, int N, int D, class REP>
 GHX_DEV void site_update(const StepArgs<T>& a, const typename REP::template Params<T>& rp, const RepDesc& rd, unsigned s) {
   using R = T;
-  constexpr int NC = 2 * N * N;
+  constexpr int NC = link_reals<T, N>();
   constexpr int NGN = N * N - 1;
   constexpr int DR = REP::DR;
   const unsigned g = rd.offset + s;

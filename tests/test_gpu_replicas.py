@@ -108,8 +108,11 @@ def test_kappa_scan_32_chains_runs_in_one_launch_per_step():
 
 def test_tempering_preserves_each_rung_marginal():
     """Canonical gate (test_replica_tempering_canonical.cpp): with swaps on, each
-    kappa rung samples the same L_link distribution as an independent chain."""
-    kap = [0.24, 0.26]
+    kappa rung samples the same L_link distribution as an independent chain.
+    Both rungs sit below the 4^4 transition (kappa ~ 0.26), where L_link is
+    unimodal and the 20-block error estimate is reliable; at the transition the
+    two-phase tunnelling time exceeds the block length."""
+    kap = [0.21, 0.23]
     res = {}
     for enabled in (False, True):
         rs = gha.ReplicaSet(4, 2, [[4, 4, 4, 4]] * 2, "fund", [2.3] * 2, kap, [0.5] * 2, seed0=77, nmd=10,
