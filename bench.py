@@ -10,6 +10,8 @@ Prints one JSON line: metric/value (site-updates/s), roofline of the MD-step
 kernel (algorithmic bytes per launch / HIP-event launch time against 8 TB/s),
 the CPU baseline (the reference compiled from its sources, or the FP64 oracle
 restatement, timed on this host on a 2-step sample of the same lattice), and
+"replicas": 32 SU(2)+fundamental chains (kappa 0.20-0.32, L 6-12) advanced by one
+launch per MD step, their throughput and the measured L_link susceptibility peaks;
 "learned_force": GPU seconds per accepted trajectory of exact-force HMC against
 multi-timescale HMC with the learned surrogate on SU(3) + {6,1} (d = 48), L=12.
 """
@@ -19,6 +21,8 @@ import os
 import subprocess
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -96,6 +100,30 @@ def cpu_baseline(workload, sample_steps=16):
             "sample": f"FP64 oracle restatement, {L}^4, {sample_steps} kick+drift MD steps ({sec:.1f} s)"}
 
 
+def replica_block(n_therm, n_meas):
+    """32 SU(2)+fundamental chains, kappa in [0.20, 0.32], L in {6, 8, 10, 12}, FP32 MD,
+    one launch per MD step for all chains, graph-replayed trajectories and kappa
+    swaps every trajectory: measured L_link susceptibility peaks and throughput."""
+    from gauge_higgs_amd import replicas
+    rs = replicas.kappa_scan()
+    m = replicas.measure_scan(rs, n_therm=n_therm, n_meas=n_meas)
+    vtot = int(sum(rs.lat.vols))
+    steps = 200
+    ms = rs.lat.time_md_steps(steps)
+    bpsite = algorithmic_bytes_per_site(4, 2, 2, 4)
+    sa = list(m["swap_acceptance"].values())
+    out = {"workload": "su2_fund_32chains_L6-12_fp32", "chains": rs.n, "sites": vtot, "beta": 2.3, "lambda": 0.5,
+           "kappa": [0.20, 0.32], "nmd": 20, "integrator": "Omelyan",
+           "md_site_updates_per_s": vtot * steps / (ms / 1e3), "ms_per_md_step": ms / steps,
+           "roofline": {"bound": "hbm", "achieved": vtot * bpsite / (ms / steps / 1e3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": vtot * bpsite / (ms / steps / 1e3) / 1e9 / HBM_PEAK_GBS, "algorithmic_bytes_per_site": bpsite},
+           "s_per_trajectory_cycle": m["s_per_cycle"], "trajectories_per_s": rs.n / m["s_per_cycle"],
+           "hmc_acceptance_mean": float(m["hmc_acceptance"].mean()), "swap_acceptance_mean": float(np.mean(sa)) if sa else None,
+           "measured_cycles": n_meas, "chi_link_peaks": replicas.chi_peaks(m)}
+    rs.lat.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +137,8 @@ def main():
     ap.add_argument("--only-learned", action="store_true", help="run only the exact vs learned-force comparison")
     ap.add_argument("--learned-meas", type=int, default=24, help="measured trajectories per chain in the comparison")
     ap.add_argument("--no-graph", action="store_true", help="launch trajectories directly instead of replaying hipGraphs")
+    ap.add_argument("--no-replicas", action="store_true", help="skip the 32-chain kappa-scan block")
+    ap.add_argument("--replica-meas", type=int, default=1000, help="measured trajectory cycles of the kappa scan")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (calibrates the HBM byte counters)")
     args = ap.parse_args()
@@ -178,9 +208,11 @@ def main():
                      "kernel": kernel, "algorithmic_bytes_per_site": bpsite, "algorithmic_bytes_per_launch": V * bpsite,
                      "stored_state_bytes_per_site": stored_bytes_per_site(D, N, d, w)},
     }
+    g.lat.close()
+    if not args.no_replicas:
+        line["replicas"] = replica_block(200, args.replica_meas)
     if not args.no_learned:
         from gauge_higgs_amd import learned
-        g.lat.close()
         line["learned_force"] = learned.compare(n_meas=args.learned_meas, use_graph=not args.no_graph)
     if not args.no_cpu:
         try:

@@ -217,6 +217,44 @@ __global__ __launch_bounds__(kBigBlock) void big_matter_kernel(BigArgs<T> a, Big
   }
 }
 
+// On-site part alone (inner steps of the multi-timescale split, F_NO_HOP):
+// -2 dV/dphi^* (scalar_higgs.hpp:111-125), pi kick and phi drift, one thread
+// per site. Same arithmetic and summation order as big_matter_kernel with no
+// hops, without a workgroup and the LDS tensor per site.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void onsite_step_kernel(const RepDesc* reps, const int* blk_rep, unsigned Vp, const T* phi_in,
+                                                           T* phi_out, T* pi, int d, double ck, double cd, int flags) {
+  using R = T;
+  const int r = blk_rep[blockIdx.x];
+  const RepDesc& rd = reps[r];
+  const unsigned s = blockIdx.x * kBlock + threadIdx.x - rd.offset;
+  if (s >= rd.vol) return;
+  const unsigned g = rd.offset + s;
+  R n2 = R(0);
+  for (int l = 0; l < d; ++l) {
+    const R xr = R(phi_in[size_t(2 * l) * Vp + g]), xi = R(phi_in[size_t(2 * l + 1) * Vp + g]);
+    n2 += xr * xr + xi * xi;
+  }
+  const R c = (flags & F_NO_ONSITE) ? R(0) : R(-2) * (R(1) + R(2.0 * rd.lambda) * (n2 - R(1)));
+  const bool abs_eps = flags & F_ABS_EPS;
+  const R ek = R(abs_eps ? ck : ck * rd.eps), ed = R(abs_eps ? cd : cd * rd.eps);
+  for (int l = 0; l < d; ++l) {
+    const R xr = R(phi_in[size_t(2 * l) * Vp + g]), xi = R(phi_in[size_t(2 * l + 1) * Vp + g]);
+    const R fr = fma(c, xr, R(0)), fi = fma(c, xi, R(0));
+    R pr = R(pi[size_t(2 * l) * Vp + g]), pim = R(pi[size_t(2 * l + 1) * Vp + g]);
+    if (flags & F_KICK) {
+      pr = fma(ek, fr, pr);
+      pim = fma(ek, fi, pim);
+      pi[size_t(2 * l) * Vp + g] = T(pr);
+      pi[size_t(2 * l + 1) * Vp + g] = T(pim);
+    }
+    if (flags & F_DRIFT) {
+      phi_out[size_t(2 * l) * Vp + g] = T(fma(ed, pr, xr));
+      phi_out[size_t(2 * l + 1) * Vp + g] = T(fma(ed, pim, xi));
+    }
+  }
+}
+
 // Per-site FP64 contributions for H with a large rep: q1 from big_matter_kernel
 // (BIG_ACTION), the rest streamed plane by plane.
 template <typename T, int N, int D>

@@ -192,7 +192,10 @@ void launch_part_step(ghx_ctx* c, double ck, double cd, bool drift, int parts, d
     flags |= F_SURR;
     l.surr_sign = surr_sign;
   }
-  if (c->big) c->bops.matter(l, c->tb, c->U[in], c->phi[in], c->phi[out], c->pi, c->G, nullptr, nullptr, ck, cd, flags, BIG_STEP);
+  if (c->big) {
+    if (flags & F_NO_HOP) c->bops.onsite(l, c->phi[in], c->phi[out], c->pi, c->rep.d, ck, cd, flags);
+    else c->bops.matter(l, c->tb, c->U[in], c->phi[in], c->phi[out], c->pi, c->G, nullptr, nullptr, ck, cd, flags, BIG_STEP);
+  }
   c->ops.md_step(l, c->U[in], c->U[out], c->P, c->phi[in], c->phi[out], c->pi, c->F, c->Fphi, ck, cd, flags);
   if (drift) c->cur = out;
 }

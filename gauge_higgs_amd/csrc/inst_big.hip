@@ -19,6 +19,10 @@ struct BigLaunch {
     hipLaunchKernelGGL(phi_drift_kernel<T>, dim3(c.nblk), dim3(kBlock), 0, c.stream, c.reps, c.blk_rep, c.Vp, (const T*)phi_in,
                        (T*)phi_out, (const T*)pi, nplanes, cd, abs_eps);
   }
+  static void onsite(const LaunchCtx& c, const void* phi_in, void* phi_out, void* pi, int d, double ck, double cd, int flags) {
+    hipLaunchKernelGGL(onsite_step_kernel<T>, dim3(c.nblk), dim3(kBlock), 0, c.stream, c.reps, c.blk_rep, c.Vp, (const T*)phi_in,
+                       (T*)phi_out, (T*)pi, d, ck, cd, flags);
+  }
 };
 template <typename T, int N, int D>
 void fill(Ops* o, BigOps* b) {
@@ -26,6 +30,7 @@ void fill(Ops* o, BigOps* b) {
   b->matter = &BigLaunch<T, N, D>::matter;
   b->action = &BigLaunch<T, N, D>::action;
   b->phi_drift = &BigLaunch<T, N, D>::phi_drift;
+  b->onsite = &BigLaunch<T, N, D>::onsite;
 }
 bool ops_big(int prec, int N, int D, Ops* o, BigOps* b) {
   if (N == 3 && D == 4) { prec ? fill<float, 3, 4>(o, b) : fill<double, 3, 4>(o, b); return true; }

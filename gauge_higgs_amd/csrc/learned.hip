@@ -474,7 +474,7 @@ int ghx_surrogate_init(ghx_ctx* ctx, uint64_t seed) {
   const int ngn = ctx->N * ctx->N - 1;
   st->nrows = size_t(ctx->D) * ctx->Vp;
   const size_t ntiles = (st->nrows + 15) / 16;
-  st->nblocks = int(std::min<size_t>((ntiles + 3) / 4, 1024));
+  st->nblocks = int(std::min<size_t>((ntiles + 3) / 4, 256));  // 1024 waves: the partial-gradient sum stays small
   st->nwaves = st->nblocks * 4;
   SCHK(hipMalloc((void**)&st->P, NPARAM * 4));
   SCHK(hipMalloc((void**)&st->m, NPARAM * 4));

@@ -32,6 +32,7 @@ struct BigOps {
                  void* Fphi, double* q1, double ck, double cd, int flags, int mode);
   void (*action)(const LaunchCtx&, const void* U, const void* P, const void* phi, const void* pi, const double* q1, int d);
   void (*phi_drift)(const LaunchCtx&, const void* phi_in, void* phi_out, const void* pi, int nplanes, double cd, int abs_eps);
+  void (*onsite)(const LaunchCtx&, const void* phi_in, void* phi_out, void* pi, int d, double ck, double cd, int flags);
 };
 
 struct Ops {

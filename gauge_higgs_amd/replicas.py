@@ -83,3 +83,45 @@ def kappa_scan(seed0=1, nmd=20, precision=FP32MD, n_chains=32, Ls=(6, 8, 10, 12)
     ladders = [[i for i in range(n_chains) if Lc[i] == L] for L in Ls]
     return ReplicaSet(4, 2, [[L] * 4 for L in Lc], "fund", [beta] * n_chains, kappas, [lam] * n_chains, seed0=seed0, nmd=nmd,
                       precision=precision, ladders=ladders)
+
+
+def measure_scan(rs, n_therm=200, n_meas=1000, use_graph=True):
+    """Run a replica set: n_therm cycles, then n_meas cycles recording every
+    chain's L_link (scalar_higgs.hpp:217-229). Returns per-chain kappa, L,
+    <L_link>, chi_link = V Var(L_link) (observables.hpp:104-145 Stats), the HMC
+    and swap acceptances, and the time per cycle (one trajectory of every chain
+    plus the swap passes, graph-replayed)."""
+    import time
+    rs.step(n_therm, use_graph=use_graph)
+    rs.lat.sync()
+    series = np.zeros((n_meas, rs.n))
+    t = 0.0
+    for k in range(n_meas):
+        t0 = time.perf_counter()
+        rs.step(1, use_graph=use_graph)
+        rs.lat.sync()
+        t += time.perf_counter() - t0
+        series[k] = rs.observables()[:, 3]
+    vols = np.array(rs.lat.vols, dtype=float)
+    return {
+        "kappa": np.array(rs.kappas), "L": np.array([e[0] for e in rs.lat.extents]), "Llink": series.mean(axis=0),
+        "chi_link": vols * series.var(axis=0), "hmc_acceptance": np.array([rs.hmc_acceptance(k) for k in range(rs.n)]),
+        "swap_acceptance": rs.pair_acceptance(), "s_per_cycle": t / n_meas, "series": series,
+    }
+
+
+def chi_peaks(m):
+    """kappa of the largest chi_link for each L, with a parabolic refinement
+    through the maximum and its two kappa neighbours."""
+    out = {}
+    for L in sorted(set(m["L"].tolist())):
+        sel = np.where(m["L"] == L)[0]
+        k, c = m["kappa"][sel], m["chi_link"][sel]
+        i = int(np.argmax(c))
+        kp = float(k[i])
+        if 0 < i < len(k) - 1:
+            den = c[i - 1] - 2 * c[i] + c[i + 1]
+            if den < 0:
+                kp = float(k[i] + 0.5 * (k[i + 1] - k[i]) * (c[i - 1] - c[i + 1]) / den)
+        out[int(L)] = {"kappa_peak": kp, "chi_max": float(c[i])}
+    return out

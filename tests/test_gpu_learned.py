@@ -270,3 +270,22 @@ def test_nested_omelyan_outer():
         c.md_evolve()
         dh.append(abs(c.hamiltonian() - h0))
     assert 2.5 < dh[0] / dh[1] < 6.0, dh
+
+
+def test_nested_m1_is_leapfrog_bigrep():
+    # inner steps of a large irrep run the per-site on-site kernel; with one inner
+    # step the split must still be the reference leapfrog up to rounding
+    def mk():
+        g = gha.GaugeHiggsHMC(3, "6,1", seed=1, D=4, N=3).configure(beta=5.7, kappa=0.05, lam=0.5, nmd=6, integ=gha.LEAPFROG,
+                                                                    reunit_each_traj=False)
+        g.hot(0.8, 12345, 0.3)
+        return g
+    a, b = mk(), mk()
+    a.refresh_momenta()
+    b.set("P", a.get("P"))
+    b.set("pi", a.get("pi"))
+    b.lat.set_nested(6, 1)
+    a.md_evolve()
+    b.md_evolve()
+    for k in ("U", "P", "phi", "pi"):
+        assert np.max(np.abs(a.get(k) - b.get(k))) < 1e-11, k
