@@ -26,7 +26,9 @@ struct BigRepTables {
   const int* row_ptr;   // CSR of W by basis index l
   const int* row_col;   // tensor index t of each entry
   const double* row_val;
-  const double* T;      // T^a_R, (N^2-1) x d x d complex (re, im)
+  const int* T_ptr;     // CSR of T^a_R by row (a*d + l): entries [T_ptr[r], T_ptr[r+1])
+  const int* T_col;     // column m of each entry
+  const double* T_val;  // complex value (re, im) of each entry
 };
 
 enum BigMode : int { BIG_STEP = 0, BIG_FORCE = 1, BIG_ACTION = 2 };
@@ -153,9 +155,9 @@ __global__ __launch_bounds__(kBigBlock) void big_matter_kernel(BigArgs<T> a, Big
     // g^a = -2 Im sum_{l,m} conj(phx_l) T^a_{lm} psi_m ; link force -kappa g^a
     for (int idx = threadIdx.x; idx < NGN * d; idx += kBigBlock) {
       const int ga = idx / d, l = idx % d;
-      const double* Tr = tb.T + (size_t(ga) * d + l) * d * 2;
       cx<R> t = {R(0), R(0)};
-      for (int m = 0; m < d; ++m) cfma(t, cx<R>{R(Tr[2 * m]), R(Tr[2 * m + 1])}, psi[m]);
+      for (int e = tb.T_ptr[idx]; e < tb.T_ptr[idx + 1]; ++e)
+        cfma(t, cx<R>{R(tb.T_val[2 * e]), R(tb.T_val[2 * e + 1])}, psi[tb.T_col[e]]);
       cx<R> z = {R(0), R(0)};
       cfmac(z, phx[l], t);
       gred[ga][l] = z.i;

@@ -96,7 +96,7 @@ bool sym2_canonical(const RepInfo& r) {
   return true;
 }
 
-// Sparse W (CSC by tensor index, CSR by basis index) and dense T^a_R on the device.
+// Sparse W (CSC by tensor index, CSR by basis index) and sparse T^a_R (CSR by row) on the device.
 int upload_tables(ghx_ctx* ctx) {
   free_tables(ctx);
   const RepInfo& r = ctx->rep;
@@ -117,16 +117,28 @@ int upload_tables(ghx_ctx* ctx) {
     }
     rptr[l + 1] = int(rcol.size());
   }
-  std::vector<double> T(r.T.size() * 2);
-  for (size_t i = 0; i < r.T.size(); ++i) { T[2 * i] = r.T[i].real(); T[2 * i + 1] = r.T[i].imag(); }
-  const void* src[7] = {cptr.data(), crow.data(), cval.data(), rptr.data(), rcol.data(), rval.data(), T.data()};
-  const size_t bytes[7] = {cptr.size() * 4, crow.size() * 4, cval.size() * 8, rptr.size() * 4, rcol.size() * 4, rval.size() * 8, T.size() * 8};
-  for (int k = 0; k < 7; ++k) {
+  // T^a_R by rows (a, l), nonzeros only (a zero entry adds exactly nothing)
+  const int ngn = ctx->N * ctx->N - 1;
+  std::vector<int> tptr(size_t(ngn) * r.d + 1, 0), tcol;
+  std::vector<double> tval;
+  for (int a = 0; a < ngn; ++a)
+    for (int l = 0; l < r.d; ++l) {
+      for (int m = 0; m < r.d; ++m) {
+        const std::complex<double> z = r.T[(size_t(a) * r.d + l) * r.d + m];
+        if (z != 0.0) { tcol.push_back(m); tval.push_back(z.real()); tval.push_back(z.imag()); }
+      }
+      tptr[size_t(a) * r.d + l + 1] = int(tcol.size());
+    }
+  const void* src[9] = {cptr.data(), crow.data(), cval.data(), rptr.data(), rcol.data(), rval.data(), tptr.data(), tcol.data(), tval.data()};
+  const size_t bytes[9] = {cptr.size() * 4, crow.size() * 4, cval.size() * 8, rptr.size() * 4, rcol.size() * 4, rval.size() * 8,
+                           tptr.size() * 4, tcol.size() * 4, tval.size() * 8};
+  for (int k = 0; k < 9; ++k) {
     HIPCHK(hipMalloc(&ctx->dtab[k], std::max<size_t>(bytes[k], 8)));
     HIPCHK(hipMemcpy(ctx->dtab[k], src[k], bytes[k], hipMemcpyHostToDevice));
   }
   ctx->tb = BigRepTables{r.d, r.nbox, int(dimT), (const int*)ctx->dtab[0], (const int*)ctx->dtab[1], (const double*)ctx->dtab[2],
-                         (const int*)ctx->dtab[3], (const int*)ctx->dtab[4], (const double*)ctx->dtab[5], (const double*)ctx->dtab[6]};
+                         (const int*)ctx->dtab[3], (const int*)ctx->dtab[4], (const double*)ctx->dtab[5], (const int*)ctx->dtab[6],
+                         (const int*)ctx->dtab[7], (const double*)ctx->dtab[8]};
   return GHX_OK;
 }
 

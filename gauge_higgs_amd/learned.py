@@ -57,8 +57,8 @@ def _chain(g, n, record=True, use_graph=True):
 RESIDUAL_GATE = 0.9  # use S only if it explains > 10 % of |F|^2 on a held-out configuration
 
 
-def compare(L=12, rep="6,1", beta=5.7, kappa=0.05, lam=0.5, nmd=16, exact_nmds=(16, 48), n_therm=40, n_meas=24,
-            splits=((8, 6),), train_configs=8, train_iters=200, lr=2e-3, seed=1, precision=_h.FP32MD,
+def compare(L=12, rep="6,1", beta=5.7, kappa=0.05, lam=0.5, nmd=16, exact_nmds=(16, 32, 48), n_therm=40, n_meas=24,
+            splits=((5, 10), (8, 6)), train_configs=8, train_iters=200, lr=2e-3, seed=1, precision=_h.FP32MD,
             therm_split=(12, 6), warm=(0.4, 0.27), use_graph=True):
     """Cost per accepted trajectory of exact-force HMC and of the multi-timescale
     chain with and without the learned surrogate, from one thermalised
