@@ -26,12 +26,20 @@ struct BigRepTables {
   const int* row_ptr;   // CSR of W by basis index l
   const int* row_col;   // tensor index t of each entry
   const double* row_val;
+  const float* col_valf;  // the W values again in FP32 (read by the FP32 kernels: half the table traffic)
+  const float* row_valf;
   const int* T_ptr;     // CSR of T^a_R by row (a*d + l): entries [T_ptr[r], T_ptr[r+1])
   const int* T_col;     // column m of each entry
   const double* T_val;  // complex value (re, im) of each entry
 };
 
 enum BigMode : int { BIG_STEP = 0, BIG_FORCE = 1, BIG_ACTION = 2 };
+
+template <typename R>
+GHX_DEV R wval(const double* vd, const float* vf, int e) {
+  if constexpr (sizeof(R) == 4) return vf[e];
+  else return vd[e];
+}
 
 template <typename T>
 struct BigArgs {
@@ -82,7 +90,7 @@ GHX_DEV void big_apply(const BigRepTables& tb, const Mat<R, N>& M, const cx<R>* 
   for (int t = threadIdx.x; t < tb.dimT; t += kBigBlock) {  // lift
     cx<R> z = {R(0), R(0)};
     for (int e = tb.col_ptr[t]; e < tb.col_ptr[t + 1]; ++e) {
-      const R w = R(tb.col_val[e]);
+      const R w = wval<R>(tb.col_val, tb.col_valf, e);
       const cx<R> x = v[tb.col_row[e]];
       z.r = fma(w, x.r, z.r);
       z.i = fma(w, x.i, z.i);
@@ -96,15 +104,24 @@ GHX_DEV void big_apply(const BigRepTables& tb, const Mat<R, N>& M, const cx<R>* 
     __syncthreads();
     blk /= N;
   }
-  for (int l = threadIdx.x; l < tb.d; l += kBigBlock) {  // project
+  // project: kProjLanes adjacent lanes share a row (a row holds ~dimT/d
+  // entries) and combine their partial sums with lane shuffles
+  constexpr int kProjLanes = 4;
+  for (int idx = threadIdx.x; idx < tb.d * kProjLanes; idx += kBigBlock) {
+    const int l = idx / kProjLanes, part = idx % kProjLanes;
     cx<R> z = {R(0), R(0)};
-    for (int e = tb.row_ptr[l]; e < tb.row_ptr[l + 1]; ++e) {
-      const R w = R(tb.row_val[e]);
+    for (int e = tb.row_ptr[l] + part; e < tb.row_ptr[l + 1]; e += kProjLanes) {
+      const R w = wval<R>(tb.row_val, tb.row_valf, e);
       const cx<R> x = X[tb.row_col[e]];
       z.r = fma(w, x.r, z.r);
       z.i = fma(w, x.i, z.i);
     }
-    out[l] = z;
+#pragma unroll
+    for (int o = 1; o < kProjLanes; o <<= 1) {
+      z.r += __shfl_xor(z.r, o, 64);
+      z.i += __shfl_xor(z.i, o, 64);
+    }
+    if (part == 0) out[l] = z;
   }
   __syncthreads();
 }

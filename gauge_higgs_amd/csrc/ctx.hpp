@@ -58,7 +58,7 @@ struct ghx_ctx {
   bool big = false;
   BigOps bops{};
   BigRepTables tb{};
-  void* dtab[9] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  void* dtab[11] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   void* G = nullptr;
   double* q1 = nullptr;
   // replica exchange

@@ -129,16 +129,18 @@ int upload_tables(ghx_ctx* ctx) {
       }
       tptr[size_t(a) * r.d + l + 1] = int(tcol.size());
     }
-  const void* src[9] = {cptr.data(), crow.data(), cval.data(), rptr.data(), rcol.data(), rval.data(), tptr.data(), tcol.data(), tval.data()};
-  const size_t bytes[9] = {cptr.size() * 4, crow.size() * 4, cval.size() * 8, rptr.size() * 4, rcol.size() * 4, rval.size() * 8,
-                           tptr.size() * 4, tcol.size() * 4, tval.size() * 8};
-  for (int k = 0; k < 9; ++k) {
+  const std::vector<float> cvalf(cval.begin(), cval.end()), rvalf(rval.begin(), rval.end());
+  const void* src[11] = {cptr.data(), crow.data(), cval.data(), rptr.data(), rcol.data(), rval.data(), cvalf.data(), rvalf.data(),
+                         tptr.data(), tcol.data(), tval.data()};
+  const size_t bytes[11] = {cptr.size() * 4, crow.size() * 4, cval.size() * 8, rptr.size() * 4, rcol.size() * 4, rval.size() * 8,
+                            cvalf.size() * 4, rvalf.size() * 4, tptr.size() * 4, tcol.size() * 4, tval.size() * 8};
+  for (int k = 0; k < 11; ++k) {
     HIPCHK(hipMalloc(&ctx->dtab[k], std::max<size_t>(bytes[k], 8)));
     HIPCHK(hipMemcpy(ctx->dtab[k], src[k], bytes[k], hipMemcpyHostToDevice));
   }
   ctx->tb = BigRepTables{r.d, r.nbox, int(dimT), (const int*)ctx->dtab[0], (const int*)ctx->dtab[1], (const double*)ctx->dtab[2],
-                         (const int*)ctx->dtab[3], (const int*)ctx->dtab[4], (const double*)ctx->dtab[5], (const int*)ctx->dtab[6],
-                         (const int*)ctx->dtab[7], (const double*)ctx->dtab[8]};
+                         (const int*)ctx->dtab[3], (const int*)ctx->dtab[4], (const double*)ctx->dtab[5], (const float*)ctx->dtab[6],
+                         (const float*)ctx->dtab[7], (const int*)ctx->dtab[8], (const int*)ctx->dtab[9], (const double*)ctx->dtab[10]};
   return GHX_OK;
 }
 
