@@ -56,6 +56,8 @@ def load():
         "ghx_set_tuning": [vp, i, i],
         "ghx_get_tuning": [vp, i, C.POINTER(C.c_int)],
         "ghx_autotune": [vp, i, C.POINTER(C.c_int), dp],
+        "ghx_set_frozen_phi": [vp, i],
+        "ghx_normalize_phi": [vp],
         "ghx_surrogate_init": [vp, u64],
         "ghx_surrogate_enable": [vp, i],
         "ghx_surrogate_nparams": [],

@@ -2,6 +2,8 @@
 //   hmc_higgs <rep> <L> <beta> <kappa> <lambda> [ntherm=80] [nmeas=200] [nmd=24] [tau=1.0] [seed=1]
 //             [--N=2] [--D=4] [--integrator=omelyan|leapfrog] [--precision=fp64|fp32] [--no-graph]
 // <rep> = fund | adj | Young rows "r1,r2,..." with optional ":real".
+#include <cstdlib>
+
 #include "cli_common.hpp"
 
 int main(int argc, char** argv) {
@@ -48,7 +50,15 @@ int main(int argc, char** argv) {
               fl.D, vol);
   std::printf("# beta=%.4f kappa=%.4f lambda=%.4f tau=%.3f nmd=%d  (kappa_c,free=1/(2D)=%.4f)\n", beta, kappa, lambda, tau, nmd,
               1.0 / (2.0 * fl.D));
-  GHX_CHECK(ctx, ghx_init_fields(ctx, 0, GHX_HOT, 0.8, 1, 12345, 0.3));  // hmc_higgs.cpp:72-73
+  // Start modes of the reference's string / multi-invariant drivers (gh_string.cpp:106-125,
+  // hmc_higgs_multi.cpp:81-90): GH_COLD -> unit links and phi = (1, 0, ...); GH_FROZEN ->
+  // |phi_x| = 1 frozen-length scalar, normalised before thermalising. Default: hot start.
+  if (std::getenv("GH_COLD")) GHX_CHECK(ctx, ghx_init_fields(ctx, 0, GHX_COLD, 0.0, 0, 0, 1.0));
+  else GHX_CHECK(ctx, ghx_init_fields(ctx, 0, GHX_HOT, 0.8, 1, 12345, 0.3));  // hmc_higgs.cpp:72-73
+  if (std::getenv("GH_FROZEN")) {
+    GHX_CHECK(ctx, ghx_set_frozen_phi(ctx, 1));
+    GHX_CHECK(ctx, ghx_normalize_phi(ctx));
+  }
   GHX_CHECK(ctx, ghx_trajectories(ctx, ntherm, fl.graph ? 1 : 0));
 
   GHX_CHECK(ctx, ghx_set_counters(ctx, 0, 0, 0));  // hmc_higgs.cpp:76

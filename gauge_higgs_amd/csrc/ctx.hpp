@@ -52,6 +52,7 @@ struct ghx_ctx {
   // surrogate link force (learned); S planes are filled by surr_fill before a kick
   void* S = nullptr;
   bool use_surr = false;
+  bool frozen = false;  // |phi_x| = 1 scalar
   void (*surr_fill)(ghx_ctx*, int buf) = nullptr;
   SurrogateState* surr_state = nullptr;
   // large-irrep path
@@ -84,6 +85,7 @@ struct ghx_ctx {
     c.G = G;
     c.S = S;
     c.surr_sign = 0.0;
+    c.frozen = frozen ? 1 : 0;
     c.stream_seg = (nrep == 1) ? stream_seg : 0;
     c.V3 = hreps[0].stride[D - 1];
     c.Lt = unsigned(hreps[0].L[D - 1]);

@@ -318,6 +318,7 @@ void launch_trajectory(ghx_ctx* c) {
   const LaunchCtx l = c->lc();
   const bool hp = has_phi(c);
   c->ops.refresh(l, c->P, c->pi, c->mc, hp ? kTagP : kTagP_pure, kTagPi);
+  if (c->frozen) c->ops.frozen_site(l, c->phi[c->cur], c->pi, 0);  // tangent momenta (gauge_higgs_hmc.hpp:172)
   (void)hipMemcpyAsync(c->Usave, c->U[0], c->nU() * c->Vp * c->tsize, hipMemcpyDeviceToDevice, c->stream);
   if (hp) (void)hipMemcpyAsync(c->phisave, c->phi[0], c->nphi() * c->Vp * c->tsize, hipMemcpyDeviceToDevice, c->stream);
   launch_hamiltonian(c, 0, c->Hi);
@@ -637,6 +638,28 @@ int ghx_refresh_momenta(ghx_ctx* ctx) {
   int rc = check_ready(ctx);
   if (rc) return rc;
   ctx->ops.refresh(ctx->lc(), ctx->P, ctx->pi, ctx->mc, has_phi(ctx) ? kTagP : kTagP_pure, kTagPi);
+  if (ctx->frozen) ctx->ops.frozen_site(ctx->lc(), ctx->phi[ctx->cur], ctx->pi, 0);
+  HIPCHK(hipGetLastError());
+  return GHX_OK;
+}
+
+// Frozen-length scalar |phi_x| = 1 (GaugeHiggsHMC::frozen_phi, gauge_higgs_hmc.hpp:38, 61-104).
+int ghx_set_frozen_phi(ghx_ctx* ctx, int on) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (on && !has_phi(ctx)) return fail(ctx, GHX_E_INVALID, "frozen-length scalar needs a scalar field");
+  if (on && ctx->big) return fail(ctx, GHX_E_UNSUPPORTED, "frozen-length scalar is not built for the large-irrep kernels");
+  ctx->frozen = on != 0;
+  drop_graph(ctx);
+  return GHX_OK;
+}
+// normalize_phi (gauge_higgs_hmc.hpp:74-84): phi_x /= |phi_x| on every replica.
+int ghx_normalize_phi(ghx_ctx* ctx) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (!has_phi(ctx)) return fail(ctx, GHX_E_INVALID, "no scalar field");
+  if (ctx->big) return fail(ctx, GHX_E_UNSUPPORTED, "frozen-length scalar is not built for the large-irrep kernels");
+  ctx->ops.frozen_site(ctx->lc(), ctx->phi[ctx->cur], ctx->pi, 1);
   HIPCHK(hipGetLastError());
   return GHX_OK;
 }

@@ -37,8 +37,10 @@ struct RepDesc {
 // F_NO_GAUGE / F_NO_HOP / F_NO_ONSITE drop force parts (multi-timescale
 // splitting); F_SURR adds surr_sign * S (a position-dependent surrogate link
 // force) to the link force.
+// F_FROZEN: |phi_x| = 1 scalar (gauge_higgs_hmc.hpp:61-104): pi is projected
+// onto the tangent space after each kick and the phi drift is the geodesic.
 enum StepFlags : int { F_KICK = 1, F_DRIFT = 2, F_FORCE = 4, F_ABS_EPS = 8, F_NO_GAUGE = 16, F_NO_HOP = 32, F_NO_ONSITE = 64,
-                       F_SURR = 128 };
+                       F_SURR = 128, F_FROZEN = 256 };
 
 template <typename T>
 struct StepArgs {
@@ -229,6 +231,33 @@ struct Nbr {
   }
 };
 
+// |phi| = 1 helpers on one site's d components (gauge_higgs_hmc.hpp:63-104).
+template <typename R, int DR>
+GHX_DEV void tangent_project(const cx<R>* ph, cx<R>* pv) {
+  R rp = R(0);
+#pragma unroll
+  for (int k = 0; k < DR; ++k) rp += ph[k].r * pv[k].r + ph[k].i * pv[k].i;  // Re(phi^dag pi)
+#pragma unroll
+  for (int k = 0; k < DR; ++k) { pv[k].r -= rp * ph[k].r; pv[k].i -= rp * ph[k].i; }
+}
+// Great-circle flow by time e: phi' = cos(r e) phi + sin(r e)/r pi, pi' = -r sin(r e) phi + cos(r e) pi.
+template <typename R, int DR>
+GHX_DEV void geodesic_drift(cx<R>* ph, cx<R>* pv, R e) {
+  R r2 = R(0);
+#pragma unroll
+  for (int k = 0; k < DR; ++k) r2 += pv[k].r * pv[k].r + pv[k].i * pv[k].i;
+  const R r = sqrt(r2);
+  const R rmin = sizeof(R) == 8 ? R(1e-300) : R(1e-30);
+  if (r < rmin) return;
+  const R c = cos(r * e), sn = sin(r * e), q = sn / r, w = -r * sn;
+#pragma unroll
+  for (int k = 0; k < DR; ++k) {
+    const cx<R> f = ph[k], p = pv[k];
+    ph[k] = {c * f.r + q * p.r, c * f.i + q * p.i};
+    pv[k] = {w * f.r + c * p.r, w * f.i + c * p.i};
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Fused MD step: kick (P -= eps F_link, pi += eps F_phi) then drift
 // (U' = exp(i eps P) U, phi' = phi + eps pi) for one site's D links and phi.
@@ -362,16 +391,22 @@ GHX_DEV void site_update(const StepArgs<T>& a, const typename REP::template Para
     const Planes<T> Pi(a.pi, Vp, 2 * DR);
     cx<R> pv[DR];
     bload_vec<T, R, DR>(Pi, g, pv);
+    const bool frozen = a.flags & F_FROZEN;
     if (a.flags & F_KICK) {
 #pragma unroll
       for (int k = 0; k < DR; ++k) { pv[k].r = fma(ek, fph[k].r, pv[k].r); pv[k].i = fma(ek, fph[k].i, pv[k].i); }
-      bstore_vec<T, R, DR>(Pi, g, pv);
+      if (frozen) tangent_project<R, DR>(phx, pv);
     }
     if (a.flags & F_DRIFT) {
+      if (frozen) {
+        geodesic_drift<R, DR>(phx, pv, ed);
+      } else {
 #pragma unroll
-      for (int k = 0; k < DR; ++k) { phx[k].r = fma(ed, pv[k].r, phx[k].r); phx[k].i = fma(ed, pv[k].i, phx[k].i); }
+        for (int k = 0; k < DR; ++k) { phx[k].r = fma(ed, pv[k].r, phx[k].r); phx[k].i = fma(ed, pv[k].i, phx[k].i); }
+      }
       bstore_vec<T, R, DR>(Planes<T>(a.phi_out, Vp, 2 * DR), g, phx);
     }
+    if ((a.flags & F_KICK) || (frozen && (a.flags & F_DRIFT))) bstore_vec<T, R, DR>(Pi, g, pv);
   }
 }
 
@@ -432,9 +467,41 @@ __global__ __launch_bounds__(kBlock) void drift_kernel(StepArgs<T> a) {
     cx<R> ph[DR], pv[DR];
     load_vec<T, R, DR>(a.phi_in, Vp, g, ph);
     load_vec<T, R, DR>(a.pi, Vp, g, pv);
+    if (a.flags & F_FROZEN) {
+      geodesic_drift<R, DR>(ph, pv, ed);
+      store_vec<T, R, DR>(a.pi, Vp, g, pv);
+    } else {
 #pragma unroll
-    for (int k = 0; k < DR; ++k) { ph[k].r = fma(ed, pv[k].r, ph[k].r); ph[k].i = fma(ed, pv[k].i, ph[k].i); }
+      for (int k = 0; k < DR; ++k) { ph[k].r = fma(ed, pv[k].r, ph[k].r); ph[k].i = fma(ed, pv[k].i, ph[k].i); }
+    }
     store_vec<T, R, DR>(a.phi_out, Vp, g, ph);
+  }
+}
+
+// |phi_x| = 1 scalar, runtime d (gauge_higgs_hmc.hpp:63-84): mode 0 projects pi
+// onto the tangent space at phi, mode 1 normalises phi.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void frozen_site_kernel(const RepDesc* reps, const int* blk_rep, unsigned Vp, T* phi, T* pi, int d,
+                                                            int mode) {
+  const int r = blk_rep[blockIdx.x];
+  const RepDesc& rd = reps[r];
+  const unsigned s = blockIdx.x * kBlock + threadIdx.x - rd.offset;
+  if (s >= rd.vol) return;
+  const unsigned g = rd.offset + s;
+  T acc = T(0);
+  for (int k = 0; k < d; ++k) {
+    const T xr = phi[size_t(2 * k) * Vp + g], xi = phi[size_t(2 * k + 1) * Vp + g];
+    if (mode == 0) acc += xr * pi[size_t(2 * k) * Vp + g] + xi * pi[size_t(2 * k + 1) * Vp + g];
+    else acc += xr * xr + xi * xi;
+  }
+  if (mode == 0) {
+    for (int k = 0; k < d; ++k) {
+      pi[size_t(2 * k) * Vp + g] -= acc * phi[size_t(2 * k) * Vp + g];
+      pi[size_t(2 * k + 1) * Vp + g] -= acc * phi[size_t(2 * k + 1) * Vp + g];
+    }
+  } else if (acc > T(0)) {
+    const T inv = T(1) / sqrt(acc);
+    for (int k = 0; k < 2 * d; ++k) phi[size_t(k) * Vp + g] *= inv;
   }
 }
 

@@ -26,6 +26,7 @@ struct Launch {
   using REPD = REPT<double, N>;
   static void md_step(const LaunchCtx& c, const void* U_in, void* U_out, void* P, const void* phi_in, void* phi_out, void* pi,
                       void* F, void* Fphi, double ck, double cd, int flags) {
+    if (c.frozen) flags |= F_FROZEN;
     StepArgs<T> a{c.reps, c.blk_rep, c.Vp, (const T*)U_in, (T*)U_out, (T*)P, (const T*)phi_in, (T*)phi_out, (T*)pi, (T*)F, (T*)Fphi,
                   ck, cd, flags, (const T*)c.G, (const T*)c.S, c.surr_sign};
     if (c.stream_seg > 0) {
@@ -41,6 +42,7 @@ struct Launch {
   }
   static void drift(const LaunchCtx& c, const void* U_in, void* U_out, const void* P, const void* phi_in, void* phi_out,
                     const void* pi, double cd, int flags) {
+    if (c.frozen) flags |= F_FROZEN;
     StepArgs<T> a{c.reps, c.blk_rep, c.Vp, (const T*)U_in, (T*)U_out, (T*)P, (const T*)phi_in, (T*)phi_out, (T*)pi, nullptr, nullptr,
                   0.0, cd, flags, nullptr, nullptr, 0.0};
     hipLaunchKernelGGL((drift_kernel<T, N, D, REP::DR>), dim3(c.nblk), dim3(kBlock), 0, c.stream, a);
@@ -53,6 +55,10 @@ struct Launch {
   static void refresh(const LaunchCtx& c, void* P, void* pi, const McState* st, unsigned long long tagP, unsigned long long tagPi) {
     hipLaunchKernelGGL((refresh_kernel<T, N, D>), dim3(c.nblk), dim3(kBlock), 0, c.stream, c.reps, c.blk_rep, c.Vp, (T*)P, (T*)pi,
                        c.d, c.real_rep, st, tagP, tagPi);
+  }
+  static void frozen_site(const LaunchCtx& c, void* phi, void* pi, int mode) {
+    hipLaunchKernelGGL(frozen_site_kernel<T>, dim3(c.nblk), dim3(kBlock), 0, c.stream, c.reps, c.blk_rep, c.Vp, (T*)phi, (T*)pi, c.d,
+                       mode);
   }
   static void init_links(const LaunchCtx& c, void* U, int replica, int mode, double sigma) {
     hipLaunchKernelGGL((init_links_kernel<T, N, D>), dim3(c.nblk), dim3(kBlock), 0, c.stream, c.reps, c.blk_rep, c.Vp, (T*)U, replica,
@@ -70,7 +76,7 @@ struct Launch {
     ActArgs a{c.reps, c.red_rep, c.red_first, c.Vp, c.partials};
     hipLaunchKernelGGL((polyakov_kernel<T, N, D>), dim3(c.nred), dim3(kRedBlock), 0, c.stream, a, (const T*)U);
   }
-  static Ops ops() { return Ops{&md_step, &drift, &action, &refresh, &init_links, &reunit, &unitarity, &polyakov}; }
+  static Ops ops() { return Ops{&md_step, &drift, &action, &refresh, &init_links, &reunit, &unitarity, &polyakov, &frozen_site}; }
 };
 
 // Dispatch helper over precision and D in {3, 4}.

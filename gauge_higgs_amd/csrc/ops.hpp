@@ -23,6 +23,7 @@ struct LaunchCtx {
   unsigned V3, Lt;  // cross-section volume and last extent of replica 0 (streaming kernel)
   const void* S;    // surrogate link force planes
   double surr_sign;
+  int frozen;       // |phi_x| = 1 scalar: adds F_FROZEN to every MD launch
 };
 
 struct BigRepTables;
@@ -46,6 +47,7 @@ struct Ops {
   void (*reunit)(const LaunchCtx&, const void* U_in, void* U_out);
   void (*unitarity)(const LaunchCtx&, const void* U);
   void (*polyakov)(const LaunchCtx&, const void* U);
+  void (*frozen_site)(const LaunchCtx&, void* phi, void* pi, int mode);
 };
 
 // Returns false when the combination is not compiled. kind: RepKind; for

@@ -80,6 +80,13 @@ class Lattices:
         """Multi-timescale leapfrog: n_outer hopping-force steps, m_inner gauge/on-site steps each (0 = off)."""
         self._check(self._lib.ghx_set_nested(self.h, n_outer, m_inner))
 
+    def set_frozen_phi(self, on=True):
+        """|phi_x| = 1 scalar: tangent momenta and geodesic drift (gauge_higgs_hmc.hpp:61-104)."""
+        self._check(self._lib.ghx_set_frozen_phi(self.h, int(on)))
+
+    def normalize_phi(self):
+        self._check(self._lib.ghx_normalize_phi(self.h))
+
     def set_seed(self, replica, seed):
         self._check(self._lib.ghx_set_seed(self.h, replica, seed))
 
@@ -275,6 +282,18 @@ class GaugeHiggsHMC:
 
     def hot(self, sigma=0.8, phi_stream=12345, phi_sigma=0.3, phi_mode=1):
         self.lat.init_fields(0, HOT, sigma, phi_mode, phi_stream, phi_sigma)
+
+    @property
+    def frozen_phi(self):
+        return getattr(self, "_frozen", False)
+
+    @frozen_phi.setter
+    def frozen_phi(self, on):
+        self.lat.set_frozen_phi(on)
+        self._frozen = bool(on)
+
+    def normalize_phi(self):
+        self.lat.normalize_phi()
 
     def cold(self, phi_c=1.0):
         self.lat.init_fields(0, COLD, 0.0, 0, 0, phi_c)

@@ -66,6 +66,13 @@ int ghx_set_integrator(ghx_ctx* ctx, int nmd, int integrator, int reunit_each_tr
  * steps that carry the gauge force and the on-site scalar force. n_outer = 0
  * restores the single-scale integrator. */
 int ghx_set_nested(ghx_ctx* ctx, int n_outer, int m_inner);
+/* Frozen-length scalar |phi_x| = 1 (GaugeHiggsHMC::frozen_phi,
+ * gauge_higgs_hmc.hpp:38, 61-104, 133, 144, 172): momenta are projected onto
+ * the tangent space at refresh and after every kick, and the phi drift is the
+ * exact great-circle flow. ghx_normalize_phi is normalize_phi() (:74-84).
+ * Not available with the large-irrep kernels (GHX_E_UNSUPPORTED). */
+int ghx_set_frozen_phi(ghx_ctx* ctx, int on);
+int ghx_normalize_phi(ghx_ctx* ctx);
 /* RNG seed of one replica (Rng ctor, rng.hpp:27). */
 int ghx_set_seed(ghx_ctx* ctx, int replica, uint64_t seed);
 

@@ -620,6 +620,7 @@ struct Engine {
   double beta = 2.3, kappa = 0.1, lambda = 0.5, tau = 1.0;
   int nmd = 20;
   bool omelyan = true, reunit_each_traj = true;
+  bool frozen = false;  // |phi_x| = 1 scalar (gauge_higgs_hmc.hpp:38)
   std::uint64_t traj_count = 0, accept_count = 0;
   double last_dH = 0.0;
 
@@ -666,6 +667,7 @@ struct Engine {
     if (!rep) { refresh_P(stream_key(0xA1, traj_count)); return; }
     refresh_P(stream_key(0xC3, traj_count));
     gaussian_field(pi, stream_key(0xD4, traj_count), 1.0);
+    if (frozen) project_pi_tangent();
   }
   double kinetic_P() const {  // fields.hpp:69-74
     double e = 0.0;
@@ -802,6 +804,43 @@ struct Engine {
     if (!rep) return kinetic_P() + gauge_action();
     return kinetic_P() + kinetic_pi() + gauge_action() + scalar_action();
   }
+  // Frozen-length scalar |phi_x| = 1 — gauge_higgs_hmc.hpp:61-104.
+  // Tangent projection: pi -= Re(phi^dag pi) phi per site.
+  void project_pi_tangent() {
+#pragma omp parallel for schedule(static)
+    for (std::int64_t s = 0; s < g.vol; ++s) {
+      double rp = 0.0;
+      for (int k = 0; k < d; ++k) rp += (std::conj(phi[s * d + k]) * pi[s * d + k]).real();
+      for (int k = 0; k < d; ++k) pi[s * d + k] -= cd(rp, 0.0) * phi[s * d + k];
+    }
+  }
+  void normalize_phi() {
+#pragma omp parallel for schedule(static)
+    for (std::int64_t s = 0; s < g.vol; ++s) {
+      double n2 = 0.0;
+      for (int k = 0; k < d; ++k) n2 += std::norm(phi[s * d + k]);
+      if (n2 > 0.0) {
+        const double inv = 1.0 / std::sqrt(n2);
+        for (int k = 0; k < d; ++k) phi[s * d + k] *= cd(inv, 0.0);
+      }
+    }
+  }
+  // Great-circle flow by eps: phi' = cos(r eps) phi + sin(r eps)/r pi, pi' = -r sin(r eps) phi + cos(r eps) pi.
+  void geodesic_drift(double eps) {
+#pragma omp parallel for schedule(static)
+    for (std::int64_t s = 0; s < g.vol; ++s) {
+      double r2 = 0.0;
+      for (int k = 0; k < d; ++k) r2 += std::norm(pi[s * d + k]);
+      const double r = std::sqrt(r2);
+      if (r < 1e-300) continue;
+      const double c = std::cos(r * eps), sn = std::sin(r * eps);
+      for (int k = 0; k < d; ++k) {
+        const cd f = phi[s * d + k], p = pi[s * d + k];
+        phi[s * d + k] = cd(c, 0.0) * f + cd(sn / r, 0.0) * p;
+        pi[s * d + k] = cd(-r * sn, 0.0) * f + cd(c, 0.0) * p;
+      }
+    }
+  }
   void kick(double eps) {  // gauge_higgs_hmc.hpp:112-134, hmc.hpp:35-42
     for (auto& v : F) v.fill(0.0);
     add_gauge_force();
@@ -813,6 +852,7 @@ struct Engine {
     scalar_force();
 #pragma omp parallel for schedule(static)
     for (std::size_t i = 0; i < pi.size(); ++i) pi[i] += eps * Fphi[i];
+    if (frozen) project_pi_tangent();
   }
   void drift(double eps) {  // gauge_higgs_hmc.hpp:136-147, hmc.hpp:44-52
 #pragma omp parallel for schedule(static)
@@ -823,6 +863,7 @@ struct Engine {
         U[s * D + mu] = expi<N>(Pm) * U[s * D + mu];
       }
     if (!rep) return;
+    if (frozen) { geodesic_drift(eps); return; }
 #pragma omp parallel for schedule(static)
     for (std::size_t i = 0; i < phi.size(); ++i) phi[i] += eps * pi[i];
   }

@@ -45,6 +45,8 @@ struct Base {
   virtual void rep_apply(const double* U, const double* phi, int dag, double* out) const = 0;
   virtual void rep_hop_g(const double* U, const double* px, const double* py, double* out) const = 0;
   virtual void rep_matrix(const double* U, double* out) const = 0;
+  virtual void set_frozen(int on) = 0;
+  virtual void normalize_phi() = 0;
 };
 
 template <int N>
@@ -110,6 +112,8 @@ struct Impl : Base {
     return acc ? 1 : 0;
   }
   void reunitarize() override { e.reunitarize_all(); }
+  void set_frozen(int on) override { e.frozen = on != 0; }
+  void normalize_phi() override { e.normalize_phi(); }
   double max_unitarity() const override { return e.max_unitarity_violation(); }
   std::uint64_t& traj_count() override { return e.traj_count; }
   std::uint64_t& accept_count() override { return e.accept_count; }
@@ -174,6 +178,9 @@ void orc_hot(void* h, double sigma) { static_cast<Base*>(h)->hot(sigma); }
 void orc_cold(void* h) { static_cast<Base*>(h)->cold(); }
 void orc_phi_gaussian(void* h, std::uint64_t stream, double sigma) { static_cast<Base*>(h)->phi_gaussian(stream, sigma); }
 void orc_phi_cold(void* h, double c) { static_cast<Base*>(h)->phi_cold(c); }
+// frozen-length scalar (gauge_higgs_hmc.hpp:38, 61-104)
+void orc_set_frozen(void* h, int on) { static_cast<Base*>(h)->set_frozen(on); }
+void orc_normalize_phi(void* h) { static_cast<Base*>(h)->normalize_phi(); }
 void orc_refresh(void* h) { static_cast<Base*>(h)->refresh(); }
 void orc_get(void* h, int which, double* out) { static_cast<Base*>(h)->get(which, out); }
 void orc_set(void* h, int which, const double* in) { static_cast<Base*>(h)->set(which, in); }

@@ -150,13 +150,14 @@ static void dump_pure(const std::string& tag, int L, double beta, int nmd) {
 // Gauge+Higgs case: fields, forces, actions, one kick/drift, trajectories.
 template <int D, int N>
 static void dump_higgs(const std::string& tag, const Representation<N>& rep, int L, double beta,
-                       double kappa, double lambda, int nmd, bool with_traj) {
+                       double kappa, double lambda, int nmd, bool with_traj, bool frozen = false) {
   std::array<int, D> ext{};
   ext.fill(L);
   GaugeHiggsHMC<D, N> h(ext, rep, 1);
   h.beta = beta; h.kappa = kappa; h.lambda = lambda; h.tau = 1.0; h.nmd = nmd;
   h.U.hot(h.rng, 0.8);
   h.phi.gaussian(h.rng, 12345, rep.real, 0.3);
+  if (frozen) { h.frozen_phi = true; h.normalize_phi(); }  // |phi_x| = 1 scalar
   put_array(tag + "_U0", flat_U(h.U));
   put_array(tag + "_phi0", flat_phi(h.phi));
   // forces on the start configuration (as in kick: zero, gauge, matter; scalar force)
@@ -185,6 +186,7 @@ static void dump_higgs(const std::string& tag, const Representation<N>& rep, int
   h.drift(0.05);
   put_array(tag + "_U_drift", flat_U(h.U));
   put_array(tag + "_phi_drift", flat_phi(h.phi));
+  if (frozen) put_array(tag + "_pi_drift", flat_phi(h.pi));  // the geodesic drift also turns pi
   h.U.u = Usave; h.phi.data = phisave; h.P.p = Psave; h.pi.data = pisave;
   if (!with_traj) return;
   for (int integ = 0; integ < 2; ++integ) {
@@ -246,6 +248,8 @@ static int dump(const char* dir) {
   dump_pure<4, 2>("su2_pure", 4, 2.3, 20);
   dump_pure<4, 3>("su3_pure", 4, 5.7, 12);
   { FundamentalRep<2> f; dump_higgs<4, 2>("su2_fund", f, 4, 2.3, 0.265, 0.5, 20, true); }
+  { FundamentalRep<2> f; dump_higgs<4, 2>("su2_fund_frozen", f, 4, 2.3, 0.6, 0.5, 20, true, true); }
+  { AdjointRep<2> a; dump_higgs<3, 2>("su2_adj_frozen", a, 4, 2.3, 0.5, 0.5, 12, true, true); }
   { AdjointRep<2> a; dump_higgs<3, 2>("su2_adj", a, 4, 2.3, 0.3, 0.5, 12, true); }
   { FundamentalRep<3> f; dump_higgs<3, 3>("su3_fund", f, 4, 5.7, 0.2, 0.5, 12, true); }
   {

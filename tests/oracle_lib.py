@@ -32,6 +32,7 @@ def lib():
             "orc_set_couplings": (None, [vp, d, d, d, d, i, i, i]),
             "orc_hot": (None, [vp, d]), "orc_cold": (None, [vp]),
             "orc_phi_gaussian": (None, [vp, u64, d]), "orc_phi_cold": (None, [vp, d]),
+            "orc_set_frozen": (None, [vp, C.c_int]), "orc_normalize_phi": (None, [vp]),
             "orc_refresh": (None, [vp]),
             "orc_get": (None, [vp, i, dp]), "orc_set": (None, [vp, i, dp]),
             "orc_forces": (None, [vp]), "orc_observables": (None, [vp, dp]), "orc_ham_parts": (None, [vp, dp]),
@@ -115,6 +116,13 @@ class Oracle:
 
     def phi_cold(self, c=1.0):
         lib().orc_phi_cold(self.h, c)
+
+    def frozen(self, on=True):
+        """|phi_x| = 1 scalar: tangent projections and geodesic drift (gauge_higgs_hmc.hpp:61-104)."""
+        lib().orc_set_frozen(self.h, int(on))
+
+    def normalize_phi(self):
+        lib().orc_normalize_phi(self.h)
 
     def refresh(self):
         lib().orc_refresh(self.h)

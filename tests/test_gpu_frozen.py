@@ -1,0 +1,83 @@
+"""Frozen-length scalar |phi_x| = 1 on the GPU (GaugeHiggsHMC::frozen_phi,
+gauge_higgs_hmc.hpp:61-104, 133, 144, 172) against the reference's own dumps:
+normalisation, tangent-projected momenta, kick + projection, geodesic drift,
+and whole trajectories of both integrators."""
+import numpy as np
+import pytest
+
+import gauge_higgs_amd as gha
+
+pytestmark = pytest.mark.gpu
+
+CASES = [("su2_fund_frozen", 4, "fund", 0.6, 20), ("su2_adj_frozen", 3, "adj", 0.5, 12)]
+
+
+def rel(a, b):
+    a = np.asarray(a)
+    a = a.view(np.float64).ravel() if np.iscomplexobj(a) else a.ravel()
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def make(D, rep, kappa, nmd, integ, prec=gha.FP64):
+    g = gha.GaugeHiggsHMC(4, rep, seed=1, D=D, N=2, precision=prec)
+    g.configure(beta=2.3, kappa=kappa, lam=0.5, tau=1.0, nmd=nmd, integ=integ)
+    g.hot(0.8, 12345, 0.3)
+    g.frozen_phi = True
+    g.normalize_phi()
+    return g
+
+
+@pytest.mark.parametrize("tag,D,rep,kappa,nmd", CASES, ids=[c[0] for c in CASES])
+def test_frozen_against_reference(goldens, tag, D, rep, kappa, nmd):
+    arr, sc = goldens
+    g = make(D, rep, kappa, nmd, gha.LEAPFROG)
+    assert rel(g.get("phi"), arr[tag + "_phi0"]) < 1e-15
+    g.refresh_momenta()
+    assert rel(g.get("pi"), arr[tag + "_pi0"]) < 1e-15
+    g.kick(0.05)
+    assert rel(g.get("pi"), arr[tag + "_pi_kick"]) < 1e-12
+    g.drift(0.05)
+    assert rel(g.get("phi"), arr[tag + "_phi_drift"]) < 1e-12
+    assert rel(g.get("pi"), arr[tag + "_pi_drift"]) < 1e-12
+    for integ, name in ((gha.LEAPFROG, "lf"), (gha.OMELYAN, "omel")):
+        g = make(D, rep, kappa, nmd, integ)
+        dh, acc = [], []
+        for _ in range(2):
+            acc.append(float(g.trajectory()))
+            dh.append(g.last_dH)
+        assert np.max(np.abs(np.array(dh) - arr[f"{tag}_{name}_dH"])) < 1e-9
+        assert np.array_equal(np.array(acc), arr[f"{tag}_{name}_acc"])
+        assert rel(g.get("U"), arr[f"{tag}_{name}_U2"]) < 1e-10
+        assert rel(g.get("phi"), arr[f"{tag}_{name}_phi2"]) < 1e-10
+
+
+@pytest.mark.parametrize("prec,tol", [(gha.FP64, 1e-13), (gha.FP32MD, 2e-6)])
+def test_frozen_length_preserved(prec, tol):
+    g = make(4, "fund", 0.6, 10, gha.OMELYAN, prec)
+    g.lat.trajectories(5, use_graph=True)
+    assert np.max(np.abs(np.linalg.norm(g.get("phi"), axis=1) - 1.0)) < 5 * tol
+    g.refresh_momenta()
+    phi, pi = g.get("phi"), g.get("pi")
+    assert np.max(np.abs(np.einsum("sk,sk->s", phi.conj(), pi).real)) < 10 * tol
+
+
+def test_frozen_reversible_and_nested():
+    for nested in (False, True):
+        g = make(4, "fund", 0.6, 8, gha.LEAPFROG)
+        g.configure(reunit_each_traj=False)
+        if nested:
+            g.lat.set_nested(4, 3)
+        g.refresh_momenta()
+        U0, phi0 = g.get("U"), g.get("phi")
+        g.md_evolve()
+        g.set("P", -g.get("P"))
+        g.set("pi", -g.get("pi"))
+        g.md_evolve()
+        assert np.max(np.abs(g.get("U") - U0)) < 1e-10
+        assert np.max(np.abs(g.get("phi") - phi0)) < 1e-10
+
+
+def test_frozen_refused_for_large_irreps():
+    g = gha.GaugeHiggsHMC(3, "6,1", seed=1, D=4, N=3)
+    with pytest.raises(gha.GhxError):
+        g.frozen_phi = True

@@ -156,3 +156,37 @@ def test_oracle_rep_matches_reference(goldens, tag, N, spec):
         assert rel(o.rep_apply(U[t], py[t]), rot[t]) < 1e-13
         assert rel(o.rep_apply(U[t], py[t], dag=True), rotd[t]) < 1e-13
         assert rel(o.rep_hop_g(U[t], px[t], py[t]), hop[t]) < 1e-12
+
+
+@pytest.mark.parametrize("tag,D,rep,kappa,nmd", [("su2_fund_frozen", 4, "fund", 0.6, 20), ("su2_adj_frozen", 3, "adj", 0.5, 12)])
+def test_frozen_scalar_matches_reference(goldens, tag, D, rep, kappa, nmd):
+    """|phi_x| = 1 scalar (gauge_higgs_hmc.hpp:61-104, 133, 144, 172): normalisation,
+    tangent-projected momenta and kicks, geodesic drift, trajectories."""
+    g, sc = goldens
+    o = O.Oracle(D, 2, 4, rep, seed=1)
+    o.couplings(beta=2.3, kappa=kappa, lam=0.5, tau=1.0, nmd=nmd, omelyan=False)
+    o.hot(0.8)
+    o.phi_gaussian(12345, 0.3)
+    o.frozen(True)
+    o.normalize_phi()
+    assert rel(o.get("phi"), g[tag + "_phi0"]) < 1e-15
+    assert np.allclose(np.linalg.norm(o.get("phi"), axis=1), 1.0, atol=1e-14)
+    o.refresh()
+    assert rel(o.get("pi"), g[tag + "_pi0"]) < 1e-15
+    o.kick(0.05)
+    assert rel(o.get("pi"), g[tag + "_pi_kick"]) < 1e-14
+    o.drift(0.05)
+    assert rel(o.get("phi"), g[tag + "_phi_drift"]) < 1e-14
+    assert rel(o.get("pi"), g[tag + "_pi_drift"]) < 1e-14
+    for integ in ("lf", "omel"):
+        o = O.Oracle(D, 2, 4, rep, seed=1)
+        o.couplings(beta=2.3, kappa=kappa, lam=0.5, tau=1.0, nmd=nmd, omelyan=(integ == "omel"))
+        o.hot(0.8)
+        o.phi_gaussian(12345, 0.3)
+        o.frozen(True)
+        o.normalize_phi()
+        res = [o.trajectory() for _ in range(2)]
+        assert np.max(np.abs(np.array([r[1] for r in res]) - g[f"{tag}_{integ}_dH"])) < 1e-10
+        assert np.array_equal(np.array([float(r[0]) for r in res]), g[f"{tag}_{integ}_acc"])
+        assert rel(o.get("U"), g[f"{tag}_{integ}_U2"]) < 1e-13
+        assert rel(o.get("phi"), g[f"{tag}_{integ}_phi2"]) < 1e-13
