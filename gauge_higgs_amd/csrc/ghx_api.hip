@@ -1093,4 +1093,113 @@ int64_t ghx_volume(const ghx_ctx* ctx) {
   return v;
 }
 
+
+// ---- checkpoint / resume (roadmap item of the reference, README.md:126-127) ----
+// Binary file, little-endian: magic "GHXCKPT1", then int32 {version, D, N,
+// n_replicas, precision, d, frozen}, int32 extents[n_replicas*D], per replica
+// {u64 seed, traj_count, accept_count; f64 last_dH, beta, kappa, lambda, tau},
+// u64 {swap seed, swap counter}, then per replica the links (reference AoS
+// [(s*D+mu)*N*N + i*N + j] complex, FP64) and phi ([s*d+k] complex, FP64).
+// Momenta are not stored: every trajectory refreshes them from its counter.
+namespace {
+constexpr char kCkptMagic[8] = {'G', 'H', 'X', 'C', 'K', 'P', 'T', '1'};
+struct CkptRep {
+  uint64_t seed, traj, acc;
+  double last_dH, beta, kappa, lambda, tau;
+};
+}  // namespace
+
+int ghx_save_checkpoint(ghx_ctx* ctx, const char* path) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (!path) return GHX_E_INVALID;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(ctx, GHX_E_INVALID, std::string("cannot open ") + path);
+  bool ok = std::fwrite(kCkptMagic, 1, 8, f) == 8;
+  const int32_t hdr[7] = {1, ctx->D, ctx->N, ctx->nrep, ctx->prec, ctx->rep.d, ctx->frozen ? 1 : 0};
+  ok = ok && std::fwrite(hdr, sizeof hdr, 1, f) == 1;
+  for (int r = 0; r < ctx->nrep && ok; ++r) {
+    int32_t L[4] = {0, 0, 0, 0};
+    for (int m = 0; m < ctx->D; ++m) L[m] = ctx->hreps[r].L[m];
+    ok = std::fwrite(L, sizeof(int32_t), ctx->D, f) == size_t(ctx->D);
+  }
+  for (int r = 0; r < ctx->nrep && ok; ++r) {
+    McState m{};
+    HIPCHK(hipMemcpy(&m, ctx->mc + r, sizeof m, hipMemcpyDeviceToHost));
+    const RepDesc& rd = ctx->hreps[r];
+    const CkptRep c{rd.seed, m.traj_count, m.accept_count, m.last_dH, rd.beta, rd.kappa, rd.lambda, rd.tau};
+    ok = std::fwrite(&c, sizeof c, 1, f) == 1;
+  }
+  SwapState sw{0, 0};
+  if (ctx->dswap) HIPCHK(hipMemcpy(&sw, ctx->dswap, sizeof sw, hipMemcpyDeviceToHost));
+  ok = ok && std::fwrite(&sw, sizeof sw, 1, f) == 1;
+  for (int r = 0; r < ctx->nrep && ok; ++r) {
+    const size_t nl = size_t(ctx->hreps[r].vol) * ctx->D * ctx->N * ctx->N * 2;
+    std::vector<double> U(nl);
+    if ((rc = get_links(ctx, r, U.data()))) { std::fclose(f); return rc; }
+    ok = std::fwrite(U.data(), 8, nl, f) == nl;
+    if (ok && has_phi(ctx)) {
+      const size_t np = size_t(ctx->hreps[r].vol) * ctx->nphi();
+      std::vector<double> ph(np);
+      if ((rc = get_planes(ctx, ctx->phi[ctx->cur], int(ctx->nphi()), r, ph.data()))) { std::fclose(f); return rc; }
+      ok = std::fwrite(ph.data(), 8, np, f) == np;
+    }
+  }
+  ok = (std::fclose(f) == 0) && ok;
+  return ok ? GHX_OK : fail(ctx, GHX_E_INVALID, std::string("write failed: ") + path);
+}
+
+int ghx_load_checkpoint(ghx_ctx* ctx, const char* path) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (!path) return GHX_E_INVALID;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(ctx, GHX_E_INVALID, std::string("cannot open ") + path);
+  auto bad = [&](const std::string& why) { std::fclose(f); return fail(ctx, GHX_E_INVALID, "checkpoint " + std::string(path) + ": " + why); };
+  char magic[8];
+  int32_t hdr[7];
+  if (std::fread(magic, 1, 8, f) != 8 || std::memcmp(magic, kCkptMagic, 8) != 0) return bad("not a ghx checkpoint");
+  if (std::fread(hdr, sizeof hdr, 1, f) != 1 || hdr[0] != 1) return bad("unknown version");
+  if (hdr[1] != ctx->D || hdr[2] != ctx->N || hdr[3] != ctx->nrep || hdr[5] != ctx->rep.d)
+    return bad("lattice dimension, colours, replica count or scalar dimension differ from this context");
+  for (int r = 0; r < ctx->nrep; ++r) {
+    int32_t L[4] = {0, 0, 0, 0};
+    if (std::fread(L, sizeof(int32_t), ctx->D, f) != size_t(ctx->D)) return bad("truncated");
+    for (int m = 0; m < ctx->D; ++m)
+      if (L[m] != ctx->hreps[r].L[m]) return bad("extents differ from this context");
+  }
+  std::vector<CkptRep> reps(ctx->nrep);
+  if (std::fread(reps.data(), sizeof(CkptRep), ctx->nrep, f) != size_t(ctx->nrep)) return bad("truncated");
+  SwapState sw{0, 0};
+  if (std::fread(&sw, sizeof sw, 1, f) != 1) return bad("truncated");
+  std::vector<std::vector<double>> Us(ctx->nrep), phs(ctx->nrep);
+  for (int r = 0; r < ctx->nrep; ++r) {
+    Us[r].resize(size_t(ctx->hreps[r].vol) * ctx->D * ctx->N * ctx->N * 2);
+    if (std::fread(Us[r].data(), 8, Us[r].size(), f) != Us[r].size()) return bad("truncated");
+    if (has_phi(ctx)) {
+      phs[r].resize(size_t(ctx->hreps[r].vol) * ctx->nphi());
+      if (std::fread(phs[r].data(), 8, phs[r].size(), f) != phs[r].size()) return bad("truncated");
+    }
+  }
+  std::fclose(f);
+  if (ctx->cur != 0) launch_settle(ctx, false);
+  for (int r = 0; r < ctx->nrep; ++r) {
+    const CkptRep& c = reps[r];
+    if ((rc = ghx_set_couplings(ctx, r, c.beta, c.kappa, c.lambda, c.tau))) return rc;
+    if ((rc = ghx_set_seed(ctx, r, c.seed))) return rc;
+    McState m{};
+    m.traj_count = c.traj;
+    m.accept_count = c.acc;
+    m.last_dH = c.last_dH;
+    HIPCHK(hipMemcpy(ctx->mc + r, &m, sizeof m, hipMemcpyHostToDevice));
+    if ((rc = set_links(ctx, r, Us[r].data()))) return rc;
+    if (has_phi(ctx) && (rc = set_planes(ctx, ctx->phi[ctx->cur], int(ctx->nphi()), r, phs[r].data()))) return rc;
+  }
+  if (ctx->dswap) HIPCHK(hipMemcpy(ctx->dswap, &sw, sizeof sw, hipMemcpyHostToDevice));
+  if (hdr[6] && !ctx->frozen && (rc = ghx_set_frozen_phi(ctx, 1))) return rc;
+  drop_graph(ctx);
+  return GHX_OK;
+}
+
 }  // extern "C"

@@ -87,6 +87,13 @@ class Lattices:
     def normalize_phi(self):
         self._check(self._lib.ghx_normalize_phi(self.h))
 
+    def save_checkpoint(self, path):
+        """Links, phi, couplings, seeds and counters of every replica (reference AoS, FP64)."""
+        self._check(self._lib.ghx_save_checkpoint(self.h, os.fsencode(path)))
+
+    def load_checkpoint(self, path):
+        self._check(self._lib.ghx_load_checkpoint(self.h, os.fsencode(path)))
+
     def set_seed(self, replica, seed):
         self._check(self._lib.ghx_set_seed(self.h, replica, seed))
 

@@ -170,6 +170,14 @@ int ghx_surrogate_train(ghx_ctx* ctx, int iters, double lr, double* loss_first, 
  * force F (rows*(N*N-1)) of one replica, rows in the reference link order. */
 int ghx_surrogate_snapshot(ghx_ctx* ctx, int replica, float* X, float* B, double* S, double* F);
 
+/* Checkpoint / resume of every replica (roadmap item, README.md:126-127): links
+ * and phi in the reference AoS layout (FP64), couplings, seeds, trajectory and
+ * accept counters, last dH, the swap-coin counter and the frozen flag, in a
+ * versioned binary file. Loading checks D, N, the replica count, extents and d
+ * against the context and returns GHX_E_INVALID on any mismatch. */
+int ghx_save_checkpoint(ghx_ctx* ctx, const char* path);
+int ghx_load_checkpoint(ghx_ctx* ctx, const char* path);
+
 int ghx_sync(ghx_ctx* ctx);
 /* Time `steps` fused MD steps (kick+drift launches) with HIP events on the
  * context's stream: total ms and the number of launches timed. */
