@@ -10,6 +10,8 @@ Prints one JSON line: metric/value (site-updates/s), roofline of the MD-step
 kernel (algorithmic bytes per launch / HIP-event launch time against 8 TB/s),
 the CPU baseline (the reference compiled from its sources, or the FP64 oracle
 restatement, timed on this host on a 2-step sample of the same lattice), and
+"precision": FP32-MD against FP64 on SU(2)+fundamental 16^4 (forces, one
+trajectory's dH and observables) and both MD throughputs;
 "replicas": 32 SU(2)+fundamental chains (kappa 0.20-0.32, L 6-12) advanced by one
 launch per MD step, their throughput and the measured L_link susceptibility peaks;
 "learned_force": GPU seconds per accepted trajectory of exact-force HMC against
@@ -124,6 +126,48 @@ def replica_block(n_therm, n_meas):
     return out
 
 
+def precision_block():
+    """SU(2)+fundamental, 16^4, beta=2.3, kappa=0.265, lambda=0.5, nmd=20 (Omelyan):
+    FP32-MD against FP64 from the same start and momenta — forces, one trajectory's
+    dH and the observables after it — plus the MD throughput of both."""
+    import gauge_higgs_amd as gha
+    out = {"workload": "su2_fund_L16", "sites": 16 ** 4}
+    runs = {}
+    for name, prec in (("fp64", gha.FP64), ("fp32", gha.FP32MD)):
+        g = gha.GaugeHiggsHMC(16, "fund", seed=1, D=4, N=2, precision=prec)
+        g.configure(beta=2.3, kappa=0.265, lam=0.5, nmd=20, reunit_each_traj=True)
+        g.hot(0.8, 12345, 1.0)
+        runs[name] = g
+    a, b = runs["fp64"], runs["fp32"]
+    b.set("U", a.get("U"))
+    b.set("phi", a.get("phi"))
+    Fa, Fpa = a.forces()
+    Fb, Fpb = b.forces()
+    out["force_rel_err"] = float(np.max(np.abs(Fb - Fa)) / np.max(np.abs(Fa)))
+    out["scalar_force_rel_err"] = float(np.max(np.abs(Fpb - Fpa)) / np.max(np.abs(Fpa)))
+    a.refresh_momenta()
+    b.set("P", a.get("P"))
+    b.set("pi", a.get("pi"))
+    for g in (a, b):
+        g.lat.set_integrator(20, gha.OMELYAN, True)
+    ha, hb = a.hamiltonian(), b.hamiltonian()
+    a.md_evolve()
+    b.md_evolve()
+    a.lat.reunitarize()
+    b.lat.reunitarize()
+    dHa, dHb = a.hamiltonian() - ha, b.hamiltonian() - hb
+    oa, ob = a.observables(), b.observables()
+    out["one_trajectory"] = {"dH_fp64": dHa, "dH_fp32": dHb, "dH_abs_err": abs(dHb - dHa), "H_fp64": ha,
+                             **{f"{k}_abs_err": abs(ob[k] - oa[k]) for k in ("plaq", "Llink", "Lphi")},
+                             **{f"{k}_fp64": oa[k] for k in ("plaq", "Llink", "Lphi")}}
+    for name, g in runs.items():
+        g.lat.time_md_steps(5)
+        ms = g.lat.time_md_steps(100)
+        out[f"md_site_updates_per_s_{name}"] = 16 ** 4 * 100 / (ms / 1e3)
+        g.lat.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,6 +182,7 @@ def main():
     ap.add_argument("--learned-meas", type=int, default=24, help="measured trajectories per chain in the comparison")
     ap.add_argument("--no-graph", action="store_true", help="launch trajectories directly instead of replaying hipGraphs")
     ap.add_argument("--no-replicas", action="store_true", help="skip the 32-chain kappa-scan block")
+    ap.add_argument("--no-precision", action="store_true", help="skip the FP32-vs-FP64 block (SU(2)+fund 16^4)")
     ap.add_argument("--replica-meas", type=int, default=1000, help="measured trajectory cycles of the kappa scan")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (calibrates the HBM byte counters)")
@@ -211,6 +256,8 @@ def main():
     g.lat.close()
     if not args.no_replicas:
         line["replicas"] = replica_block(200, args.replica_meas)
+    if not args.no_precision:
+        line["precision"] = precision_block()
     if not args.no_learned:
         from gauge_higgs_amd import learned
         line["learned_force"] = learned.compare(n_meas=args.learned_meas, use_graph=not args.no_graph)
