@@ -55,31 +55,33 @@ struct TEntry {
   double re, im;
 };
 
-// Q^a_x = phi_x^dag T^a_R phi_x (real), one thread per site.
-template <typename T, int N>
-__global__ void surr_charge_kernel(const RepDesc* reps, const int* blk_rep, unsigned Vp, const T* phi, const TEntry* te, int nte, float* Q) {
-  constexpr int NGN = N * N - 1;
+// Q^a_x = phi_x^dag T^a_R phi_x (real): one thread per (site, a); te holds the
+// nonzero T^a_R entries sorted by a, entries [te_off[a], te_off[a+1]).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void surr_charge_kernel(const RepDesc* reps, const int* blk_rep, unsigned Vp, const T* phi,
+                                                             const TEntry* te, const int* te_off, float* Q) {
   const int r = blk_rep[blockIdx.x];
+  const int a = blockIdx.y;
   const RepDesc& rd = reps[r];
   const unsigned s = blockIdx.x * kBlock + threadIdx.x - rd.offset;
   if (s >= rd.vol) return;
   const unsigned g = rd.offset + s;
-  float q[NGN];
-  for (int a = 0; a < NGN; ++a) q[a] = 0.f;
-  for (int e = 0; e < nte; ++e) {
+  float q = 0.f;
+  for (int e = te_off[a]; e < te_off[a + 1]; ++e) {
     const TEntry t = te[e];
     const float xr = float(phi[size_t(2 * t.l) * Vp + g]), xi = float(phi[size_t(2 * t.l + 1) * Vp + g]);
     const float yr = float(phi[size_t(2 * t.m) * Vp + g]), yi = float(phi[size_t(2 * t.m + 1) * Vp + g]);
     // Re( conj(x) (tr + i ti) y )
     const float zr = float(t.re) * yr - float(t.im) * yi, zi = float(t.re) * yi + float(t.im) * yr;
-    q[t.a] += xr * zr + xi * zi;
+    q += xr * zr + xi * zi;
   }
-  for (int a = 0; a < NGN; ++a) Q[size_t(a) * Vp + g] = q[a];
+  Q[size_t(a) * Vp + g] = q;
 }
 
-// Features X[row][16] and bases B[row][4][N^2-1] of link row = mu*Vp + g.
+// Features X[row][16] and bases B[row][4][N^2-1] of link row = mu*Vp + g;
+// one thread per link (blockIdx.y = mu).
 template <typename T, int N, int D>
-__global__ void surr_feature_kernel(const RepDesc* reps, const int* blk_rep, unsigned Vp, const T* U, const T* phi, int d, const float* Q,
+__global__ __launch_bounds__(kBlock) void surr_feature_kernel(const RepDesc* reps, const int* blk_rep, unsigned Vp, const T* U, const T* phi, int d, const float* Q,
                                     float* X, float* B) {
   using R = float;
   constexpr int NGN = N * N - 1;
@@ -94,7 +96,8 @@ __global__ void surr_feature_kernel(const RepDesc* reps, const int* blk_rep, uns
   cx<R> qx[NGN];
   for (int a = 0; a < NGN; ++a) qx[a] = {Q[size_t(a) * Vp + g], 0.f};
   const Mat<R, N> Qx = calg_to_mat<R, N>(qx);
-  for (int mu = 0; mu < D; ++mu) {
+  {
+    const int mu = blockIdx.y;
     const size_t row = size_t(mu) * Vp + g;
     float* x = X + row * kNF;
     float* b = B + row * kNB * NGN;
@@ -374,6 +377,7 @@ struct SurrogateState {
   double *lpart = nullptr, *loss = nullptr;
   float *X = nullptr, *B = nullptr, *Q = nullptr;
   TEntry* te = nullptr;
+  int* te_off = nullptr;
   int nte = 0, nwaves = 0, nblocks = 0;
   long step = 0;
   size_t nrows = 0;
@@ -386,9 +390,9 @@ using namespace ghx;
 template <typename T, int N, int D>
 void feats_T(ghx_ctx* c, int b) {
   SurrogateState* st = c->surr_state;
-  hipLaunchKernelGGL((surr_charge_kernel<T, N>), dim3(c->nblk), dim3(kBlock), 0, c->stream, c->dreps, c->dblk_rep, c->Vp,
-                     (const T*)c->phi[b], st->te, st->nte, st->Q);
-  hipLaunchKernelGGL((surr_feature_kernel<T, N, D>), dim3(c->nblk), dim3(kBlock), 0, c->stream, c->dreps, c->dblk_rep, c->Vp,
+  hipLaunchKernelGGL((surr_charge_kernel<T>), dim3(c->nblk, N * N - 1), dim3(kBlock), 0, c->stream, c->dreps, c->dblk_rep, c->Vp,
+                     (const T*)c->phi[b], st->te, st->te_off, st->Q);
+  hipLaunchKernelGGL((surr_feature_kernel<T, N, D>), dim3(c->nblk, D), dim3(kBlock), 0, c->stream, c->dreps, c->dblk_rep, c->Vp,
                      (const T*)c->U[b], (const T*)c->phi[b], c->rep.d, st->Q, st->X, st->B);
 }
 void launch_features(ghx_ctx* c, int b) {
@@ -446,7 +450,7 @@ void surrogate_free(ghx_ctx* c) {
   SurrogateState* st = c->surr_state;
   if (!st) return;
   for (void* p : {(void*)st->P, (void*)st->m, (void*)st->v, (void*)st->grad, (void*)st->part, (void*)st->lpart, (void*)st->loss,
-                  (void*)st->X, (void*)st->B, (void*)st->Q, (void*)st->te})
+                  (void*)st->X, (void*)st->B, (void*)st->Q, (void*)st->te, (void*)st->te_off})
     if (p) (void)hipFree(p);
   if (c->S) { (void)hipFree(c->S); c->S = nullptr; }
   delete st;
@@ -502,8 +506,13 @@ int ghx_surrogate_init(ghx_ctx* ctx, uint64_t seed) {
         if (std::abs(z) > 1e-14) te.push_back(TEntry{a, l, m, z.real(), z.imag()});
       }
   st->nte = int(te.size());
+  std::vector<int> off(ngn + 1, 0);
+  for (const TEntry& t : te) ++off[t.a + 1];
+  for (int a = 0; a < ngn; ++a) off[a + 1] += off[a];
   SCHK(hipMalloc((void**)&st->te, std::max<size_t>(te.size(), 1) * sizeof(TEntry)));
   SCHK(hipMemcpy(st->te, te.data(), te.size() * sizeof(TEntry), hipMemcpyHostToDevice));
+  SCHK(hipMalloc((void**)&st->te_off, off.size() * sizeof(int)));
+  SCHK(hipMemcpy(st->te_off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
   // initial weights: N(0, 1/fan_in) from the counter RNG, W3 small, biases zero
   std::vector<float> P(NPARAM, 0.f);
   auto gauss = [&](uint64_t k) {
