@@ -102,13 +102,13 @@ def cpu_baseline(workload, sample_steps=16):
             "sample": f"FP64 oracle restatement, {L}^4, {sample_steps} kick+drift MD steps ({sec:.1f} s)"}
 
 
-def replica_block(n_therm, n_meas):
+def replica_block(n_therm, n_meas, use_graph=True):
     """32 SU(2)+fundamental chains, kappa in [0.20, 0.32], L in {6, 8, 10, 12}, FP32 MD,
     one launch per MD step for all chains, graph-replayed trajectories and kappa
     swaps every trajectory: measured L_link susceptibility peaks and throughput."""
     from gauge_higgs_amd import replicas
     rs = replicas.kappa_scan()
-    m = replicas.measure_scan(rs, n_therm=n_therm, n_meas=n_meas)
+    m = replicas.measure_scan(rs, n_therm=n_therm, n_meas=n_meas, use_graph=use_graph)
     vtot = int(sum(rs.lat.vols))
     steps = 200
     ms = rs.lat.time_md_steps(steps)
@@ -255,7 +255,7 @@ def main():
     }
     g.lat.close()
     if not args.no_replicas:
-        line["replicas"] = replica_block(200, args.replica_meas)
+        line["replicas"] = replica_block(200, args.replica_meas, use_graph=not args.no_graph)
     if not args.no_precision:
         line["precision"] = precision_block()
     if not args.no_learned:
