@@ -607,6 +607,111 @@ Rep<N> make_rep(std::string spec) {
 // HMC state and sweeps — fields.hpp, scalar_field.hpp, gauge_wilson.hpp,
 // scalar_higgs.hpp, hmc.hpp, gauge_higgs_hmc.hpp, observables.hpp
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Multi-invariant quartic potential — scalar_invariants.hpp:20-208.
+// V = -mu2 phi^dag phi + sum_c f_c ||P_c(M)||^2, M = phi phi^dag, P_c the
+// eigenprojector of Chat(M) = sum_a [T^a,[T^a,M]] on channel c (a Lagrange
+// polynomial in Chat); dV/dphi^* = -mu2 phi + sum_c 2 f_c P_c(M) phi.
+// The channel eigenvalues are found here by diagonalising Chat on the real
+// space of Hermitian d x d matrices with cyclic Jacobi (the reference runs
+// Lanczos from a random Hermitian start; both return the distinct C2's).
+// ---------------------------------------------------------------------------
+inline double re_tr_dag(const DM& A, const DM& B) {  // linalg.hpp:252-256
+  double s = 0.0;
+  for (std::size_t i = 0; i < A.a.size(); ++i) s += (std::conj(A.a[i]) * B.a[i]).real();
+  return s;
+}
+inline std::vector<double> jacobi_eigenvalues(std::vector<double> A, int n) {
+  auto at = [&](int i, int j) -> double& { return A[std::size_t(i) * n + j]; };
+  for (int sweep = 0; sweep < 200; ++sweep) {
+    double off = 0.0;
+    for (int p = 0; p < n; ++p) for (int q = p + 1; q < n; ++q) off += at(p, q) * at(p, q);
+    if (off < 1e-30) break;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        if (std::fabs(at(p, q)) < 1e-300) continue;
+        const double th = (at(q, q) - at(p, p)) / (2.0 * at(p, q));
+        const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+        for (int k = 0; k < n; ++k) { const double a = at(k, p), b = at(k, q); at(k, p) = c * a - sn * b; at(k, q) = sn * a + c * b; }
+        for (int k = 0; k < n; ++k) { const double a = at(p, k), b = at(q, k); at(p, k) = c * a - sn * b; at(q, k) = sn * a + c * b; }
+      }
+  }
+  std::vector<double> ev(n);
+  for (int i = 0; i < n; ++i) ev[i] = at(i, i);
+  return ev;
+}
+
+template <int N>
+struct Channels {
+  const Rep<N>* rep;
+  int d;
+  std::vector<double> lambda;
+  explicit Channels(const Rep<N>& R) : rep(&R), d(R.d) {
+    // orthonormal basis of Hermitian d x d matrices (real inner product Re Tr(A^dag B))
+    std::vector<DM> B;
+    for (int i = 0; i < d; ++i) { DM e(d, d); e(i, i) = 1.0; B.push_back(e); }
+    const double h = 1.0 / std::sqrt(2.0);
+    for (int i = 0; i < d; ++i)
+      for (int j = i + 1; j < d; ++j) {
+        DM e(d, d); e(i, j) = h; e(j, i) = h; B.push_back(e);
+        DM f(d, d); f(i, j) = cd(0, -h); f(j, i) = cd(0, h); B.push_back(f);
+      }
+    const int n = int(B.size());
+    std::vector<double> C(std::size_t(n) * n);
+    for (int q = 0; q < n; ++q) {
+      const DM CB = casimir(B[q]);
+      for (int p = 0; p < n; ++p) C[std::size_t(p) * n + q] = re_tr_dag(B[p], CB);
+    }
+    std::vector<double> ev = jacobi_eigenvalues(C, n);
+    std::sort(ev.begin(), ev.end());
+    for (double e : ev) if (lambda.empty() || std::fabs(e - lambda.back()) > 1e-4) lambda.push_back(e);  // as scalar_invariants.hpp:175-176
+  }
+  int n_channels() const { return int(lambda.size()); }
+  DM casimir(const DM& M) const {  // scalar_invariants.hpp:31-39
+    DM out(d, d);
+    for (const DM& T : rep->T) {
+      const DM c1 = T * M - M * T;
+      out = out + (T * c1 - c1 * T);
+    }
+    return out;
+  }
+  DM proj(int c, const DM& M) const {  // scalar_invariants.hpp:42-50
+    DM X = M;
+    for (int cp = 0; cp < n_channels(); ++cp) {
+      if (cp == c) continue;
+      const DM CX = casimir(X);
+      DM Y(d, d);
+      for (std::size_t i = 0; i < Y.a.size(); ++i) Y.a[i] = (CX.a[i] - X.a[i] * lambda[cp]) * (1.0 / (lambda[c] - lambda[cp]));
+      X = Y;
+    }
+    return X;
+  }
+  static DM outer(const DV& x) {
+    DM M(int(x.size()), int(x.size()));
+    for (std::size_t a = 0; a < x.size(); ++a)
+      for (std::size_t b = 0; b < x.size(); ++b) M(int(a), int(b)) = x[a] * std::conj(x[b]);
+    return M;
+  }
+  double value(const DV& x, const std::vector<double>& f, double mu2) const {  // scalar_invariants.hpp:62-67
+    double v = -mu2 * vnorm2(x);
+    const DM M = outer(x);
+    for (int c = 0; c < n_channels(); ++c) if (f[c] != 0.0) v += f[c] * re_tr_dag(M, proj(c, M));
+    return v;
+  }
+  DV grad(const DV& x, const std::vector<double>& f, double mu2) const {  // scalar_invariants.hpp:70-87
+    const DM M = outer(x);
+    DV g(x.size());
+    for (std::size_t e = 0; e < x.size(); ++e) g[e] = cd(-mu2, 0) * x[e];
+    for (int c = 0; c < n_channels(); ++c) {
+      if (f[c] == 0.0) continue;
+      const DV PMx = matvec(proj(c, M), x);
+      for (std::size_t e = 0; e < x.size(); ++e) g[e] += cd(2.0 * f[c], 0) * PMx[e];
+    }
+    return g;
+  }
+};
+
 template <int N>
 struct Engine {
   Geom g;
@@ -718,8 +823,18 @@ struct Engine {
       }
   }
   // scalar_higgs.hpp:111-125 (QuarticPotential)
-  double pot_value(const DV& x) const { const double n2 = vnorm2(x); return n2 + lambda * (n2 - 1.0) * (n2 - 1.0); }
+  // on-site potential: the quartic default (scalar_higgs.hpp:26-39) or the plugged-in
+  // multi-invariant one (GaugeHiggsHMC::potential, gauge_higgs_hmc.hpp:27)
+  const Channels<N>* multi = nullptr;
+  std::vector<double> multi_f;
+  double multi_mu2 = 0.0;
+  double pot_value(const DV& x) const {
+    if (multi) return multi->value(x, multi_f, multi_mu2);
+    const double n2 = vnorm2(x);
+    return n2 + lambda * (n2 - 1.0) * (n2 - 1.0);
+  }
   DV pot_grad(const DV& x) const {
+    if (multi) return multi->grad(x, multi_f, multi_mu2);
     const double n2 = vnorm2(x);
     const double c = 1.0 + 2.0 * lambda * (n2 - 1.0);
     DV gr(x.size());

@@ -47,6 +47,8 @@ struct Base {
   virtual void rep_matrix(const double* U, double* out) const = 0;
   virtual void set_frozen(int on) = 0;
   virtual void normalize_phi() = 0;
+  virtual int set_multi(const double* f, int nf, double mu2, double* lambda_out) = 0;
+  virtual int multi_eval(const double* phi, double* V, double* dV) const = 0;
 };
 
 template <int N>
@@ -113,6 +115,28 @@ struct Impl : Base {
   }
   void reunitarize() override { e.reunitarize_all(); }
   void set_frozen(int on) override { e.frozen = on != 0; }
+  std::unique_ptr<Channels<N>> ch;
+  // nf < 0: only report the channel count and eigenvalues
+  int set_multi(const double* f, int nf, double mu2, double* lambda_out) override {
+    if (!rep) return -1;
+    if (!ch) ch = std::make_unique<Channels<N>>(*rep);
+    if (lambda_out) for (int c = 0; c < ch->n_channels(); ++c) lambda_out[c] = ch->lambda[c];
+    if (nf < 0) return ch->n_channels();
+    if (nf != ch->n_channels()) return -1;
+    e.multi = ch.get();
+    e.multi_f.assign(f, f + nf);
+    e.multi_mu2 = mu2;
+    return ch->n_channels();
+  }
+  int multi_eval(const double* phi, double* V, double* dV) const override {
+    if (!e.multi) return -1;
+    DV x(rep->d);
+    for (int k = 0; k < rep->d; ++k) x[k] = cd(phi[2 * k], phi[2 * k + 1]);
+    *V = e.pot_value(x);
+    const DV g = e.pot_grad(x);
+    for (int k = 0; k < rep->d; ++k) { dV[2 * k] = g[k].real(); dV[2 * k + 1] = g[k].imag(); }
+    return 0;
+  }
   void normalize_phi() override { e.normalize_phi(); }
   double max_unitarity() const override { return e.max_unitarity_violation(); }
   std::uint64_t& traj_count() override { return e.traj_count; }
@@ -181,6 +205,11 @@ void orc_phi_cold(void* h, double c) { static_cast<Base*>(h)->phi_cold(c); }
 // frozen-length scalar (gauge_higgs_hmc.hpp:38, 61-104)
 void orc_set_frozen(void* h, int on) { static_cast<Base*>(h)->set_frozen(on); }
 void orc_normalize_phi(void* h) { static_cast<Base*>(h)->normalize_phi(); }
+// multi-invariant potential (scalar_invariants.hpp): returns the channel count (or -1)
+int orc_set_multi(void* h, const double* f, int nf, double mu2, double* lambda_out) {
+  return static_cast<Base*>(h)->set_multi(f, nf, mu2, lambda_out);
+}
+int orc_multi_eval(void* h, const double* phi, double* V, double* dV) { return static_cast<Base*>(h)->multi_eval(phi, V, dV); }
 void orc_refresh(void* h) { static_cast<Base*>(h)->refresh(); }
 void orc_get(void* h, int which, double* out) { static_cast<Base*>(h)->get(which, out); }
 void orc_set(void* h, int which, const double* in) { static_cast<Base*>(h)->set(which, in); }

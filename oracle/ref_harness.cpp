@@ -8,6 +8,7 @@
 #include "hmc/hmc.hpp"
 #include "hmc/gauge_higgs_hmc.hpp"
 #include "measure/observables.hpp"
+#include "action/scalar_invariants.hpp"
 #include "rep/rep_fundamental.hpp"
 #include "rep/rep_adjoint.hpp"
 #include "rep/rep_general.hpp"
@@ -205,6 +206,58 @@ static void dump_higgs(const std::string& tag, const Representation<N>& rep, int
   }
 }
 
+// Multi-invariant quartic potential (scalar_invariants.hpp): channels, V and
+// dV/dphi^* on random phi, then the HMC with this potential plugged in.
+template <int D, int N>
+static void dump_multi(const std::string& tag, const Representation<N>& rep, int L, double beta, double kappa,
+                       double mu2, int nmd) {
+  CasimirChannels<N> ch(rep);
+  put_scalar(tag + "_nch", ch.n_channels());
+  put_array(tag + "_lambda", ch.lambda);
+  std::vector<double> f;
+  for (int c = 0; c < ch.n_channels(); ++c) f.push_back(0.2 + 0.15 * c);
+  put_array(tag + "_f", f);
+  MultiInvariantPotential<N> pot(ch, f, mu2);
+  std::mt19937_64 rng(41);
+  std::normal_distribution<double> g(0.0, 0.7);
+  std::vector<double> px, val, dv, dvu;
+  for (int t = 0; t < 3; ++t) {
+    DVec x(rep.d);
+    for (int k = 0; k < rep.d; ++k) x(k) = Complex(g(rng), rep.real ? 0.0 : g(rng));
+    for (int k = 0; k < rep.d; ++k) { px.push_back(x(k).real()); px.push_back(x(k).imag()); }
+    val.push_back(pot.value(x));
+    const DVec a = pot.dV_dphibar(x), b = ch.dV_dphibar(x, f, mu2);
+    for (int k = 0; k < rep.d; ++k) { dv.push_back(a(k).real()); dv.push_back(a(k).imag()); dvu.push_back(b(k).real()); dvu.push_back(b(k).imag()); }
+  }
+  put_array(tag + "_px", px); put_array(tag + "_V", val); put_array(tag + "_dV", dv); put_array(tag + "_dV_uncached", dvu);
+  std::array<int, D> ext{};
+  ext.fill(L);
+  GaugeHiggsHMC<D, N> h(ext, rep, 1);
+  h.beta = beta; h.kappa = kappa; h.tau = 1.0; h.nmd = nmd; h.potential = &pot;
+  h.U.hot(h.rng, 0.8);
+  h.phi.gaussian(h.rng, 12345, rep.real, 0.3);
+  h.Flink.zero();
+  add_gauge_force<D, N>(h.U, beta, h.Flink);
+  add_matter_link_force<D, N>(h.phi, h.U, rep, kappa, h.Flink);
+  scalar_force<D, N>(h.phi, h.U, rep, kappa, pot, h.Fphi);
+  put_array(tag + "_Fphi", flat_phi(h.Fphi));
+  put_scalar(tag + "_SH", scalar_action<D, N>(h.phi, h.U, rep, kappa, pot));
+  h.refresh_momenta();
+  put_scalar(tag + "_H0", h.hamiltonian());
+  const auto Usave = h.U.u; const auto phisave = h.phi.data;
+  for (int integ = 0; integ < 2; ++integ) {
+    h.U.u = Usave; h.phi.data = phisave; h.traj_count = 0; h.accept_count = 0;
+    h.integ = integ ? Integrator::Omelyan2MN : Integrator::Leapfrog;
+    const std::string t = tag + (integ ? "_omel" : "_lf");
+    std::vector<double> dH, acc;
+    for (int k = 0; k < 2; ++k) { acc.push_back(h.trajectory()); dH.push_back(h.last_dH); }
+    put_array(t + "_dH", dH);
+    put_array(t + "_acc", acc);
+    put_array(t + "_U2", flat_U(h.U));
+    put_array(t + "_phi2", flat_phi(h.phi));
+  }
+}
+
 template <int N>
 static void dump_rep(const std::string& tag, const GeneralRep<N>& rep, std::uint64_t seed) {
   put_scalar(tag + "_d", rep.d);
@@ -250,6 +303,9 @@ static int dump(const char* dir) {
   { FundamentalRep<2> f; dump_higgs<4, 2>("su2_fund", f, 4, 2.3, 0.265, 0.5, 20, true); }
   { FundamentalRep<2> f; dump_higgs<4, 2>("su2_fund_frozen", f, 4, 2.3, 0.6, 0.5, 20, true, true); }
   { AdjointRep<2> a; dump_higgs<3, 2>("su2_adj_frozen", a, 4, 2.3, 0.5, 0.5, 12, true, true); }
+  { FundamentalRep<2> f; dump_multi<4, 2>("multi_su2_fund", f, 4, 2.3, 0.2, 0.4, 12); }
+  { AdjointRep<2> a; dump_multi<3, 2>("multi_su2_adj", a, 4, 2.3, 0.2, 0.4, 12); }
+  { GeneralRep<3> s2({2}); dump_multi<4, 3>("multi_su3_2", s2, 4, 5.7, 0.1, 0.5, 12); }
   { AdjointRep<2> a; dump_higgs<3, 2>("su2_adj", a, 4, 2.3, 0.3, 0.5, 12, true); }
   { FundamentalRep<3> f; dump_higgs<3, 3>("su3_fund", f, 4, 5.7, 0.2, 0.5, 12, true); }
   {

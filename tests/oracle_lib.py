@@ -33,6 +33,7 @@ def lib():
             "orc_hot": (None, [vp, d]), "orc_cold": (None, [vp]),
             "orc_phi_gaussian": (None, [vp, u64, d]), "orc_phi_cold": (None, [vp, d]),
             "orc_set_frozen": (None, [vp, C.c_int]), "orc_normalize_phi": (None, [vp]),
+            "orc_set_multi": (C.c_int, [vp, dp, C.c_int, d, dp]), "orc_multi_eval": (C.c_int, [vp, dp, dp, dp]),
             "orc_refresh": (None, [vp]),
             "orc_get": (None, [vp, i, dp]), "orc_set": (None, [vp, i, dp]),
             "orc_forces": (None, [vp]), "orc_observables": (None, [vp, dp]), "orc_ham_parts": (None, [vp, dp]),
@@ -123,6 +124,25 @@ class Oracle:
 
     def normalize_phi(self):
         lib().orc_normalize_phi(self.h)
+
+    def channels(self):
+        """Distinct adjoint-Casimir eigenvalues on R x Rbar (scalar_invariants.hpp:20-52)."""
+        lam = np.zeros(64)
+        n = lib().orc_set_multi(self.h, None, -1, 0.0, lam.ctypes.data_as(C.POINTER(C.c_double)))
+        return lam[:n]
+
+    def multi_potential(self, f, mu2):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        n = lib().orc_set_multi(self.h, f.ctypes.data_as(C.POINTER(C.c_double)), len(f), mu2, None)
+        assert n == len(f), "wrong number of channel couplings"
+
+    def multi_eval(self, phi):
+        x = np.ascontiguousarray(np.asarray(phi, dtype=np.complex128)).view(np.float64)
+        V = C.c_double()
+        dV = np.zeros(x.size)
+        dp = C.POINTER(C.c_double)
+        assert lib().orc_multi_eval(self.h, x.ctypes.data_as(dp), C.byref(V), dV.ctypes.data_as(dp)) == 0
+        return V.value, dV.view(np.complex128)
 
     def refresh(self):
         lib().orc_refresh(self.h)
