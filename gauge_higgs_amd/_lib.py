@@ -57,6 +57,8 @@ def load():
         "ghx_get_tuning": [vp, i, C.POINTER(C.c_int)],
         "ghx_autotune": [vp, i, C.POINTER(C.c_int), dp],
         "ghx_set_frozen_phi": [vp, i],
+        "ghx_casimir_channels": [i, C.c_char_p, dp, C.POINTER(C.c_int)],
+        "ghx_set_multi_potential": [vp, dp, i, d],
         "ghx_save_checkpoint": [vp, C.c_char_p],
         "ghx_load_checkpoint": [vp, C.c_char_p],
         "ghx_normalize_phi": [vp],

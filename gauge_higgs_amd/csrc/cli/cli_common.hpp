@@ -1,6 +1,7 @@
 // Shared pieces of the drop-in CLIs: argument helpers, the reference's Stats
 // accumulator (observables.hpp:104-145) and error handling around the C ABI.
 #pragma once
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -44,6 +45,64 @@ struct Stats {  // observables.hpp:104-145
     double v = 0; for (double b : bm) v += (b - mb) * (b - mb);
     v /= (bm.size() - 1);
     return std::sqrt(v / bm.size());
+  }
+};
+
+// Autocorrelation analysis of a Monte Carlo series (autocorr.hpp:53-184):
+// biased autocovariance, Sokal's self-consistent window M >= 6 tau_int(M)
+// with the lag capped at 2000, N_eff = N / (2 tau_int) and the error
+// sqrt(2 tau_int Var / N).
+struct Autocorr {
+  static constexpr double kSokal = 6.0;
+  static constexpr int kLagCap = 2000;
+  static constexpr double kDegen = 1e-24;  // C(0) below kDegen <x^2>: no fluctuations
+  static double mean(const std::vector<double>& x) {
+    double s = 0;
+    for (double v : x) s += v;
+    return x.empty() ? 0.0 : s / x.size();
+  }
+  static double meansq(const std::vector<double>& x) {
+    double s = 0;
+    for (double v : x) s += v * v;
+    return x.empty() ? 0.0 : s / x.size();
+  }
+  static bool degenerate(double c0, double msq) { return c0 <= kDegen * msq + 1e-300; }
+  static std::vector<double> autocov(const std::vector<double>& x, int maxlag) {
+    const long N = long(x.size());
+    if (N <= 0) return {};
+    const long L = (maxlag < 0 || maxlag > N - 1) ? N - 1 : maxlag;
+    const double m = mean(x);
+    std::vector<double> C(L + 1);
+    for (long t = 0; t <= L; ++t) {
+      double s = 0;
+      for (long i = 0; i + t < N; ++i) s += (x[i] - m) * (x[i + t] - m);
+      C[t] = s / double(N);
+    }
+    return C;
+  }
+  static double tau_int(const std::vector<double>& x, int* window = nullptr) {
+    const int N = int(x.size());
+    if (N < 2) { if (window) *window = 0; return 0.5; }
+    std::vector<double> rho = autocov(x, std::min(N - 1, kLagCap));
+    if (degenerate(rho[0], meansq(x))) { if (window) *window = 0; return 0.5; }
+    const double c0 = rho[0];
+    for (double& v : rho) v /= c0;
+    const int Mmax = int(rho.size()) - 1;
+    double tau = 0.5;
+    int W = Mmax;
+    for (int M = 1; M <= Mmax; ++M) {
+      tau += rho[M];
+      if (double(M) >= kSokal * tau) { W = M; break; }
+    }
+    if (window) *window = W;
+    return tau < 0.5 ? 0.5 : tau;
+  }
+  static double neff(const std::vector<double>& x) { return x.empty() ? 0.0 : x.size() / (2.0 * tau_int(x)); }
+  static double error(const std::vector<double>& x) {
+    if (x.size() < 2) return 0.0;
+    const double var = autocov(x, 0)[0];
+    if (degenerate(var, meansq(x))) return 0.0;
+    return std::sqrt(2.0 * tau_int(x) * var / double(x.size()));
   }
 };
 

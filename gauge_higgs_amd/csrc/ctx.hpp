@@ -53,6 +53,12 @@ struct ghx_ctx {
   void* S = nullptr;
   bool use_surr = false;
   bool frozen = false;  // |phi_x| = 1 scalar
+  // multi-invariant potential (scalar_invariants.hpp): superoperator, on-site force and value buffers
+  bool multi = false;
+  double mu2 = 0.0;
+  void* dAq = nullptr;
+  void* dFon = nullptr;
+  double* dVon = nullptr;
   void (*surr_fill)(ghx_ctx*, int buf) = nullptr;
   SurrogateState* surr_state = nullptr;
   // large-irrep path
@@ -86,6 +92,11 @@ struct ghx_ctx {
     c.S = S;
     c.surr_sign = 0.0;
     c.frozen = frozen ? 1 : 0;
+    c.multi = multi ? 1 : 0;
+    c.Aq = dAq;
+    c.Fon = dFon;
+    c.Von = dVon;
+    c.mu2 = mu2;
     c.stream_seg = (nrep == 1) ? stream_seg : 0;
     c.V3 = hreps[0].stride[D - 1];
     c.Lt = unsigned(hreps[0].L[D - 1]);

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "ctx.hpp"
+#include "potential.hpp"
 
 namespace {
 constexpr double kOmelyanLambda = 0.1931833275037836;  // config.hpp:25
@@ -38,6 +39,11 @@ void free_fields(ghx_ctx* c) {
   for (void** p : {&c->U[0], &c->U[1], &c->Usave, &c->P, &c->phi[0], &c->phi[1], &c->phisave, &c->pi, &c->F, &c->Fphi, &c->G,
                    (void**)&c->q1})
     if (*p) { (void)hipFree(*p); *p = nullptr; }
+}
+void free_multi(ghx_ctx* c) {
+  for (void** p : {&c->dAq, &c->dFon, (void**)&c->dVon})
+    if (*p) { (void)hipFree(*p); *p = nullptr; }
+  c->multi = false;
 }
 void free_tables(ghx_ctx* c) {
   for (auto& p : c->dtab)
@@ -506,6 +512,7 @@ int ghx_destroy(ghx_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   drop_graph(ctx);
   surrogate_free(ctx);
+  free_multi(ctx);
   free_fields(ctx);
   free_tables(ctx);
   for (void* p : {(void*)ctx->dreps, (void*)ctx->dblk_rep, (void*)ctx->dred_rep, (void*)ctx->dred_first, (void*)ctx->partials,
@@ -1083,6 +1090,58 @@ int ghx_rep_query(int N, const char* spec, int* d, int* nality, int* real_rep, l
   if (W) std::copy(r.W.begin(), r.W.end(), W);
   if (T)
     for (size_t i = 0; i < r.T.size(); ++i) { T[2 * i] = r.T[i].real(); T[2 * i + 1] = r.T[i].imag(); }
+  return GHX_OK;
+}
+
+// Channels of the multi-invariant potential (scalar_invariants.hpp:20-52): the
+// distinct eigenvalues of M -> sum_a [T^a_R, [T^a_R, M]]; host only.
+int ghx_casimir_channels(int N, const char* spec, double* C2, int* n) {
+  if ((N != 2 && N != 3) || !n) return GHX_E_INVALID;
+  RepInfo r;
+  try {
+    r = build_rep(N, spec ? spec : "");
+  } catch (const std::exception&) {
+    return GHX_E_REP;
+  }
+  if (r.d == 0) return GHX_E_INVALID;
+  const std::vector<double> lam = casimir_channels(r);
+  *n = int(lam.size());
+  if (C2) std::copy(lam.begin(), lam.end(), C2);
+  return GHX_OK;
+}
+
+// Plug in the multi-invariant potential V = -mu2 phi^dag phi + sum_c f_c V_c
+// (MultiInvariantPotential, scalar_invariants.hpp:182-208, as GaugeHiggsHMC::potential):
+// nf must equal the number of channels; f == NULL restores the quartic potential.
+int ghx_set_multi_potential(ghx_ctx* ctx, const double* f, int nf, double mu2) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  drop_graph(ctx);
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  free_multi(ctx);
+  if (!f) return GHX_OK;
+  if (!has_phi(ctx)) return fail(ctx, GHX_E_INVALID, "multi-invariant potential needs a scalar field");
+  if (ctx->big) return fail(ctx, GHX_E_UNSUPPORTED, "multi-invariant potential is not built for the large-irrep kernels");
+  const std::vector<double> lam = casimir_channels(ctx->rep);
+  if (nf != int(lam.size()))
+    return fail(ctx, GHX_E_INVALID, "need " + std::to_string(lam.size()) + " channel couplings, got " + std::to_string(nf));
+  const std::vector<std::complex<double>> A = multi_superop(ctx->rep, lam, std::vector<double>(f, f + nf));
+  std::vector<double> Ad(A.size() * 2);
+  for (size_t i = 0; i < A.size(); ++i) { Ad[2 * i] = A[i].real(); Ad[2 * i + 1] = A[i].imag(); }
+  if (ctx->tsize == 8) {
+    HIPCHK(hipMalloc(&ctx->dAq, Ad.size() * 8));
+    HIPCHK(hipMemcpy(ctx->dAq, Ad.data(), Ad.size() * 8, hipMemcpyHostToDevice));
+  } else {
+    const std::vector<float> Af(Ad.begin(), Ad.end());
+    HIPCHK(hipMalloc(&ctx->dAq, Af.size() * 4));
+    HIPCHK(hipMemcpy(ctx->dAq, Af.data(), Af.size() * 4, hipMemcpyHostToDevice));
+  }
+  HIPCHK(hipMalloc(&ctx->dFon, ctx->nphi() * ctx->Vp * ctx->tsize));
+  HIPCHK(hipMemset(ctx->dFon, 0, ctx->nphi() * ctx->Vp * ctx->tsize));
+  HIPCHK(hipMalloc((void**)&ctx->dVon, size_t(ctx->Vp) * 8));
+  HIPCHK(hipMemset(ctx->dVon, 0, size_t(ctx->Vp) * 8));
+  ctx->mu2 = mu2;
+  ctx->multi = true;
   return GHX_OK;
 }
 

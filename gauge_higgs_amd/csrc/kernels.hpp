@@ -39,8 +39,10 @@ struct RepDesc {
 // force) to the link force.
 // F_FROZEN: |phi_x| = 1 scalar (gauge_higgs_hmc.hpp:61-104): pi is projected
 // onto the tangent space after each kick and the phi drift is the geodesic.
+// F_EXT_ONSITE: the on-site force -2 dV/dphi^* comes from the Fon planes (a
+// plugged-in potential, e.g. the multi-invariant one) instead of the quartic.
 enum StepFlags : int { F_KICK = 1, F_DRIFT = 2, F_FORCE = 4, F_ABS_EPS = 8, F_NO_GAUGE = 16, F_NO_HOP = 32, F_NO_ONSITE = 64,
-                       F_SURR = 128, F_FROZEN = 256 };
+                       F_SURR = 128, F_FROZEN = 256, F_EXT_ONSITE = 512 };
 
 template <typename T>
 struct StepArgs {
@@ -60,6 +62,7 @@ struct StepArgs {
   const T* G;     // external matter link force planes (REP_EXT)
   const T* S;     // surrogate link force planes (F_SURR)
   double surr_sign;
+  const T* Fon;   // on-site scalar force planes (F_EXT_ONSITE)
 };
 
 // Link storage. FP64 (parity mode) stores all N rows. FP32 MD stores the first
@@ -290,9 +293,13 @@ GHX_DEV void site_update(const StepArgs<T>& a, const typename REP::template Para
     R n2 = R(0);
 #pragma unroll
     for (int k = 0; k < DR; ++k) n2 += phx[k].r * phx[k].r + phx[k].i * phx[k].i;
-    const R c = (a.flags & F_NO_ONSITE) ? R(0) : R(-2) * (R(1) + R(2.0 * rd.lambda) * (n2 - R(1)));  // -2 dV/dphi^*
+    if ((a.flags & F_EXT_ONSITE) && !(a.flags & F_NO_ONSITE)) {
+      bload_vec<T, R, DR>(Planes<T>(a.Fon, Vp, 2 * DR), g, fph);
+    } else {
+      const R c = (a.flags & F_NO_ONSITE) ? R(0) : R(-2) * (R(1) + R(2.0 * rd.lambda) * (n2 - R(1)));  // -2 dV/dphi^*
 #pragma unroll
-    for (int k = 0; k < DR; ++k) fph[k] = cscale(phx[k], c);
+      for (int k = 0; k < DR; ++k) fph[k] = cscale(phx[k], c);
+    }
   }
   const bool do_gauge = !(a.flags & F_NO_GAUGE), do_hop = !(a.flags & F_NO_HOP);
 
@@ -505,6 +512,55 @@ __global__ __launch_bounds__(kBlock) void frozen_site_kernel(const RepDesc* reps
   }
 }
 
+// Multi-invariant on-site term (scalar_invariants.hpp:62-87, 119-146) with the
+// phi-independent superoperator A (d^2 x d^2, complex, row (p*d+q), column
+// (i*d+j)): vM = vec(phi phi^dag), g = -mu2 phi + reshape(A vM) phi,
+// V = -mu2 |phi|^2 + Re <vM, A vM> / 2. Writes Fon = -2 g (the on-site
+// force planes) and/or Von = V (FP64, for the action).
+template <typename T, int DR>
+__global__ __launch_bounds__(kBlock) void multi_onsite_kernel(const RepDesc* reps, const int* blk_rep, unsigned Vp,
+                                                             const T* __restrict__ phi, const T* __restrict__ A, double mu2,
+                                                             T* Fon, double* Von) {
+  using R = T;
+  const int r = blk_rep[blockIdx.x];
+  const RepDesc& rd = reps[r];
+  const unsigned s = blockIdx.x * kBlock + threadIdx.x - rd.offset;
+  if (s >= rd.vol) return;
+  const unsigned g = rd.offset + s;
+  cx<R> ph[DR];
+  load_vec<T, R, DR>(phi, Vp, g, ph);
+  cx<R> vM[DR * DR];
+  R n2 = R(0);
+#pragma unroll
+  for (int i = 0; i < DR; ++i) {
+    n2 += ph[i].r * ph[i].r + ph[i].i * ph[i].i;
+#pragma unroll
+    for (int j = 0; j < DR; ++j) vM[i * DR + j] = cmul(ph[i], cconj(ph[j]));
+  }
+  const R m2 = R(mu2);
+  R v2 = R(0);
+#pragma unroll 1
+  for (int p = 0; p < DR; ++p) {
+    const cx<R> fp = {R(phi[size_t(2 * p) * Vp + g]), R(phi[size_t(2 * p + 1) * Vp + g])};
+    cx<R> gp = {-m2 * fp.r, -m2 * fp.i};
+#pragma unroll
+    for (int q = 0; q < DR; ++q) {
+      const T* row = A + size_t(2 * (p * DR + q)) * (DR * DR);
+      cx<R> am = {R(0), R(0)};
+#pragma unroll
+      for (int k = 0; k < DR * DR; ++k) cfma(am, cx<R>{row[2 * k], row[2 * k + 1]}, vM[k]);
+      cfma(gp, am, ph[q]);
+      const cx<R> w = cmul(cconj(fp), ph[q]);  // conj(vM_pq)
+      v2 += w.r * am.r - w.i * am.i;
+    }
+    if (Fon) {
+      Fon[size_t(2 * p) * Vp + g] = T(R(-2) * gp.r);
+      Fon[size_t(2 * p + 1) * Vp + g] = T(R(-2) * gp.i);
+    }
+  }
+  if (Von) Von[g] = double(-m2 * n2) + 0.5 * double(v2);
+}
+
 // phi' = phi + eps pi for a runtime number of planes (large reps).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void phi_drift_kernel(const RepDesc* reps, const int* blk_rep, unsigned Vp, const T* phi_in,
@@ -547,6 +603,7 @@ struct ActArgs {
   const unsigned* red_first;  // reduction block -> first global site
   unsigned Vp;
   double* partials;  // [nblocks][kNQ]
+  const double* Von = nullptr;  // per-site potential of a plugged-in potential (else the quartic)
 };
 
 // Per-site FP64 contributions to H and the observables:
@@ -597,7 +654,7 @@ __global__ __launch_bounds__(kRedBlock) void action_kernel(ActArgs a, const T* _
       double n2 = 0.0, p2 = 0.0;
 #pragma unroll
       for (int k = 0; k < DR; ++k) { n2 += px[k].r * px[k].r + px[k].i * px[k].i; p2 += pv[k].r * pv[k].r + pv[k].i * pv[k].i; }
-      q[2] = n2 + rd.lambda * (n2 - 1.0) * (n2 - 1.0);
+      q[2] = a.Von ? a.Von[g] : n2 + rd.lambda * (n2 - 1.0) * (n2 - 1.0);
       q[3] = n2;
       q[5] = p2;
     }

@@ -27,8 +27,14 @@ struct Launch {
   static void md_step(const LaunchCtx& c, const void* U_in, void* U_out, void* P, const void* phi_in, void* phi_out, void* pi,
                       void* F, void* Fphi, double ck, double cd, int flags) {
     if (c.frozen) flags |= F_FROZEN;
+    if constexpr (REP::DR > 0) {
+      if (c.multi && !(flags & F_NO_ONSITE) && (flags & (F_KICK | F_FORCE))) {
+        multi_onsite(c, phi_in, c.Fon, nullptr);
+        flags |= F_EXT_ONSITE;
+      }
+    }
     StepArgs<T> a{c.reps, c.blk_rep, c.Vp, (const T*)U_in, (T*)U_out, (T*)P, (const T*)phi_in, (T*)phi_out, (T*)pi, (T*)F, (T*)Fphi,
-                  ck, cd, flags, (const T*)c.G, (const T*)c.S, c.surr_sign};
+                  ck, cd, flags, (const T*)c.G, (const T*)c.S, c.surr_sign, (const T*)c.Fon};
     if (c.stream_seg > 0) {
       const unsigned V3 = c.V3, Lt = c.Lt;
       unsigned ntile = (V3 + kBlock - 1) / kBlock;
@@ -44,11 +50,23 @@ struct Launch {
                     const void* pi, double cd, int flags) {
     if (c.frozen) flags |= F_FROZEN;
     StepArgs<T> a{c.reps, c.blk_rep, c.Vp, (const T*)U_in, (T*)U_out, (T*)P, (const T*)phi_in, (T*)phi_out, (T*)pi, nullptr, nullptr,
-                  0.0, cd, flags, nullptr, nullptr, 0.0};
+                  0.0, cd, flags, nullptr, nullptr, 0.0, nullptr};
     hipLaunchKernelGGL((drift_kernel<T, N, D, REP::DR>), dim3(c.nblk), dim3(kBlock), 0, c.stream, a);
+  }
+  // Multi-invariant on-site force planes and/or per-site potential from phi.
+  static void multi_onsite(const LaunchCtx& c, const void* phi, void* Fon, double* Von) {
+    if constexpr (REP::DR > 0)
+      hipLaunchKernelGGL((multi_onsite_kernel<T, REP::DR>), dim3(c.nblk), dim3(kBlock), 0, c.stream, c.reps, c.blk_rep, c.Vp,
+                         (const T*)phi, (const T*)c.Aq, c.mu2, (T*)Fon, Von);
   }
   static void action(const LaunchCtx& c, const void* U, const void* P, const void* phi, const void* pi) {
     ActArgs a{c.reps, c.red_rep, c.red_first, c.Vp, c.partials};
+    if constexpr (REP::DR > 0) {
+      if (c.multi) {
+        multi_onsite(c, phi, nullptr, c.Von);
+        a.Von = c.Von;
+      }
+    }
     hipLaunchKernelGGL((action_kernel<T, N, D, REPD>), dim3(c.nred), dim3(kRedBlock), 0, c.stream, a, (const T*)U, (const T*)P,
                        (const T*)phi, (const T*)pi, ParamFill<REPD, double>::make(c));
   }

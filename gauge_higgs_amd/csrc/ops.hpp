@@ -24,6 +24,11 @@ struct LaunchCtx {
   const void* S;    // surrogate link force planes
   double surr_sign;
   int frozen;       // |phi_x| = 1 scalar: adds F_FROZEN to every MD launch
+  int multi;        // multi-invariant potential in use (on-site force from Fon)
+  const void* Aq;   // its superoperator (complex, storage type)
+  void* Fon;        // on-site force planes
+  double* Von;      // per-site potential (FP64)
+  double mu2;
 };
 
 struct BigRepTables;

@@ -80,6 +80,15 @@ class Lattices:
         """Multi-timescale leapfrog: n_outer hopping-force steps, m_inner gauge/on-site steps each (0 = off)."""
         self._check(self._lib.ghx_set_nested(self.h, n_outer, m_inner))
 
+    def set_multi_potential(self, f, mu2):
+        """Multi-invariant quartic potential V = -mu2 |phi|^2 + sum_c f_c V_c (scalar_invariants.hpp);
+        f=None restores the quartic potential."""
+        if f is None:
+            self._check(self._lib.ghx_set_multi_potential(self.h, None, 0, 0.0))
+            return
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        self._check(self._lib.ghx_set_multi_potential(self.h, _dp(f), len(f), mu2))
+
     def set_frozen_phi(self, on=True):
         """|phi_x| = 1 scalar: tangent momenta and geodesic drift (gauge_higgs_hmc.hpp:61-104)."""
         self._check(self._lib.ghx_set_frozen_phi(self.h, int(on)))
@@ -339,6 +348,18 @@ class GaugeHiggsHMC:
 
     def forces(self):
         return self.lat.forces()
+
+
+def casimir_channels(N, spec):
+    """Channel Casimirs of R x Rbar for the multi-invariant potential (host only)."""
+    lib = _lib.load()
+    n = C.c_int()
+    rc = lib.ghx_casimir_channels(N, spec.encode(), None, C.byref(n))
+    if rc != 0:
+        raise GhxError(f"ghx_casimir_channels failed ({rc})")
+    out = np.zeros(n.value)
+    lib.ghx_casimir_channels(N, spec.encode(), _dp(out), C.byref(n))
+    return out
 
 
 def GaugeHMC(extents, seed, D=4, N=2, precision=FP64):

@@ -73,6 +73,15 @@ int ghx_set_nested(ghx_ctx* ctx, int n_outer, int m_inner);
  * Not available with the large-irrep kernels (GHX_E_UNSUPPORTED). */
 int ghx_set_frozen_phi(ghx_ctx* ctx, int on);
 int ghx_normalize_phi(ghx_ctx* ctx);
+/* Multi-invariant quartic potential (MultiInvariantPotential,
+ * scalar_invariants.hpp:20-208, plugged in as GaugeHiggsHMC::potential):
+ * V = -mu2 phi^dag phi + sum_c f_c ||P_c(phi phi^dag)||^2 over the channels c of
+ * R x Rbar (eigenspaces of M -> sum_a [T^a_R,[T^a_R,M]]). ghx_casimir_channels
+ * lists the channel Casimirs (host only; C2 may be NULL to get the count);
+ * ghx_set_multi_potential needs one f_c per channel, f == NULL restores the
+ * quartic potential. Not available with the large-irrep kernels. */
+int ghx_casimir_channels(int N, const char* spec, double* C2, int* n);
+int ghx_set_multi_potential(ghx_ctx* ctx, const double* f, int nf, double mu2);
 /* RNG seed of one replica (Rng ctor, rng.hpp:27). */
 int ghx_set_seed(ghx_ctx* ctx, int replica, uint64_t seed);
 

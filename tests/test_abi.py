@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_clis_built():
-    for name in ("gh_hmc", "hmc_higgs"):
+    for name in ("gh_hmc", "hmc_higgs", "hmc_higgs_multi"):
         assert os.access(os.path.join(ROOT, "gauge_higgs_amd", "bin", name), os.X_OK)
 
 
@@ -79,3 +79,15 @@ def test_python_layer_fails_loudly_without_library(tmp_path, monkeypatch):
     monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "missing.so"))
     with pytest.raises(RuntimeError):
         _lib.load()
+
+
+@pytest.mark.parametrize("tag,N,spec", [("multi_su2_fund", 2, "fund"), ("multi_su2_adj", 2, "adj"), ("multi_su3_2", 3, "2")])
+def test_casimir_channels_match_reference(goldens, tag, N, spec):
+    """Host construction of the multi-invariant potential's channels against the
+    reference's own Lanczos result (scalar_invariants.hpp:150-179)."""
+    arr, sc = goldens
+    import gauge_higgs_amd as gha
+
+    lam = gha.casimir_channels(N, spec)
+    assert len(lam) == sc[tag + "_nch"]
+    assert np.max(np.abs(lam - arr[tag + "_lambda"])) < 1e-9
