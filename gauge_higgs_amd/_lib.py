@@ -42,6 +42,10 @@ def load():
         "ghx_trajectories": [vp, i, i],
         "ghx_observables": [vp, dp],
         "ghx_max_unitarity": [vp, i, dp],
+        "ghx_wilson_loop": [vp, C.c_char_p, i, i, dp],
+        "ghx_series_analysis": [dp, C.c_long, i, dp],
+        "ghx_autocorr_function": [dp, C.c_long, i, dp, C.POINTER(C.c_long)],
+        "ghx_polyakov_loop_rep": [vp, C.c_char_p, i, dp],
         "ghx_reunitarize": [vp],
         "ghx_get_fields": [vp, i, dp, dp, dp, dp],
         "ghx_set_fields": [vp, i, dp, dp, dp, dp],

@@ -76,7 +76,7 @@ int main(int argc, char** argv) {
 
   GHX_CHECK(ctx, ghx_set_counters(ctx, 0, 0, 0));
   cli::Stats plaq, Lphi, Llink;
-  std::vector<double> pl_series, ll_series;
+  std::vector<double> pl_series;
   double sExp = 0, obs[6], dH = 0;
   uint64_t tc = 0, acc = 0;
   for (int t = 0; t < nmeas; ++t) {
@@ -84,22 +84,22 @@ int main(int argc, char** argv) {
     GHX_CHECK(ctx, ghx_get_counters(ctx, 0, &tc, &acc, &dH));
     GHX_CHECK(ctx, ghx_observables(ctx, obs));
     plaq.add(obs[0]); Lphi.add(obs[4]); Llink.add(obs[3]);
-    pl_series.push_back(obs[0]); ll_series.push_back(obs[3]);
+    pl_series.push_back(obs[0]);
     sExp += std::exp(-dH);
   }
   if (!ts_out.empty()) {
     std::ofstream fo(ts_out);
     fo << "# traj plaq L_link  (rep=" << name << " beta=" << beta << " kappa=" << kappa << " mu2=" << mu2 << " nmd=" << nmd << ")\n";
-    for (int t = 0; t < nmeas; ++t) fo << t << ' ' << pl_series[t] << ' ' << ll_series[t] << '\n';
+    for (int t = 0; t < nmeas; ++t) fo << t << ' ' << pl_series[t] << ' ' << Llink.x[t] << '\n';
   }
   long long vol = 1;
   for (int m = 0; m < fl.D; ++m) vol *= Lext;
   int W = 0;
-  const double tau_ll = cli::Autocorr::tau_int(ll_series, &W);
+  const double tau_ll = Llink.tau_int(&W);
   std::printf("plaquette   = %.6f +/- %.6f\n", plaq.mean(), plaq.binned_error());
   std::printf("L_phi       = %.6f +/- %.6f\n", Lphi.mean(), Lphi.binned_error());
-  std::printf("L_link      = %.6f +/- %.6f   chi_link=%.4f\n", Llink.mean(), cli::Autocorr::error(ll_series), Llink.susceptibility(double(vol)));
-  std::printf("autocorr    = tau_int(L_link)=%.2f window=%d  N_eff=%.1f / %d meas\n", tau_ll, W, cli::Autocorr::neff(ll_series), nmeas);
+  std::printf("L_link      = %.6f +/- %.6f   chi_link=%.4f\n", Llink.mean(), Llink.autocorr_error(), Llink.susceptibility(double(vol)));
+  std::printf("autocorr    = tau_int(L_link)=%.2f window=%d  N_eff=%.1f / %d meas\n", tau_ll, W, Llink.neff(), nmeas);
   std::printf("acceptance  = %.4f   <exp(-dH)>=%.5f\n", tc ? double(acc) / tc : 0.0, sExp / nmeas);
   ghx_destroy(ctx);
   return 0;

@@ -9,6 +9,7 @@
 #include "../../include/ghx.h"
 #include "bigrep.hpp"
 #include "kernels.hpp"
+#include "measure.hpp"
 #include "ops.hpp"
 #include "rep_build.hpp"
 
@@ -28,6 +29,7 @@ struct ghx_ctx {
   RepInfo rep;
   bool ops_ok = false;
   Ops ops{};
+  MeasOps mops{};  // Wilson / Polyakov loops in a probe irrep
   // integrator
   int nmd = 20, integ = GHX_OMELYAN;
   bool reunit = true;

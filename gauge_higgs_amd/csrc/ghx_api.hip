@@ -169,6 +169,7 @@ int select_ops(ghx_ctx* ctx) {
       }
       break;
   }
+  if (ok) ok = meas_ops(p, ctx->N, ctx->D, &ctx->mops);
   ctx->ops_ok = ok;
   if (!ok)
     return fail(ctx, GHX_E_UNSUPPORTED, "no compiled kernels for N=" + std::to_string(ctx->N) + " D=" + std::to_string(ctx->D) +
@@ -783,6 +784,74 @@ int ghx_observables(ghx_ctx* ctx, double* out) {
     o[3] = has_phi(ctx) ? q[1] / (V * ctx->D) : 0.0;
     o[4] = has_phi(ctx) ? q[3] / V : 0.0;
     o[5] = pl[r] / ((V / rd.L[ctx->D - 1]) * ctx->N);
+  }
+  return GHX_OK;
+}
+
+namespace {
+// Probe irrep of a measurement from a rep spec; the Young-row rules and
+// messages are those of the scalar's representation. Returns d_R' or 0 (with
+// the context's error set).
+long parse_probe(ghx_ctx* ctx, const char* spec, Probe* p) {
+  const std::string s = spec ? spec : "";
+  *p = Probe{0, 0, {0, 0}};
+  if (s == "fund") { p->kind = 1; return ctx->N; }
+  if (s == "adj") { p->kind = 2; return long(ctx->N) * ctx->N - 1; }
+  try {
+    const std::vector<int> rows = parse_young_rows(ctx->N, s, nullptr);
+    if (rows[0] + 1 > kProbeMaxH) throw std::runtime_error("probe rep: first row longer than " + std::to_string(kProbeMaxH - 1));
+    p->kind = 3;
+    p->nrows = int(rows.size());
+    for (size_t i = 0; i < rows.size(); ++i) p->rows[i] = rows[i];
+    return young_dimension(ctx->N, rows);
+  } catch (const std::exception& ex) {
+    fail(ctx, GHX_E_REP, ex.what());
+    return 0;
+  }
+}
+
+// Sum the per-block partials of every replica (fixed order) and copy them out.
+int replica_sums(ghx_ctx* ctx, std::vector<double>& out) {
+  hipLaunchKernelGGL(finalize_kernel, dim3(ctx->nrep), dim3(kRedBlock), 0, ctx->stream, ctx->dreps, ctx->partials, 1, ctx->Hwork);
+  out.assign(ctx->nrep, 0.0);
+  HIPCHK(hipMemcpyAsync(out.data(), ctx->Hwork, sizeof(double) * ctx->nrep, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return GHX_OK;
+}
+}  // namespace
+
+int ghx_wilson_loop(ghx_ctx* ctx, const char* probe, int Ra, int Rb, double* W) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (!W || Ra < 0 || Rb < 0) return GHX_E_INVALID;
+  Probe p;
+  const long d = parse_probe(ctx, probe, &p);
+  if (d == 0) return GHX_E_REP;
+  ctx->mops.wilson(ctx->lc(), ctx->U[ctx->cur], p, Ra, Rb);
+  HIPCHK(hipGetLastError());
+  std::vector<double> s;
+  if ((rc = replica_sums(ctx, s))) return rc;
+  for (int r = 0; r < ctx->nrep; ++r) {
+    const double nloop = double(ctx->hreps[r].vol) * (ctx->D * (ctx->D - 1) / 2);
+    W[r] = s[r] / (nloop * double(d));
+  }
+  return GHX_OK;
+}
+
+int ghx_polyakov_loop_rep(ghx_ctx* ctx, const char* probe, int tdir, double* P) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (!P || tdir < 0 || tdir >= ctx->D) return GHX_E_INVALID;
+  Probe p;
+  const long d = parse_probe(ctx, probe, &p);
+  if (d == 0) return GHX_E_REP;
+  ctx->mops.polyakov(ctx->lc(), ctx->U[ctx->cur], p, tdir);
+  HIPCHK(hipGetLastError());
+  std::vector<double> s;
+  if ((rc = replica_sums(ctx, s))) return rc;
+  for (int r = 0; r < ctx->nrep; ++r) {
+    const RepDesc& rd = ctx->hreps[r];
+    P[r] = s[r] / (double(rd.vol / unsigned(rd.L[tdir])) * double(d));
   }
   return GHX_OK;
 }

@@ -145,6 +145,47 @@ void young_basis(const TensorSpace& ts, const std::vector<int>& rows, std::vecto
 
 }  // namespace
 
+std::vector<int> parse_young_rows(int N, std::string spec, bool* real) {
+  bool re = false;
+  const auto colon = spec.find(':');
+  if (colon != std::string::npos) { re = spec.substr(colon + 1) == "real"; spec = spec.substr(0, colon); }
+  if (real) *real = re;
+  std::vector<int> rows;
+  std::size_t p = 0;
+  while (p <= spec.size()) {
+    const std::size_t q = spec.find(',', p);
+    const std::string tok = spec.substr(p, q == std::string::npos ? std::string::npos : q - p);
+    if (!tok.empty()) rows.push_back(std::atoi(tok.c_str()));
+    if (q == std::string::npos) break;
+    p = q + 1;
+  }
+  if (rows.empty()) throw std::runtime_error("bad rep spec '" + spec + "'");
+  while (!rows.empty() && rows.back() == 0) rows.pop_back();
+  if (rows.empty()) throw std::runtime_error("GeneralRep: empty diagram (singlet not supported)");
+  for (std::size_t i = 0; i + 1 < rows.size(); ++i)
+    if (rows[i] < rows[i + 1]) throw std::runtime_error("GeneralRep: rows must be non-increasing");
+  if (int(rows.size()) > N - 1) throw std::runtime_error("GeneralRep: > N-1 rows (strip full columns of N)");
+  const int n = std::accumulate(rows.begin(), rows.end(), 0);
+  double dimT = 1.0;
+  for (int k = 0; k < n; ++k) dimT *= N;
+  if (dimT > 200000.0) throw std::runtime_error("GeneralRep: N^n exceeds cap; irrep too large for tensor build");
+  return rows;
+}
+
+long young_dimension(int N, const std::vector<int>& rows) {
+  // hook-content formula: prod over boxes (N + column - row) / hook length
+  double num = 1.0, den = 1.0;
+  for (std::size_t i = 0; i < rows.size(); ++i)
+    for (int j = 0; j < rows[i]; ++j) {
+      int below = 0;
+      for (std::size_t k = i + 1; k < rows.size(); ++k)
+        if (rows[k] > j) ++below;
+      num *= double(N + j - int(i));
+      den *= double(rows[i] - j - 1 + below + 1);
+    }
+  return long(std::llround(num / den));
+}
+
 RepInfo build_rep(int N, const std::string& spec_in) {
   RepInfo r;
   r.N = N;
@@ -177,26 +218,9 @@ RepInfo build_rep(int N, const std::string& spec_in) {
     return r;
   }
   bool real = false;
-  const auto colon = spec.find(':');
-  if (colon != std::string::npos) { real = spec.substr(colon + 1) == "real"; spec = spec.substr(0, colon); }
-  std::vector<int> rows;
-  std::size_t p = 0;
-  while (p <= spec.size()) {
-    const std::size_t q = spec.find(',', p);
-    const std::string tok = spec.substr(p, q == std::string::npos ? std::string::npos : q - p);
-    if (!tok.empty()) rows.push_back(std::atoi(tok.c_str()));
-    if (q == std::string::npos) break;
-    p = q + 1;
-  }
-  if (rows.empty()) throw std::runtime_error("bad rep spec '" + spec + "'");
-  while (!rows.empty() && rows.back() == 0) rows.pop_back();
-  if (rows.empty()) throw std::runtime_error("GeneralRep: empty diagram (singlet not supported)");
-  for (std::size_t i = 0; i + 1 < rows.size(); ++i)
-    if (rows[i] < rows[i + 1]) throw std::runtime_error("GeneralRep: rows must be non-increasing");
-  if (int(rows.size()) > N - 1) throw std::runtime_error("GeneralRep: > N-1 rows (strip full columns of N)");
+  const std::vector<int> rows = parse_young_rows(N, spec, &real);
   const int n = std::accumulate(rows.begin(), rows.end(), 0);
   TensorSpace ts(N, n);
-  if (ts.dim > 200000) throw std::runtime_error("GeneralRep: N^n exceeds cap; irrep too large for tensor build");
   std::vector<std::vector<double>> W;
   if (rows.size() == 1) symmetric_basis(ts, W); else young_basis(ts, rows, W);
   r.kind = 3; r.rows = rows; r.nbox = n; r.dimT = ts.dim; r.d = int(W.size()); r.nality = n % N; r.real = real;

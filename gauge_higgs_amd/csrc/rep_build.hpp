@@ -29,6 +29,12 @@ struct RepInfo {
 // with the reference's messages for invalid diagrams.
 RepInfo build_rep(int N, const std::string& spec);
 
+// Young rows of a "r1,r2,...[:real]" spec, validated with the reference's
+// messages (rep_general.hpp:58-69, 338-345); *real receives the suffix flag.
+std::vector<int> parse_young_rows(int N, std::string spec, bool* real);
+// Dimension of the SU(N) irrep with these rows (hook-content formula).
+long young_dimension(int N, const std::vector<int>& rows);
+
 // Fundamental generators in the reference order (algebra.hpp:24-50), N x N each.
 std::vector<std::complex<double>> fundamental_generators(int N);
 

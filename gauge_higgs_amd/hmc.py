@@ -149,6 +149,21 @@ class Lattices:
         self._check(self._lib.ghx_observables(self.h, _dp(out)))
         return out.reshape(self.nrep, 6)
 
+    def wilson_loops(self, probe, Ra, Rb):
+        """<(1/d_R') Re Tr D^(R')(U_loop)> of the Ra x Rb loops, per replica
+        (observables.hpp:50-81). probe: "fund", "adj" or Young rows "r1[,r2]"."""
+        out = np.zeros(self.nrep)
+        self._check(self._lib.ghx_wilson_loop(self.h, probe.encode(), Ra, Rb, _dp(out)))
+        return out
+
+    def polyakov_loops_rep(self, probe, tdir=None):
+        """Polyakov loop in direction tdir (default D-1) traced in the probe
+        irrep, per replica (observables.hpp:83-101)."""
+        out = np.zeros(self.nrep)
+        t = self.D - 1 if tdir is None else tdir
+        self._check(self._lib.ghx_polyakov_loop_rep(self.h, probe.encode(), t, _dp(out)))
+        return out
+
     def max_unitarity(self, replica=0):
         v = C.c_double()
         self._check(self._lib.ghx_max_unitarity(self.h, replica, C.byref(v)))
@@ -340,6 +355,16 @@ class GaugeHiggsHMC:
         o = self.lat.observables_all()[0]
         return dict(zip(["plaq", "Sg", "SH", "Llink", "Lphi", "poly"], o))
 
+    # measurement layer (observables.hpp:36-101)
+    def wilson_loop_fund(self, Ra, Rb):
+        return float(self.lat.wilson_loops("fund", Ra, Rb)[0])
+
+    def wilson_loop_rep(self, probe, Ra, Rb):
+        return float(self.lat.wilson_loops(probe, Ra, Rb)[0])
+
+    def polyakov_loop_rep(self, probe, tdir=None):
+        return float(self.lat.polyakov_loops_rep(probe, tdir)[0])
+
     def get(self, name):
         return self.lat.get(name)
 
@@ -348,6 +373,36 @@ class GaugeHiggsHMC:
 
     def forces(self):
         return self.lat.forces()
+
+
+SERIES_KEYS = ("mean", "var", "binder", "binned_error", "tau_int", "window", "n_eff", "naive_error",
+               "autocorr_error", "blocking_error")
+
+
+def series_analysis(x, block=1):
+    """Stats and autocorrelation analysis of one Monte Carlo series
+    (observables.hpp:104-145, autocorr.hpp:55-212), computed by the library."""
+    lib = _lib.load()
+    x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    out = np.zeros(10)
+    rc = lib.ghx_series_analysis(_dp(x), x.size, int(block), _dp(out))
+    if rc != 0:
+        raise GhxError(f"ghx_series_analysis failed ({rc})")
+    d = dict(zip(SERIES_KEYS, out))
+    d["window"] = int(d["window"])
+    return d
+
+
+def autocorr_function(x, maxlag=-1):
+    """rho(t) = C(t)/C(0) for t = 0..maxlag (autocorr.hpp:93-123)."""
+    lib = _lib.load()
+    x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    rho = np.zeros(max(x.size, 1))
+    n = C.c_long()
+    rc = lib.ghx_autocorr_function(_dp(x), x.size, int(maxlag), _dp(rho), C.byref(n))
+    if rc != 0:
+        raise GhxError(f"ghx_autocorr_function failed ({rc})")
+    return rho[: n.value].copy()
 
 
 def casimir_channels(N, spec):

@@ -114,6 +114,26 @@ int ghx_trajectories(ghx_ctx* ctx, int n, int use_graph);
 /* Observables per replica: [plaquette, S_g, S_H, L_link, L_phi, Polyakov] x n_replicas
  * (gauge_wilson.hpp:51-65, scalar_higgs.hpp:217-239, observables.hpp:18-34). */
 int ghx_observables(ghx_ctx* ctx, double* out);
+/* Wilson loop traced in a probe irrep, per replica (observables.hpp:36-81, wilson_loop_fund /
+ * wilson_loop_rep): W[r] = < (1/d_R') Re Tr D^(R')(U_loop) > over sites and planes mu < nu of the
+ * Ra x Rb rectangle (Ra links along mu, Rb along nu). probe: "fund" (wilson_loop_fund), "adj" or
+ * Young rows "r1[,r2][:real]"; an invalid spec gives GHX_E_REP with the GeneralRep message.
+ * W has n_replicas entries. */
+int ghx_wilson_loop(ghx_ctx* ctx, const char* probe, int Ra, int Rb, double* W);
+/* Polyakov loop in direction tdir traced in a probe irrep, per replica (observables.hpp:83-101,
+ * polyakov_loop_rep): < (1/d_R') Re Tr D^(R')(prod_t U_tdir) > over the sites with x_tdir = 0. */
+int ghx_polyakov_loop_rep(ghx_ctx* ctx, const char* probe, int tdir, double* P);
+/* Analysis of one observable's Monte Carlo series x[0..n) (host only, no context):
+ * out[10] = {mean, population variance, Binder cumulant U4, block-jackknife error with
+ * 16 bins (observables.hpp:104-145 Stats), tau_int with Sokal's window M >= 6 tau_int(M)
+ * and lags capped at 2000, the window W, N_eff = n / (2 tau_int), naive error sqrt(Var/n),
+ * autocorrelation-aware error sqrt(2 tau_int Var / n), blocking error with block size
+ * `block`} (autocorr.hpp:55-212), with the reference's values for empty, length-1 and
+ * constant series. */
+int ghx_series_analysis(const double* x, long n, int block, double* out);
+/* Normalised autocorrelation rho(t) = C(t)/C(0), t = 0..min(maxlag, n-1) (every lag when
+ * maxlag < 0), biased 1/n autocovariance (autocorr.hpp:93-123); *nlag = entries written. */
+int ghx_autocorr_function(const double* x, long n, int maxlag, double* rho, long* nlag);
 /* max ||U^dag U - 1|| of one replica (fields.hpp:38-42). */
 int ghx_max_unitarity(ghx_ctx* ctx, int replica, double* out);
 /* Reunitarize all links (fields.hpp:36, sun.hpp:49-72). */

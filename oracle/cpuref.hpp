@@ -906,6 +906,49 @@ struct Engine {
     }
     return sum / (double(count) * N);
   }
+  // observables.hpp:36-48: path-ordered product around the Ra x Rb rectangle
+  // based at x in the (mu, nu) plane (Ra steps along mu, Rb along nu, back).
+  Mat<N> wilson_rect(std::int64_t x, int mu, int nu, int Ra, int Rb) const {
+    Mat<N> W = Mat<N>::eye();
+    std::int64_t s = x;
+    for (int i = 0; i < Ra; ++i) { W = W * U[s * D + mu]; s = g.up(s, mu); }
+    for (int j = 0; j < Rb; ++j) { W = W * U[s * D + nu]; s = g.up(s, nu); }
+    for (int i = 0; i < Ra; ++i) { s = g.dn(s, mu); W = W * U[s * D + mu].dag(); }
+    for (int j = 0; j < Rb; ++j) { s = g.dn(s, nu); W = W * U[s * D + nu].dag(); }
+    return W;
+  }
+  // Re Tr D^(R)(M) / d_R; probe null = fundamental (observables.hpp:50-81).
+  static double probe_trace(const Rep<N>* probe, const Mat<N>& M) {
+    if (!probe) return M.tr().real();
+    return probe->rep_matrix(M).tr().real();
+  }
+  // observables.hpp:50-81: <(1/d_R) Re Tr D^(R)(W(Ra,Rb))> over sites and planes mu < nu.
+  double wilson_loop(const Rep<N>* probe, int Ra, int Rb) const {
+    double sum = 0.0;
+    std::int64_t count = 0;
+    for (std::int64_t x = 0; x < g.vol; ++x)
+      for (int mu = 0; mu < D; ++mu)
+        for (int nu = mu + 1; nu < D; ++nu) {
+          sum += probe_trace(probe, wilson_rect(x, mu, nu, Ra, Rb));
+          ++count;
+        }
+    return sum / (double(count) * (probe ? probe->d : N));
+  }
+  // observables.hpp:83-101: Polyakov loop in direction tdir traced in the probe rep.
+  double polyakov_rep(const Rep<N>* probe, int tdir) const {
+    const int Lt = g.L[tdir];
+    double sum = 0.0;
+    std::int64_t count = 0;
+    for (std::int64_t s = 0; s < g.vol; ++s) {
+      if (g.coord(s, tdir) != 0) continue;
+      Mat<N> Pl = U[s * D + tdir];
+      std::int64_t cur = g.up(s, tdir);
+      for (int t = 1; t < Lt; ++t) { Pl = Pl * U[cur * D + tdir]; cur = g.up(cur, tdir); }
+      sum += probe_trace(probe, Pl);
+      ++count;
+    }
+    return sum / (double(count) * (probe ? probe->d : N));
+  }
   void reunitarize_all() { for (auto& m : U) reunitarize<N>(m); }
   double max_unitarity_violation() const { double w = 0.0; for (const auto& m : U) w = std::max(w, unitarity_violation<N>(m)); return w; }
 

@@ -49,6 +49,9 @@ struct Base {
   virtual void normalize_phi() = 0;
   virtual int set_multi(const double* f, int nf, double mu2, double* lambda_out) = 0;
   virtual int multi_eval(const double* phi, double* V, double* dV) const = 0;
+  virtual double wilson_loop(const char* probe, int Ra, int Rb) const = 0;
+  virtual double polyakov_rep(const char* probe, int tdir) const = 0;
+  virtual void wilson_rect(long x, int mu, int nu, int Ra, int Rb, double* out) const = 0;
 };
 
 template <int N>
@@ -137,6 +140,21 @@ struct Impl : Base {
     for (int k = 0; k < rep->d; ++k) { dV[2 * k] = g[k].real(); dV[2 * k + 1] = g[k].imag(); }
     return 0;
   }
+  // probe spec "fund" | "adj" | Young rows (as make_rep); "fund" uses the plain trace
+  double wilson_loop(const char* probe, int Ra, int Rb) const override {
+    if (std::string(probe) == "fund") return e.wilson_loop(nullptr, Ra, Rb);
+    const Rep<N> r = make_rep<N>(probe);
+    return e.wilson_loop(&r, Ra, Rb);
+  }
+  double polyakov_rep(const char* probe, int tdir) const override {
+    if (std::string(probe) == "fund") return e.polyakov_rep(nullptr, tdir);
+    const Rep<N> r = make_rep<N>(probe);
+    return e.polyakov_rep(&r, tdir);
+  }
+  void wilson_rect(long x, int mu, int nu, int Ra, int Rb, double* out) const override {
+    const Mat<N> W = e.wilson_rect(x, mu, nu, Ra, Rb);
+    for (int i = 0; i < N * N; ++i) { out[2 * i] = W.a[i].real(); out[2 * i + 1] = W.a[i].imag(); }
+  }
   void normalize_phi() override { e.normalize_phi(); }
   double max_unitarity() const override { return e.max_unitarity_violation(); }
   std::uint64_t& traj_count() override { return e.traj_count; }
@@ -210,6 +228,18 @@ int orc_set_multi(void* h, const double* f, int nf, double mu2, double* lambda_o
   return static_cast<Base*>(h)->set_multi(f, nf, mu2, lambda_out);
 }
 int orc_multi_eval(void* h, const double* phi, double* V, double* dV) { return static_cast<Base*>(h)->multi_eval(phi, V, dV); }
+// measurement layer (observables.hpp:36-101); NaN with orc_last_error set on a bad probe spec
+double orc_wilson_loop(void* h, const char* probe, int Ra, int Rb) {
+  try { return static_cast<Base*>(h)->wilson_loop(probe, Ra, Rb); } catch (const std::exception& ex) { g_err = ex.what(); }
+  return std::nan("");
+}
+double orc_polyakov_rep(void* h, const char* probe, int tdir) {
+  try { return static_cast<Base*>(h)->polyakov_rep(probe, tdir); } catch (const std::exception& ex) { g_err = ex.what(); }
+  return std::nan("");
+}
+void orc_wilson_rect(void* h, long x, int mu, int nu, int Ra, int Rb, double* out) {
+  static_cast<Base*>(h)->wilson_rect(x, mu, nu, Ra, Rb, out);
+}
 void orc_refresh(void* h) { static_cast<Base*>(h)->refresh(); }
 void orc_get(void* h, int which, double* out) { static_cast<Base*>(h)->get(which, out); }
 void orc_set(void* h, int which, const double* in) { static_cast<Base*>(h)->set(which, in); }

@@ -8,10 +8,12 @@
 #include "hmc/hmc.hpp"
 #include "hmc/gauge_higgs_hmc.hpp"
 #include "measure/observables.hpp"
+#include "measure/autocorr.hpp"
 #include "action/scalar_invariants.hpp"
 #include "rep/rep_fundamental.hpp"
 #include "rep/rep_adjoint.hpp"
 #include "rep/rep_general.hpp"
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <fstream>
@@ -206,6 +208,63 @@ static void dump_higgs(const std::string& tag, const Representation<N>& rep, int
   }
 }
 
+// Measurement layer (observables.hpp:36-101) on a thermalised pure-gauge
+// field of an anisotropic lattice: Wilson loops and Polyakov loops traced in
+// several probe irreps. Keys: <tag>_W_<probe>_<Ra>x<Rb>, <tag>_P_<probe>_t<dir>.
+template <int D, int N>
+static void dump_loops(const std::string& tag, const std::array<int, D>& ext, double beta, int ntraj,
+                       const std::vector<std::pair<std::string, const Representation<N>*>>& probes,
+                       const std::vector<std::pair<int, int>>& sizes, const std::vector<std::string>& small_only) {
+  GaugeHMC<D, N> h(ext, 3);
+  h.beta = beta; h.nmd = 12;
+  h.U.hot(h.rng, 0.35);
+  for (int t = 0; t < ntraj; ++t) h.trajectory();
+  put_array(tag + "_U", flat_U(h.U));
+  for (const auto& sz : sizes)
+    put_scalar(tag + "_Wfund_" + std::to_string(sz.first) + "x" + std::to_string(sz.second),
+               wilson_loop_fund<D, N>(h.U, sz.first, sz.second));
+  for (const auto& pr : probes) {
+    const bool small = std::find(small_only.begin(), small_only.end(), pr.first) != small_only.end();
+    for (const auto& sz : sizes) {
+      if (small && sz.first * sz.second > 4) continue;
+      put_scalar(tag + "_W_" + pr.first + "_" + std::to_string(sz.first) + "x" + std::to_string(sz.second),
+                 wilson_loop_rep<D, N>(h.U, *pr.second, sz.first, sz.second));
+    }
+    for (int t = 0; t < D; ++t) put_scalar(tag + "_P_" + pr.first + "_t" + std::to_string(t), polyakov_loop_rep<D, N>(h.U, *pr.second, t));
+  }
+  // one rectangle product for the path order (x = 5, mu = 0, nu = D-1, 2 x 3)
+  const Cmat<N> W = wilson_rect<D, N>(h.U, 5, 0, D - 1, 2, 3);
+  std::vector<double> w;
+  for (const auto& z : W.a) { w.push_back(z.real()); w.push_back(z.imag()); }
+  put_array(tag + "_rect", w);
+}
+
+// Time-series analysis (observables.hpp:104-145 Stats, autocorr.hpp:55-212) of
+// AR(1) series x_{t+1} = a x_t + sqrt(1-a^2) eta_t and a constant series.
+static void dump_series(const std::string& tag, double a, int n, std::uint64_t seed) {
+  std::vector<Real> x(static_cast<std::size_t>(n));
+  if (a >= 0.0) {
+    std::mt19937_64 rng(seed);
+    std::normal_distribution<Real> g(0.0, 1.0);
+    const Real s = std::sqrt(1.0 - a * a);
+    x[0] = g(rng);
+    for (int t = 1; t < n; ++t) x[t] = a * x[t - 1] + s * g(rng);
+  } else {
+    std::fill(x.begin(), x.end(), 3.7);
+  }
+  put_array(tag + "_x", x);
+  Stats st;
+  for (Real v : x) st.add(v);
+  int W = 0;
+  const Real tau = tau_int(x, &W);
+  put_array(tag + "_out", {st.mean(), st.var(), st.binder(), st.binned_error(), tau, double(W),
+                           effective_sample_size(x), naive_error(x), autocorr_error(x)});
+  std::vector<double> be;
+  for (int bs : {1, 3, 16, 50}) be.push_back(blocking_error(x, bs));
+  put_array(tag + "_blocking", be);
+  put_array(tag + "_rho", autocorr_function(x, 12));
+}
+
 // Multi-invariant quartic potential (scalar_invariants.hpp): channels, V and
 // dV/dphi^* on random phi, then the HMC with this potential plugged in.
 template <int D, int N>
@@ -326,6 +385,22 @@ static int dump(const char* dir) {
     dump_rep<3>("rep_su3_61", s, 34);
     dump_higgs<4, 3>("su3_61", s, 3, 5.7, 0.05, 0.5, 16, false);
   }
+  {
+    const std::vector<std::pair<int, int>> sizes = {{1, 1}, {1, 2}, {2, 1}, {2, 2}, {1, 3}, {3, 2}};
+    FundamentalRep<3> f; AdjointRep<3> a;
+    GeneralRep<3> s2({2}), s11({1, 1}), s21({2, 1}), s3({3}), s22({2, 2}), s61({6, 1});
+    dump_loops<4, 3>("loops_su3", {4, 3, 4, 5}, 5.7, 3,
+                     {{"fund", &f}, {"adj", &a}, {"2", &s2}, {"11", &s11}, {"21", &s21}, {"3", &s3}, {"22", &s22}, {"61", &s61}},
+                     sizes, {"61"});
+    FundamentalRep<2> f2; AdjointRep<2> a2;
+    GeneralRep<2> t2({2}), t3({3}), t4({4});
+    dump_loops<3, 2>("loops_su2", {5, 4, 3}, 2.3, 3, {{"fund", &f2}, {"adj", &a2}, {"2", &t2}, {"3", &t3}, {"4", &t4}}, sizes, {});
+  }
+  dump_series("series_white", 0.0, 1000, 0xA1u);
+  dump_series("series_ar05", 0.5, 4000, 0xB2u);
+  dump_series("series_ar09", 0.9, 8000, 0xC3u);
+  dump_series("series_ar099", 0.99, 3000, 0xD4u);
+  dump_series("series_const", -1.0, 1000, 0);
   std::ofstream m(std::string(dir) + "/manifest.json");
   m << "{\n" << g_manifest.str() << "\n}\n";
   return 0;

@@ -44,6 +44,8 @@ def lib():
             "orc_rep_T": (None, [vp, dp]), "orc_rep_W": (None, [vp, dp]),
             "orc_rep_apply": (None, [vp, dp, dp, i, dp]), "orc_rep_hop_g": (None, [vp, dp, dp, dp, dp]),
             "orc_rep_matrix": (None, [vp, dp, dp]),
+            "orc_wilson_loop": (d, [vp, C.c_char_p, i, i]), "orc_polyakov_rep": (d, [vp, C.c_char_p, i]),
+            "orc_wilson_rect": (None, [vp, C.c_long, i, i, i, i, dp]),
             "orc_splitmix64": (u64, [u64]), "orc_key": (u64, [u64, u64, u64, u64, u64]),
             "orc_uniform": (d, [u64, u64]), "orc_gauss": (d, [u64, u64]),
             "orc_gauss_many": (None, [u64, C.POINTER(C.c_uint64), C.c_long, dp]),
@@ -222,6 +224,24 @@ class Oracle:
         out = np.zeros(2 * self.d * self.d)
         lib().orc_rep_matrix(self.h, _dp(U.view(np.float64)), _dp(out))
         return out.view(np.complex128).reshape(self.d, self.d)
+
+    # measurement layer (observables.hpp:36-101)
+    def wilson_loop(self, probe, Ra, Rb):
+        v = lib().orc_wilson_loop(self.h, probe.encode(), Ra, Rb)
+        if np.isnan(v):
+            raise RuntimeError(lib().orc_last_error().decode())
+        return v
+
+    def polyakov_rep(self, probe, tdir):
+        v = lib().orc_polyakov_rep(self.h, probe.encode(), tdir)
+        if np.isnan(v):
+            raise RuntimeError(lib().orc_last_error().decode())
+        return v
+
+    def wilson_rect(self, x, mu, nu, Ra, Rb):
+        out = np.zeros(2 * self.N * self.N)
+        lib().orc_wilson_rect(self.h, x, mu, nu, Ra, Rb, _dp(out))
+        return out.view(np.complex128).reshape(self.N, self.N)
 
 
 def splitmix64(x):

@@ -8,7 +8,9 @@ expression, but g++ contracts FMAs per call site, so a few ulps differ).
 """
 import numpy as np
 import pytest
+import scipy.linalg
 
+import loop_cases as LC
 import oracle_lib as O
 
 # (tag, D, N, rep, L, beta, kappa, nmd, with_traj)
@@ -233,3 +235,65 @@ def test_multi_invariant_potential_matches_reference(goldens, case):
         assert np.max(np.abs(np.array([r[1] for r in res]) - g[f"{tag}_{integ}_dH"])) < 1e-8
         assert np.array_equal(np.array([float(r[0]) for r in res]), g[f"{tag}_{integ}_acc"])
         assert rel(o.get("phi"), g[f"{tag}_{integ}_phi2"]) < 1e-11
+
+
+@pytest.mark.parametrize("case", LC.CASES, ids=[c[0] for c in LC.CASES])
+def test_wilson_and_polyakov_loops_match_reference(goldens, case):
+    """Measurement layer (observables.hpp:36-101): the rectangle path product,
+    wilson_loop_fund, wilson_loop_rep and polyakov_loop_rep in every probe irrep
+    on the reference's thermalised field. The d = 48 probe {6,1} costs ~15 s per
+    Wilson-loop size in the tensor restatement, so only its Polyakov loop runs
+    here; the GPU test covers its Wilson loops against the same fixtures."""
+    g, sc = goldens
+    tag, D, N, ext, labels = case
+    o = O.Oracle(D, N, list(ext))
+    o.set("U", g[tag + "_U"].view(np.complex128).reshape(-1, N, N))
+    r = o.wilson_rect(5, 0, D - 1, 2, 3)
+    assert np.max(np.abs(r.ravel() - g[tag + "_rect"].view(np.complex128))) < 1e-14
+    for Ra, Rb in LC.SIZES:
+        assert abs(o.wilson_loop("fund", Ra, Rb) - sc[f"{tag}_Wfund_{Ra}x{Rb}"]) < 1e-14
+    for lab in labels:
+        spec = LC.SPEC[lab]
+        if lab != "61":
+            for Ra, Rb in LC.sizes_for(lab):
+                assert abs(o.wilson_loop(spec, Ra, Rb) - sc[f"{tag}_W_{lab}_{Ra}x{Rb}"]) < 1e-14, (lab, Ra, Rb)
+        for t in range(D) if lab != "61" else [D - 1]:
+            assert abs(o.polyakov_rep(spec, t) - sc[f"{tag}_P_{lab}_t{t}"]) < 1e-14, (lab, t)
+
+
+def test_loops_closed_forms():
+    """What the mathematics fixes: on the cold lattice every loop in every probe
+    is 1; the 1x1 fundamental loop is the average plaquette; a random gauge
+    transformation leaves the loops unchanged; the conjugate probe {1,1} of
+    SU(3) gives the same real part as the fundamental."""
+    o = O.Oracle(4, 3, [3, 4, 3, 4])
+    o.cold()
+    for spec in ("fund", "adj", "2", "2,1", "3"):
+        assert abs(o.wilson_loop(spec, 2, 1) - 1.0) < 1e-14
+        assert abs(o.polyakov_rep(spec, 1) - 1.0) < 1e-14
+    o.hot(0.4)
+    assert abs(o.wilson_loop("fund", 1, 1) - o.observables()["plaq"]) < 1e-14
+    U = o.get("U")
+    before = [o.wilson_loop(s, 2, 2) for s in ("fund", "adj", "2")] + [o.polyakov_rep("2,1", 0)]
+    assert abs(o.wilson_loop("1,1", 2, 1) - o.wilson_loop("fund", 2, 1)) < 1e-14
+    # gauge transformation U_mu(x) -> G(x) U_mu(x) G(x+mu)^dag with random SU(3) G
+    rng = np.random.default_rng(5)
+    ext = [3, 4, 3, 4]
+    V = int(np.prod(ext))
+    Gs = []
+    for _ in range(V):
+        H = rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3))
+        H = 0.5 * (H + H.conj().T)
+        Gs.append(scipy.linalg.expm(1j * H))
+    strides = np.cumprod([1] + ext[:-1])
+    Ut = U.reshape(V, 4, 3, 3).copy()
+    for s in range(V):
+        x = [(s // strides[m]) % ext[m] for m in range(4)]
+        for mu in range(4):
+            y = list(x)
+            y[mu] = (y[mu] + 1) % ext[mu]
+            sy = sum(y[m] * strides[m] for m in range(4))
+            Ut[s, mu] = Gs[s] @ U.reshape(V, 4, 3, 3)[s, mu] @ Gs[sy].conj().T
+    o.set("U", Ut.reshape(-1, 3, 3))
+    after = [o.wilson_loop(s, 2, 2) for s in ("fund", "adj", "2")] + [o.polyakov_rep("2,1", 0)]
+    assert np.max(np.abs(np.array(before) - np.array(after))) < 1e-13
