@@ -177,6 +177,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
     ap.add_argument("--stream-seg", type=int, default=None, help="force a streaming segment (implies --no-tune)")
+    ap.add_argument("--block-order", type=int, default=None, choices=(0, 1, 2),
+                    help="force the block walk of the MD-step kernel (implies --no-tune)")
     ap.add_argument("--no-learned", action="store_true", help="skip the exact vs learned-force HMC comparison")
     ap.add_argument("--only-learned", action="store_true", help="run only the exact vs learned-force comparison")
     ap.add_argument("--learned-meas", type=int, default=24, help="measured trajectories per chain in the comparison")
@@ -193,7 +195,7 @@ def main():
         from gauge_higgs_amd import learned
         print(json.dumps({"learned_force": learned.compare(n_meas=args.learned_meas, use_graph=not args.no_graph)}))
         return 0
-    if args.stream_seg is not None:
+    if args.stream_seg is not None or args.block_order is not None:
         args.no_tune = True
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU path is not built.", file=sys.stderr)
@@ -208,12 +210,15 @@ def main():
     g.hot(0.8, 12345, 1.0)
     g.refresh_momenta()
     V = g.lat.vols[0]
-    tuned = {"stream_seg": args.stream_seg or 0}
+    tuned = {"stream_seg": args.stream_seg or 0, "block_order": args.block_order or 0}
     if args.stream_seg is not None:
         g.lat.set_stream_segment(args.stream_seg)
+    if args.block_order is not None:
+        g.lat.set_block_order(args.block_order)
+        tuned["block_order"] = g.lat.block_order()
     if not args.no_tune:
         seg, tms = g.lat.autotune(steps=5)  # launch-shape autotuner (fields restored)
-        tuned = {"stream_seg": seg, "ms_per_step_at_tune": tms}
+        tuned = {"stream_seg": seg, "block_order": g.lat.block_order(), "ms_per_step_at_tune": tms}
     if args.warmup:
         g.lat.time_md_steps(args.warmup)
     g.lat.sync()

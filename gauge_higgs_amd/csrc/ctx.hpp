@@ -49,6 +49,7 @@ struct ghx_ctx {
   std::string err;
   const char* kernel_path = "";
   int stream_seg = 0;  // streaming MD kernel (0 = one site per thread)
+  int block_order = 0;  // BlockWalk order of the MD-step kernel (single lattice)
   // multi-timescale integrator (0 = single scale)
   int n_outer = 0, m_inner = 1;
   // surrogate link force (learned); S planes are filled by surr_fill before a kick
@@ -85,6 +86,22 @@ struct ghx_ctx {
   size_t nU() const { return size_t(D) * 2 * N * link_rows(); }
   size_t nP() const { return size_t(D) * (N * N - 1); }
   size_t nphi() const { return size_t(2) * rep.d; }
+  // The requested block walk when it applies (one lattice whose slices and
+  // (D-2)-planes are whole blocks and whose last extent splits over the 8 XCDs),
+  // else storage order.
+  BlockWalk block_walk() const {
+    BlockWalk w;
+    if (block_order == 0 || nrep != 1) return w;
+    const RepDesc& r = hreps[0];
+    const unsigned Lt = unsigned(r.L[D - 1]), V3 = r.stride[D - 1], plane = r.stride[D - 2];
+    if (V3 % kBlock || plane % kBlock || Lt % 8 || nblk != V3 / kBlock * Lt) return w;
+    w.order = block_order;
+    w.nbt = V3 / kBlock;
+    w.tt = Lt / 8;
+    w.ngy = plane / kBlock;
+    w.lz = unsigned(r.L[D - 2]);
+    return w;
+  }
   LaunchCtx lc() const {
     LaunchCtx c;
     c.stream = stream; c.nblk = nblk; c.nred = nred; c.Vp = Vp;
@@ -102,6 +119,7 @@ struct ghx_ctx {
     c.stream_seg = (nrep == 1) ? stream_seg : 0;
     c.V3 = hreps[0].stride[D - 1];
     c.Lt = unsigned(hreps[0].L[D - 1]);
+    c.walk = block_walk();
     return c;
   }
 };

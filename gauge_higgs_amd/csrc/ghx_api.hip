@@ -1053,12 +1053,19 @@ int ghx_set_tuning(ghx_ctx* ctx, int key, int value) {
     drop_graph(ctx);
     return GHX_OK;
   }
+  if (key == GHX_TUNE_BLOCK_ORDER) {
+    if (value < 0 || value > 2) return fail(ctx, GHX_E_INVALID, "block order must be 0, 1 or 2");
+    ctx->block_order = value;
+    drop_graph(ctx);
+    return GHX_OK;
+  }
   return fail(ctx, GHX_E_INVALID, "unknown tuning key");
 }
 
 int ghx_get_tuning(const ghx_ctx* ctx, int key, int* value) {
   if (!ctx || !value) return GHX_E_INVALID;
   if (key == GHX_TUNE_STREAM_SEG) { *value = ctx->stream_seg; return GHX_OK; }
+  if (key == GHX_TUNE_BLOCK_ORDER) { *value = ctx->block_walk().order; return GHX_OK; }
   return GHX_E_INVALID;
 }
 
@@ -1080,17 +1087,23 @@ int ghx_autotune(ghx_ctx* ctx, int steps, int* best_seg, double* best_ms) {
     HIPCHK(hipMemcpyAsync(ctx->phisave, ctx->phi[0], nphb, hipMemcpyDeviceToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(pis, ctx->pi, nphb, hipMemcpyDeviceToDevice, ctx->stream));
   }
-  std::vector<int> cand = {0};
+  // candidates (stream segment, block order); the walks only for one site per thread
+  std::vector<std::pair<int, int>> cand = {{0, 0}};
+  for (int order : {1, 2}) {
+    ctx->block_order = order;
+    if (ctx->block_walk().order == order) cand.push_back({0, order});
+  }
   if (ctx->nrep == 1)
     for (int sg : {1, 2, 4, 8, 16, 32, 64})
-      if (sg <= ctx->hreps[0].L[ctx->D - 1]) cand.push_back(sg);
-  int best = ctx->stream_seg;
+      if (sg <= ctx->hreps[0].L[ctx->D - 1]) cand.push_back({sg, 0});
+  int best = ctx->stream_seg, best_order = 0;
   double bms = 1e300;
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  for (int sg : cand) {
+  for (const auto& [sg, order] : cand) {
     ctx->stream_seg = sg;
+    ctx->block_order = order;
     launch_step(ctx, 1.0, 1.0, true);  // warm
     HIPCHK(hipEventRecord(e0, ctx->stream));
     for (int i = 0; i < steps; ++i) launch_step(ctx, 1.0, 1.0, true);
@@ -1098,11 +1111,12 @@ int ghx_autotune(ghx_ctx* ctx, int steps, int* best_seg, double* best_ms) {
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-    if (ms < bms) { bms = ms; best = sg; }
+    if (ms < bms) { bms = ms; best = sg; best_order = order; }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   ctx->stream_seg = best;
+  ctx->block_order = best_order;
   ctx->cur = 0;
   HIPCHK(hipMemcpyAsync(ctx->U[0], ctx->Usave, nUb, hipMemcpyDeviceToDevice, ctx->stream));
   HIPCHK(hipMemcpyAsync(ctx->P, Ps, nPb, hipMemcpyDeviceToDevice, ctx->stream));

@@ -10,6 +10,7 @@
 // Replicas are laid back to back; each starts on a block boundary so a block
 // never straddles two lattices (blk_rep maps block -> replica).
 #pragma once
+#include "ops.hpp"
 #include "reps.hpp"
 
 namespace ghx {
@@ -63,6 +64,7 @@ struct StepArgs {
   const T* S;     // surrogate link force planes (F_SURR)
   double surr_sign;
   const T* Fon;   // on-site scalar force planes (F_EXT_ONSITE)
+  BlockWalk walk{};  // block order of md_step_kernel
 };
 
 // Link storage. FP64 (parity mode) stores all N rows. FP32 MD stores the first
@@ -214,6 +216,17 @@ GHX_DEV unsigned xcd_block(unsigned b, unsigned n) {
   if (b < q * kXcd) return (b % kXcd) * q + b / kXcd;
 #endif
   return b;
+}
+
+// Storage-order block of the p-th block in the walk w (see BlockWalk).
+GHX_DEV unsigned walk_block(unsigned p, const BlockWalk& w) {
+  if (w.order == 0) return p;
+  const unsigned span = w.nbt * w.tt;
+  const unsigned chunk = p / span, rem = p % span;
+  unsigned j = rem / w.tt;
+  const unsigned t = chunk * w.tt + rem % w.tt;
+  if (w.order == 2) j = (j % w.lz) * w.ngy + j / w.lz;
+  return t * w.nbt + j;
 }
 
 // Site -> neighbour offsets for a periodic lattice (geometry.hpp:20-38 order:
@@ -421,7 +434,7 @@ GHX_DEV void site_update(const StepArgs<T>& a, const typename REP::template Para
 // One site per thread; blocks renumbered XCD-aware.
 template <typename T, int N, int D, class REP>
 __global__ __launch_bounds__(kBlock, GHX_MD_WAVES) void md_step_kernel(StepArgs<T> a, typename REP::template Params<T> rp) {
-  const unsigned bid = xcd_block(blockIdx.x, gridDim.x);
+  const unsigned bid = walk_block(xcd_block(blockIdx.x, gridDim.x), a.walk);
   const int r = a.blk_rep[bid];
   const RepDesc& rd = a.reps[r];
   const unsigned s = bid * kBlock + threadIdx.x - rd.offset;

@@ -34,7 +34,7 @@ struct Launch {
       }
     }
     StepArgs<T> a{c.reps, c.blk_rep, c.Vp, (const T*)U_in, (T*)U_out, (T*)P, (const T*)phi_in, (T*)phi_out, (T*)pi, (T*)F, (T*)Fphi,
-                  ck, cd, flags, (const T*)c.G, (const T*)c.S, c.surr_sign, (const T*)c.Fon};
+                  ck, cd, flags, (const T*)c.G, (const T*)c.S, c.surr_sign, (const T*)c.Fon, c.walk};
     if (c.stream_seg > 0) {
       const unsigned V3 = c.V3, Lt = c.Lt;
       unsigned ntile = (V3 + kBlock - 1) / kBlock;

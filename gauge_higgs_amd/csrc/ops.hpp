@@ -8,6 +8,20 @@ namespace ghx {
 struct RepDesc;
 struct McState;
 
+// Order in which the MD-step kernel walks the blocks of a single lattice
+// (D >= 3, extents L_0..L_{D-1}, blocks of 128 sites in storage order).
+//  0: storage order (the last direction slowest);
+//  1: each XCD's contiguous share of Lt/8 slices of the last direction is
+//     walked with that direction fastest, so a block's +-t neighbour blocks run
+//     1-3 blocks earlier or later on the same XCD (still in its L2) instead of
+//     a whole slice of blocks apart;
+//  2: as 1, and inside a slice direction D-2 runs before the lower directions.
+// nbt = blocks per slice, tt = Lt/8, ngy = blocks per (D-2)-plane, lz = L_{D-2}.
+struct BlockWalk {
+  int order = 0;
+  unsigned nbt = 1, tt = 1, ngy = 1, lz = 1;
+};
+
 struct LaunchCtx {
   hipStream_t stream;
   unsigned nblk, nred, Vp;
@@ -21,6 +35,7 @@ struct LaunchCtx {
   const void* G;    // external matter link force planes (large reps)
   int stream_seg;   // > 0: single-lattice streaming kernel with this many t-slices per block
   unsigned V3, Lt;  // cross-section volume and last extent of replica 0 (streaming kernel)
+  BlockWalk walk;   // block order of the MD-step kernel
   const void* S;    // surrogate link force planes
   double surr_sign;
   int frozen;       // |phi_x| = 1 scalar: adds F_FROZEN to every MD launch

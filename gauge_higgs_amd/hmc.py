@@ -188,6 +188,15 @@ class Lattices:
         self._check(self._lib.ghx_get_tuning(self.h, 0, C.byref(v)))
         return v.value
 
+    def set_block_order(self, order):
+        """Block walk of the one-site-per-thread MD kernel (0, 1, 2; see ghx.h)."""
+        self._check(self._lib.ghx_set_tuning(self.h, 1, order))
+
+    def block_order(self):
+        v = C.c_int()
+        self._check(self._lib.ghx_get_tuning(self.h, 1, C.byref(v)))
+        return v.value
+
     def autotune(self, steps=10):
         """Pick the fastest MD-step launch shape on the live fields (fields restored)."""
         best, ms = C.c_int(), C.c_double()

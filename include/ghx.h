@@ -170,10 +170,14 @@ int ghx_swap_stats(ghx_ctx* ctx, int* npairs, int* pairs, uint64_t* attempts, ui
 
 /* Launch-shape tuning (no counterpart in the reference, which runs OpenMP
  * loops): GHX_TUNE_STREAM_SEG = number of slices of the last direction one
- * workgroup walks in the MD-step kernel (0 = one site per thread). The
- * autotuner times every candidate for `steps` MD steps on the live fields,
- * restores the fields, and keeps the fastest. */
-enum { GHX_TUNE_STREAM_SEG = 0 };
+ * workgroup walks in the MD-step kernel (0 = one site per thread);
+ * GHX_TUNE_BLOCK_ORDER = order in which the one-site-per-thread kernel walks the
+ * blocks of a single lattice (0 storage order; 1 each XCD's share of slices
+ * walked with the last direction fastest; 2 as 1 with direction D-2 next;
+ * get_tuning reports 0 where the order does not apply). The autotuner times
+ * every candidate for `steps` MD steps on the live fields, restores the
+ * fields, and keeps the fastest. */
+enum { GHX_TUNE_STREAM_SEG = 0, GHX_TUNE_BLOCK_ORDER = 1 };
 int ghx_set_tuning(ghx_ctx* ctx, int key, int value);
 int ghx_get_tuning(const ghx_ctx* ctx, int key, int* value);
 int ghx_autotune(ghx_ctx* ctx, int steps, int* best_seg, double* best_ms_per_step);

@@ -251,6 +251,33 @@ def test_streaming_kernel_equals_site_kernel(seg, prec):
         assert np.array_equal(out[0][k], out[1][k])
 
 
+@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("prec", [gha.FP64, gha.FP32MD])
+def test_block_walk_equals_storage_order(order, prec):
+    """Walking the blocks in another order changes only the schedule: every
+    site update reads the step's input fields and writes its own site, so the
+    trajectory is bitwise equal. 16x8x4x8 is the smallest shape the walk
+    applies to (whole-block slices and planes, Lt divisible by 8)."""
+    out = []
+    for o in (0, order):
+        g = gha.GaugeHiggsHMC([16, 8, 4, 8], "2", seed=9, D=4, N=3, precision=prec).configure(beta=5.7, kappa=0.1, lam=0.5, nmd=4)
+        g.lat.set_block_order(o)
+        assert g.lat.block_order() == o
+        g.hot(0.8, 12345, 0.5)
+        g.trajectory()
+        g.lat.trajectories(2, use_graph=True)
+        out.append((g.last_dH, g.get("U"), g.get("phi"), g.get("P")))
+    assert out[0][0] == out[1][0]
+    for k in (1, 2, 3):
+        assert np.array_equal(out[0][k], out[1][k])
+
+
+def test_block_walk_falls_back_where_it_does_not_apply():
+    g = gha.GaugeHiggsHMC([4, 6, 4, 10], "2", seed=9, D=4, N=3)
+    g.lat.set_block_order(1)
+    assert g.lat.block_order() == 0
+
+
 def test_autotune_restores_fields():
     g = gha.GaugeHiggsHMC(8, "2", seed=2, D=4, N=3, precision=gha.FP32MD).configure(beta=5.7, kappa=0.1, lam=0.5)
     g.hot(0.8, 12345, 0.5)
