@@ -82,7 +82,7 @@ struct ghx_ctx {
   unsigned maxvol = 0;
 
   // link planes per site: FP32 MD stores rows 0..N-2 (the last row is rebuilt on load)
-  int link_rows() const { return tsize == 4 ? N - 1 : N; }
+  int link_rows() const { return (tsize == 4 && GHX_COMPRESS_LINKS) ? N - 1 : N; }
   size_t nU() const { return size_t(D) * 2 * N * link_rows(); }
   size_t nP() const { return size_t(D) * (N * N - 1); }
   size_t nphi() const { return size_t(2) * rep.d; }

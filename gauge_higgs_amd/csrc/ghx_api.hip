@@ -1209,8 +1209,7 @@ int ghx_set_multi_potential(ghx_ctx* ctx, const double* f, int nf, double mu2) {
   if (nf != int(lam.size()))
     return fail(ctx, GHX_E_INVALID, "need " + std::to_string(lam.size()) + " channel couplings, got " + std::to_string(nf));
   const std::vector<std::complex<double>> A = multi_superop(ctx->rep, lam, std::vector<double>(f, f + nf));
-  std::vector<double> Ad(A.size() * 2);
-  for (size_t i = 0; i < A.size(); ++i) { Ad[2 * i] = A[i].real(); Ad[2 * i + 1] = A[i].imag(); }
+  const std::vector<double> Ad = hermitian_superop(ctx->rep.d, A);
   if (ctx->tsize == 8) {
     HIPCHK(hipMalloc(&ctx->dAq, Ad.size() * 8));
     HIPCHK(hipMemcpy(ctx->dAq, Ad.data(), Ad.size() * 8, hipMemcpyHostToDevice));

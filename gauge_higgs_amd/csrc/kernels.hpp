@@ -72,8 +72,11 @@ struct StepArgs {
 //   SU(2): row 1 = (-conj(U01), conj(U00));  SU(3): row 2 = conj(row 0 x row 1).
 // This cuts link traffic by 1/3 (SU(3)) or 1/2 (SU(2)) in the bandwidth-bound
 // MD step; the reconstructed matrix is unitary to the rounding of rows 0..N-2.
+#ifndef GHX_COMPRESS_LINKS
+#define GHX_COMPRESS_LINKS 1  // FP32 links keep N-1 rows
+#endif
 template <typename T, int N>
-constexpr int link_rows() { return sizeof(T) == 4 ? N - 1 : N; }
+constexpr int link_rows() { return (sizeof(T) == 4 && GHX_COMPRESS_LINKS) ? N - 1 : N; }
 template <typename T, int N>
 constexpr int link_reals() { return 2 * N * link_rows<T, N>(); }
 
@@ -525,50 +528,84 @@ __global__ __launch_bounds__(kBlock) void frozen_site_kernel(const RepDesc* reps
   }
 }
 
-// Multi-invariant on-site term (scalar_invariants.hpp:62-87, 119-146) with the
-// phi-independent superoperator A (d^2 x d^2, complex, row (p*d+q), column
-// (i*d+j)): vM = vec(phi phi^dag), g = -mu2 phi + reshape(A vM) phi,
-// V = -mu2 |phi|^2 + Re <vM, A vM> / 2. Writes Fon = -2 g (the on-site
-// force planes) and/or Von = V (FP64, for the action).
+// Multi-invariant on-site term (scalar_invariants.hpp:62-87, 119-146):
+// g = -mu2 phi + A(M) phi with M = phi phi^dag, V = -mu2 |phi|^2 + Tr(M A(M)) / 2.
+// A maps Hermitian matrices to Hermitian ones, so it acts as a real d^2 x d^2
+// matrix A_R on the coordinates m_k = Tr(E_k M) in an orthonormal Hermitian
+// basis (k < d: diagonal units; then per pair i < j, row-major, S and K with
+// m_S = sqrt2 Re M_ij, m_K = sqrt2 Im M_ij). t = A_R m gives T = A(M):
+// T_ii = t_i, T_ij = (t_S + i t_K)/sqrt2, T_ji = conj(T_ij). That is d^4 real
+// FMA per site instead of d^4 complex ones; A_R is read with uniform (scalar)
+// loads. Writes Fon = -2 g (the on-site force planes) and/or Von = V (FP64).
 template <typename T, int DR>
 __global__ __launch_bounds__(kBlock) void multi_onsite_kernel(const RepDesc* reps, const int* blk_rep, unsigned Vp,
                                                              const T* __restrict__ phi, const T* __restrict__ A, double mu2,
                                                              T* Fon, double* Von) {
   using R = T;
+  constexpr int DD = DR * DR;
   const int r = blk_rep[blockIdx.x];
   const RepDesc& rd = reps[r];
   const unsigned s = blockIdx.x * kBlock + threadIdx.x - rd.offset;
   if (s >= rd.vol) return;
   const unsigned g = rd.offset + s;
+  const R h = R(0.70710678118654752440);  // 1/sqrt2
   cx<R> ph[DR];
   load_vec<T, R, DR>(phi, Vp, g, ph);
-  cx<R> vM[DR * DR];
+  R m[DD];
   R n2 = R(0);
 #pragma unroll
   for (int i = 0; i < DR; ++i) {
-    n2 += ph[i].r * ph[i].r + ph[i].i * ph[i].i;
+    m[i] = ph[i].r * ph[i].r + ph[i].i * ph[i].i;
+    n2 += m[i];
+  }
+  {
+    int k = DR;
 #pragma unroll
-    for (int j = 0; j < DR; ++j) vM[i * DR + j] = cmul(ph[i], cconj(ph[j]));
+    for (int i = 0; i < DR; ++i)
+#pragma unroll
+      for (int j = i + 1; j < DR; ++j, k += 2) {
+        const cx<R> z = cmul(ph[i], cconj(ph[j]));
+        m[k] = R(2) * h * z.r;
+        m[k + 1] = R(2) * h * z.i;
+      }
   }
   const R m2 = R(mu2);
+  cx<R> gv[DR];
+#pragma unroll
+  for (int i = 0; i < DR; ++i) gv[i] = {-m2 * ph[i].r, -m2 * ph[i].i};
   R v2 = R(0);
-#pragma unroll 1
-  for (int p = 0; p < DR; ++p) {
-    const cx<R> fp = {R(phi[size_t(2 * p) * Vp + g]), R(phi[size_t(2 * p + 1) * Vp + g])};
-    cx<R> gp = {-m2 * fp.r, -m2 * fp.i};
 #pragma unroll
-    for (int q = 0; q < DR; ++q) {
-      const T* row = A + size_t(2 * (p * DR + q)) * (DR * DR);
-      cx<R> am = {R(0), R(0)};
+  for (int k = 0; k < DR; ++k) {  // diagonal units
+    R t = R(0);
 #pragma unroll
-      for (int k = 0; k < DR * DR; ++k) cfma(am, cx<R>{row[2 * k], row[2 * k + 1]}, vM[k]);
-      cfma(gp, am, ph[q]);
-      const cx<R> w = cmul(cconj(fp), ph[q]);  // conj(vM_pq)
-      v2 += w.r * am.r - w.i * am.i;
-    }
-    if (Fon) {
-      Fon[size_t(2 * p) * Vp + g] = T(R(-2) * gp.r);
-      Fon[size_t(2 * p + 1) * Vp + g] = T(R(-2) * gp.i);
+    for (int l = 0; l < DD; ++l) t = fma(A[k * DD + l], m[l], t);
+    v2 = fma(t, m[k], v2);
+    gv[k].r = fma(t, ph[k].r, gv[k].r);
+    gv[k].i = fma(t, ph[k].i, gv[k].i);
+  }
+  {
+    int k = DR;
+#pragma unroll
+    for (int i = 0; i < DR; ++i)
+#pragma unroll
+      for (int j = i + 1; j < DR; ++j, k += 2) {
+        R tS = R(0), tK = R(0);
+#pragma unroll
+        for (int l = 0; l < DD; ++l) {
+          tS = fma(A[k * DD + l], m[l], tS);
+          tK = fma(A[(k + 1) * DD + l], m[l], tK);
+        }
+        v2 = fma(tS, m[k], fma(tK, m[k + 1], v2));
+        const cx<R> Tij = {h * tS, h * tK};
+        cfma(gv[i], Tij, ph[j]);
+        cfma(gv[j], cconj(Tij), ph[i]);
+      }
+  }
+  if (Fon) {
+#pragma unroll
+    for (int i = 0; i < DR; ++i) {
+      Fon[size_t(2 * i) * Vp + g] = T(R(-2) * gv[i].r);
+      Fon[size_t(2 * i + 1) * Vp + g] = T(R(-2) * gv[i].i);
     }
   }
   if (Von) Von[g] = double(-m2 * n2) + 0.5 * double(v2);

@@ -126,4 +126,43 @@ std::vector<cplx> multi_superop(const RepInfo& r, const std::vector<double>& lam
   return A;
 }
 
+namespace {
+// E_k of the orthonormal Hermitian basis as a d x d matrix.
+std::vector<cplx> hermitian_unit(int d, int k) {
+  std::vector<cplx> E(size_t(d) * d, cplx(0, 0));
+  if (k < d) { E[size_t(k) * d + k] = 1.0; return E; }
+  const double h = std::sqrt(0.5);
+  int pr = (k - d) / 2;
+  const bool imag = (k - d) % 2 == 1;
+  int i = 0;
+  while (pr >= d - 1 - i) { pr -= d - 1 - i; ++i; }
+  const int j = i + 1 + pr;
+  E[size_t(i) * d + j] = imag ? cplx(0, h) : cplx(h, 0);
+  E[size_t(j) * d + i] = imag ? cplx(0, -h) : cplx(h, 0);
+  return E;
+}
+}  // namespace
+
+std::vector<double> hermitian_superop(int d, const std::vector<cplx>& A) {
+  const int dd = d * d;
+  std::vector<std::vector<cplx>> E(dd);
+  for (int k = 0; k < dd; ++k) E[size_t(k)] = hermitian_unit(d, k);
+  std::vector<double> AR(size_t(dd) * dd);
+  std::vector<cplx> T(dd);
+  for (int l = 0; l < dd; ++l) {
+    for (int p = 0; p < dd; ++p) {  // T = A vec(E_l)
+      cplx s(0, 0);
+      for (int q = 0; q < dd; ++q) s += A[size_t(p) * dd + q] * E[size_t(l)][size_t(q)];
+      T[size_t(p)] = s;
+    }
+    for (int k = 0; k < dd; ++k) {  // Tr(E_k T) = sum_ij (E_k)_ij T_ji
+      cplx tr(0, 0);
+      for (int i = 0; i < d; ++i)
+        for (int j = 0; j < d; ++j) tr += E[size_t(k)][size_t(i) * d + j] * T[size_t(j) * d + i];
+      AR[size_t(k) * dd + l] = tr.real();
+    }
+  }
+  return AR;
+}
+
 }  // namespace ghx
