@@ -57,6 +57,7 @@ struct BigArgs {
   double ck, cd;
   int flags;             // F_KICK / F_DRIFT / F_ABS_EPS as in StepArgs
   int mode;
+  const T* Fon = nullptr;  // on-site force planes of a plugged-in potential (else the quartic)
 };
 
 // In-place (M on slot s) over the whole tensor: each thread owns whole fibres
@@ -215,8 +216,13 @@ __global__ __launch_bounds__(kBigBlock) void big_matter_kernel(BigArgs<T> a, Big
   const R ek = R(abs_eps ? a.ck : a.ck * rd.eps), ed = R(abs_eps ? a.cd : a.cd * rd.eps);
   for (int l = threadIdx.x; l < d; l += kBigBlock) {
     cx<R> f = fph[l];
-    f.r = fma(c, phx[l].r, f.r);
-    f.i = fma(c, phx[l].i, f.i);
+    if (!a.Fon) {
+      f.r = fma(c, phx[l].r, f.r);
+      f.i = fma(c, phx[l].i, f.i);
+    } else if (!(a.flags & F_NO_ONSITE)) {
+      f.r += R(a.Fon[size_t(2 * l) * Vp + g]);
+      f.i += R(a.Fon[size_t(2 * l + 1) * Vp + g]);
+    }
     if (a.mode == BIG_FORCE) {
       a.Fphi[size_t(2 * l) * Vp + g] = T(f.r);
       a.Fphi[size_t(2 * l + 1) * Vp + g] = T(f.i);
@@ -242,7 +248,8 @@ __global__ __launch_bounds__(kBigBlock) void big_matter_kernel(BigArgs<T> a, Big
 // hops, without a workgroup and the LDS tensor per site.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void onsite_step_kernel(const RepDesc* reps, const int* blk_rep, unsigned Vp, const T* phi_in,
-                                                           T* phi_out, T* pi, int d, double ck, double cd, int flags) {
+                                                           T* phi_out, T* pi, int d, double ck, double cd, int flags,
+                                                           const T* Fon) {
   using R = T;
   const int r = blk_rep[blockIdx.x];
   const RepDesc& rd = reps[r];
@@ -259,7 +266,12 @@ __global__ __launch_bounds__(kBlock) void onsite_step_kernel(const RepDesc* reps
   const R ek = R(abs_eps ? ck : ck * rd.eps), ed = R(abs_eps ? cd : cd * rd.eps);
   for (int l = 0; l < d; ++l) {
     const R xr = R(phi_in[size_t(2 * l) * Vp + g]), xi = R(phi_in[size_t(2 * l + 1) * Vp + g]);
-    const R fr = fma(c, xr, R(0)), fi = fma(c, xi, R(0));
+    R fr = fma(c, xr, R(0)), fi = fma(c, xi, R(0));
+    if (Fon) {
+      const bool on = !(flags & F_NO_ONSITE);
+      fr = on ? R(Fon[size_t(2 * l) * Vp + g]) : R(0);
+      fi = on ? R(Fon[size_t(2 * l + 1) * Vp + g]) : R(0);
+    }
     R pr = R(pi[size_t(2 * l) * Vp + g]), pim = R(pi[size_t(2 * l + 1) * Vp + g]);
     if (flags & F_KICK) {
       pr = fma(ek, fr, pr);
@@ -278,7 +290,8 @@ __global__ __launch_bounds__(kBlock) void onsite_step_kernel(const RepDesc* reps
 // (BIG_ACTION), the rest streamed plane by plane.
 template <typename T, int N, int D>
 __global__ __launch_bounds__(kRedBlock) void big_action_kernel(ActArgs a, const T* __restrict__ U, const T* __restrict__ P,
-                                                              const T* __restrict__ phi, const T* __restrict__ pi, const double* q1, int d) {
+                                                              const T* __restrict__ phi, const T* __restrict__ pi, const double* q1, int d,
+                                                              const double* Von) {
   using R = double;
   constexpr int NGN = N * N - 1;
   const int r = a.red_rep[blockIdx.x];
@@ -305,11 +318,92 @@ __global__ __launch_bounds__(kRedBlock) void big_action_kernel(ActArgs a, const 
       n2 += x * x;
       p2 += y * y;
     }
-    q[2] = n2 + rd.lambda * (n2 - 1.0) * (n2 - 1.0);
+    q[2] = Von ? Von[g] : n2 + rd.lambda * (n2 - 1.0) * (n2 - 1.0);
     q[3] = n2;
     q[5] = p2;
   }
   block_reduce_store<kNQ>(q, a.partials + size_t(blockIdx.x) * kNQ);
+}
+
+// Multi-invariant on-site term for a runtime irrep dimension d <= kMultiMaxD
+// (the large-irrep path): the Hermitian-basis form of multi_onsite_kernel
+// (A_R real, d^2 x d^2, on m_k = Tr(E_k phi phi^dag)) with the per-site
+// vectors in LDS. A block holds SB sites, one per thread: m (d^2 x SB), phi
+// and the accumulated g (d x SB each). t_k = sum_l A_R[k][l] m_l reads A_R
+// with uniform loads; each t_k is applied at once (E_k has at most two
+// entries), so t is never stored.
+constexpr int kMultiMaxD = 16;
+template <typename T, int SB>
+__global__ __launch_bounds__(SB) void multi_onsite_lds_kernel(const RepDesc* reps, const int* blk_rep, unsigned Vp, int d,
+                                                             const T* __restrict__ phi, const T* __restrict__ A, double mu2,
+                                                             T* Fon, double* Von) {
+  using R = T;
+  extern __shared__ unsigned char smem_raw[];
+  const int dd = d * d;
+  R* m = reinterpret_cast<R*>(smem_raw);                 // [dd][SB]
+  cx<R>* ph = reinterpret_cast<cx<R>*>(m + size_t(dd) * SB);  // [d][SB]
+  cx<R>* gv = ph + size_t(d) * SB;                      // [d][SB]
+  const unsigned g0 = blockIdx.x * SB;                  // SB divides kBlock: one replica per block
+  const int r = blk_rep[g0 / kBlock];
+  const RepDesc& rd = reps[r];
+  const int t = threadIdx.x;
+  const unsigned g = g0 + t;
+  const bool live = g - rd.offset < rd.vol;
+  const R h = R(0.70710678118654752440);
+  const R m2 = R(mu2);
+  R n2 = R(0);
+  for (int i = 0; i < d; ++i) {
+    cx<R> x = {R(0), R(0)};
+    if (live) x = {R(phi[size_t(2 * i) * Vp + g]), R(phi[size_t(2 * i + 1) * Vp + g])};
+    ph[i * SB + t] = x;
+    gv[i * SB + t] = {-m2 * x.r, -m2 * x.i};
+    m[i * SB + t] = x.r * x.r + x.i * x.i;
+    n2 += m[i * SB + t];
+  }
+  {
+    int k = d;
+    for (int i = 0; i < d; ++i)
+      for (int j = i + 1; j < d; ++j, k += 2) {
+        const cx<R> z = cmul(ph[i * SB + t], cconj(ph[j * SB + t]));
+        m[k * SB + t] = R(2) * h * z.r;
+        m[(k + 1) * SB + t] = R(2) * h * z.i;
+      }
+  }
+  R v2 = R(0);
+  for (int k = 0; k < d; ++k) {  // diagonal units
+    R tk = R(0);
+    const T* row = A + size_t(k) * dd;
+    for (int l = 0; l < dd; ++l) tk = fma(R(row[l]), m[l * SB + t], tk);
+    v2 = fma(tk, m[k * SB + t], v2);
+    cx<R>& gk = gv[k * SB + t];
+    gk.r = fma(tk, ph[k * SB + t].r, gk.r);
+    gk.i = fma(tk, ph[k * SB + t].i, gk.i);
+  }
+  {
+    int k = d;
+    for (int i = 0; i < d; ++i)
+      for (int j = i + 1; j < d; ++j, k += 2) {
+        R tS = R(0), tK = R(0);
+        const T* rS = A + size_t(k) * dd;
+        const T* rK = rS + dd;
+        for (int l = 0; l < dd; ++l) {
+          const R ml = m[l * SB + t];
+          tS = fma(R(rS[l]), ml, tS);
+          tK = fma(R(rK[l]), ml, tK);
+        }
+        v2 = fma(tS, m[k * SB + t], fma(tK, m[(k + 1) * SB + t], v2));
+        const cx<R> Tij = {h * tS, h * tK};
+        cfma(gv[i * SB + t], Tij, ph[j * SB + t]);
+        cfma(gv[j * SB + t], cconj(Tij), ph[i * SB + t]);
+      }
+  }
+  if (!live) return;
+  if (Fon)
+    for (int i = 0; i < d; ++i) {
+      Fon[size_t(2 * i) * Vp + g] = T(R(-2) * gv[i * SB + t].r);
+      Fon[size_t(2 * i + 1) * Vp + g] = T(R(-2) * gv[i * SB + t].i);
+    }
+  if (Von) Von[g] = double(-m2 * n2) + 0.5 * double(v2);
 }
 
 }  // namespace ghx

@@ -1204,7 +1204,9 @@ int ghx_set_multi_potential(ghx_ctx* ctx, const double* f, int nf, double mu2) {
   free_multi(ctx);
   if (!f) return GHX_OK;
   if (!has_phi(ctx)) return fail(ctx, GHX_E_INVALID, "multi-invariant potential needs a scalar field");
-  if (ctx->big) return fail(ctx, GHX_E_UNSUPPORTED, "multi-invariant potential is not built for the large-irrep kernels");
+  if (ctx->big && ctx->rep.d > kMultiMaxD)
+    return fail(ctx, GHX_E_UNSUPPORTED, "multi-invariant potential is built for irreps of dimension <= " + std::to_string(kMultiMaxD) +
+                                            " (this one has " + std::to_string(ctx->rep.d) + ")");
   const std::vector<double> lam = casimir_channels(ctx->rep);
   if (nf != int(lam.size()))
     return fail(ctx, GHX_E_INVALID, "need " + std::to_string(lam.size()) + " channel couplings, got " + std::to_string(nf));

@@ -7,21 +7,46 @@ struct BigLaunch {
   static void matter(const LaunchCtx& c, const BigRepTables& tb, const void* U, const void* phi_in, void* phi_out, void* pi, void* G,
                      void* Fphi, double* q1, double ck, double cd, int flags, int mode) {
     BigArgs<T> a{c.reps, nullptr, c.blk_rep, c.Vp, (const T*)U, (const T*)phi_in, (T*)phi_out, (T*)pi, (T*)G, (T*)Fphi, q1, ck, cd, flags, mode};
+    if (c.multi && mode != BIG_ACTION && !(flags & F_NO_ONSITE) && (mode == BIG_FORCE || (flags & F_KICK))) {
+      multi(c, tb.d, phi_in, c.Fon, nullptr);
+      a.Fon = (const T*)c.Fon;
+    }
     const size_t lds = size_t(tb.dimT) * sizeof(cx<T>);
     hipLaunchKernelGGL((big_matter_kernel<T, N, D>), dim3(c.Vp), dim3(kBigBlock), lds, c.stream, a, tb);
   }
   static void action(const LaunchCtx& c, const void* U, const void* P, const void* phi, const void* pi, const double* q1, int d) {
     ActArgs a{c.reps, c.red_rep, c.red_first, c.Vp, c.partials};
+    if (c.multi) multi(c, d, phi, nullptr, c.Von);
     hipLaunchKernelGGL((big_action_kernel<T, N, D>), dim3(c.nred), dim3(kRedBlock), 0, c.stream, a, (const T*)U, (const T*)P,
-                       (const T*)phi, (const T*)pi, q1, d);
+                       (const T*)phi, (const T*)pi, q1, d, c.multi ? (const double*)c.Von : nullptr);
+  }
+  // Multi-invariant on-site force planes and/or per-site potential (d <= kMultiMaxD).
+  // Sites per block: the largest of 64, 32, 16 whose LDS (d^2 + 4d reals per site) fits 64 KB.
+  static void multi(const LaunchCtx& c, int d, const void* phi, void* Fon, double* Von) {
+    const size_t per = size_t(d * d + 4 * d) * sizeof(T);
+    const unsigned nsites = c.nblk * unsigned(kBlock);
+    if (per * 64 <= 65536)
+      hipLaunchKernelGGL((multi_onsite_lds_kernel<T, 64>), dim3(nsites / 64), dim3(64), per * 64, c.stream, c.reps, c.blk_rep, c.Vp,
+                         d, (const T*)phi, (const T*)c.Aq, c.mu2, (T*)Fon, Von);
+    else if (per * 32 <= 65536)
+      hipLaunchKernelGGL((multi_onsite_lds_kernel<T, 32>), dim3(nsites / 32), dim3(32), per * 32, c.stream, c.reps, c.blk_rep, c.Vp,
+                         d, (const T*)phi, (const T*)c.Aq, c.mu2, (T*)Fon, Von);
+    else
+      hipLaunchKernelGGL((multi_onsite_lds_kernel<T, 16>), dim3(nsites / 16), dim3(16), per * 16, c.stream, c.reps, c.blk_rep, c.Vp,
+                         d, (const T*)phi, (const T*)c.Aq, c.mu2, (T*)Fon, Von);
   }
   static void phi_drift(const LaunchCtx& c, const void* phi_in, void* phi_out, const void* pi, int nplanes, double cd, int abs_eps) {
     hipLaunchKernelGGL(phi_drift_kernel<T>, dim3(c.nblk), dim3(kBlock), 0, c.stream, c.reps, c.blk_rep, c.Vp, (const T*)phi_in,
                        (T*)phi_out, (const T*)pi, nplanes, cd, abs_eps);
   }
   static void onsite(const LaunchCtx& c, const void* phi_in, void* phi_out, void* pi, int d, double ck, double cd, int flags) {
+    const T* Fon = nullptr;
+    if (c.multi) {
+      if (!(flags & F_NO_ONSITE) && (flags & F_KICK)) multi(c, d, phi_in, c.Fon, nullptr);
+      Fon = (const T*)c.Fon;
+    }
     hipLaunchKernelGGL(onsite_step_kernel<T>, dim3(c.nblk), dim3(kBlock), 0, c.stream, c.reps, c.blk_rep, c.Vp, (const T*)phi_in,
-                       (T*)phi_out, (T*)pi, d, ck, cd, flags);
+                       (T*)phi_out, (T*)pi, d, ck, cd, flags, Fon);
   }
 };
 template <typename T, int N, int D>

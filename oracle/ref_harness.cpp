@@ -365,6 +365,8 @@ static int dump(const char* dir) {
   { FundamentalRep<2> f; dump_multi<4, 2>("multi_su2_fund", f, 4, 2.3, 0.2, 0.4, 12); }
   { AdjointRep<2> a; dump_multi<3, 2>("multi_su2_adj", a, 4, 2.3, 0.2, 0.4, 12); }
   { GeneralRep<3> s2({2}); dump_multi<4, 3>("multi_su3_2", s2, 4, 5.7, 0.1, 0.5, 12); }
+  { GeneralRep<3> s21({2, 1}); dump_multi<3, 3>("multi_su3_21", s21, 3, 5.7, 0.1, 0.5, 10); }
+  { GeneralRep<2> s12({12}); dump_multi<3, 2>("multi_su2_12", s12, 3, 2.3, 0.1, 0.4, 10); }
   { AdjointRep<2> a; dump_higgs<3, 2>("su2_adj", a, 4, 2.3, 0.3, 0.5, 12, true); }
   { FundamentalRep<3> f; dump_higgs<3, 3>("su3_fund", f, 4, 5.7, 0.2, 0.5, 12, true); }
   {

@@ -14,8 +14,12 @@ import oracle_lib as O
 
 pytestmark = pytest.mark.gpu
 
-MULTI = [("multi_su2_fund", 4, 2, "fund", 2.3, 0.2, 0.4, 12), ("multi_su2_adj", 3, 2, "adj", 2.3, 0.2, 0.4, 12),
-         ("multi_su3_2", 4, 3, "2", 5.7, 0.1, 0.5, 12)]
+# (tag, D, N, rep, beta, kappa, mu2, nmd, L, s): s scales the tolerances (the
+# degree-12 Lagrange projectors of d = 13 lose ~1e-10 relative to roundoff).
+# The last two run on the large-irrep kernels (LDS multi kernel, runtime d).
+MULTI = [("multi_su2_fund", 4, 2, "fund", 2.3, 0.2, 0.4, 12, 4, 1.0), ("multi_su2_adj", 3, 2, "adj", 2.3, 0.2, 0.4, 12, 4, 1.0),
+         ("multi_su3_2", 4, 3, "2", 5.7, 0.1, 0.5, 12, 4, 1.0), ("multi_su3_21", 3, 3, "2,1", 5.7, 0.1, 0.5, 10, 3, 1.0),
+         ("multi_su2_12", 3, 2, "12", 2.3, 0.1, 0.4, 10, 3, 1e3)]
 IDS = [c[0] for c in MULTI]
 
 
@@ -28,8 +32,8 @@ def rel(a, b):
 
 
 def make(case, integ=gha.LEAPFROG, prec=gha.FP64, f=None):
-    tag, D, N, rep, beta, kappa, mu2, nmd = case
-    g = gha.GaugeHiggsHMC(4, rep, seed=1, D=D, N=N, precision=prec)
+    tag, D, N, rep, beta, kappa, mu2, nmd, L, _ = case
+    g = gha.GaugeHiggsHMC(L, rep, seed=1, D=D, N=N, precision=prec)
     g.configure(beta=beta, kappa=kappa, lam=0.5, tau=1.0, nmd=nmd, integ=integ)
     g.lat.set_multi_potential(f, mu2)
     g.hot(0.8, 12345, 0.3)
@@ -39,30 +43,30 @@ def make(case, integ=gha.LEAPFROG, prec=gha.FP64, f=None):
 @pytest.mark.parametrize("case", MULTI, ids=IDS)
 def test_against_reference(goldens, case):
     arr, sc = goldens
-    tag = case[0]
+    tag, s = case[0], case[-1]
     f = arr[tag + "_f"]
     g = make(case, f=f)
     _, Fp = g.forces()
-    assert rel(Fp, arr[tag + "_Fphi"]) < 1e-11
-    assert abs(g.observables()["SH"] - sc[tag + "_SH"]) < 1e-11 * max(1.0, abs(sc[tag + "_SH"]))
+    assert rel(Fp, arr[tag + "_Fphi"]) < 1e-11 * s
+    assert abs(g.observables()["SH"] - sc[tag + "_SH"]) < 1e-11 * s * max(1.0, abs(sc[tag + "_SH"]))
     g.refresh_momenta()
-    assert abs(g.hamiltonian() - sc[tag + "_H0"]) < 1e-11 * abs(sc[tag + "_H0"])
+    assert abs(g.hamiltonian() - sc[tag + "_H0"]) < 1e-11 * s * abs(sc[tag + "_H0"])
     for integ, name in ((gha.LEAPFROG, "lf"), (gha.OMELYAN, "omel")):
         g = make(case, integ, f=f)
         dh, acc = [], []
         for _ in range(2):
             acc.append(float(g.trajectory()))
             dh.append(g.last_dH)
-        assert np.max(np.abs(np.array(dh) - arr[f"{tag}_{name}_dH"])) < 1e-8
+        assert np.max(np.abs(np.array(dh) - arr[f"{tag}_{name}_dH"])) < 1e-8 * max(1.0, s / 100)
         assert np.array_equal(np.array(acc), arr[f"{tag}_{name}_acc"])
-        assert rel(g.get("phi"), arr[f"{tag}_{name}_phi2"]) < 1e-10
+        assert rel(g.get("phi"), arr[f"{tag}_{name}_phi2"]) < 1e-10 * s
 
 
 @pytest.mark.parametrize("case", MULTI, ids=IDS)
 def test_against_oracle_and_fp32(case):
-    tag, D, N, rep, beta, kappa, mu2, nmd = case
+    tag, D, N, rep, beta, kappa, mu2, nmd, L, s = case
     f = 0.1 + 0.25 * np.arange(len(gha.casimir_channels(N, rep)))
-    o = O.Oracle(D, N, 4, rep, seed=1)
+    o = O.Oracle(D, N, L, rep, seed=1)
     o.multi_potential(f, mu2)
     o.couplings(beta=beta, kappa=kappa, lam=0.5, tau=1.0, nmd=6, omelyan=True)
     o.hot(0.8)
@@ -71,10 +75,10 @@ def test_against_oracle_and_fp32(case):
     g.configure(nmd=6)
     F, Fp = g.forces()
     Fo, Fpo = o.forces()
-    assert rel(F, Fo) < 1e-12 and rel(Fp, Fpo) < 1e-12
+    assert rel(F, Fo) < 1e-12 * s and rel(Fp, Fpo) < 1e-12 * s
     acc = g.trajectory()
     a_o, dH_o, _, _ = o.trajectory()
-    assert abs(g.last_dH - dH_o) < 1e-9 and acc == a_o
+    assert abs(g.last_dH - dH_o) < 1e-9 * s and acc == a_o
     h = make(case, gha.OMELYAN, prec=gha.FP32MD, f=f)
     h.set("U", g.get("U"))
     h.set("phi", g.get("phi"))
@@ -82,7 +86,7 @@ def test_against_oracle_and_fp32(case):
     o.set("phi", h.get("phi"))
     F32, Fp32 = h.forces()
     Fo, Fpo = o.forces()
-    assert rel(F32, Fo) < 2e-5 and rel(Fp32, Fpo) < 2e-5
+    assert rel(F32, Fo) < 2e-5 * max(1.0, s / 100) and rel(Fp32, Fpo) < 2e-5 * max(1.0, s / 100)
 
 
 def test_frozen_with_multi_reversible():
@@ -99,6 +103,39 @@ def test_frozen_with_multi_reversible():
     g.md_evolve()
     assert np.max(np.abs(g.get("U") - U0)) < 1e-10 and np.max(np.abs(g.get("phi") - phi0)) < 1e-10
     assert np.max(np.abs(np.linalg.norm(g.get("phi"), axis=1) - 1)) < 1e-12
+
+
+def test_large_irrep_multi_nested_and_limits():
+    """The multi-timescale split on a large irrep moves the multi-invariant
+    on-site force to the inner scale (onsite_step_kernel reads the same
+    planes): one inner step reproduces the single-scale leapfrog to rounding,
+    and the split is reversible. Irreps above the LDS kernel's bound are
+    refused."""
+    case = MULTI[3]
+    f = [0.2, 0.35, 0.5, 0.65]
+    a = make(case, gha.LEAPFROG, f=f)
+    b = make(case, gha.LEAPFROG, f=f)
+    a.refresh_momenta()
+    b.set("P", a.get("P"))
+    b.set("pi", a.get("pi"))
+    b.lat.set_nested(case[7], 1)
+    a.md_evolve()
+    b.md_evolve()
+    for k in ("U", "P", "phi", "pi"):
+        assert np.max(np.abs(a.get(k) - b.get(k))) < 1e-11, k
+    g = make(case, gha.LEAPFROG, f=f)
+    g.configure(reunit_each_traj=False)
+    g.lat.set_nested(2, 3)
+    g.refresh_momenta()
+    U0, phi0 = g.get("U"), g.get("phi")
+    g.md_evolve()
+    g.set("P", -g.get("P"))
+    g.set("pi", -g.get("pi"))
+    g.md_evolve()
+    assert np.max(np.abs(g.get("U") - U0)) < 1e-10 and np.max(np.abs(g.get("phi") - phi0)) < 1e-10
+    g = gha.GaugeHiggsHMC(3, "6,1", seed=1, D=3, N=3)
+    with pytest.raises(gha.GhxError, match="dimension <= 16"):
+        g.lat.set_multi_potential(np.ones(4), 0.5)
 
 
 def test_restore_quartic():

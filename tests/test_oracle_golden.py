@@ -194,8 +194,12 @@ def test_frozen_scalar_matches_reference(goldens, tag, D, rep, kappa, nmd):
         assert rel(o.get("phi"), g[f"{tag}_{integ}_phi2"]) < 1e-13
 
 
-MULTI = [("multi_su2_fund", 4, 2, "fund", 2.3, 0.2, 0.4, 12), ("multi_su2_adj", 3, 2, "adj", 2.3, 0.2, 0.4, 12),
-         ("multi_su3_2", 4, 3, "2", 5.7, 0.1, 0.5, 12)]
+# (tag, D, N, rep, beta, kappa, mu2, nmd, L, s): s scales the tolerances. At d = 13
+# the degree-12 Lagrange projectors lose ~1e-10 relative to roundoff, in the
+# reference as here (scalar_invariants.hpp:96-99 quotes ~2e-8 for its cache).
+MULTI = [("multi_su2_fund", 4, 2, "fund", 2.3, 0.2, 0.4, 12, 4, 1.0), ("multi_su2_adj", 3, 2, "adj", 2.3, 0.2, 0.4, 12, 4, 1.0),
+         ("multi_su3_2", 4, 3, "2", 5.7, 0.1, 0.5, 12, 4, 1.0), ("multi_su3_21", 3, 3, "2,1", 5.7, 0.1, 0.5, 10, 3, 1.0),
+         ("multi_su2_12", 3, 2, "12", 2.3, 0.1, 0.4, 10, 3, 1e3)]
 
 
 @pytest.mark.parametrize("case", MULTI, ids=[c[0] for c in MULTI])
@@ -204,11 +208,11 @@ def test_multi_invariant_potential_matches_reference(goldens, case):
     into the HMC: channel Casimirs, V and dV/dphi^* on random phi, the scalar
     force and action, and trajectories of both integrators."""
     g, sc = goldens
-    tag, D, N, rep, beta, kappa, mu2, nmd = case
-    o = O.Oracle(D, N, 4, rep, seed=1)
+    tag, D, N, rep, beta, kappa, mu2, nmd, L, s = case
+    o = O.Oracle(D, N, L, rep, seed=1)
     lam = o.channels()
     assert len(lam) == sc[tag + "_nch"]
-    assert np.max(np.abs(lam - g[tag + "_lambda"])) < 1e-9
+    assert np.max(np.abs(lam - g[tag + "_lambda"])) < 1e-9 * s
     f = g[tag + "_f"]
     o.multi_potential(f, mu2)
     px = g[tag + "_px"].view(np.complex128).reshape(3, -1)
@@ -216,25 +220,25 @@ def test_multi_invariant_potential_matches_reference(goldens, case):
     dvu = g[tag + "_dV_uncached"].view(np.complex128).reshape(3, -1)
     for t in range(3):
         V, dV = o.multi_eval(px[t])
-        assert abs(V - g[tag + "_V"][t]) < 1e-12 * max(1.0, abs(V))
-        assert np.max(np.abs(dV - dvu[t])) < 1e-12 * np.max(np.abs(dvu[t]))
-        assert np.max(np.abs(dV - dv[t])) < 1e-10 * np.max(np.abs(dv[t]))  # reference's superoperator cache: roundoff-level
+        assert abs(V - g[tag + "_V"][t]) < 1e-12 * s * max(1.0, abs(V))
+        assert np.max(np.abs(dV - dvu[t])) < 1e-12 * s * np.max(np.abs(dvu[t]))
+        assert np.max(np.abs(dV - dv[t])) < 1e-10 * s * np.max(np.abs(dv[t]))  # reference's superoperator cache: roundoff-level
     o.couplings(beta=beta, kappa=kappa, lam=0.5, tau=1.0, nmd=nmd, omelyan=False)
     o.hot(0.8)
     o.phi_gaussian(12345, 0.3)
     F, Fp = o.forces()
-    assert rel(Fp, g[tag + "_Fphi"]) < 1e-10
-    assert abs(o.observables()["SH"] - sc[tag + "_SH"]) < 1e-10 * max(1, abs(sc[tag + "_SH"]))
+    assert rel(Fp, g[tag + "_Fphi"]) < 1e-10 * s
+    assert abs(o.observables()["SH"] - sc[tag + "_SH"]) < 1e-10 * s * max(1, abs(sc[tag + "_SH"]))
     for integ in ("lf", "omel"):
-        o = O.Oracle(D, N, 4, rep, seed=1)
+        o = O.Oracle(D, N, L, rep, seed=1)
         o.multi_potential(f, mu2)
         o.couplings(beta=beta, kappa=kappa, lam=0.5, tau=1.0, nmd=nmd, omelyan=(integ == "omel"))
         o.hot(0.8)
         o.phi_gaussian(12345, 0.3)
         res = [o.trajectory() for _ in range(2)]
-        assert np.max(np.abs(np.array([r[1] for r in res]) - g[f"{tag}_{integ}_dH"])) < 1e-8
+        assert np.max(np.abs(np.array([r[1] for r in res]) - g[f"{tag}_{integ}_dH"])) < 1e-8 * max(1.0, s / 100)
         assert np.array_equal(np.array([float(r[0]) for r in res]), g[f"{tag}_{integ}_acc"])
-        assert rel(o.get("phi"), g[f"{tag}_{integ}_phi2"]) < 1e-11
+        assert rel(o.get("phi"), g[f"{tag}_{integ}_phi2"]) < 1e-11 * s
 
 
 @pytest.mark.parametrize("case", LC.CASES, ids=[c[0] for c in LC.CASES])
