@@ -232,12 +232,55 @@ __global__ __launch_bounds__(kBigBlock) void big_matter_kernel(BigArgs<T> a, Big
     if (a.flags & F_KICK) {
       pr = fma(ek, f.r, pr);
       pim = fma(ek, f.i, pim);
-      a.pi[size_t(2 * l) * Vp + g] = T(pr);
-      a.pi[size_t(2 * l + 1) * Vp + g] = T(pim);
+    }
+    vin[l] = {pr, pim};  // the site's momenta, kicked
+  }
+  if (a.mode == BIG_FORCE) return;
+  const bool frozen = a.flags & F_FROZEN;
+  __syncthreads();
+  if (frozen) {
+    // |phi| = 1 (gauge_higgs_hmc.hpp:61-104): project the kicked pi onto the
+    // tangent space, then move along the great circle
+    __shared__ R red;
+    if (a.flags & F_KICK) {
+      if (threadIdx.x == 0) {
+        R rp = R(0);
+        for (int l = 0; l < d; ++l) rp += phx[l].r * vin[l].r + phx[l].i * vin[l].i;
+        red = rp;
+      }
+      __syncthreads();
+      const R rp = red;
+      for (int l = threadIdx.x; l < d; l += kBigBlock) { vin[l].r -= rp * phx[l].r; vin[l].i -= rp * phx[l].i; }
+      __syncthreads();
     }
     if (a.flags & F_DRIFT) {
-      a.phi_out[size_t(2 * l) * Vp + g] = T(fma(ed, pr, phx[l].r));
-      a.phi_out[size_t(2 * l + 1) * Vp + g] = T(fma(ed, pim, phx[l].i));
+      if (threadIdx.x == 0) {
+        R r2 = R(0);
+        for (int l = 0; l < d; ++l) r2 += vin[l].r * vin[l].r + vin[l].i * vin[l].i;
+        red = r2;
+      }
+      __syncthreads();
+      const R rr = sqrt(red);
+      const R rmin = sizeof(R) == 8 ? R(1e-300) : R(1e-30);
+      if (rr >= rmin) {
+        const R cs = cos(rr * ed), sn = sin(rr * ed), q = sn / rr, w = -rr * sn;
+        for (int l = threadIdx.x; l < d; l += kBigBlock) {
+          const cx<R> fv = phx[l], pv = vin[l];
+          phx[l] = {cs * fv.r + q * pv.r, cs * fv.i + q * pv.i};
+          vin[l] = {w * fv.r + cs * pv.r, w * fv.i + cs * pv.i};
+        }
+      }
+    }
+  }
+  for (int l = threadIdx.x; l < d; l += kBigBlock) {
+    if ((a.flags & F_KICK) || (frozen && (a.flags & F_DRIFT))) {
+      a.pi[size_t(2 * l) * Vp + g] = T(vin[l].r);
+      a.pi[size_t(2 * l + 1) * Vp + g] = T(vin[l].i);
+    }
+    if (a.flags & F_DRIFT) {
+      const cx<R> ph = frozen ? phx[l] : cx<R>{fma(ed, vin[l].r, phx[l].r), fma(ed, vin[l].i, phx[l].i)};
+      a.phi_out[size_t(2 * l) * Vp + g] = T(ph.r);
+      a.phi_out[size_t(2 * l + 1) * Vp + g] = T(ph.i);
     }
   }
 }
@@ -264,6 +307,8 @@ __global__ __launch_bounds__(kBlock) void onsite_step_kernel(const RepDesc* reps
   const R c = (flags & F_NO_ONSITE) ? R(0) : R(-2) * (R(1) + R(2.0 * rd.lambda) * (n2 - R(1)));
   const bool abs_eps = flags & F_ABS_EPS;
   const R ek = R(abs_eps ? ck : ck * rd.eps), ed = R(abs_eps ? cd : cd * rd.eps);
+  const bool frozen = flags & F_FROZEN;
+  R rp = R(0);
   for (int l = 0; l < d; ++l) {
     const R xr = R(phi_in[size_t(2 * l) * Vp + g]), xi = R(phi_in[size_t(2 * l + 1) * Vp + g]);
     R fr = fma(c, xr, R(0)), fi = fma(c, xi, R(0));
@@ -279,11 +324,40 @@ __global__ __launch_bounds__(kBlock) void onsite_step_kernel(const RepDesc* reps
       pi[size_t(2 * l) * Vp + g] = T(pr);
       pi[size_t(2 * l + 1) * Vp + g] = T(pim);
     }
+    if (frozen) {
+      rp += xr * pr + xi * pim;
+      continue;
+    }
     if (flags & F_DRIFT) {
       phi_out[size_t(2 * l) * Vp + g] = T(fma(ed, pr, xr));
       phi_out[size_t(2 * l + 1) * Vp + g] = T(fma(ed, pim, xi));
     }
   }
+  if (!frozen) return;
+  // |phi| = 1: tangent projection of the kicked pi, then the great-circle drift
+  // (gauge_higgs_hmc.hpp:61-104), over the site's planes in pi (in place)
+  R r2 = R(0);
+  for (int l = 0; l < d; ++l) {
+    R pr = R(pi[size_t(2 * l) * Vp + g]), pim = R(pi[size_t(2 * l + 1) * Vp + g]);
+    if (flags & F_KICK) {
+      pr -= rp * R(phi_in[size_t(2 * l) * Vp + g]);
+      pim -= rp * R(phi_in[size_t(2 * l + 1) * Vp + g]);
+      pi[size_t(2 * l) * Vp + g] = T(pr);
+      pi[size_t(2 * l + 1) * Vp + g] = T(pim);
+    }
+    r2 += pr * pr + pim * pim;
+  }
+  if (!(flags & F_DRIFT)) return;
+  const R rr = sqrt(r2);
+  const R rmin = sizeof(R) == 8 ? R(1e-300) : R(1e-30);
+  const bool move = rr >= rmin;
+  const R cs = move ? cos(rr * ed) : R(1), sn = move ? sin(rr * ed) : R(0), q = move ? sn / rr : R(0), w = -rr * sn;
+  for (int l = 0; l < d; ++l)
+    for (int k = 0; k < 2; ++k) {
+      const R fv = R(phi_in[size_t(2 * l + k) * Vp + g]), pv = R(pi[size_t(2 * l + k) * Vp + g]);
+      phi_out[size_t(2 * l + k) * Vp + g] = T(move ? cs * fv + q * pv : fv);
+      pi[size_t(2 * l + k) * Vp + g] = T(move ? w * fv + cs * pv : pv);
+    }
 }
 
 // Per-site FP64 contributions for H with a large rep: q1 from big_matter_kernel

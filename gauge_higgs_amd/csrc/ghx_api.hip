@@ -656,7 +656,6 @@ int ghx_set_frozen_phi(ghx_ctx* ctx, int on) {
   int rc = check_ready(ctx);
   if (rc) return rc;
   if (on && !has_phi(ctx)) return fail(ctx, GHX_E_INVALID, "frozen-length scalar needs a scalar field");
-  if (on && ctx->big) return fail(ctx, GHX_E_UNSUPPORTED, "frozen-length scalar is not built for the large-irrep kernels");
   ctx->frozen = on != 0;
   drop_graph(ctx);
   return GHX_OK;
@@ -666,7 +665,6 @@ int ghx_normalize_phi(ghx_ctx* ctx) {
   int rc = check_ready(ctx);
   if (rc) return rc;
   if (!has_phi(ctx)) return fail(ctx, GHX_E_INVALID, "no scalar field");
-  if (ctx->big) return fail(ctx, GHX_E_UNSUPPORTED, "frozen-length scalar is not built for the large-irrep kernels");
   ctx->ops.frozen_site(ctx->lc(), ctx->phi[ctx->cur], ctx->pi, 1);
   HIPCHK(hipGetLastError());
   return GHX_OK;

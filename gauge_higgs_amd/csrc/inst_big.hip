@@ -6,6 +6,7 @@ template <typename T, int N, int D>
 struct BigLaunch {
   static void matter(const LaunchCtx& c, const BigRepTables& tb, const void* U, const void* phi_in, void* phi_out, void* pi, void* G,
                      void* Fphi, double* q1, double ck, double cd, int flags, int mode) {
+    if (c.frozen) flags |= F_FROZEN;
     BigArgs<T> a{c.reps, nullptr, c.blk_rep, c.Vp, (const T*)U, (const T*)phi_in, (T*)phi_out, (T*)pi, (T*)G, (T*)Fphi, q1, ck, cd, flags, mode};
     if (c.multi && mode != BIG_ACTION && !(flags & F_NO_ONSITE) && (mode == BIG_FORCE || (flags & F_KICK))) {
       multi(c, tb.d, phi_in, c.Fon, nullptr);
@@ -37,9 +38,10 @@ struct BigLaunch {
   }
   static void phi_drift(const LaunchCtx& c, const void* phi_in, void* phi_out, const void* pi, int nplanes, double cd, int abs_eps) {
     hipLaunchKernelGGL(phi_drift_kernel<T>, dim3(c.nblk), dim3(kBlock), 0, c.stream, c.reps, c.blk_rep, c.Vp, (const T*)phi_in,
-                       (T*)phi_out, (const T*)pi, nplanes, cd, abs_eps);
+                       (T*)phi_out, (T*)const_cast<void*>(pi), nplanes, cd, abs_eps, c.frozen);
   }
   static void onsite(const LaunchCtx& c, const void* phi_in, void* phi_out, void* pi, int d, double ck, double cd, int flags) {
+    if (c.frozen) flags |= F_FROZEN;
     const T* Fon = nullptr;
     if (c.multi) {
       if (!(flags & F_NO_ONSITE) && (flags & F_KICK)) multi(c, d, phi_in, c.Fon, nullptr);

@@ -614,14 +614,29 @@ __global__ __launch_bounds__(kBlock) void multi_onsite_kernel(const RepDesc* rep
 // phi' = phi + eps pi for a runtime number of planes (large reps).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void phi_drift_kernel(const RepDesc* reps, const int* blk_rep, unsigned Vp, const T* phi_in,
-                                                          T* phi_out, const T* pi, int nplanes, double cd, int abs_eps) {
+                                                          T* phi_out, T* pi, int nplanes, double cd, int abs_eps, int frozen) {
   const int r = blk_rep[blockIdx.x];
   const RepDesc& rd = reps[r];
   const unsigned s = blockIdx.x * kBlock + threadIdx.x - rd.offset;
   if (s >= rd.vol) return;
   const unsigned g = rd.offset + s;
   const T e = T(abs_eps ? cd : cd * rd.eps);
-  for (int c = 0; c < nplanes; ++c) phi_out[size_t(c) * Vp + g] = phi_in[size_t(c) * Vp + g] + e * pi[size_t(c) * Vp + g];
+  if (!frozen) {
+    for (int c = 0; c < nplanes; ++c) phi_out[size_t(c) * Vp + g] = phi_in[size_t(c) * Vp + g] + e * pi[size_t(c) * Vp + g];
+    return;
+  }
+  // |phi| = 1: great-circle drift of phi and pi (gauge_higgs_hmc.hpp:86-104)
+  T r2 = T(0);
+  for (int c = 0; c < nplanes; ++c) r2 += pi[size_t(c) * Vp + g] * pi[size_t(c) * Vp + g];
+  const T rr = sqrt(r2);
+  const T rmin = sizeof(T) == 8 ? T(1e-300) : T(1e-30);
+  const bool move = rr >= rmin;
+  const T cs = move ? cos(rr * e) : T(1), sn = move ? sin(rr * e) : T(0), q = move ? sn / rr : T(0), w = -rr * sn;
+  for (int c = 0; c < nplanes; ++c) {
+    const T fv = phi_in[size_t(c) * Vp + g], pv = pi[size_t(c) * Vp + g];
+    phi_out[size_t(c) * Vp + g] = move ? cs * fv + q * pv : fv;
+    pi[size_t(c) * Vp + g] = move ? w * fv + cs * pv : pv;
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -362,6 +362,8 @@ static int dump(const char* dir) {
   { FundamentalRep<2> f; dump_higgs<4, 2>("su2_fund", f, 4, 2.3, 0.265, 0.5, 20, true); }
   { FundamentalRep<2> f; dump_higgs<4, 2>("su2_fund_frozen", f, 4, 2.3, 0.6, 0.5, 20, true, true); }
   { AdjointRep<2> a; dump_higgs<3, 2>("su2_adj_frozen", a, 4, 2.3, 0.5, 0.5, 12, true, true); }
+  { GeneralRep<2> s4({4}); dump_higgs<3, 2>("su2_4_frozen", s4, 3, 2.3, 0.5, 0.5, 10, true, true); }
+  { GeneralRep<3> s21({2, 1}); dump_higgs<3, 3>("su3_21_frozen", s21, 3, 5.7, 0.3, 0.5, 10, true, true); }
   { FundamentalRep<2> f; dump_multi<4, 2>("multi_su2_fund", f, 4, 2.3, 0.2, 0.4, 12); }
   { AdjointRep<2> a; dump_multi<3, 2>("multi_su2_adj", a, 4, 2.3, 0.2, 0.4, 12); }
   { GeneralRep<3> s2({2}); dump_multi<4, 3>("multi_su3_2", s2, 4, 5.7, 0.1, 0.5, 12); }

@@ -160,13 +160,17 @@ def test_oracle_rep_matches_reference(goldens, tag, N, spec):
         assert rel(o.rep_hop_g(U[t], px[t], py[t]), hop[t]) < 1e-12
 
 
-@pytest.mark.parametrize("tag,D,rep,kappa,nmd", [("su2_fund_frozen", 4, "fund", 0.6, 20), ("su2_adj_frozen", 3, "adj", 0.5, 12)])
-def test_frozen_scalar_matches_reference(goldens, tag, D, rep, kappa, nmd):
+FROZEN = [("su2_fund_frozen", 4, 2, "fund", 2.3, 0.6, 20, 4), ("su2_adj_frozen", 3, 2, "adj", 2.3, 0.5, 12, 4),
+          ("su2_4_frozen", 3, 2, "4", 2.3, 0.5, 10, 3), ("su3_21_frozen", 3, 3, "2,1", 5.7, 0.3, 10, 3)]
+
+
+@pytest.mark.parametrize("tag,D,N,rep,beta,kappa,nmd,L", FROZEN, ids=[c[0] for c in FROZEN])
+def test_frozen_scalar_matches_reference(goldens, tag, D, N, rep, beta, kappa, nmd, L):
     """|phi_x| = 1 scalar (gauge_higgs_hmc.hpp:61-104, 133, 144, 172): normalisation,
     tangent-projected momenta and kicks, geodesic drift, trajectories."""
     g, sc = goldens
-    o = O.Oracle(D, 2, 4, rep, seed=1)
-    o.couplings(beta=2.3, kappa=kappa, lam=0.5, tau=1.0, nmd=nmd, omelyan=False)
+    o = O.Oracle(D, N, L, rep, seed=1)
+    o.couplings(beta=beta, kappa=kappa, lam=0.5, tau=1.0, nmd=nmd, omelyan=False)
     o.hot(0.8)
     o.phi_gaussian(12345, 0.3)
     o.frozen(True)
@@ -181,8 +185,8 @@ def test_frozen_scalar_matches_reference(goldens, tag, D, rep, kappa, nmd):
     assert rel(o.get("phi"), g[tag + "_phi_drift"]) < 1e-14
     assert rel(o.get("pi"), g[tag + "_pi_drift"]) < 1e-14
     for integ in ("lf", "omel"):
-        o = O.Oracle(D, 2, 4, rep, seed=1)
-        o.couplings(beta=2.3, kappa=kappa, lam=0.5, tau=1.0, nmd=nmd, omelyan=(integ == "omel"))
+        o = O.Oracle(D, N, L, rep, seed=1)
+        o.couplings(beta=beta, kappa=kappa, lam=0.5, tau=1.0, nmd=nmd, omelyan=(integ == "omel"))
         o.hot(0.8)
         o.phi_gaussian(12345, 0.3)
         o.frozen(True)
