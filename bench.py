@@ -53,14 +53,18 @@ def stored_bytes_per_site(D, N, d, w):
     return 2 * (D * 2 * N * nr * w + D * (N * N - 1) * w + 2 * (2 * d) * w)
 
 
-def load_traffic(workload, kernel):
-    """HBM bytes per launch of `kernel` from the committed PMC summary (FETCH_SIZE
-    and WRITE_SIZE per dispatch, scaled by the copy-kernel calibration)."""
+def load_traffic(workload, kernel, block_order=0):
+    """HBM bytes per launch of `kernel` (launched with `block_order`) from the
+    committed PMC summary (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the
+    copy-kernel calibration)."""
     path = os.path.join(ROOT, "profiles", f"md_step_{workload}_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get(kernel, {}).get("hbm_bytes_per_launch")
+        entry = json.load(f).get(kernel, {})
+    if "by_block_order" in entry:
+        entry = entry["by_block_order"].get(str(block_order), {})
+    return entry.get("hbm_bytes_per_launch")
 
 
 def copy_calibration(mib):
@@ -217,7 +221,7 @@ def main():
         g.lat.set_block_order(args.block_order)
         tuned["block_order"] = g.lat.block_order()
     if not args.no_tune:
-        seg, tms = g.lat.autotune(steps=5)  # launch-shape autotuner (fields restored)
+        seg, tms = g.lat.autotune(steps=20)  # launch-shape autotuner (fields restored)
         tuned = {"stream_seg": seg, "block_order": g.lat.block_order(), "ms_per_step_at_tune": tms}
     if args.warmup:
         g.lat.time_md_steps(args.warmup)
@@ -237,7 +241,7 @@ def main():
     launch_s = sec / args.steps
     achieved = V * bpsite / launch_s / 1e9
     kernel = "md_stream_kernel" if tuned.get("stream_seg") else "md_step_kernel"
-    traffic = load_traffic(args.workload, kernel)
+    traffic = load_traffic(args.workload, kernel, tuned.get("block_order", 0))
     line = {
         "metric": "MD site-updates/s; GPU-s per accepted traj, learned vs exact force (1 GPU)",
         "value": value,

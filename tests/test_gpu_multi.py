@@ -158,3 +158,10 @@ def test_cli_hmc_higgs_multi(tmp_path):
     assert len(open(ts).read().strip().splitlines()) == 11
     bad = subprocess.run([exe, "2", "4", "5.7", "0.1", "0.5", "0.2,0.35"], capture_output=True, text=True, timeout=60)
     assert bad.returncode == 1 and "need 3 couplings" in bad.stderr
+    # a large irrep (SU(2) spin 6, 13 channels) with the frozen scalar, as in discrete-subgroup runs
+    out = subprocess.run([exe, "12", "--N=2"], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0 and "13 quartic channels" in out.stdout
+    out = subprocess.run([exe, "12", "3", "2.3", "0.3", "0.4", "auto", "2", "4", "6", "1.0", "1", "--N=2", "--D=3"],
+                         capture_output=True, text=True, timeout=120, env=dict(os.environ, GH_FROZEN="1"))
+    assert out.returncode == 0, out.stderr
+    assert "FROZEN" in out.stdout and "acceptance" in out.stdout
