@@ -12,6 +12,7 @@ the CPU baseline (the reference compiled from its sources, or the FP64 oracle
 restatement, timed on this host on a 2-step sample of the same lattice), and
 "precision": FP32-MD against FP64 on SU(2)+fundamental 16^4 (forces, one
 trajectory's dH and observables) and both MD throughputs;
+"pure_gauge": SU(2) pure gauge 8^4 FP64 throughput and plaquette at beta=2.3;
 "replicas": 32 SU(2)+fundamental chains (kappa 0.20-0.32, L 6-12) advanced by one
 launch per MD step, their throughput and the measured L_link susceptibility peaks;
 "learned_force": GPU seconds per accepted trajectory of exact-force HMC against
@@ -112,6 +113,8 @@ def replica_block(n_therm, n_meas, use_graph=True):
     swaps every trajectory: measured L_link susceptibility peaks and throughput."""
     from gauge_higgs_amd import replicas
     rs = replicas.kappa_scan()
+    rs.lat.autotune(steps=20)  # launch shape (fields restored)
+    launch = {"stream_seg": rs.lat.stream_segment(), "block_order": rs.lat.block_order(), "link_lanes": rs.lat.link_lanes()}
     m = replicas.measure_scan(rs, n_therm=n_therm, n_meas=n_meas, use_graph=use_graph)
     vtot = int(sum(rs.lat.vols))
     steps = 200
@@ -125,7 +128,7 @@ def replica_block(n_therm, n_meas, use_graph=True):
                         "frac": vtot * bpsite / (ms / steps / 1e3) / 1e9 / HBM_PEAK_GBS, "algorithmic_bytes_per_site": bpsite},
            "s_per_trajectory_cycle": m["s_per_cycle"], "trajectories_per_s": rs.n / m["s_per_cycle"],
            "hmc_acceptance_mean": float(m["hmc_acceptance"].mean()), "swap_acceptance_mean": float(np.mean(sa)) if sa else None,
-           "measured_cycles": n_meas, "chi_link_peaks": replicas.chi_peaks(m)}
+           "measured_cycles": n_meas, "chi_link_peaks": replicas.chi_peaks(m), "launch": launch}
     rs.lat.close()
     return out
 
@@ -165,10 +168,38 @@ def precision_block():
                              **{f"{k}_abs_err": abs(ob[k] - oa[k]) for k in ("plaq", "Llink", "Lphi")},
                              **{f"{k}_fp64": oa[k] for k in ("plaq", "Llink", "Lphi")}}
     for name, g in runs.items():
+        g.lat.autotune(steps=20)
         g.lat.time_md_steps(5)
         ms = g.lat.time_md_steps(100)
         out[f"md_site_updates_per_s_{name}"] = 16 ** 4 * 100 / (ms / 1e3)
+        out[f"launch_{name}"] = {"block_order": g.lat.block_order(), "link_lanes": g.lat.link_lanes(),
+                                 "stream_seg": g.lat.stream_segment()}
         g.lat.close()
+    return out
+
+
+def pure_gauge_block():
+    """SU(2) pure gauge (GaugeHMC semantics), 8^4, beta=2.3, FP64, Omelyan nmd=20:
+    MD throughput at the tuned launch shape and the plaquette after 60 trajectories
+    from the hot start (published value for SU(2) at beta=2.3: 0.603)."""
+    import gauge_higgs_amd as gha
+    g = gha.GaugeHiggsHMC(8, "", seed=1, D=4, N=2, precision=gha.FP64)
+    g.configure(beta=2.3, nmd=20, tau=1.0)
+    g.hot(0.8, 12345, 1.0)
+    g.lat.autotune(steps=20)
+    g.lat.trajectories(20, use_graph=True)
+    plaq = []
+    for _ in range(40):
+        g.lat.trajectories(1, use_graph=True)
+        plaq.append(g.observables()["plaq"])
+    g.refresh_momenta()
+    g.lat.time_md_steps(20)
+    steps = 500
+    ms = g.lat.time_md_steps(steps)
+    out = {"workload": "su2_pure_L8_fp64", "sites": 8 ** 4, "md_site_updates_per_s": 8 ** 4 * steps / (ms / 1e3),
+           "ms_per_md_step": ms / steps, "plaquette": float(np.mean(plaq)), "plaquette_err": float(np.std(plaq) / np.sqrt(len(plaq) / 4)),
+           "launch": {"block_order": g.lat.block_order(), "link_lanes": g.lat.link_lanes(), "stream_seg": g.lat.stream_segment()}}
+    g.lat.close()
     return out
 
 
@@ -222,7 +253,8 @@ def main():
         tuned["block_order"] = g.lat.block_order()
     if not args.no_tune:
         seg, tms = g.lat.autotune(steps=20)  # launch-shape autotuner (fields restored)
-        tuned = {"stream_seg": seg, "block_order": g.lat.block_order(), "ms_per_step_at_tune": tms}
+        tuned = {"stream_seg": seg, "block_order": g.lat.block_order(), "link_lanes": g.lat.link_lanes(),
+                 "ms_per_step_at_tune": tms}
     if args.warmup:
         g.lat.time_md_steps(args.warmup)
     g.lat.sync()
@@ -240,7 +272,7 @@ def main():
     bpsite = algorithmic_bytes_per_site(D, N, d, w)
     launch_s = sec / args.steps
     achieved = V * bpsite / launch_s / 1e9
-    kernel = "md_stream_kernel" if tuned.get("stream_seg") else "md_step_kernel"
+    kernel = "md_stream_kernel" if tuned.get("stream_seg") else ("md_link_kernel" if tuned.get("link_lanes") else "md_step_kernel")
     traffic = load_traffic(args.workload, kernel, tuned.get("block_order", 0))
     line = {
         "metric": "MD site-updates/s; GPU-s per accepted traj, learned vs exact force (1 GPU)",
@@ -267,6 +299,7 @@ def main():
         line["replicas"] = replica_block(200, args.replica_meas, use_graph=not args.no_graph)
     if not args.no_precision:
         line["precision"] = precision_block()
+        line["pure_gauge"] = pure_gauge_block()
     if not args.no_learned:
         from gauge_higgs_amd import learned
         line["learned_force"] = learned.compare(n_meas=args.learned_meas, use_graph=not args.no_graph)

@@ -50,6 +50,7 @@ struct ghx_ctx {
   const char* kernel_path = "";
   int stream_seg = 0;  // streaming MD kernel (0 = one site per thread)
   int block_order = 0;  // BlockWalk order of the MD-step kernel (single lattice)
+  int link_lanes = 0;   // link-parallel MD step (one lane per link) for small lattices
   // multi-timescale integrator (0 = single scale)
   int n_outer = 0, m_inner = 1;
   // surrogate link force (learned); S planes are filled by surr_fill before a kick
@@ -120,6 +121,7 @@ struct ghx_ctx {
     c.V3 = hreps[0].stride[D - 1];
     c.Lt = unsigned(hreps[0].L[D - 1]);
     c.walk = block_walk();
+    c.link_lanes = link_lanes;
     return c;
   }
 };

@@ -1057,6 +1057,12 @@ int ghx_set_tuning(ghx_ctx* ctx, int key, int value) {
     drop_graph(ctx);
     return GHX_OK;
   }
+  if (key == GHX_TUNE_LINK_LANES) {
+    if (value < 0 || value > 1) return fail(ctx, GHX_E_INVALID, "link lanes must be 0 or 1");
+    ctx->link_lanes = value;
+    drop_graph(ctx);
+    return GHX_OK;
+  }
   return fail(ctx, GHX_E_INVALID, "unknown tuning key");
 }
 
@@ -1064,6 +1070,7 @@ int ghx_get_tuning(const ghx_ctx* ctx, int key, int* value) {
   if (!ctx || !value) return GHX_E_INVALID;
   if (key == GHX_TUNE_STREAM_SEG) { *value = ctx->stream_seg; return GHX_OK; }
   if (key == GHX_TUNE_BLOCK_ORDER) { *value = ctx->block_walk().order; return GHX_OK; }
+  if (key == GHX_TUNE_LINK_LANES) { *value = ctx->link_lanes; return GHX_OK; }
   return GHX_E_INVALID;
 }
 
@@ -1085,12 +1092,14 @@ int ghx_autotune(ghx_ctx* ctx, int steps, int* best_seg, double* best_ms) {
     HIPCHK(hipMemcpyAsync(ctx->phisave, ctx->phi[0], nphb, hipMemcpyDeviceToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(pis, ctx->pi, nphb, hipMemcpyDeviceToDevice, ctx->stream));
   }
-  // candidates (stream segment, block order); the walks only for one site per thread
+  // candidates (stream segment, block order or -1 for the link-parallel step);
+  // the walks only for one site per thread
   std::vector<std::pair<int, int>> cand = {{0, 0}};
   for (int order : {1, 2}) {
     ctx->block_order = order;
     if (ctx->block_walk().order == order) cand.push_back({0, order});
   }
+  if (!ctx->big) cand.push_back({0, -1});
   if (ctx->nrep == 1)
     for (int sg : {1, 2, 4, 8, 16, 32, 64})
       if (sg <= ctx->hreps[0].L[ctx->D - 1]) cand.push_back({sg, 0});
@@ -1101,7 +1110,8 @@ int ghx_autotune(ghx_ctx* ctx, int steps, int* best_seg, double* best_ms) {
   HIPCHK(hipEventCreate(&e1));
   for (const auto& [sg, order] : cand) {
     ctx->stream_seg = sg;
-    ctx->block_order = order;
+    ctx->block_order = order < 0 ? 0 : order;
+    ctx->link_lanes = order < 0 ? 1 : 0;
     launch_step(ctx, 1.0, 1.0, true);  // warm
     HIPCHK(hipEventRecord(e0, ctx->stream));
     for (int i = 0; i < steps; ++i) launch_step(ctx, 1.0, 1.0, true);
@@ -1114,7 +1124,8 @@ int ghx_autotune(ghx_ctx* ctx, int steps, int* best_seg, double* best_ms) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   ctx->stream_seg = best;
-  ctx->block_order = best_order;
+  ctx->block_order = best_order < 0 ? 0 : best_order;
+  ctx->link_lanes = best_order < 0 ? 1 : 0;
   ctx->cur = 0;
   HIPCHK(hipMemcpyAsync(ctx->U[0], ctx->Usave, nUb, hipMemcpyDeviceToDevice, ctx->stream));
   HIPCHK(hipMemcpyAsync(ctx->P, Ps, nPb, hipMemcpyDeviceToDevice, ctx->stream));

@@ -285,154 +285,174 @@ GHX_DEV void geodesic_drift(cx<R>* ph, cx<R>* pv, R e) {
 // updates P and pi in place. Gauge force: gauge_wilson.hpp:12-27,71-83;
 // matter forces: scalar_higgs.hpp:156-213; drift: gauge_higgs_hmc.hpp:136-147.
 // ---------------------------------------------------------------------------
+// The site's scalar: phi_x into phx and the on-site term -2 dV/dphi^* into fph
+// (the quartic, or the Fon planes of a plugged-in potential).
+template <typename T, int DR>
+GHX_DEV void site_phi_begin(const StepArgs<T>& a, const RepDesc& rd, unsigned g, cx<T>* phx, cx<T>* fph) {
+  using R = T;
+  const Planes<T> Ph(a.phi_in, a.Vp, 2 * DR);
+  bload_vec<T, R, DR>(Ph, g, phx);
+  R n2 = R(0);
+#pragma unroll
+  for (int k = 0; k < DR; ++k) n2 += phx[k].r * phx[k].r + phx[k].i * phx[k].i;
+  if ((a.flags & F_EXT_ONSITE) && !(a.flags & F_NO_ONSITE)) {
+    bload_vec<T, R, DR>(Planes<T>(a.Fon, a.Vp, 2 * DR), g, fph);
+  } else {
+    const R c = (a.flags & F_NO_ONSITE) ? R(0) : R(-2) * (R(1) + R(2.0 * rd.lambda) * (n2 - R(1)));  // -2 dV/dphi^*
+#pragma unroll
+    for (int k = 0; k < DR; ++k) fph[k] = cscale(phx[k], c);
+  }
+}
+
+// One link of the fused MD step: link mu at site g. Adds the site's hop terms
+// through this link to fph (the scalar force of the site), writes the link
+// force (F_FORCE) or kicks P_mu and drifts U_mu.
 template <typename T, int N, int D, class REP>
-GHX_DEV void site_update(const StepArgs<T>& a, const typename REP::template Params<T>& rp, const RepDesc& rd, unsigned s) {
+GHX_DEV void link_update(const StepArgs<T>& a, const typename REP::template Params<T>& rp, const RepDesc& rd, unsigned g,
+                         const Nbr<D>& nb, int mu, const cx<T>* phx, cx<T>* fph) {
   using R = T;
   constexpr int NC = link_reals<T, N>();
   constexpr int NGN = N * N - 1;
   constexpr int DR = REP::DR;
-  const unsigned g = rd.offset + s;
   const unsigned Vp = a.Vp;
-  const Nbr<D> nb(rd, s);
   const bool abs_eps = a.flags & F_ABS_EPS;
   const R ek = R(abs_eps ? a.ck : a.ck * rd.eps);
   const R ed = R(abs_eps ? a.cd : a.cd * rd.eps);
   const R bN = R(rd.beta / N);
+  const R k2 = R(2.0 * rd.kappa);
   const Planes<T> Uin(a.U_in, Vp, D * NC), Uout(a.U_out, Vp, D * NC), Pp(a.P, Vp, D * NGN);
-
-  cx<R> phx[DR > 0 ? DR : 1], fph[DR > 0 ? DR : 1];
-  R k2 = R(0);
-  if constexpr (DR > 0) {
-    k2 = R(2.0 * rd.kappa);
+  const bool do_gauge = !(a.flags & F_NO_GAUGE), do_hop = !(a.flags & F_NO_HOP);
+  const Mat<R, N> Ux = bload_link<T, R, N>(Uin, mu, g);
+  const unsigned gpm = g + nb.up[mu];
+  // matter first (its temporaries die before the staple sum is built)
+  Mat<R, N> M = mzero<R, N>();
+  if constexpr (DR > 0) if (do_hop) {
     const Planes<T> Ph(a.phi_in, Vp, 2 * DR);
-    bload_vec<T, R, DR>(Ph, g, phx);
-    R n2 = R(0);
+    {
+      cx<R> phy[DR], psi[DR];
+      bload_vec<T, R, DR>(Ph, gpm, phy);
+      REP::fwd_M(Ux, phy, phx, psi, M, rp);
 #pragma unroll
-    for (int k = 0; k < DR; ++k) n2 += phx[k].r * phx[k].r + phx[k].i * phx[k].i;
-    if ((a.flags & F_EXT_ONSITE) && !(a.flags & F_NO_ONSITE)) {
-      bload_vec<T, R, DR>(Planes<T>(a.Fon, Vp, 2 * DR), g, fph);
-    } else {
-      const R c = (a.flags & F_NO_ONSITE) ? R(0) : R(-2) * (R(1) + R(2.0 * rd.lambda) * (n2 - R(1)));  // -2 dV/dphi^*
+      for (int k = 0; k < DR; ++k) { fph[k].r = fma(k2, psi[k].r, fph[k].r); fph[k].i = fma(k2, psi[k].i, fph[k].i); }
+    }
+    {
+      const unsigned gmm = g + nb.dn[mu];
+      cx<R> phb[DR], chi[DR];
+      bload_vec<T, R, DR>(Ph, gmm, phb);
+      const Mat<R, N> Ub = bload_link<T, R, N>(Uin, mu, gmm);
+      REP::apply_dag(Ub, phb, chi, rp);
 #pragma unroll
-      for (int k = 0; k < DR; ++k) fph[k] = cscale(phx[k], c);
+      for (int k = 0; k < DR; ++k) { fph[k].r = fma(k2, chi[k].r, fph[k].r); fph[k].i = fma(k2, chi[k].i, fph[k].i); }
     }
   }
-  const bool do_gauge = !(a.flags & F_NO_GAUGE), do_hop = !(a.flags & F_NO_HOP);
-
-#pragma unroll 1
-  for (int mu = 0; mu < D; ++mu) {
-    const Mat<R, N> Ux = bload_link<T, R, N>(Uin, mu, g);
-    const unsigned gpm = g + nb.up[mu];
-    // matter first (its temporaries die before the staple sum is built)
-    Mat<R, N> M = mzero<R, N>();
-    if constexpr (DR > 0) if (do_hop) {
-      const Planes<T> Ph(a.phi_in, Vp, 2 * DR);
-      {
-        cx<R> phy[DR], psi[DR];
-        bload_vec<T, R, DR>(Ph, gpm, phy);
-        REP::fwd_M(Ux, phy, phx, psi, M, rp);
-#pragma unroll
-        for (int k = 0; k < DR; ++k) { fph[k].r = fma(k2, psi[k].r, fph[k].r); fph[k].i = fma(k2, psi[k].i, fph[k].i); }
-      }
-      {
-        const unsigned gmm = g + nb.dn[mu];
-        cx<R> phb[DR], chi[DR];
-        bload_vec<T, R, DR>(Ph, gmm, phb);
-        const Mat<R, N> Ub = bload_link<T, R, N>(Uin, mu, gmm);
-        REP::apply_dag(Ub, phb, chi, rp);
-#pragma unroll
-        for (int k = 0; k < DR; ++k) { fph[k].r = fma(k2, chi[k].r, fph[k].r); fph[k].i = fma(k2, chi[k].i, fph[k].i); }
-      }
-    }
-    Mat<R, N> Sig = mzero<R, N>();
+  Mat<R, N> Sig = mzero<R, N>();
 #if GHX_NU_UNROLL
 #pragma unroll
 #else
 #pragma unroll 1
 #endif
-    for (int nu = 0; nu < (do_gauge ? D : 0); ++nu) {
-      if (nu == mu) continue;
-      // forward staple U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag
-      {
-        const Mat<R, N> AB = mul_nd(bload_link<T, R, N>(Uin, nu, gpm), bload_link<T, R, N>(Uin, mu, g + nb.up[nu]));
-        mac_nd(Sig, AB, bload_link<T, R, N>(Uin, nu, g));
-      }
-      // backward staple U_nu(x+mu-nu)^dag U_mu(x-nu)^dag U_nu(x-nu)
-      {
-        const unsigned gmn = g + nb.dn[nu];
-        const Mat<R, N> AB = mul_dd(bload_link<T, R, N>(Uin, nu, gpm + nb.dn[nu]), bload_link<T, R, N>(Uin, mu, gmn));
-        mac_nn(Sig, AB, bload_link<T, R, N>(Uin, nu, gmn));
-      }
+  for (int nu = 0; nu < (do_gauge ? D : 0); ++nu) {
+    if (nu == mu) continue;
+    // forward staple U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag
+    {
+      const Mat<R, N> AB = mul_nd(bload_link<T, R, N>(Uin, nu, gpm), bload_link<T, R, N>(Uin, mu, g + nb.up[nu]));
+      mac_nd(Sig, AB, bload_link<T, R, N>(Uin, nu, g));
     }
-    Mat<R, N> X = mul_nn(Ux, Sig);
-#pragma unroll
-    for (int i = 0; i < N; ++i)
-#pragma unroll
-      for (int j = 0; j < N; ++j) {
-        X.e[i][j] = cscale(X.e[i][j], bN);
-        if constexpr (DR > 0) {
-          X.e[i][j].r = fma(k2, M.e[i][j].r, X.e[i][j].r);
-          X.e[i][j].i = fma(k2, M.e[i][j].i, X.e[i][j].i);
-        }
-      }
-    R f[NGN];
-    im_tr_gen<R, N>(X, f);
-    if constexpr (REP::KIND == REP_EXT) {
-      if (do_hop) {
-        const Planes<T> Gp(a.G, Vp, D * NGN);
-#pragma unroll
-        for (int c = 0; c < NGN; ++c) f[c] += R(Gp.ld(g, unsigned(mu * NGN + c)));
-      }
-    }
-    if (a.flags & F_SURR) {
-      const Planes<T> Sp(a.S, Vp, D * NGN);
-      const R sg = R(a.surr_sign);
-#pragma unroll
-      for (int c = 0; c < NGN; ++c) f[c] = fma(sg, R(Sp.ld(g, unsigned(mu * NGN + c))), f[c]);
-    }
-    if (a.flags & F_FORCE) {
-      const Planes<T> Fp(a.F_out, Vp, D * NGN);
-#pragma unroll
-      for (int c = 0; c < NGN; ++c) Fp.st(g, unsigned(mu * NGN + c), T(f[c]));
-      continue;
-    }
-    R p[NGN];
-#pragma unroll
-    for (int c = 0; c < NGN; ++c) p[c] = R(Pp.ld(g, unsigned(mu * NGN + c)));
-    if (a.flags & F_KICK) {
-#pragma unroll
-      for (int c = 0; c < NGN; ++c) { p[c] = fma(-ek, f[c], p[c]); Pp.st(g, unsigned(mu * NGN + c), T(p[c])); }
-    }
-    if (a.flags & F_DRIFT) {
-      const Mat<R, N> H = alg_to_mat<R, N>(p, ed);
-      const Mat<R, N> E = expi<R, N>(H);
-      bstore_link<T, R, N>(Uout, mu, g, mul_nn(E, Ux));
+    // backward staple U_nu(x+mu-nu)^dag U_mu(x-nu)^dag U_nu(x-nu)
+    {
+      const unsigned gmn = g + nb.dn[nu];
+      const Mat<R, N> AB = mul_dd(bload_link<T, R, N>(Uin, nu, gpm + nb.dn[nu]), bload_link<T, R, N>(Uin, mu, gmn));
+      mac_nn(Sig, AB, bload_link<T, R, N>(Uin, nu, gmn));
     }
   }
-  if constexpr (DR > 0) {
-    if (a.flags & F_FORCE) { bstore_vec<T, R, DR>(Planes<T>(a.Fphi_out, Vp, 2 * DR), g, fph); return; }
-    const Planes<T> Pi(a.pi, Vp, 2 * DR);
-    cx<R> pv[DR];
-    bload_vec<T, R, DR>(Pi, g, pv);
-    const bool frozen = a.flags & F_FROZEN;
-    if (a.flags & F_KICK) {
+  Mat<R, N> X = mul_nn(Ux, Sig);
 #pragma unroll
-      for (int k = 0; k < DR; ++k) { pv[k].r = fma(ek, fph[k].r, pv[k].r); pv[k].i = fma(ek, fph[k].i, pv[k].i); }
-      if (frozen) tangent_project<R, DR>(phx, pv);
-    }
-    if (a.flags & F_DRIFT) {
-      if (frozen) {
-        geodesic_drift<R, DR>(phx, pv, ed);
-      } else {
+  for (int i = 0; i < N; ++i)
 #pragma unroll
-        for (int k = 0; k < DR; ++k) { phx[k].r = fma(ed, pv[k].r, phx[k].r); phx[k].i = fma(ed, pv[k].i, phx[k].i); }
+    for (int j = 0; j < N; ++j) {
+      X.e[i][j] = cscale(X.e[i][j], bN);
+      if constexpr (DR > 0) {
+        X.e[i][j].r = fma(k2, M.e[i][j].r, X.e[i][j].r);
+        X.e[i][j].i = fma(k2, M.e[i][j].i, X.e[i][j].i);
       }
-      bstore_vec<T, R, DR>(Planes<T>(a.phi_out, Vp, 2 * DR), g, phx);
     }
-    if ((a.flags & F_KICK) || (frozen && (a.flags & F_DRIFT))) bstore_vec<T, R, DR>(Pi, g, pv);
+  R f[NGN];
+  im_tr_gen<R, N>(X, f);
+  if constexpr (REP::KIND == REP_EXT) {
+    if (do_hop) {
+      const Planes<T> Gp(a.G, Vp, D * NGN);
+#pragma unroll
+      for (int c = 0; c < NGN; ++c) f[c] += R(Gp.ld(g, unsigned(mu * NGN + c)));
+    }
+  }
+  if (a.flags & F_SURR) {
+    const Planes<T> Sp(a.S, Vp, D * NGN);
+    const R sg = R(a.surr_sign);
+#pragma unroll
+    for (int c = 0; c < NGN; ++c) f[c] = fma(sg, R(Sp.ld(g, unsigned(mu * NGN + c))), f[c]);
+  }
+  if (a.flags & F_FORCE) {
+    const Planes<T> Fp(a.F_out, Vp, D * NGN);
+#pragma unroll
+    for (int c = 0; c < NGN; ++c) Fp.st(g, unsigned(mu * NGN + c), T(f[c]));
+    return;
+  }
+  R p[NGN];
+#pragma unroll
+  for (int c = 0; c < NGN; ++c) p[c] = R(Pp.ld(g, unsigned(mu * NGN + c)));
+  if (a.flags & F_KICK) {
+#pragma unroll
+    for (int c = 0; c < NGN; ++c) { p[c] = fma(-ek, f[c], p[c]); Pp.st(g, unsigned(mu * NGN + c), T(p[c])); }
+  }
+  if (a.flags & F_DRIFT) {
+    const Mat<R, N> H = alg_to_mat<R, N>(p, ed);
+    const Mat<R, N> E = expi<R, N>(H);
+    bstore_link<T, R, N>(Uout, mu, g, mul_nn(E, Ux));
   }
 }
 
+// The site's scalar after its links: the F_FORCE output, or the pi kick (with
+// the tangent projection of a frozen scalar) and the phi drift.
+template <typename T, int DR>
+GHX_DEV void site_phi_end(const StepArgs<T>& a, const RepDesc& rd, unsigned g, cx<T>* phx, const cx<T>* fph) {
+  using R = T;
+  const unsigned Vp = a.Vp;
+  const bool abs_eps = a.flags & F_ABS_EPS;
+  const R ek = R(abs_eps ? a.ck : a.ck * rd.eps);
+  const R ed = R(abs_eps ? a.cd : a.cd * rd.eps);
+  if (a.flags & F_FORCE) { bstore_vec<T, R, DR>(Planes<T>(a.Fphi_out, Vp, 2 * DR), g, fph); return; }
+  const Planes<T> Pi(a.pi, Vp, 2 * DR);
+  cx<R> pv[DR];
+  bload_vec<T, R, DR>(Pi, g, pv);
+  const bool frozen = a.flags & F_FROZEN;
+  if (a.flags & F_KICK) {
+#pragma unroll
+    for (int k = 0; k < DR; ++k) { pv[k].r = fma(ek, fph[k].r, pv[k].r); pv[k].i = fma(ek, fph[k].i, pv[k].i); }
+    if (frozen) tangent_project<R, DR>(phx, pv);
+  }
+  if (a.flags & F_DRIFT) {
+    if (frozen) {
+      geodesic_drift<R, DR>(phx, pv, ed);
+    } else {
+#pragma unroll
+      for (int k = 0; k < DR; ++k) { phx[k].r = fma(ed, pv[k].r, phx[k].r); phx[k].i = fma(ed, pv[k].i, phx[k].i); }
+    }
+    bstore_vec<T, R, DR>(Planes<T>(a.phi_out, Vp, 2 * DR), g, phx);
+  }
+  if ((a.flags & F_KICK) || (frozen && (a.flags & F_DRIFT))) bstore_vec<T, R, DR>(Pi, g, pv);
+}
+
+template <typename T, int N, int D, class REP>
+GHX_DEV void site_update(const StepArgs<T>& a, const typename REP::template Params<T>& rp, const RepDesc& rd, unsigned s) {
+  constexpr int DR = REP::DR;
+  const unsigned g = rd.offset + s;
+  const Nbr<D> nb(rd, s);
+  cx<T> phx[DR > 0 ? DR : 1], fph[DR > 0 ? DR : 1];
+  if constexpr (DR > 0) site_phi_begin<T, DR>(a, rd, g, phx, fph);
+#pragma unroll 1
+  for (int mu = 0; mu < D; ++mu) link_update<T, N, D, REP>(a, rp, rd, g, nb, mu, phx, fph);
+  if constexpr (DR > 0) site_phi_end<T, DR>(a, rd, g, phx, fph);
+}
 
 // One site per thread; blocks renumbered XCD-aware.
 template <typename T, int N, int D, class REP>
@@ -464,6 +484,53 @@ __global__ __launch_bounds__(kBlock, GHX_MD_WAVES) void md_stream_kernel(StepArg
   const int t1 = t0 + seg < Lt ? t0 + seg : Lt;
 #pragma unroll 1
   for (int t = t0; t < t1; ++t) site_update<T, N, D, REP>(a, rp, rd, s3 + unsigned(t) * V3);
+}
+
+// Link-parallel variant for lattices too small to fill the GPU with one thread
+// per site: a workgroup of D waves covers 64 sites, wave mu runs link_update
+// for link mu of every site (so mu is uniform within a wave and every access
+// stays a 256-byte row of a plane). The waves' hop terms of the scalar force
+// meet in LDS; wave 0 adds them in direction order, then the on-site term,
+// and updates the site's scalar. D times the waves of md_step_kernel to hide
+// load latency; the same arithmetic except for the order in which the scalar
+// force terms are summed.
+constexpr int kLinkSites = 64;  // sites per workgroup (one wave per direction)
+template <typename T, int N, int D, class REP>
+__global__ __launch_bounds__(kLinkSites * D) void md_link_kernel(StepArgs<T> a, typename REP::template Params<T> rp) {
+  constexpr int DR = REP::DR;
+  const unsigned bid = xcd_block(blockIdx.x, gridDim.x);
+  const int mu = __builtin_amdgcn_readfirstlane(int(threadIdx.x) / kLinkSites);
+  const int lane = int(threadIdx.x) % kLinkSites;
+  const unsigned gsite = bid * kLinkSites + unsigned(lane);
+  const int r = a.blk_rep[gsite / kBlock];
+  const RepDesc& rd = a.reps[r];
+  const unsigned s = gsite - rd.offset;
+  const bool live = s < rd.vol;
+  cx<T> phx[DR > 0 ? DR : 1], fph[DR > 0 ? DR : 1];
+#pragma unroll
+  for (int k = 0; k < (DR > 0 ? DR : 1); ++k) fph[k] = {T(0), T(0)};
+  if (live) {
+    const Nbr<D> nb(rd, s);
+    if constexpr (DR > 0) bload_vec<T, T, DR>(Planes<T>(a.phi_in, a.Vp, 2 * DR), gsite, phx);
+    link_update<T, N, D, REP>(a, rp, rd, gsite, nb, mu, phx, fph);
+  }
+  if constexpr (DR > 0) {
+    __shared__ cx<T> part[D][DR][kLinkSites];
+#pragma unroll
+    for (int k = 0; k < DR; ++k) part[mu][k][lane] = fph[k];
+    __syncthreads();
+    if (mu != 0 || !live) return;
+    cx<T> on[DR];
+    site_phi_begin<T, DR>(a, rd, gsite, phx, on);
+#pragma unroll
+    for (int k = 0; k < DR; ++k) {
+      cx<T> t = part[0][k][lane];
+#pragma unroll
+      for (int m = 1; m < D; ++m) { t.r += part[m][k][lane].r; t.i += part[m][k][lane].i; }
+      fph[k] = {on[k].r + t.r, on[k].i + t.i};
+    }
+    site_phi_end<T, DR>(a, rd, gsite, phx, fph);
+  }
 }
 
 // Drift alone (U' = exp(i eps P) U, phi' = phi + eps pi) — for the kick/drift API.

@@ -42,6 +42,9 @@ struct Launch {
       const unsigned nseg = (Lt + unsigned(c.stream_seg) - 1) / unsigned(c.stream_seg);
       hipLaunchKernelGGL((md_stream_kernel<T, N, D, REP>), dim3(ntile * nseg), dim3(kBlock), 0, c.stream, a, ParamFill<REP, T>::make(c),
                          ntile, c.stream_seg);
+    } else if (c.link_lanes) {
+      hipLaunchKernelGGL((md_link_kernel<T, N, D, REP>), dim3(c.nblk * (kBlock / kLinkSites)), dim3(kLinkSites * D), 0, c.stream, a,
+                         ParamFill<REP, T>::make(c));
     } else {
       hipLaunchKernelGGL((md_step_kernel<T, N, D, REP>), dim3(c.nblk), dim3(kBlock), 0, c.stream, a, ParamFill<REP, T>::make(c));
     }

@@ -36,6 +36,7 @@ struct LaunchCtx {
   int stream_seg;   // > 0: single-lattice streaming kernel with this many t-slices per block
   unsigned V3, Lt;  // cross-section volume and last extent of replica 0 (streaming kernel)
   BlockWalk walk;   // block order of the MD-step kernel
+  int link_lanes;   // 1: link-parallel MD step (one lane per link)
   const void* S;    // surrogate link force planes
   double surr_sign;
   int frozen;       // |phi_x| = 1 scalar: adds F_FROZEN to every MD launch

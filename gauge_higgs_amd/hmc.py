@@ -197,6 +197,15 @@ class Lattices:
         self._check(self._lib.ghx_get_tuning(self.h, 1, C.byref(v)))
         return v.value
 
+    def set_link_lanes(self, on):
+        """One lane per link (four per site) in the MD step, for small lattices."""
+        self._check(self._lib.ghx_set_tuning(self.h, 2, int(on)))
+
+    def link_lanes(self):
+        v = C.c_int()
+        self._check(self._lib.ghx_get_tuning(self.h, 2, C.byref(v)))
+        return v.value
+
     def autotune(self, steps=10):
         """Pick the fastest MD-step launch shape on the live fields (fields restored)."""
         best, ms = C.c_int(), C.c_double()

@@ -174,10 +174,12 @@ int ghx_swap_stats(ghx_ctx* ctx, int* npairs, int* pairs, uint64_t* attempts, ui
  * GHX_TUNE_BLOCK_ORDER = order in which the one-site-per-thread kernel walks the
  * blocks of a single lattice (0 storage order; 1 each XCD's share of slices
  * walked with the last direction fastest; 2 as 1 with direction D-2 next;
- * get_tuning reports 0 where the order does not apply). The autotuner times
+ * get_tuning reports 0 where the order does not apply); GHX_TUNE_LINK_LANES = 1
+ * runs the MD step with one lane per link (four per site) instead of one
+ * thread per site, for lattices too small to fill the GPU. The autotuner times
  * every candidate for `steps` MD steps on the live fields, restores the
  * fields, and keeps the fastest. */
-enum { GHX_TUNE_STREAM_SEG = 0, GHX_TUNE_BLOCK_ORDER = 1 };
+enum { GHX_TUNE_STREAM_SEG = 0, GHX_TUNE_BLOCK_ORDER = 1, GHX_TUNE_LINK_LANES = 2 };
 int ghx_set_tuning(ghx_ctx* ctx, int key, int value);
 int ghx_get_tuning(const ghx_ctx* ctx, int key, int* value);
 int ghx_autotune(ghx_ctx* ctx, int steps, int* best_seg, double* best_ms_per_step);

@@ -278,6 +278,61 @@ def test_block_walk_falls_back_where_it_does_not_apply():
     assert g.lat.block_order() == 0
 
 
+LINK_CASES = [("fund", 2, 4), ("2", 3, 4), ("adj", 2, 3), ("", 2, 4), ("adj", 3, 4), ("1,1", 3, 3)]
+
+
+@pytest.mark.parametrize("rep,N,D", LINK_CASES, ids=[f"{r or 'pure'}-N{n}-D{d}" for r, n, d in LINK_CASES])
+@pytest.mark.parametrize("prec,tol", [(gha.FP64, 1e-12), (gha.FP32MD, 2e-5)], ids=["fp64", "fp32"])
+def test_link_lanes_equal_site_kernel(rep, N, D, prec, tol):
+    """The link-parallel MD step (one lane per link) does the per-link
+    arithmetic of the site kernel and sums the scalar force in another order:
+    forces, and fields after a trajectory, agree to rounding."""
+    out = []
+    for lanes in (0, 1):
+        g = gha.GaugeHiggsHMC([4, 6, 4, 5][:D], rep, seed=9, D=D, N=N, precision=prec)
+        g.configure(beta=2.3 if N == 2 else 5.7, kappa=0.2, lam=0.5, nmd=6)
+        g.lat.set_link_lanes(lanes)
+        assert g.lat.link_lanes() == lanes
+        g.hot(0.8, 12345, 0.5)
+        F, Fp = g.forces()
+        g.trajectory()
+        out.append((F, Fp, g.last_dH, g.get("U"), g.get("phi") if rep else None, g.get("P")))
+    a, b = out
+    assert relmax(b[0], a[0]) < tol
+    if rep:
+        assert relmax(b[1], a[1]) < tol
+    assert abs(b[2] - a[2]) < 1e3 * tol
+    for k in (3, 5):
+        assert relmax(b[k], a[k]) < 1e2 * tol
+    if rep:
+        assert relmax(b[4], a[4]) < 1e2 * tol
+
+
+def test_link_lanes_with_frozen_multi_nested():
+    """The link-parallel step with the frozen scalar, the multi-invariant
+    potential and the multi-timescale split (which passes F_NO_HOP /
+    F_NO_GAUGE / F_NO_ONSITE): a reversible trajectory, close to the site
+    kernel's."""
+    res = []
+    for lanes in (0, 1):
+        g = gha.GaugeHiggsHMC(4, "2", seed=3, D=4, N=3).configure(beta=5.7, kappa=0.2, lam=0.5, nmd=4, reunit_each_traj=False)
+        g.lat.set_link_lanes(lanes)
+        g.lat.set_multi_potential([0.2, 0.35, 0.5], 0.4)
+        g.hot(0.8, 12345, 0.5)
+        g.frozen_phi = True
+        g.normalize_phi()
+        g.lat.set_nested(3, 2)
+        g.refresh_momenta()
+        U0, phi0 = g.get("U"), g.get("phi")
+        g.md_evolve()
+        res.append((g.get("U"), g.get("phi")))
+        g.set("P", -g.get("P"))
+        g.set("pi", -g.get("pi"))
+        g.md_evolve()
+        assert np.max(np.abs(g.get("U") - U0)) < 1e-10 and np.max(np.abs(g.get("phi") - phi0)) < 1e-10
+    assert relmax(res[1][0], res[0][0]) < 1e-11 and relmax(res[1][1], res[0][1]) < 1e-11
+
+
 def test_autotune_restores_fields():
     g = gha.GaugeHiggsHMC(8, "2", seed=2, D=4, N=3, precision=gha.FP32MD).configure(beta=5.7, kappa=0.1, lam=0.5)
     g.hot(0.8, 12345, 0.5)
