@@ -31,6 +31,23 @@ struct BigRepTables {
   const int* T_ptr;     // CSR of T^a_R by row (a*d + l): entries [T_ptr[r], T_ptr[r+1])
   const int* T_col;     // column m of each entry
   const double* T_val;  // complex value (re, im) of each entry
+  // symmetric-power engine (symrep.hpp): D(U) = invc Yc^T (Sym^k1(U) (x) Sym^k2(U)) Yc
+  struct Sym {
+    int on = 0;           // 1: sym_matter_kernel replaces big_matter_kernel
+    int k1 = 0, k2 = 0, s1 = 0, s2 = 1, dimS = 0;
+    const int* rec_off = nullptr;  // level k's recipes start at rec_off[k] (k = 0..k1+1)
+    const int4* recipe = nullptr;  // per level entry: sources in level k-1 (or -1) and the column j
+    const double* scale1 = nullptr;  // sqrt(m!/n!) per entry of level k1 (FP64 and FP32 copies)
+    const float* scale1f = nullptr;
+    const double* scale2 = nullptr;  // the same for level k2
+    const float* scale2f = nullptr;
+    const double* Yc = nullptr;    // dimS x d
+    const float* Ycf = nullptr;
+    double invc = 0.0;
+    void* lift = nullptr;          // Yc phi per site, site-major: [Vp][dimS] complex, storage type
+    void* hops = nullptr;          // the 2D hops of every site, site-major: [Vp][2D][d] complex, storage type
+    int lds_Ca = 0, lds_Cb = 0, lds_S2 = 0, lds_V = 0, lds_T1 = 0, lds_W = 0, lds_Ws = 0, lds_total = 0;  // cx units
+  } sym;
 };
 
 enum BigMode : int { BIG_STEP = 0, BIG_FORCE = 1, BIG_ACTION = 2 };
@@ -127,6 +144,98 @@ GHX_DEV void big_apply(const BigRepTables& tb, const Mat<R, N>& M, const cx<R>* 
   __syncthreads();
 }
 
+// The site's scalar after its hop terms (fph, in LDS, shared by the whole
+// workgroup): the on-site term -2 dV/dphi^* (scalar_higgs.hpp:111-125, or the
+// Fon planes of a plugged-in potential), then the F_FORCE output or the pi kick
+// and phi drift, with the tangent projection and great-circle drift of a
+// frozen scalar. vin is workgroup scratch of d entries.
+template <typename T>
+GHX_DEV void big_phi_finish(const BigArgs<T>& a, const RepDesc& rd, unsigned g, int d, int nthreads, cx<T>* phx, cx<T>* fph,
+                            cx<T>* vin) {
+  using R = T;
+  const unsigned Vp = a.Vp;
+  // on-site term -2 dV/dphi^* (scalar_higgs.hpp:111-125)
+  __shared__ R n2s;
+  if (threadIdx.x == 0) {
+    R n2 = R(0);
+    for (int l = 0; l < d; ++l) n2 += phx[l].r * phx[l].r + phx[l].i * phx[l].i;
+    n2s = n2;
+  }
+  __syncthreads();
+  const R c = (a.flags & F_NO_ONSITE) ? R(0) : R(-2) * (R(1) + R(2.0 * rd.lambda) * (n2s - R(1)));
+  const bool abs_eps = a.flags & F_ABS_EPS;
+  const R ek = R(abs_eps ? a.ck : a.ck * rd.eps), ed = R(abs_eps ? a.cd : a.cd * rd.eps);
+  for (int l = threadIdx.x; l < d; l += nthreads) {
+    cx<R> f = fph[l];
+    if (!a.Fon) {
+      f.r = fma(c, phx[l].r, f.r);
+      f.i = fma(c, phx[l].i, f.i);
+    } else if (!(a.flags & F_NO_ONSITE)) {
+      f.r += R(a.Fon[size_t(2 * l) * Vp + g]);
+      f.i += R(a.Fon[size_t(2 * l + 1) * Vp + g]);
+    }
+    if (a.mode == BIG_FORCE) {
+      a.Fphi[size_t(2 * l) * Vp + g] = T(f.r);
+      a.Fphi[size_t(2 * l + 1) * Vp + g] = T(f.i);
+      continue;
+    }
+    R pr = R(a.pi[size_t(2 * l) * Vp + g]), pim = R(a.pi[size_t(2 * l + 1) * Vp + g]);
+    if (a.flags & F_KICK) {
+      pr = fma(ek, f.r, pr);
+      pim = fma(ek, f.i, pim);
+    }
+    vin[l] = {pr, pim};  // the site's momenta, kicked
+  }
+  if (a.mode == BIG_FORCE) return;
+  const bool frozen = a.flags & F_FROZEN;
+  __syncthreads();
+  if (frozen) {
+    // |phi| = 1 (gauge_higgs_hmc.hpp:61-104): project the kicked pi onto the
+    // tangent space, then move along the great circle
+    __shared__ R red;
+    if (a.flags & F_KICK) {
+      if (threadIdx.x == 0) {
+        R rp = R(0);
+        for (int l = 0; l < d; ++l) rp += phx[l].r * vin[l].r + phx[l].i * vin[l].i;
+        red = rp;
+      }
+      __syncthreads();
+      const R rp = red;
+      for (int l = threadIdx.x; l < d; l += nthreads) { vin[l].r -= rp * phx[l].r; vin[l].i -= rp * phx[l].i; }
+      __syncthreads();
+    }
+    if (a.flags & F_DRIFT) {
+      if (threadIdx.x == 0) {
+        R r2 = R(0);
+        for (int l = 0; l < d; ++l) r2 += vin[l].r * vin[l].r + vin[l].i * vin[l].i;
+        red = r2;
+      }
+      __syncthreads();
+      const R rr = sqrt(red);
+      const R rmin = sizeof(R) == 8 ? R(1e-300) : R(1e-30);
+      if (rr >= rmin) {
+        const R cs = cos(rr * ed), sn = sin(rr * ed), q = sn / rr, w = -rr * sn;
+        for (int l = threadIdx.x; l < d; l += nthreads) {
+          const cx<R> fv = phx[l], pv = vin[l];
+          phx[l] = {cs * fv.r + q * pv.r, cs * fv.i + q * pv.i};
+          vin[l] = {w * fv.r + cs * pv.r, w * fv.i + cs * pv.i};
+        }
+      }
+    }
+  }
+  for (int l = threadIdx.x; l < d; l += nthreads) {
+    if ((a.flags & F_KICK) || (frozen && (a.flags & F_DRIFT))) {
+      a.pi[size_t(2 * l) * Vp + g] = T(vin[l].r);
+      a.pi[size_t(2 * l + 1) * Vp + g] = T(vin[l].i);
+    }
+    if (a.flags & F_DRIFT) {
+      const cx<R> ph = frozen ? phx[l] : cx<R>{fma(ed, vin[l].r, phx[l].r), fma(ed, vin[l].i, phx[l].i)};
+      a.phi_out[size_t(2 * l) * Vp + g] = T(ph.r);
+      a.phi_out[size_t(2 * l + 1) * Vp + g] = T(ph.i);
+    }
+  }
+}
+
 template <typename T, int N, int D>
 __global__ __launch_bounds__(kBigBlock) void big_matter_kernel(BigArgs<T> a, BigRepTables tb) {
   using R = T;
@@ -203,86 +312,7 @@ __global__ __launch_bounds__(kBigBlock) void big_matter_kernel(BigArgs<T> a, Big
     if (threadIdx.x == 0) a.q1[g] = q1s;
     return;
   }
-  // on-site term -2 dV/dphi^* (scalar_higgs.hpp:111-125)
-  __shared__ R n2s;
-  if (threadIdx.x == 0) {
-    R n2 = R(0);
-    for (int l = 0; l < d; ++l) n2 += phx[l].r * phx[l].r + phx[l].i * phx[l].i;
-    n2s = n2;
-  }
-  __syncthreads();
-  const R c = (a.flags & F_NO_ONSITE) ? R(0) : R(-2) * (R(1) + R(2.0 * rd.lambda) * (n2s - R(1)));
-  const bool abs_eps = a.flags & F_ABS_EPS;
-  const R ek = R(abs_eps ? a.ck : a.ck * rd.eps), ed = R(abs_eps ? a.cd : a.cd * rd.eps);
-  for (int l = threadIdx.x; l < d; l += kBigBlock) {
-    cx<R> f = fph[l];
-    if (!a.Fon) {
-      f.r = fma(c, phx[l].r, f.r);
-      f.i = fma(c, phx[l].i, f.i);
-    } else if (!(a.flags & F_NO_ONSITE)) {
-      f.r += R(a.Fon[size_t(2 * l) * Vp + g]);
-      f.i += R(a.Fon[size_t(2 * l + 1) * Vp + g]);
-    }
-    if (a.mode == BIG_FORCE) {
-      a.Fphi[size_t(2 * l) * Vp + g] = T(f.r);
-      a.Fphi[size_t(2 * l + 1) * Vp + g] = T(f.i);
-      continue;
-    }
-    R pr = R(a.pi[size_t(2 * l) * Vp + g]), pim = R(a.pi[size_t(2 * l + 1) * Vp + g]);
-    if (a.flags & F_KICK) {
-      pr = fma(ek, f.r, pr);
-      pim = fma(ek, f.i, pim);
-    }
-    vin[l] = {pr, pim};  // the site's momenta, kicked
-  }
-  if (a.mode == BIG_FORCE) return;
-  const bool frozen = a.flags & F_FROZEN;
-  __syncthreads();
-  if (frozen) {
-    // |phi| = 1 (gauge_higgs_hmc.hpp:61-104): project the kicked pi onto the
-    // tangent space, then move along the great circle
-    __shared__ R red;
-    if (a.flags & F_KICK) {
-      if (threadIdx.x == 0) {
-        R rp = R(0);
-        for (int l = 0; l < d; ++l) rp += phx[l].r * vin[l].r + phx[l].i * vin[l].i;
-        red = rp;
-      }
-      __syncthreads();
-      const R rp = red;
-      for (int l = threadIdx.x; l < d; l += kBigBlock) { vin[l].r -= rp * phx[l].r; vin[l].i -= rp * phx[l].i; }
-      __syncthreads();
-    }
-    if (a.flags & F_DRIFT) {
-      if (threadIdx.x == 0) {
-        R r2 = R(0);
-        for (int l = 0; l < d; ++l) r2 += vin[l].r * vin[l].r + vin[l].i * vin[l].i;
-        red = r2;
-      }
-      __syncthreads();
-      const R rr = sqrt(red);
-      const R rmin = sizeof(R) == 8 ? R(1e-300) : R(1e-30);
-      if (rr >= rmin) {
-        const R cs = cos(rr * ed), sn = sin(rr * ed), q = sn / rr, w = -rr * sn;
-        for (int l = threadIdx.x; l < d; l += kBigBlock) {
-          const cx<R> fv = phx[l], pv = vin[l];
-          phx[l] = {cs * fv.r + q * pv.r, cs * fv.i + q * pv.i};
-          vin[l] = {w * fv.r + cs * pv.r, w * fv.i + cs * pv.i};
-        }
-      }
-    }
-  }
-  for (int l = threadIdx.x; l < d; l += kBigBlock) {
-    if ((a.flags & F_KICK) || (frozen && (a.flags & F_DRIFT))) {
-      a.pi[size_t(2 * l) * Vp + g] = T(vin[l].r);
-      a.pi[size_t(2 * l + 1) * Vp + g] = T(vin[l].i);
-    }
-    if (a.flags & F_DRIFT) {
-      const cx<R> ph = frozen ? phx[l] : cx<R>{fma(ed, vin[l].r, phx[l].r), fma(ed, vin[l].i, phx[l].i)};
-      a.phi_out[size_t(2 * l) * Vp + g] = T(ph.r);
-      a.phi_out[size_t(2 * l + 1) * Vp + g] = T(ph.i);
-    }
-  }
+  big_phi_finish<T>(a, rd, g, d, kBigBlock, phx, fph, vin);
 }
 
 // On-site part alone (inner steps of the multi-timescale split, F_NO_HOP):

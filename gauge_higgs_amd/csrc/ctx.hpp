@@ -70,6 +70,9 @@ struct ghx_ctx {
   BigOps bops{};
   BigRepTables tb{};
   void* dtab[11] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  std::vector<void*> dsym;   // symmetric-power engine tables and lifted scalar
+  bool sym_valid = false;    // the engine applies to this irrep (host-checked)
+  bool sym_enabled = true;   // GHX_TUNE_SYM_ENGINE
   void* G = nullptr;
   double* q1 = nullptr;
   // replica exchange

@@ -16,6 +16,7 @@
 
 #include "ctx.hpp"
 #include "potential.hpp"
+#include "symrep.hpp"
 
 namespace {
 constexpr double kOmelyanLambda = 0.1931833275037836;  // config.hpp:25
@@ -48,6 +49,10 @@ void free_multi(ghx_ctx* c) {
 void free_tables(ghx_ctx* c) {
   for (auto& p : c->dtab)
     if (p) { (void)hipFree(p); p = nullptr; }
+  for (void* p : c->dsym) (void)hipFree(p);
+  c->dsym.clear();
+  c->tb.sym = BigRepTables::Sym{};
+  c->sym_valid = false;
 }
 
 int alloc_zero(ghx_ctx* ctx, void** p, size_t bytes) {
@@ -147,6 +152,54 @@ int upload_tables(ghx_ctx* ctx) {
   ctx->tb = BigRepTables{r.d, r.nbox, int(dimT), (const int*)ctx->dtab[0], (const int*)ctx->dtab[1], (const double*)ctx->dtab[2],
                          (const int*)ctx->dtab[3], (const int*)ctx->dtab[4], (const double*)ctx->dtab[5], (const float*)ctx->dtab[6],
                          (const float*)ctx->dtab[7], (const int*)ctx->dtab[8], (const int*)ctx->dtab[9], (const double*)ctx->dtab[10]};
+  // symmetric-power engine, when the irrep admits it and passes the host check
+  const SymBuild sb = build_sym(r);
+  ctx->sym_valid = sb.ok;
+  if (sb.ok) {
+    const std::vector<float> Ycf(sb.Yc.begin(), sb.Yc.end()), s1f(sb.scale1.begin(), sb.scale1.end()),
+        s2f(sb.scale2.begin(), sb.scale2.end());
+    const void* ssrc[8] = {sb.rec_off.data(), sb.recipe.data(), sb.scale1.data(), s1f.data(), sb.scale2.data(), s2f.data(),
+                           sb.Yc.data(), Ycf.data()};
+    const size_t sbytes[8] = {sb.rec_off.size() * 4, sb.recipe.size() * 4, sb.scale1.size() * 8, s1f.size() * 4,
+                              sb.scale2.size() * 8, s2f.size() * 4, sb.Yc.size() * 8, Ycf.size() * 4};
+    void* dp[9] = {};
+    for (int k = 0; k < 8; ++k) {
+      HIPCHK(hipMalloc(&dp[k], std::max<size_t>(sbytes[k], 16)));
+      ctx->dsym.push_back(dp[k]);
+      HIPCHK(hipMemcpy(dp[k], ssrc[k], sbytes[k], hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMalloc(&dp[8], size_t(2) * sb.dimS * ctx->Vp * ctx->tsize));
+    ctx->dsym.push_back(dp[8]);
+    void* hops = nullptr;
+    HIPCHK(hipMalloc(&hops, size_t(2) * 2 * ctx->D * r.d * ctx->Vp * ctx->tsize));
+    ctx->dsym.push_back(hops);
+    ctx->tb.sym.hops = hops;
+    BigRepTables::Sym& st = ctx->tb.sym;
+    st.k1 = sb.k1; st.k2 = sb.k2; st.s1 = sb.s1; st.s2 = sb.s2; st.dimS = sb.dimS;
+    st.rec_off = (const int*)dp[0]; st.recipe = (const int4*)dp[1];
+    st.scale1 = (const double*)dp[2]; st.scale1f = (const float*)dp[3]; st.scale2 = (const double*)dp[4]; st.scale2f = (const float*)dp[5];
+    st.Yc = (const double*)dp[6]; st.Ycf = (const float*)dp[7];
+    st.invc = sb.invc;
+    st.lift = dp[8];
+    // LDS (cx units): Ca holds the levels of k1's parity, Cb the others; once
+    // the build is done, V, T1 and W reuse Cb when it is large enough
+    int ca = 0, cb = 0;
+    for (int k = 0; k <= sb.k1; ++k) {
+      const int sz = (sb.rec_off[k + 1] - sb.rec_off[k]);
+      if ((sb.k1 - k) % 2 == 0) ca = std::max(ca, sz);
+      else cb = std::max(cb, sz);
+    }
+    st.lds_Ca = 0;
+    st.lds_Cb = ca;
+    st.lds_S2 = ca + cb;
+    const int after_s2 = st.lds_S2 + sb.s2 * sb.s2;
+    st.lds_V = cb >= 3 * sb.dimS ? st.lds_Cb : after_s2;
+    st.lds_T1 = st.lds_V + sb.dimS;
+    st.lds_W = st.lds_T1 + sb.dimS;
+    st.lds_Ws = st.lds_W + sb.dimS;
+    st.lds_total = std::max(after_s2, st.lds_Ws);
+  }
+  ctx->tb.sym.on = (sb.ok && ctx->sym_enabled) ? 1 : 0;
   return GHX_OK;
 }
 
@@ -1063,6 +1116,13 @@ int ghx_set_tuning(ghx_ctx* ctx, int key, int value) {
     drop_graph(ctx);
     return GHX_OK;
   }
+  if (key == GHX_TUNE_SYM_ENGINE) {
+    if (value < 0 || value > 1) return fail(ctx, GHX_E_INVALID, "symmetric-power engine must be 0 or 1");
+    ctx->sym_enabled = value != 0;
+    ctx->tb.sym.on = (ctx->sym_valid && ctx->sym_enabled) ? 1 : 0;
+    drop_graph(ctx);
+    return GHX_OK;
+  }
   return fail(ctx, GHX_E_INVALID, "unknown tuning key");
 }
 
@@ -1071,6 +1131,7 @@ int ghx_get_tuning(const ghx_ctx* ctx, int key, int* value) {
   if (key == GHX_TUNE_STREAM_SEG) { *value = ctx->stream_seg; return GHX_OK; }
   if (key == GHX_TUNE_BLOCK_ORDER) { *value = ctx->block_walk().order; return GHX_OK; }
   if (key == GHX_TUNE_LINK_LANES) { *value = ctx->link_lanes; return GHX_OK; }
+  if (key == GHX_TUNE_SYM_ENGINE) { *value = ctx->tb.sym.on; return GHX_OK; }
   return GHX_E_INVALID;
 }
 

@@ -1,6 +1,7 @@
 // Large tensor irreps (matrix-free apply in LDS, one workgroup per site).
 #include "bigrep.hpp"
 #include "launch.hpp"
+#include "symrep_dev.hpp"
 namespace ghx {
 template <typename T, int N, int D>
 struct BigLaunch {
@@ -11,6 +12,18 @@ struct BigLaunch {
     if (c.multi && mode != BIG_ACTION && !(flags & F_NO_ONSITE) && (mode == BIG_FORCE || (flags & F_KICK))) {
       multi(c, tb.d, phi_in, c.Fon, nullptr);
       a.Fon = (const T*)c.Fon;
+    }
+    if (tb.sym.on) {  // symmetric-power engine: lift every site's scalar, one wave per link, then one per site
+      T* hops = static_cast<T*>(tb.sym.hops);
+      if (!(flags & F_NO_HOP)) {
+        const size_t lds = size_t(tb.d) * kSymWave * sizeof(cx<T>) + size_t(kSymWave) * (2 * kLiftChunk + 1) * sizeof(T);
+        hipLaunchKernelGGL(sym_lift_kernel<T>, dim3(c.Vp / kSymWave, (tb.sym.dimS + kLiftChunk - 1) / kLiftChunk), dim3(kSymWave), lds,
+                           c.stream, c.Vp, (const T*)phi_in, tb.d, tb.sym);
+        hipLaunchKernelGGL((sym_link_kernel<T, N, D>), dim3(c.Vp, D), dim3(kSymWave), size_t(tb.sym.lds_total) * sizeof(cx<T>),
+                           c.stream, a, tb, hops, mode == BIG_ACTION ? 0 : 1);  // the action needs the forward hops only
+      }
+      hipLaunchKernelGGL((sym_finish_kernel<T, N, D>), dim3(c.Vp), dim3(kSymWave), 0, c.stream, a, tb, (const T*)hops);
+      return;
     }
     const size_t lds = size_t(tb.dimT) * sizeof(cx<T>);
     hipLaunchKernelGGL((big_matter_kernel<T, N, D>), dim3(c.Vp), dim3(kBigBlock), lds, c.stream, a, tb);

@@ -1,0 +1,49 @@
+// Symmetric-power engine for a large tensor irrep R = Young rows (k1[, k2]).
+//
+// The irrep spanned by the reference's basis W (rep_general.hpp:338-411) sits,
+// up to a unitary intertwiner, inside S = Sym^k1(V) (x) Sym^k2(V): the slots of
+// row 1 symmetrised, then those of row 2. With the orthonormal monomial bases
+// |m> = sqrt(m!/k!) sum_{tuples of type m} |t> of Sym^k, the map
+// Yc = B^T W^T : C^d -> S is an isometry up to a scale c (Schur), and
+//   D^(R)(U) = (1/c) Yc^T (Sym^k1(U) (x) Sym^k2(U)) Yc.
+// Sym^k(U) in that basis is diag(sqrt(m!)) C_k diag(1/sqrt(n!)), where column n
+// of C_k holds the coefficients of prod_j (sum_i U_ij x_i)^(n_j); C_k follows
+// from C_(k-1) one linear factor at a time:
+//   C_k[m][n] = sum_i U[i][j] C_(k-1)[m - e_i][n - e_j],  j = first index with n_j > 0.
+// A tensor apply costs n one-slot passes over N^n entries; this costs
+// sum_k s_k^2 N + s1 s2 (s1 + s2) per apply (s_k = dim Sym^k), about 12x fewer
+// operations for SU(3) {6,1} (d = 48, N^n = 2187, dim S = 84).
+#pragma once
+#include <vector>
+
+#include "rep_build.hpp"
+
+namespace ghx {
+
+constexpr int kSymMaxS = 36;      // largest dim Sym^k of the engine (SU(3): k <= 7)
+constexpr int kSymMaxDimS = 256;  // largest dim S
+
+struct SymBuild {
+  bool ok = false;
+  int N = 0, k1 = 0, k2 = 0, s1 = 0, s2 = 1, dimS = 0, d = 0;
+  double invc = 0.0;
+  std::vector<double> Yc;       // dimS x d, row-major (real: W and the monomial bases are real)
+  std::vector<int> lvl_off;     // offset of level k in the per-level tables, k = 0..k1+1
+  std::vector<int> colj, colsrc;  // per (level k, column n): j and the index of n - e_j at level k-1
+  std::vector<int> rowsrc;        // per (level k, row m) x N: index of m - e_i at level k-1, or -1
+  std::vector<double> fnorm;      // per (level k, m): sqrt(m!)
+  // device form of the recursion: per level k = 1..k1 and entry e = m s_k + n,
+  // {src_0, .., src_(N-1), j} with src_i = the entry (m - e_i, n - e_j) of level
+  // k-1 (row-major) or -1; entries of level k start at rec_off[k]
+  std::vector<int> recipe;        // 4 ints per entry (N <= 3)
+  std::vector<int> rec_off;       // k = 0..k1+1
+  std::vector<double> scale1, scale2;  // sqrt(m!/n!) per entry of levels k1 and k2
+  double max_err = 0.0;           // host check against the tensor apply on generic links
+};
+
+// Builds the engine for a tensor irrep and checks D^(R)(U) against the tensor
+// apply W U^(x n) W^T on three generic links; ok = false when the irrep has
+// more than two rows, exceeds the size limits or fails the check (1e-10).
+SymBuild build_sym(const RepInfo& r);
+
+}  // namespace ghx

@@ -206,6 +206,16 @@ class Lattices:
         self._check(self._lib.ghx_get_tuning(self.h, 2, C.byref(v)))
         return v.value
 
+    def set_sym_engine(self, on):
+        """Large tensor irreps: apply D(U) through symmetric powers of U (1) or
+        slot by slot on the tensor (0)."""
+        self._check(self._lib.ghx_set_tuning(self.h, 3, int(on)))
+
+    def sym_engine(self):
+        v = C.c_int()
+        self._check(self._lib.ghx_get_tuning(self.h, 3, C.byref(v)))
+        return v.value
+
     def autotune(self, steps=10):
         """Pick the fastest MD-step launch shape on the live fields (fields restored)."""
         best, ms = C.c_int(), C.c_double()

@@ -74,7 +74,7 @@ def compare(L=12, rep="6,1", beta=5.7, kappa=0.05, lam=0.5, nmd=16, exact_nmds=(
     each split (n_outer, m_inner) runs with the Omelyan outer scale, with the
     surrogate forced on ("surrogate_on") and without it ("split_no_surrogate").
 
-    The learned chain keeps the exact force as fallback and judge: it uses the
+    The learned chain keeps the exact force as fallback and arbiter: it uses the
     surrogate only when its held-out residual |S - F|^2 / |F|^2 is below
     RESIDUAL_GATE, otherwise the coarse scale carries the exact hopping force
     alone; "learned" reports whichever of the two measured chains that is."""

@@ -176,10 +176,13 @@ int ghx_swap_stats(ghx_ctx* ctx, int* npairs, int* pairs, uint64_t* attempts, ui
  * walked with the last direction fastest; 2 as 1 with direction D-2 next;
  * get_tuning reports 0 where the order does not apply); GHX_TUNE_LINK_LANES = 1
  * runs the MD step with one lane per link (four per site) instead of one
- * thread per site, for lattices too small to fill the GPU. The autotuner times
- * every candidate for `steps` MD steps on the live fields, restores the
- * fields, and keeps the fastest. */
-enum { GHX_TUNE_STREAM_SEG = 0, GHX_TUNE_BLOCK_ORDER = 1, GHX_TUNE_LINK_LANES = 2 };
+ * thread per site, for lattices too small to fill the GPU; GHX_TUNE_SYM_ENGINE
+ * = 1 (default where it applies) applies a large tensor irrep through
+ * symmetric powers of the link, D(U) = Yc^T (Sym^k1(U) x Sym^k2(U)) Yc / c, instead of
+ * slot by slot on the N^n tensor (get_tuning reports whether it is in use). The
+ * autotuner times every candidate for `steps` MD steps on the live fields,
+ * restores the fields, and keeps the fastest. */
+enum { GHX_TUNE_STREAM_SEG = 0, GHX_TUNE_BLOCK_ORDER = 1, GHX_TUNE_LINK_LANES = 2, GHX_TUNE_SYM_ENGINE = 3 };
 int ghx_set_tuning(ghx_ctx* ctx, int key, int value);
 int ghx_get_tuning(const ghx_ctx* ctx, int key, int* value);
 int ghx_autotune(ghx_ctx* ctx, int steps, int* best_seg, double* best_ms_per_step);
