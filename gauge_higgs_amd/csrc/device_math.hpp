@@ -137,6 +137,43 @@ template <typename R, int N> GHX_DEV void mac_nd(Mat<R, N>& Acc, const Mat<R, N>
 #pragma unroll
       for (int k = 0; k < N; ++k) cfmcc(Acc.e[i][j], A.e[i][k], B.e[j][k]);
 }
+// Acc += A^dag B
+template <typename R, int N> GHX_DEV void mac_dn(Mat<R, N>& Acc, const Mat<R, N>& A, const Mat<R, N>& B) {
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+#pragma unroll
+      for (int k = 0; k < N; ++k) cfmac(Acc.e[i][j], A.e[k][i], B.e[k][j]);
+}
+// Rows 0..N-2 of A B and of A B^dag (the last row of A is not read; the last
+// row of the result is left unset).
+template <typename R, int N> GHX_DEV Mat<R, N> mul_nn_top(const Mat<R, N>& A, const Mat<R, N>& B) {
+  Mat<R, N> C;
+#pragma unroll
+  for (int i = 0; i < N - 1; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      cx<R> s = cmul(A.e[i][0], B.e[0][j]);
+#pragma unroll
+      for (int k = 1; k < N; ++k) cfma(s, A.e[i][k], B.e[k][j]);
+      C.e[i][j] = s;
+    }
+  return C;
+}
+template <typename R, int N> GHX_DEV Mat<R, N> mul_nd_top(const Mat<R, N>& A, const Mat<R, N>& B) {
+  Mat<R, N> C;
+#pragma unroll
+  for (int i = 0; i < N - 1; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      cx<R> s = {R(0), R(0)};
+#pragma unroll
+      for (int k = 0; k < N; ++k) cfmcc(s, A.e[i][k], B.e[j][k]);
+      C.e[i][j] = s;
+    }
+  return C;
+}
 template <typename R, int N> GHX_DEV R retrace(const Mat<R, N>& A) {
   R t = A.e[0][0].r;
 #pragma unroll

@@ -169,7 +169,9 @@ struct Planes {
   GHX_DEV T ld(unsigned g, unsigned c) const { return bload<T>(r, g * unsigned(sizeof(T)), c * pstride); }
   GHX_DEV void st(unsigned g, unsigned c, T v) const { bstore(v, r, g * unsigned(sizeof(T)), c * pstride); }
 };
-template <typename T, typename R, int N>
+// FULL = false leaves the last row of a compressed link unset (for a left
+// factor whose product keeps only its first rows).
+template <typename T, typename R, int N, bool FULL = true>
 GHX_DEV Mat<R, N> bload_link(const Planes<T>& U, int mu, unsigned g) {
   constexpr int NR = link_rows<T, N>(), LR = link_reals<T, N>();
   Mat<R, N> m;
@@ -180,7 +182,7 @@ GHX_DEV Mat<R, N> bload_link(const Planes<T>& U, int mu, unsigned g) {
       m.e[i][j].r = R(U.ld(g, unsigned(mu * LR + 2 * (i * N + j))));
       m.e[i][j].i = R(U.ld(g, unsigned(mu * LR + 2 * (i * N + j) + 1)));
     }
-  if constexpr (NR < N) rebuild_last_row<R, N>(m);
+  if constexpr (NR < N && FULL) rebuild_last_row<R, N>(m);
   return m;
 }
 template <typename T, typename R, int N>
@@ -346,6 +348,10 @@ GHX_DEV void link_update(const StepArgs<T>& a, const typename REP::template Para
     }
   }
   Mat<R, N> Sig = mzero<R, N>();
+  // Compressed SU(3) links: a product of two links is SU(3), so its last row is
+  // conj(row 0 x row 1). Forming rows 0, 1 only and rebuilding row 2 skips the
+  // left factor's rebuild and a third of the product.
+  constexpr bool kTopRows = N == 3 && link_rows<T, N>() == 2;
 #if GHX_NU_UNROLL
 #pragma unroll
 #else
@@ -355,14 +361,26 @@ GHX_DEV void link_update(const StepArgs<T>& a, const typename REP::template Para
     if (nu == mu) continue;
     // forward staple U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag
     {
-      const Mat<R, N> AB = mul_nd(bload_link<T, R, N>(Uin, nu, gpm), bload_link<T, R, N>(Uin, mu, g + nb.up[nu]));
+      Mat<R, N> AB;
+      if constexpr (kTopRows) {  // rows 0, 1 of the product from rows 0, 1 of U_nu(x+mu); row 2 by the SU(3) cross product
+        AB = mul_nd_top(bload_link<T, R, N, false>(Uin, nu, gpm), bload_link<T, R, N>(Uin, mu, g + nb.up[nu]));
+        rebuild_last_row<R, N>(AB);
+      } else {
+        AB = mul_nd(bload_link<T, R, N>(Uin, nu, gpm), bload_link<T, R, N>(Uin, mu, g + nb.up[nu]));
+      }
       mac_nd(Sig, AB, bload_link<T, R, N>(Uin, nu, g));
     }
-    // backward staple U_nu(x+mu-nu)^dag U_mu(x-nu)^dag U_nu(x-nu)
+    // backward staple U_nu(x+mu-nu)^dag U_mu(x-nu)^dag U_nu(x-nu) = (U_mu(x-nu) U_nu(x+mu-nu))^dag U_nu(x-nu)
     {
       const unsigned gmn = g + nb.dn[nu];
-      const Mat<R, N> AB = mul_dd(bload_link<T, R, N>(Uin, nu, gpm + nb.dn[nu]), bload_link<T, R, N>(Uin, mu, gmn));
-      mac_nn(Sig, AB, bload_link<T, R, N>(Uin, nu, gmn));
+      if constexpr (kTopRows) {
+        Mat<R, N> GF = mul_nn_top(bload_link<T, R, N, false>(Uin, mu, gmn), bload_link<T, R, N>(Uin, nu, gpm + nb.dn[nu]));
+        rebuild_last_row<R, N>(GF);
+        mac_dn(Sig, GF, bload_link<T, R, N>(Uin, nu, gmn));
+      } else {
+        const Mat<R, N> AB = mul_dd(bload_link<T, R, N>(Uin, nu, gpm + nb.dn[nu]), bload_link<T, R, N>(Uin, mu, gmn));
+        mac_nn(Sig, AB, bload_link<T, R, N>(Uin, nu, gmn));
+      }
     }
   }
   Mat<R, N> X = mul_nn(Ux, Sig);
