@@ -174,6 +174,23 @@ template <typename R, int N> GHX_DEV Mat<R, N> mul_nd_top(const Mat<R, N>& A, co
     }
   return C;
 }
+// Rows 0..N-2 of Acc += A B^dag and of Acc += A^dag B.
+template <typename R, int N> GHX_DEV void mac_nd_top(Mat<R, N>& Acc, const Mat<R, N>& A, const Mat<R, N>& B) {
+#pragma unroll
+  for (int i = 0; i < N - 1; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+#pragma unroll
+      for (int k = 0; k < N; ++k) cfmcc(Acc.e[i][j], A.e[i][k], B.e[j][k]);
+}
+template <typename R, int N> GHX_DEV void mac_dn_top(Mat<R, N>& Acc, const Mat<R, N>& A, const Mat<R, N>& B) {
+#pragma unroll
+  for (int i = 0; i < N - 1; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+#pragma unroll
+      for (int k = 0; k < N; ++k) cfmac(Acc.e[i][j], A.e[k][i], B.e[k][j]);
+}
 template <typename R, int N> GHX_DEV R retrace(const Mat<R, N>& A) {
   R t = A.e[0][0].r;
 #pragma unroll

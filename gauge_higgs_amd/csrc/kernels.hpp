@@ -348,10 +348,14 @@ GHX_DEV void link_update(const StepArgs<T>& a, const typename REP::template Para
     }
   }
   Mat<R, N> Sig = mzero<R, N>();
-  // Compressed SU(3) links: a product of two links is SU(3), so its last row is
-  // conj(row 0 x row 1). Forming rows 0, 1 only and rebuilding row 2 skips the
-  // left factor's rebuild and a third of the product.
-  constexpr bool kTopRows = N == 3 && link_rows<T, N>() == 2;
+  // Compressed links: a product of two SU(N) links is SU(N), so its last row
+  // follows from the others (SU(3): conj(row 0 x row 1); SU(2): (-conj(c01),
+  // conj(c00))). Forming rows 0..N-2 only skips the left factor's rebuild and
+  // 1/N of the product. The SU(2) rule is linear, so it also holds for sums of
+  // SU(2) matrices: the staple sum keeps row 0 only and the gauge part of
+  // U Sigma as well (half the complex MACs of the gauge force).
+  constexpr bool kTopRows = (N == 3 || N == 2) && link_rows<T, N>() == N - 1;
+  constexpr bool kQuat = N == 2 && kTopRows;
 #if GHX_NU_UNROLL
 #pragma unroll
 #else
@@ -368,7 +372,8 @@ GHX_DEV void link_update(const StepArgs<T>& a, const typename REP::template Para
       } else {
         AB = mul_nd(bload_link<T, R, N>(Uin, nu, gpm), bload_link<T, R, N>(Uin, mu, g + nb.up[nu]));
       }
-      mac_nd(Sig, AB, bload_link<T, R, N>(Uin, nu, g));
+      if constexpr (kQuat) mac_nd_top(Sig, AB, bload_link<T, R, N>(Uin, nu, g));
+      else mac_nd(Sig, AB, bload_link<T, R, N>(Uin, nu, g));
     }
     // backward staple U_nu(x+mu-nu)^dag U_mu(x-nu)^dag U_nu(x-nu) = (U_mu(x-nu) U_nu(x+mu-nu))^dag U_nu(x-nu)
     {
@@ -376,14 +381,22 @@ GHX_DEV void link_update(const StepArgs<T>& a, const typename REP::template Para
       if constexpr (kTopRows) {
         Mat<R, N> GF = mul_nn_top(bload_link<T, R, N, false>(Uin, mu, gmn), bload_link<T, R, N>(Uin, nu, gpm + nb.dn[nu]));
         rebuild_last_row<R, N>(GF);
-        mac_dn(Sig, GF, bload_link<T, R, N>(Uin, nu, gmn));
+        if constexpr (kQuat) mac_dn_top(Sig, GF, bload_link<T, R, N>(Uin, nu, gmn));
+        else mac_dn(Sig, GF, bload_link<T, R, N>(Uin, nu, gmn));
       } else {
         const Mat<R, N> AB = mul_dd(bload_link<T, R, N>(Uin, nu, gpm + nb.dn[nu]), bload_link<T, R, N>(Uin, mu, gmn));
         mac_nn(Sig, AB, bload_link<T, R, N>(Uin, nu, gmn));
       }
     }
   }
-  Mat<R, N> X = mul_nn(Ux, Sig);
+  Mat<R, N> X;
+  if constexpr (kQuat) {  // Sigma and U Sigma are sums of SU(2) matrices
+    rebuild_last_row<R, N>(Sig);
+    X = mul_nn_top(Ux, Sig);
+    rebuild_last_row<R, N>(X);
+  } else {
+    X = mul_nn(Ux, Sig);
+  }
 #pragma unroll
   for (int i = 0; i < N; ++i)
 #pragma unroll
