@@ -33,20 +33,19 @@ struct BigRepTables {
   const double* T_val;  // complex value (re, im) of each entry
   // symmetric-power engine (symrep.hpp): D(U) = invc Yc^T (Sym^k1(U) (x) Sym^k2(U)) Yc
   struct Sym {
-    int on = 0;           // 1: sym_matter_kernel replaces big_matter_kernel
-    int k1 = 0, k2 = 0, s1 = 0, s2 = 1, dimS = 0;
-    const int* rec_off = nullptr;  // level k's recipes start at rec_off[k] (k = 0..k1+1)
-    const int4* recipe = nullptr;  // per level entry: sources in level k-1 (or -1) and the column j
-    const double* scale1 = nullptr;  // sqrt(m!/n!) per entry of level k1 (FP64 and FP32 copies)
-    const float* scale1f = nullptr;
-    const double* scale2 = nullptr;  // the same for level k2
-    const float* scale2f = nullptr;
-    const double* Yc = nullptr;    // dimS x d
+    int on = 0;           // 1: the engine replaces big_matter_kernel
+    int s1 = 0, s2 = 1, dimS = 0, npass = 0;
+    const int4* prog = nullptr;       // LDS program: npass x 64 entries {s0, s1, s2, (dst << 2) | j}
+    const double* pre = nullptr;      // [2][dimS]: 1/F (forward hop) and F (adjoint), FP64 and FP32 copies
+    const float* pref = nullptr;
+    const double* Yp = nullptr;       // [2][ng][d][4] (q = 4 g + i): Yc F / c and Yc F^-1 / c
+    int ng = 0;
+    const float* Ypf = nullptr;
+    const double* Yc = nullptr;       // dimS x d (the lift)
     const float* Ycf = nullptr;
-    double invc = 0.0;
     void* lift = nullptr;          // Yc phi per site, site-major: [Vp][dimS] complex, storage type
     void* hops = nullptr;          // the 2D hops of every site, site-major: [Vp][2D][d] complex, storage type
-    int lds_Ca = 0, lds_Cb = 0, lds_S2 = 0, lds_V = 0, lds_T1 = 0, lds_W = 0, lds_Ws = 0, lds_total = 0;  // cx units
+    int lds_one = 0, lds_S1 = 0, lds_S2 = 0, lds_V = 0, lds_T1 = 0, lds_W = 0, lds_total = 0;  // cx units
   } sym;
 };
 

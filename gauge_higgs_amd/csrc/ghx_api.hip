@@ -152,54 +152,39 @@ int upload_tables(ghx_ctx* ctx) {
   ctx->tb = BigRepTables{r.d, r.nbox, int(dimT), (const int*)ctx->dtab[0], (const int*)ctx->dtab[1], (const double*)ctx->dtab[2],
                          (const int*)ctx->dtab[3], (const int*)ctx->dtab[4], (const double*)ctx->dtab[5], (const float*)ctx->dtab[6],
                          (const float*)ctx->dtab[7], (const int*)ctx->dtab[8], (const int*)ctx->dtab[9], (const double*)ctx->dtab[10]};
-  // symmetric-power engine, when the irrep admits it and passes the host check
+  // symmetric-power engine, when the irrep admits it and passes the host checks
   const SymBuild sb = build_sym(r);
-  ctx->sym_valid = sb.ok;
-  if (sb.ok) {
-    const std::vector<float> Ycf(sb.Yc.begin(), sb.Yc.end()), s1f(sb.scale1.begin(), sb.scale1.end()),
-        s2f(sb.scale2.begin(), sb.scale2.end());
-    const void* ssrc[8] = {sb.rec_off.data(), sb.recipe.data(), sb.scale1.data(), s1f.data(), sb.scale2.data(), s2f.data(),
-                           sb.Yc.data(), Ycf.data()};
-    const size_t sbytes[8] = {sb.rec_off.size() * 4, sb.recipe.size() * 4, sb.scale1.size() * 8, s1f.size() * 4,
-                              sb.scale2.size() * 8, s2f.size() * 4, sb.Yc.size() * 8, Ycf.size() * 4};
-    void* dp[9] = {};
-    for (int k = 0; k < 8; ++k) {
+  const SymProgram sp = sb.ok ? sym_program(sb) : SymProgram{};
+  ctx->sym_valid = sb.ok && sp.ok;
+  if (ctx->sym_valid) {
+    const std::vector<float> Ycf(sb.Yc.begin(), sb.Yc.end()), pref(sp.pre.begin(), sp.pre.end()), Ypf(sp.Yp.begin(), sp.Yp.end());
+    const void* ssrc[7] = {sp.prog.data(), sp.pre.data(), pref.data(), sp.Yp.data(), Ypf.data(), sb.Yc.data(), Ycf.data()};
+    const size_t sbytes[7] = {sp.prog.size() * 4, sp.pre.size() * 8, pref.size() * 4, sp.Yp.size() * 8, Ypf.size() * 4,
+                              sb.Yc.size() * 8, Ycf.size() * 4};
+    void* dp[7] = {};
+    for (int k = 0; k < 7; ++k) {
       HIPCHK(hipMalloc(&dp[k], std::max<size_t>(sbytes[k], 16)));
       ctx->dsym.push_back(dp[k]);
       HIPCHK(hipMemcpy(dp[k], ssrc[k], sbytes[k], hipMemcpyHostToDevice));
     }
-    HIPCHK(hipMalloc(&dp[8], size_t(2) * sb.dimS * ctx->Vp * ctx->tsize));
-    ctx->dsym.push_back(dp[8]);
+    void* lift = nullptr;
+    HIPCHK(hipMalloc(&lift, size_t(2) * sb.dimS * ctx->Vp * ctx->tsize));
+    ctx->dsym.push_back(lift);
     void* hops = nullptr;
     HIPCHK(hipMalloc(&hops, size_t(2) * 2 * ctx->D * r.d * ctx->Vp * ctx->tsize));
     ctx->dsym.push_back(hops);
-    ctx->tb.sym.hops = hops;
     BigRepTables::Sym& st = ctx->tb.sym;
-    st.k1 = sb.k1; st.k2 = sb.k2; st.s1 = sb.s1; st.s2 = sb.s2; st.dimS = sb.dimS;
-    st.rec_off = (const int*)dp[0]; st.recipe = (const int4*)dp[1];
-    st.scale1 = (const double*)dp[2]; st.scale1f = (const float*)dp[3]; st.scale2 = (const double*)dp[4]; st.scale2f = (const float*)dp[5];
-    st.Yc = (const double*)dp[6]; st.Ycf = (const float*)dp[7];
-    st.invc = sb.invc;
-    st.lift = dp[8];
-    // LDS (cx units): Ca holds the levels of k1's parity, Cb the others; once
-    // the build is done, V, T1 and W reuse Cb when it is large enough
-    int ca = 0, cb = 0;
-    for (int k = 0; k <= sb.k1; ++k) {
-      const int sz = (sb.rec_off[k + 1] - sb.rec_off[k]);
-      if ((sb.k1 - k) % 2 == 0) ca = std::max(ca, sz);
-      else cb = std::max(cb, sz);
-    }
-    st.lds_Ca = 0;
-    st.lds_Cb = ca;
-    st.lds_S2 = ca + cb;
-    const int after_s2 = st.lds_S2 + sb.s2 * sb.s2;
-    st.lds_V = cb >= 3 * sb.dimS ? st.lds_Cb : after_s2;
-    st.lds_T1 = st.lds_V + sb.dimS;
-    st.lds_W = st.lds_T1 + sb.dimS;
-    st.lds_Ws = st.lds_W + sb.dimS;
-    st.lds_total = std::max(after_s2, st.lds_Ws);
+    st.s1 = sb.s1; st.s2 = sb.s2; st.dimS = sb.dimS; st.npass = sp.npass; st.ng = sp.ng;
+    st.prog = (const int4*)dp[0];
+    st.pre = (const double*)dp[1]; st.pref = (const float*)dp[2];
+    st.Yp = (const double*)dp[3]; st.Ypf = (const float*)dp[4];
+    st.Yc = (const double*)dp[5]; st.Ycf = (const float*)dp[6];
+    st.lift = lift;
+    st.hops = hops;
+    st.lds_one = sp.lds_one; st.lds_S1 = sp.lds_S1; st.lds_S2 = sp.lds_S2;
+    st.lds_V = sp.lds_V; st.lds_T1 = sp.lds_T1; st.lds_W = sp.lds_W; st.lds_total = sp.lds_total;
   }
-  ctx->tb.sym.on = (sb.ok && ctx->sym_enabled) ? 1 : 0;
+  ctx->tb.sym.on = (ctx->sym_valid && ctx->sym_enabled) ? 1 : 0;
   return GHX_OK;
 }
 

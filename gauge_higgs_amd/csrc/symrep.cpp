@@ -3,6 +3,7 @@
 
 #include <cmath>
 #include <complex>
+#include <algorithm>
 #include <map>
 
 namespace ghx {
@@ -70,6 +71,24 @@ std::vector<cplx> tensor_apply(const std::vector<cplx>& U, std::vector<cplx> v, 
     v.swap(w);
   }
   return v;
+}
+
+// A generic unitary N x N matrix: Gram-Schmidt of a fixed complex matrix.
+std::vector<cplx> generic_sun(int N, int trial) {
+  std::vector<cplx> U(size_t(N) * N);
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < N; ++j) U[size_t(i) * N + j] = cplx(std::sin(1.3 * i + 0.7 * j + trial), std::cos(0.9 * i - 1.1 * j + 2.0 * trial));
+  for (int i = 0; i < N; ++i) {
+    for (int p = 0; p < i; ++p) {
+      cplx dot(0, 0);
+      for (int j = 0; j < N; ++j) dot += std::conj(U[size_t(p) * N + j]) * U[size_t(i) * N + j];
+      for (int j = 0; j < N; ++j) U[size_t(i) * N + j] -= dot * U[size_t(p) * N + j];
+    }
+    double nr = 0.0;
+    for (int j = 0; j < N; ++j) nr += std::norm(U[size_t(i) * N + j]);
+    for (int j = 0; j < N; ++j) U[size_t(i) * N + j] /= std::sqrt(nr);
+  }
+  return U;
 }
 
 }  // namespace
@@ -186,20 +205,7 @@ SymBuild build_sym(const RepInfo& r) {
   // check against the tensor apply on generic links
   double worst = 0.0;
   for (int trial = 0; trial < 3; ++trial) {
-    // a generic SU(N) element: Gram-Schmidt of a fixed complex matrix
-    std::vector<cplx> U(size_t(N) * N);
-    for (int i = 0; i < N; ++i)
-      for (int j = 0; j < N; ++j) U[size_t(i) * N + j] = cplx(std::sin(1.3 * i + 0.7 * j + trial), std::cos(0.9 * i - 1.1 * j + 2.0 * trial));
-    for (int i = 0; i < N; ++i) {
-      for (int p = 0; p < i; ++p) {
-        cplx dot(0, 0);
-        for (int j = 0; j < N; ++j) dot += std::conj(U[size_t(p) * N + j]) * U[size_t(i) * N + j];
-        for (int j = 0; j < N; ++j) U[size_t(i) * N + j] -= dot * U[size_t(p) * N + j];
-      }
-      double nr = 0.0;
-      for (int j = 0; j < N; ++j) nr += std::norm(U[size_t(i) * N + j]);
-      for (int j = 0; j < N; ++j) U[size_t(i) * N + j] /= std::sqrt(nr);
-    }
+    const std::vector<cplx> U = generic_sun(N, trial);
     const std::vector<cplx> S1 = sym_matrix(s, U, s.k1);
     const std::vector<cplx> S2 = s.k2 > 0 ? sym_matrix(s, U, s.k2) : std::vector<cplx>(1, cplx(1, 0));
     for (int l = 0; l < d; ++l) {
@@ -229,6 +235,130 @@ SymBuild build_sym(const RepInfo& r) {
   s.max_err = worst;
   s.ok = worst < 1e-10;
   return s;
+}
+
+SymProgram sym_program(const SymBuild& s) {
+  SymProgram p;
+  const int N = s.N, k1 = s.k1, k2 = s.k2;
+  auto sk = [&](int k) { return s.lvl_off[k + 1] - s.lvl_off[k]; };
+  // level k lives in Ca when k1 - k is even, so that C_k1 ends in Ca
+  int ca = 0, cb = 0;
+  for (int k = 0; k <= k1; ++k) ((k1 - k) % 2 == 0 ? ca : cb) = std::max((k1 - k) % 2 == 0 ? ca : cb, sk(k) * sk(k));
+  const int Ca = kSymLdsBase, Cb = Ca + ca;
+  auto base = [&](int k) { return (k1 - k) % 2 == 0 ? Ca : Cb; };
+  const bool copy2 = k2 > 0 && k2 < k1;
+  p.lds_S1 = Ca;
+  p.lds_S2 = copy2 ? Cb + cb : (k2 == k1 ? Ca : 0);
+  const int after = Cb + cb + (copy2 ? s.s2 * s.s2 : 0);
+  p.ng = (s.dimS + 3) / 4;
+  p.lds_V = cb >= 2 * s.dimS + 4 * p.ng ? Cb : after;  // Cb is free once the build is done
+  p.lds_T1 = p.lds_V + s.dimS;
+  p.lds_W = p.lds_T1 + s.dimS;  // 4 ng entries, zero beyond dimS
+  p.lds_total = std::max(after, p.lds_W + 4 * p.ng);
+  p.lds_one = base(0);
+  auto flush = [&](std::vector<int>& pass) {  // pad a pass to 64 entries (idle lanes write slot 1)
+    while (pass.size() < size_t(4 * 64)) { pass.push_back(0); pass.push_back(0); pass.push_back(0); pass.push_back((1 << 2) | 3); }
+    p.prog.insert(p.prog.end(), pass.begin(), pass.end());
+    pass.clear();
+    ++p.npass;
+  };
+  std::vector<int> pass;
+  for (int k = 1; k <= k1; ++k) {
+    const int n_k = sk(k), sp = sk(k - 1), off = s.lvl_off[k];
+    for (int m = 0; m < n_k; ++m)
+      for (int n = 0; n < n_k; ++n) {
+        const int np = s.colsrc[off + n];
+        int src[3] = {0, 0, 0};
+        for (int i = 0; i < N; ++i) {
+          const int ri = s.rowsrc[size_t(off + m) * N + i];
+          if (ri >= 0) src[i] = base(k - 1) + ri * sp + np;
+        }
+        pass.insert(pass.end(), {src[0], src[1], src[2], ((base(k) + m * n_k + n) << 2) | s.colj[off + n]});
+        if (pass.size() == size_t(4 * 64)) flush(pass);
+      }
+    if (!pass.empty()) flush(pass);
+    if (k == k2 && copy2) {  // save C_k2 before level k2 + 2 overwrites it
+      for (int e = 0; e < s.s2 * s.s2; ++e) {
+        pass.insert(pass.end(), {base(k2) + e, 0, 0, ((p.lds_S2 + e) << 2) | 3});
+        if (pass.size() == size_t(4 * 64)) flush(pass);
+      }
+      if (!pass.empty()) flush(pass);
+    }
+  }
+  if (p.npass % 2) flush(pass);  // an even number of passes (an idle one at the end)
+  // F over q = (a, b) and the folded lift / projection tables
+  std::vector<double> F(s.dimS);
+  const int o1 = s.lvl_off[k1], o2 = s.lvl_off[k2];
+  for (int a = 0; a < s.s1; ++a)
+    for (int b = 0; b < s.s2; ++b) F[size_t(a * s.s2 + b)] = s.fnorm[o1 + a] * (k2 > 0 ? s.fnorm[o2 + b] : 1.0);
+  p.pre.resize(size_t(2) * s.dimS);
+  p.Yp.assign(size_t(2) * p.ng * s.d * 4, 0.0);
+  auto yp = [&](int dir, int q, int l) -> double& { return p.Yp[((size_t(dir) * p.ng + q / 4) * s.d + l) * 4 + q % 4]; };
+  for (int q = 0; q < s.dimS; ++q) {
+    p.pre[q] = 1.0 / F[q];
+    p.pre[s.dimS + q] = F[q];
+    for (int l = 0; l < s.d; ++l) {
+      const double y = s.Yc[size_t(q) * s.d + l] * s.invc;
+      yp(0, q, l) = y * F[q];
+      yp(1, q, l) = y / F[q];
+    }
+  }
+  // run the program on the host, as the device does, against the engine's
+  // matrices (already checked against the tensor apply): every column of
+  // D(U) and of D(U)^dag on two generic links
+  double worst = 0.0;
+  for (int trial = 0; trial < 2; ++trial) {
+    const std::vector<cplx> U = generic_sun(N, trial + 7);
+    std::vector<cplx> L(size_t(p.lds_total), cplx(0, 0));
+    L[size_t(p.lds_one)] = cplx(1, 0);
+    for (int i = 0; i < N; ++i) {
+      for (int j = 0; j < N; ++j) L[size_t(kSymLdsUs + 4 * i + j)] = U[size_t(i) * N + j];
+      L[size_t(kSymLdsUs + 4 * i + 3)] = cplx(i == 0 ? 1 : 0, 0);
+    }
+    for (int ps = 0; ps < p.npass; ++ps) {
+      std::vector<cplx> out(64);
+      for (int t = 0; t < 64; ++t) {
+        const int* e = &p.prog[size_t(ps * 64 + t) * 4];
+        const int j = e[3] & 3;
+        cplx acc(0, 0);
+        for (int i = 0; i < N; ++i) acc += L[size_t(kSymLdsUs + 4 * i + j)] * L[size_t(e[i])];
+        out[size_t(t)] = acc;
+      }
+      for (int t = 0; t < 64; ++t) L[size_t(p.prog[size_t(ps * 64 + t) * 4 + 3] >> 2)] = out[size_t(t)];
+      L[0] = cplx(0, 0);
+    }
+    const std::vector<cplx> S1 = sym_matrix(s, U, k1);
+    const std::vector<cplx> S2 = k2 > 0 ? sym_matrix(s, U, k2) : std::vector<cplx>(1, cplx(1, 0));
+    for (int dir = 0; dir < 2; ++dir)
+      for (int l = 0; l < s.d; ++l) {
+        // device path: V = pre (Yc e_l), W = (C1 x C2) V (or its adjoint), out = Yp^T W
+        std::vector<cplx> V(s.dimS), W(s.dimS, cplx(0, 0)), ref(s.dimS, cplx(0, 0));
+        for (int q = 0; q < s.dimS; ++q) V[size_t(q)] = p.pre[size_t(dir) * s.dimS + q] * s.Yc[size_t(q) * s.d + l];
+        for (int a = 0; a < s.s1; ++a)
+          for (int b = 0; b < s.s2; ++b)
+            for (int a2 = 0; a2 < s.s1; ++a2)
+              for (int b2 = 0; b2 < s.s2; ++b2) {
+                const cplx c1 = dir ? std::conj(L[size_t(p.lds_S1 + a2 * s.s1 + a)]) : L[size_t(p.lds_S1 + a * s.s1 + a2)];
+                const cplx c2 = k2 == 0 ? cplx(1, 0)
+                                        : (dir ? std::conj(L[size_t(p.lds_S2 + b2 * s.s2 + b)]) : L[size_t(p.lds_S2 + b * s.s2 + b2)]);
+                W[size_t(a * s.s2 + b)] += c1 * c2 * V[size_t(a2 * s.s2 + b2)];
+                const cplx r1 = dir ? std::conj(S1[size_t(a2) * s.s1 + a]) : S1[size_t(a) * s.s1 + a2];
+                const cplx r2 = dir ? std::conj(S2[size_t(b2) * s.s2 + b]) : S2[size_t(b) * s.s2 + b2];
+                ref[size_t(a * s.s2 + b)] += r1 * r2 * s.Yc[size_t(a2 * s.s2 + b2) * s.d + l];
+              }
+        for (int k = 0; k < s.d; ++k) {
+          cplx dev(0, 0), eng(0, 0);
+          for (int q = 0; q < s.dimS; ++q) {
+            dev += yp(dir, q, k) * W[size_t(q)];
+            eng += s.Yc[size_t(q) * s.d + k] * s.invc * ref[size_t(q)];
+          }
+          worst = std::max(worst, std::abs(dev - eng));
+        }
+      }
+  }
+  p.max_err = worst;
+  p.ok = worst < 1e-10;
+  return p;
 }
 
 }  // namespace ghx

@@ -46,4 +46,29 @@ struct SymBuild {
 // more than two rows, exceeds the size limits or fails the check (1e-10).
 SymBuild build_sym(const RepInfo& r);
 
+// The device form. The unnormalised C_k are built in LDS by a program of
+// passes, 64 entries per pass (one per lane), each entry
+//   L[dst] = Us[0][j] L[s0] + Us[1][j] L[s1] + Us[2][j] L[s2]
+// over LDS complex slots: slot 0 holds zero (a missing source), slot 1 takes
+// the writes of idle lanes, Us (N rows x 4 columns from slot kSymLdsUs) holds
+// the link with a fourth column e_0 (j = 3: a plain copy, used to save C_k2).
+// The normalisations diag(sqrt(m!)) are moved out of the build:
+//   S1 (x) S2 = F (C1 (x) C2) F^-1,  F = diag(sqrt(a!) sqrt(b!)) over q = (a, b),
+// so the forward hop lifts V = F^-1 Yc phi and projects with Yc F / c, the
+// adjoint lifts F Yc phi and projects with Yc F^-1 / c.
+constexpr int kSymLdsUs = 4;
+constexpr int kSymLdsBase = 16;
+struct SymProgram {
+  std::vector<int> prog;           // 4 ints per entry: s0, s1, s2, (dst << 2) | j
+  int npass = 0;                   // even (padded with an idle pass)
+  int lds_one = 0;                 // level 0's entry (1)
+  int lds_S1 = 0, lds_S2 = 0, lds_V = 0, lds_T1 = 0, lds_W = 0, lds_total = 0;
+  std::vector<double> pre;         // [2][dimS]: 1/F (forward), F (adjoint)
+  std::vector<double> Yp;          // [2][ng][d][4], q = 4 g + i: Yc F / c (forward), Yc F^-1 / c (adjoint); zero for q >= dimS
+  int ng = 0;                      // groups of 4 components (dimS rounded up)
+  bool ok = false;                 // the program, run on the host, reproduces D(U) and D(U)^dag (1e-10)
+  double max_err = 0.0;
+};
+SymProgram sym_program(const SymBuild& s);
+
 }  // namespace ghx

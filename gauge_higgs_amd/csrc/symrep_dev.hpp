@@ -1,20 +1,31 @@
 // Device side of the symmetric-power engine (symrep.hpp) for large tensor
-// irreps: D(U) phi = invc Yc^T (Sym^k1(U) (x) Sym^k2(U)) Yc phi.
+// irreps: D(U) phi = Yc^T (Sym^k1(U) (x) Sym^k2(U)) Yc phi / c.
 //
-// sym_lift_kernel stores Yc phi for every site (dim S complex planes). In
-// sym_link_kernel one wave takes one link: it builds Sym^k(U) (levels k =
-// 1..k1 in LDS, the row-2 power saved on the way), applies it and its adjoint
-// to the lifted scalars at both ends as S1 V S2^T (V the lifted vector viewed
-// as s1 x s2) and projects back with Yc^T. sym_finish_kernel (one wave per
-// site, a lane per component) turns a site's 2D hops into the matter link
-// force, the scalar force and the scalar's update.
+// sym_lift_kernel stores Yc phi for every site. In sym_link_kernel one wave
+// takes one link: it runs the host-built LDS program that forms the
+// unnormalised C_k1 (and C_k2) of the link, applies C1 (x) C2 and its adjoint
+// to the lifted scalars at both ends as C1 V C2^T (V viewed as s1 x s2, scaled
+// by F^-1 or F on load) and projects back with the tables Yc F^+-1 / c.
+// sym_finish_kernel (one wave per site, a lane per component) turns a site's
+// 2D hops into the matter link force, the scalar force and the scalar's update.
 #pragma once
 #include "bigrep.hpp"
+#include "symrep.hpp"
 
 namespace ghx {
 
 constexpr int kSymWave = 64;
 constexpr int kLiftChunk = 16;  // lifted components per block of sym_lift_kernel
+
+// The link kernel runs one wave per workgroup. LDS instructions of a wave
+// complete in issue order, so ordering one step's LDS writes before the next
+// step's reads needs only a wave-scope fence (no s_barrier, and no wait for the
+// global load prefetched for the next step).
+GHX_DEV void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // One complex value as a single 8- or 16-byte access (p at an even index).
 template <typename T>
@@ -29,50 +40,45 @@ GHX_DEV void st_cx(T* p, cx<T> v) {
   *reinterpret_cast<V*>(p) = V{v.r, v.i};
 }
 
-// Sym^k1(U) (normalised) into Ca; Sym^k2(U) into S2 when the irrep has a second
-// row. Ca holds the levels of the parity of k1 (s1^2 entries), Cb the others;
-// each level entry gathers its N sources of the previous level by recipe. The
-// link goes through LDS (Us, N x N) so that its runtime column index j reads
-// LDS rather than a private array.
+// C_k1 (and C_k2) of the link U by the LDS program: one entry per lane per
+// pass, L[dst] = sum_i Us[i][j] L[s_i]. Slot 0 is zero, slot 1 takes idle lanes' writes, Us holds U
+// with a fourth column e_0 (j = 3 copies).
 template <typename R, int N>
-GHX_DEV cx<R>* sym_build(const BigRepTables::Sym& st, const Mat<R, N>& U, cx<R>* Us, cx<R>* Ca, cx<R>* Cb, cx<R>* S2) {
+GHX_DEV void sym_build(const BigRepTables::Sym& st, const Mat<R, N>& U, cx<R>* L) {
   const int t = threadIdx.x;
-  if (t == 0)
+  if (t == 0) {
+    L[0] = {R(0), R(0)};
+    L[st.lds_one] = {R(1), R(0)};
 #pragma unroll
-    for (int i = 0; i < N; ++i)
+    for (int i = 0; i < N; ++i) {
 #pragma unroll
-      for (int j = 0; j < N; ++j) Us[i * N + j] = U.e[i][j];
-  // level k lives in Ca when k1 - k is even (so the result lands in Ca)
-  cx<R>* prev = (st.k1 % 2 == 0) ? Ca : Cb;
-  cx<R>* cur = (st.k1 % 2 == 0) ? Cb : Ca;
-  if (t == 0) prev[0] = {R(1), R(0)};
-  __syncthreads();
-  for (int k = 1; k <= st.k1; ++k) {
-    const int r0 = st.rec_off[k], ne = st.rec_off[k + 1] - r0;
-    for (int e = t; e < ne; e += kSymWave) {
-      const int4 rc = st.recipe[r0 + e];
-      cx<R> acc = {R(0), R(0)};
-      if (rc.x >= 0) cfma(acc, Us[rc.w], prev[rc.x]);
-      if (rc.y >= 0) cfma(acc, Us[N + rc.w], prev[rc.y]);
-      if constexpr (N > 2)
-        if (rc.z >= 0) cfma(acc, Us[2 * N + rc.w], prev[rc.z]);
-      cur[e] = acc;
+      for (int j = 0; j < N; ++j) L[kSymLdsUs + 4 * i + j] = U.e[i][j];
+      L[kSymLdsUs + 4 * i + 3] = {R(i == 0 ? 1 : 0), R(0)};
     }
-    __syncthreads();
-    if (k == st.k2)
-      for (int e = t; e < ne; e += kSymWave)
-        S2[e] = cscale(cur[e], sizeof(R) == 4 ? R(st.scale2f[e]) : R(st.scale2[e]));
-    cx<R>* tmp = prev;
-    prev = cur;
-    cur = tmp;
   }
-  const int ne = st.s1 * st.s1;
-  for (int e = t; e < ne; e += kSymWave) prev[e] = cscale(prev[e], sizeof(R) == 4 ? R(st.scale1f[e]) : R(st.scale1[e]));
-  __syncthreads();
-  return prev;
+  wave_sync();
+  const int4* pg = st.prog + t;
+  const int last = st.npass - 1;
+  auto run = [&](const int4 rc) {
+    const cx<R>* u = L + kSymLdsUs + (rc.w & 3);
+    cx<R> acc = cmul(u[0], L[rc.x]);
+    cfma(acc, u[4], L[rc.y]);
+    if constexpr (N > 2) cfma(acc, u[8], L[rc.z]);
+    L[rc.w >> 2] = acc;
+    wave_sync();
+  };
+  // two register sets, each refilled right after use: a pass's entries load
+  // while the other set's pass runs
+  int4 X = pg[0], Y = pg[min(1, last) * kSymWave];
+  for (int p = 0; p < st.npass; p += 2) {  // npass is even (the host pads it)
+    run(X);
+    X = pg[min(p + 2, last) * kSymWave];
+    run(Y);
+    Y = pg[min(p + 3, last) * kSymWave];
+  }
 }
 
-// W = (S1 (x) S2) V = S1 V S2^T, or its adjoint S1^dag V conj(S2) (V, W: s1 x s2).
+// W = (C1 (x) C2) V = C1 V C2^T, or its adjoint C1^dag V conj(C2) (V, W: s1 x s2).
 template <typename R>
 GHX_DEV void sym_apply(const BigRepTables::Sym& st, const cx<R>* S1, const cx<R>* S2, const cx<R>* V, cx<R>* T1, cx<R>* W,
                        bool dag) {
@@ -82,8 +88,10 @@ GHX_DEV void sym_apply(const BigRepTables::Sym& st, const cx<R>* S1, const cx<R>
   for (int e = t, a = t / s2, b = t % s2; e < s1 * s2; e += kSymWave) {
     cx<R> acc = {R(0), R(0)};
     if (!dag) {
+#pragma unroll 4
       for (int a2 = 0; a2 < s1; ++a2) cfma(acc, S1[a * s1 + a2], V[a2 * s2 + b]);
     } else {
+#pragma unroll 4
       for (int a2 = 0; a2 < s1; ++a2) cfmac(acc, S1[a2 * s1 + a], V[a2 * s2 + b]);
     }
     out1[e] = acc;
@@ -91,7 +99,7 @@ GHX_DEV void sym_apply(const BigRepTables::Sym& st, const cx<R>* S1, const cx<R>
     b += db;
     if (b >= s2) { b -= s2; ++a; }
   }
-  __syncthreads();
+  wave_sync();
   if (s2 == 1) return;
   for (int e = t, a = t / s2, b = t % s2; e < s1 * s2; e += kSymWave, a += da, b += db) {
     if (b >= s2) { b -= s2; ++a; }
@@ -103,24 +111,36 @@ GHX_DEV void sym_apply(const BigRepTables::Sym& st, const cx<R>* S1, const cx<R>
     }
     W[e] = acc;
   }
-  __syncthreads();
+  wave_sync();
 }
 
-// out[l] = invc sum_q Yc[q][l] W[q], l < d (threads over l).
+// out[l] = sum_q Yp[q][l] W[q], l < d (threads over l; Yp has 1/c folded in).
+// Yp is grouped [g][l][4] (q = 4 g + i), so a lane reads 4 consecutive q of
+// its row in one 16- or 32-byte access; W is zero from dimS to 4 ng.
 template <typename R>
-GHX_DEV void sym_project(const BigRepTables::Sym& st, int d, const cx<R>* W, cx<R>* out) {
-  const R invc = R(st.invc);
+GHX_DEV void sym_project(const R* __restrict__ Yp, int ng, int d, const cx<R>* W, cx<R>* out) {
   for (int l = threadIdx.x; l < d; l += kSymWave) {
+    const R* y = Yp + 4 * l;
     cx<R> acc = {R(0), R(0)};
-#pragma unroll 4
-    for (int q = 0; q < st.dimS; ++q) {
-      const R y = sizeof(R) == 4 ? R(st.Ycf[q * d + l]) : R(st.Yc[q * d + l]);
-      acc.r = fma(y, W[q].r, acc.r);
-      acc.i = fma(y, W[q].i, acc.i);
+#pragma unroll 2
+    for (int g = 0; g < ng; ++g, y += 4 * d) {
+      R v[4];
+      if constexpr (sizeof(R) == 4) {
+        const float4 f = *reinterpret_cast<const float4*>(y);
+        v[0] = f.x; v[1] = f.y; v[2] = f.z; v[3] = f.w;
+      } else {
+        const double2 f0 = *reinterpret_cast<const double2*>(y), f1 = *reinterpret_cast<const double2*>(y + 2);
+        v[0] = f0.x; v[1] = f0.y; v[2] = f1.x; v[3] = f1.y;
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        acc.r = fma(v[i], W[4 * g + i].r, acc.r);
+        acc.i = fma(v[i], W[4 * g + i].i, acc.i);
+      }
     }
-    out[l] = cscale(acc, invc);
+    out[l] = acc;
   }
-  __syncthreads();
+  wave_sync();
 }
 
 // Yc phi of every site, site-major: lift[2 (g dimS + q) + (re, im)], so that
@@ -171,14 +191,10 @@ __global__ __launch_bounds__(kSymWave) void sym_link_kernel(BigArgs<T> a, BigRep
   const BigRepTables::Sym& st = tb.sym;
   extern __shared__ unsigned char smem_raw[];
   cx<R>* L = reinterpret_cast<cx<R>*>(smem_raw);
-  cx<R>* Ca = L + st.lds_Ca;
-  cx<R>* Cb = L + st.lds_Cb;
-  cx<R>* S2 = L + st.lds_S2;
   cx<R>* V = L + st.lds_V;
   cx<R>* T1 = L + st.lds_T1;
   cx<R>* W = L + st.lds_W;
   __shared__ cx<R> res[kBigMaxD];
-  __shared__ cx<R> Us[N * N];
   const unsigned g = blockIdx.x, Vp = a.Vp;
   const int r = a.blk_rep[g / kBlock];
   const RepDesc& rd = a.reps[r];
@@ -190,20 +206,24 @@ __global__ __launch_bounds__(kSymWave) void sym_link_kernel(BigArgs<T> a, BigRep
 #pragma unroll
   for (int m = 1; m < D; ++m) up = (mu == m) ? nb.up[m] : up;
   const unsigned gp = g + up;
-  const Mat<R, N> Uk = load_link<T, R, N>(a.U, Vp, mu, g);
-  const cx<R>* S1 = sym_build<R, N>(st, Uk, Us, Ca, Cb, S2);
+  sym_build<R, N>(st, load_link<T, R, N>(a.U, Vp, mu, g), L);
+  const cx<R>* S1 = L + st.lds_S1;
+  const cx<R>* S2 = L + st.lds_S2;
+  for (int q = st.dimS + t; q < 4 * st.ng; q += kSymWave) W[q] = {R(0), R(0)};  // the projection's padding
   const T* lift = static_cast<const T*>(st.lift);
   for (int dir = 0; dir <= both; ++dir) {
     const unsigned gv = dir ? g : gp;   // the scalar's site
     const unsigned go = dir ? gp : g;   // the site that receives the hop
     const T* lv = lift + size_t(gv) * 2 * st.dimS;
-    for (int q = t; q < st.dimS; q += kSymWave) V[q] = ld_cx(lv + 2 * q);
-    __syncthreads();
+    const R* pre = (sizeof(R) == 4 ? reinterpret_cast<const R*>(st.pref) : reinterpret_cast<const R*>(st.pre)) + dir * st.dimS;
+    for (int q = t; q < st.dimS; q += kSymWave) V[q] = cscale(ld_cx(lv + 2 * q), pre[q]);
+    wave_sync();
     sym_apply<R>(st, S1, S2, V, T1, W, dir != 0);
-    sym_project<R>(st, d, W, res);
+    const R* Yp = (sizeof(R) == 4 ? reinterpret_cast<const R*>(st.Ypf) : reinterpret_cast<const R*>(st.Yp)) + size_t(dir) * st.ng * d * 4;
+    sym_project<R>(Yp, st.ng, d, W, res);
     T* ho = out + 2 * ((size_t(go) * 2 * D + 2 * mu + dir) * d);
     for (int l = t; l < d; l += kSymWave) st_cx(ho + 2 * l, res[l]);
-    __syncthreads();
+    wave_sync();
   }
 }
 
