@@ -46,6 +46,12 @@ struct BigRepTables {
     void* lift = nullptr;          // Yc phi per site, site-major: [Vp][dimS] complex, storage type
     void* hops = nullptr;          // the 2D hops of every site, site-major: [Vp][2D][d] complex, storage type
     int lds_one = 0, lds_S1 = 0, lds_S2 = 0, lds_V = 0, lds_T1 = 0, lds_W = 0, lds_total = 0;  // cx units
+    // g^a by lanes: generator a owns lanes [a lpg, (a+1) lpg); entry k of lane t
+    // is (l | m << 16, T^a_lm) at [k][t], gK entries per lane (zero padded)
+    const int* glm = nullptr;
+    const double* gval = nullptr;     // complex, FP64 and FP32 copies
+    const float* gvalf = nullptr;
+    int gK = 0, glpg = 0;
   } sym;
 };
 

@@ -183,6 +183,42 @@ int upload_tables(ghx_ctx* ctx) {
     st.hops = hops;
     st.lds_one = sp.lds_one; st.lds_S1 = sp.lds_S1; st.lds_S2 = sp.lds_S2;
     st.lds_V = sp.lds_V; st.lds_T1 = sp.lds_T1; st.lds_W = sp.lds_W; st.lds_total = sp.lds_total;
+    // the nonzeros of T^a_R dealt round-robin to the lanes of generator a
+    const int lpg = ngn <= 4 ? 16 : 8;
+    std::vector<std::vector<int>> lm(64);
+    std::vector<std::vector<std::complex<double>>> val(64);
+    for (int a = 0; a < ngn; ++a) {
+      int i = 0;
+      for (int l = 0; l < r.d; ++l)
+        for (int m = 0; m < r.d; ++m) {
+          const std::complex<double> z = r.T[(size_t(a) * r.d + l) * r.d + m];
+          if (z == 0.0) continue;
+          const int lane = a * lpg + (i++ % lpg);
+          lm[lane].push_back(l | (m << 16));
+          val[lane].push_back(z);
+        }
+    }
+    size_t K = 1;
+    for (int t = 0; t < 64; ++t) K = std::max(K, lm[t].size());
+    std::vector<int> glm(K * 64, 0);
+    std::vector<double> gval(K * 64 * 2, 0.0);
+    for (int t = 0; t < 64; ++t)
+      for (size_t k = 0; k < lm[t].size(); ++k) {
+        glm[k * 64 + t] = lm[t][k];
+        gval[(k * 64 + t) * 2] = val[t][k].real();
+        gval[(k * 64 + t) * 2 + 1] = val[t][k].imag();
+      }
+    const std::vector<float> gvalf(gval.begin(), gval.end());
+    const void* gsrc[3] = {glm.data(), gval.data(), gvalf.data()};
+    const size_t gbytes[3] = {glm.size() * 4, gval.size() * 8, gvalf.size() * 4};
+    void* gp[3] = {};
+    for (int k = 0; k < 3; ++k) {
+      HIPCHK(hipMalloc(&gp[k], gbytes[k]));
+      ctx->dsym.push_back(gp[k]);
+      HIPCHK(hipMemcpy(gp[k], gsrc[k], gbytes[k], hipMemcpyHostToDevice));
+    }
+    st.glm = (const int*)gp[0]; st.gval = (const double*)gp[1]; st.gvalf = (const float*)gp[2];
+    st.gK = int(K); st.glpg = lpg;
   }
   ctx->tb.sym.on = (ctx->sym_valid && ctx->sym_enabled) ? 1 : 0;
   return GHX_OK;

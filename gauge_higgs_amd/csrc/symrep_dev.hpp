@@ -245,7 +245,7 @@ template <typename T, int N, int D>
 __global__ __launch_bounds__(kSymWave) void sym_finish_kernel(BigArgs<T> a, BigRepTables tb, const T* __restrict__ hops) {
   using R = T;
   constexpr int NGN = N * N - 1;
-  __shared__ cx<R> psi[kBigMaxD];
+  __shared__ cx<R> psi[kSymWave], phs[kSymWave];
   const unsigned g = blockIdx.x, Vp = a.Vp;
   const int r = a.blk_rep[g / kBlock];
   const RepDesc& rd = a.reps[r];
@@ -268,34 +268,33 @@ __global__ __launch_bounds__(kSymWave) void sym_finish_kernel(BigArgs<T> a, BigR
     return;
   }
   const R k2 = R(2.0 * rd.kappa);
+  const BigRepTables::Sym& st = tb.sym;
+  const int lpg = st.glpg;
+  const int* glm = st.glm + l;
+  const R* gval = (sizeof(R) == 4 ? reinterpret_cast<const R*>(st.gvalf) : reinterpret_cast<const R*>(st.gval)) + 2 * l;
+  phs[l] = ph;
   cx<R> f = {R(0), R(0)};
   for (int mu = 0; mu < nmu; ++mu) {
     const cx<R> ps = hop(2 * mu), ch = hop(2 * mu + 1);
     psi[l] = ps;
-    __syncthreads();
-    // (T^a psi)_l from the CSR rows (a, l), then -kappa g^a summed over l
-    R ga[NGN];
-#pragma unroll
-    for (int c = 0; c < NGN; ++c) {
-      cx<R> tt = {R(0), R(0)};
-      if (on)
-        for (int e = tb.T_ptr[c * d + l]; e < tb.T_ptr[c * d + l + 1]; ++e)
-          cfma(tt, cx<R>{R(tb.T_val[2 * e]), R(tb.T_val[2 * e + 1])}, psi[tb.T_col[e]]);
-      cx<R> z = {R(0), R(0)};
-      cfmac(z, ph, tt);
-      ga[c] = wave_sum(z.i);
+    wave_sync();
+    // g^a = -2 Im sum_lm conj(phi_l) T^a_lm psi_m over the lanes of generator a
+    R gp = R(0);
+#pragma unroll 4
+    for (int k = 0; k < st.gK; ++k) {
+      const int e = glm[k * kSymWave];
+      const cx<R> v = ld_cx(gval + 2 * k * kSymWave);
+      const cx<R> x = phs[e & 0xffff], y = psi[e >> 16];
+      const R zr = x.r * y.r + x.i * y.i, zi = x.r * y.i - x.i * y.r;
+      gp = fma(v.r, zi, fma(v.i, zr, gp));
     }
-    if (l < NGN) {
-      R v = ga[0];
-#pragma unroll
-      for (int c = 1; c < NGN; ++c) v = (l == c) ? ga[c] : v;
-      a.G[size_t(mu * NGN + l) * Vp + g] = T(R(-rd.kappa) * (R(-2) * v));
-    }
+    for (int o = 1; o < lpg; o <<= 1) gp += __shfl_xor(gp, o);
+    if (l % lpg == 0 && l / lpg < NGN) a.G[size_t(mu * NGN + l / lpg) * Vp + g] = T(R(-rd.kappa) * (R(-2) * gp));
     f.r += k2 * ps.r;
     f.i += k2 * ps.i;
     f.r += k2 * ch.r;
     f.i += k2 * ch.i;
-    __syncthreads();
+    wave_sync();
   }
   // on-site term -2 dV/dphi^* (scalar_higgs.hpp:111-125) or the plugged-in potential's planes
   const R n2 = wave_sum(ph.r * ph.r + ph.i * ph.i);
