@@ -141,32 +141,6 @@ SymBuild build_sym(const RepInfo& r) {
         s.rowsrc[size_t(g) * N + i] = idx[k - 1].at(mi);
       }
     }
-  // device recipes and normalisations
-  s.rec_off.assign(s.k1 + 2, 0);
-  for (int k = 0; k <= s.k1; ++k) s.rec_off[k + 1] = s.rec_off[k] + int(lv[k].size() * lv[k].size());
-  s.recipe.assign(size_t(s.rec_off[s.k1 + 1]) * 4, -1);
-  for (int k = 1; k <= s.k1; ++k) {
-    const int sk = int(lv[k].size()), sp = int(lv[k - 1].size()), off = s.lvl_off[k];
-    for (int m = 0; m < sk; ++m)
-      for (int n = 0; n < sk; ++n) {
-        int* rc = &s.recipe[size_t(s.rec_off[k] + m * sk + n) * 4];
-        const int np = s.colsrc[off + n];
-        for (int i = 0; i < N; ++i) {
-          const int ri = s.rowsrc[size_t(off + m) * N + i];
-          rc[i] = ri >= 0 ? ri * sp + np : -1;
-        }
-        rc[3] = s.colj[off + n];
-      }
-  }
-  auto scales = [&](int k) {
-    std::vector<double> sc;
-    const int sk = int(lv[k].size()), off = s.lvl_off[k];
-    for (int m = 0; m < sk; ++m)
-      for (int n = 0; n < sk; ++n) sc.push_back(s.fnorm[off + m] / s.fnorm[off + n]);
-    return sc;
-  };
-  s.scale1 = scales(s.k1);
-  s.scale2 = s.k2 > 0 ? scales(s.k2) : std::vector<double>(1, 1.0);
   // Yc[(a,b)][l] = sum over tuples t (row-1 slots then row-2 slots) of the
   // normalised monomial vectors times W_l[t]
   const int n = s.k1 + s.k2;

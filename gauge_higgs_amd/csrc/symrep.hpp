@@ -32,12 +32,6 @@ struct SymBuild {
   std::vector<int> colj, colsrc;  // per (level k, column n): j and the index of n - e_j at level k-1
   std::vector<int> rowsrc;        // per (level k, row m) x N: index of m - e_i at level k-1, or -1
   std::vector<double> fnorm;      // per (level k, m): sqrt(m!)
-  // device form of the recursion: per level k = 1..k1 and entry e = m s_k + n,
-  // {src_0, .., src_(N-1), j} with src_i = the entry (m - e_i, n - e_j) of level
-  // k-1 (row-major) or -1; entries of level k start at rec_off[k]
-  std::vector<int> recipe;        // 4 ints per entry (N <= 3)
-  std::vector<int> rec_off;       // k = 0..k1+1
-  std::vector<double> scale1, scale2;  // sqrt(m!/n!) per entry of levels k1 and k2
   double max_err = 0.0;           // host check against the tensor apply on generic links
 };
 
