@@ -224,11 +224,14 @@ SymProgram sym_program(const SymBuild& s) {
   p.lds_S1 = Ca;
   p.lds_S2 = copy2 ? Cb + cb : (k2 == k1 ? Ca : 0);
   const int after = Cb + cb + (copy2 ? s.s2 * s.s2 : 0);
+  // V, T1 and W hold both hops of a link, P = 4 ng entries per direction; W
+  // takes V's place once C1 V is formed (or is T1 itself for one-row irreps)
   p.ng = (s.dimS + 3) / 4;
-  p.lds_V = cb >= 2 * s.dimS + 4 * p.ng ? Cb : after;  // Cb is free once the build is done
-  p.lds_T1 = p.lds_V + s.dimS;
-  p.lds_W = p.lds_T1 + s.dimS;  // 4 ng entries, zero beyond dimS
-  p.lds_total = std::max(after, p.lds_W + 4 * p.ng);
+  const int P = 4 * p.ng;
+  p.lds_V = cb >= 4 * P ? Cb : after;  // Cb is free once the build is done
+  p.lds_T1 = p.lds_V + 2 * P;
+  p.lds_W = s.s2 == 1 ? p.lds_T1 : p.lds_V;
+  p.lds_total = std::max(after, p.lds_T1 + 2 * P);
   p.lds_one = base(0);
   auto flush = [&](std::vector<int>& pass) {  // pad a pass to 64 entries (idle lanes write slot 1)
     while (pass.size() < size_t(4 * 64)) { pass.push_back(0); pass.push_back(0); pass.push_back(0); pass.push_back((1 << 2) | 3); }

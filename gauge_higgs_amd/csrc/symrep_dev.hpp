@@ -78,52 +78,55 @@ GHX_DEV void sym_build(const BigRepTables::Sym& st, const Mat<R, N>& U, cx<R>* L
   }
 }
 
-// W = (C1 (x) C2) V = C1 V C2^T, or its adjoint C1^dag V conj(C2) (V, W: s1 x s2).
+// Both hops of a link in one sweep. Direction 0 (the forward hop) needs
+// (C1 (x) C2) V0 = C1 V0 C2^T, direction 1 (the adjoint) C1^dag V1 conj(C2).
+// Direction 1 is carried conjugated (V1, T1 and W hold the conjugates of the
+// true values), so that both are plain products that differ only in how a
+// lane walks C1 and C2:
+//   conj(C1^dag V1) = C1^T conj(V1),   conj(T conj(C2)) = conj(T) C2,
+// and the projection conjugates its result back (Yp is real). Regions V, T1
+// and W hold the two directions P = 4 ng entries apart (V, W: s1 x s2).
 template <typename R>
-GHX_DEV void sym_apply(const BigRepTables::Sym& st, const cx<R>* S1, const cx<R>* S2, const cx<R>* V, cx<R>* T1, cx<R>* W,
-                       bool dag) {
-  const int t = threadIdx.x, s1 = st.s1, s2 = st.s2;
-  const int da = kSymWave / s2, db = kSymWave % s2;  // (a, b) of e advance by (da, db) per pass
+GHX_DEV void sym_apply2(const BigRepTables::Sym& st, const cx<R>* S1, const cx<R>* S2, const cx<R>* V, cx<R>* T1, cx<R>* W) {
+  const int t = threadIdx.x, s1 = st.s1, s2 = st.s2, dimS = st.dimS, P = 4 * st.ng;
   cx<R>* out1 = s2 == 1 ? W : T1;
-  for (int e = t, a = t / s2, b = t % s2; e < s1 * s2; e += kSymWave) {
+  for (int e = t; e < 2 * dimS; e += kSymWave) {
+    const int dir = e >= dimS, q = e - dir * dimS, a = q / s2, b = q - a * s2;
+    const cx<R>* c = S1 + (dir ? a : a * s1);  // row a of C1, or column a
+    const int cs = dir ? s1 : 1;
+    const cx<R>* v = V + dir * P + b;
     cx<R> acc = {R(0), R(0)};
-    if (!dag) {
 #pragma unroll 4
-      for (int a2 = 0; a2 < s1; ++a2) cfma(acc, S1[a * s1 + a2], V[a2 * s2 + b]);
-    } else {
-#pragma unroll 4
-      for (int a2 = 0; a2 < s1; ++a2) cfmac(acc, S1[a2 * s1 + a], V[a2 * s2 + b]);
-    }
-    out1[e] = acc;
-    a += da;
-    b += db;
-    if (b >= s2) { b -= s2; ++a; }
+    for (int a2 = 0; a2 < s1; ++a2, c += cs, v += s2) cfma(acc, *c, *v);
+    out1[dir * P + q] = acc;
   }
   wave_sync();
   if (s2 == 1) return;
-  for (int e = t, a = t / s2, b = t % s2; e < s1 * s2; e += kSymWave, a += da, b += db) {
-    if (b >= s2) { b -= s2; ++a; }
+  for (int e = t; e < 2 * dimS; e += kSymWave) {
+    const int dir = e >= dimS, q = e - dir * dimS, a = q / s2, b = q - a * s2;
+    const cx<R>* c = S2 + (dir ? b : b * s2);  // row b of C2, or column b
+    const int cs = dir ? s2 : 1;
+    const cx<R>* u = T1 + dir * P + a * s2;
     cx<R> acc = {R(0), R(0)};
-    if (!dag) {
-      for (int b2 = 0; b2 < s2; ++b2) cfma(acc, S2[b * s2 + b2], T1[a * s2 + b2]);
-    } else {
-      for (int b2 = 0; b2 < s2; ++b2) cfmac(acc, S2[b2 * s2 + b], T1[a * s2 + b2]);
-    }
-    W[e] = acc;
+    for (int b2 = 0; b2 < s2; ++b2, c += cs) cfma(acc, *c, u[b2]);
+    W[dir * P + q] = acc;
   }
   wave_sync();
 }
 
-// out[l] = sum_q Yp[q][l] W[q], l < d (threads over l; Yp has 1/c folded in).
-// Yp is grouped [g][l][4] (q = 4 g + i), so a lane reads 4 consecutive q of
-// its row in one 16- or 32-byte access; W is zero from dimS to 4 ng.
+// out[dir][l] = sum_q Yp[dir][q][l] W[dir][q] (lanes over (dir, l); Yp has 1/c
+// folded in; direction 1 conjugated back). Yp is grouped [g][l][4]
+// (q = 4 g + i), so a lane reads 4 consecutive q of its row in one 16- or
+// 32-byte access; W is zero from dimS to 4 ng in each direction.
 template <typename R>
-GHX_DEV void sym_project(const R* __restrict__ Yp, int ng, int d, const cx<R>* W, cx<R>* out) {
-  for (int l = threadIdx.x; l < d; l += kSymWave) {
-    const R* y = Yp + 4 * l;
+GHX_DEV void sym_project2(const R* __restrict__ Yp, int ng, int d, int nd, const cx<R>* W, cx<R>* out) {
+  for (int e = threadIdx.x; e < nd * d; e += kSymWave) {
+    const int dir = e >= d, l = e - dir * d;
+    const R* y = Yp + (size_t(dir) * ng * d + l) * 4;
+    const cx<R>* w = W + dir * 4 * ng;
     cx<R> acc = {R(0), R(0)};
 #pragma unroll 2
-    for (int g = 0; g < ng; ++g, y += 4 * d) {
+    for (int g = 0; g < ng; ++g, y += 4 * d, w += 4) {
       R v[4];
       if constexpr (sizeof(R) == 4) {
         const float4 f = *reinterpret_cast<const float4*>(y);
@@ -134,11 +137,11 @@ GHX_DEV void sym_project(const R* __restrict__ Yp, int ng, int d, const cx<R>* W
       }
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        acc.r = fma(v[i], W[4 * g + i].r, acc.r);
-        acc.i = fma(v[i], W[4 * g + i].i, acc.i);
+        acc.r = fma(v[i], w[i].r, acc.r);
+        acc.i = fma(v[i], w[i].i, acc.i);
       }
     }
-    out[l] = acc;
+    out[e] = dir ? cconj(acc) : acc;
   }
   wave_sync();
 }
@@ -194,7 +197,7 @@ __global__ __launch_bounds__(kSymWave) void sym_link_kernel(BigArgs<T> a, BigRep
   cx<R>* V = L + st.lds_V;
   cx<R>* T1 = L + st.lds_T1;
   cx<R>* W = L + st.lds_W;
-  __shared__ cx<R> res[kBigMaxD];
+  __shared__ cx<R> res[2 * kBigMaxD];
   const unsigned g = blockIdx.x, Vp = a.Vp;
   const int r = a.blk_rep[g / kBlock];
   const RepDesc& rd = a.reps[r];
@@ -209,21 +212,26 @@ __global__ __launch_bounds__(kSymWave) void sym_link_kernel(BigArgs<T> a, BigRep
   sym_build<R, N>(st, load_link<T, R, N>(a.U, Vp, mu, g), L);
   const cx<R>* S1 = L + st.lds_S1;
   const cx<R>* S2 = L + st.lds_S2;
-  for (int q = st.dimS + t; q < 4 * st.ng; q += kSymWave) W[q] = {R(0), R(0)};  // the projection's padding
+  const int P = 4 * st.ng, dimS = st.dimS, nd = both + 1;
+  for (int e = t; e < 2 * P; e += kSymWave)  // the projection's padding
+    if (e % P >= dimS) W[e] = {R(0), R(0)};
+  // V0 = F^-1 Yc phi(x+mu) for the forward hop of x, V1 = conj(F Yc phi(x)) for
+  // the adjoint hop of x+mu
   const T* lift = static_cast<const T*>(st.lift);
-  for (int dir = 0; dir <= both; ++dir) {
-    const unsigned gv = dir ? g : gp;   // the scalar's site
-    const unsigned go = dir ? gp : g;   // the site that receives the hop
-    const T* lv = lift + size_t(gv) * 2 * st.dimS;
-    const R* pre = (sizeof(R) == 4 ? reinterpret_cast<const R*>(st.pref) : reinterpret_cast<const R*>(st.pre)) + dir * st.dimS;
-    for (int q = t; q < st.dimS; q += kSymWave) V[q] = cscale(ld_cx(lv + 2 * q), pre[q]);
-    wave_sync();
-    sym_apply<R>(st, S1, S2, V, T1, W, dir != 0);
-    const R* Yp = (sizeof(R) == 4 ? reinterpret_cast<const R*>(st.Ypf) : reinterpret_cast<const R*>(st.Yp)) + size_t(dir) * st.ng * d * 4;
-    sym_project<R>(Yp, st.ng, d, W, res);
-    T* ho = out + 2 * ((size_t(go) * 2 * D + 2 * mu + dir) * d);
-    for (int l = t; l < d; l += kSymWave) st_cx(ho + 2 * l, res[l]);
-    wave_sync();
+  const R* pre = sizeof(R) == 4 ? reinterpret_cast<const R*>(st.pref) : reinterpret_cast<const R*>(st.pre);
+  for (int e = t; e < nd * dimS; e += kSymWave) {
+    const int dir = e >= dimS, q = e - dir * dimS;
+    const cx<R> v = cscale(ld_cx(lift + size_t(dir ? g : gp) * 2 * dimS + 2 * q), pre[e]);
+    V[dir * P + q] = dir ? cconj(v) : v;
+  }
+  wave_sync();
+  sym_apply2<R>(st, S1, S2, V, T1, W);
+  const R* Yp = sizeof(R) == 4 ? reinterpret_cast<const R*>(st.Ypf) : reinterpret_cast<const R*>(st.Yp);
+  sym_project2<R>(Yp, st.ng, d, nd, W, res);
+  for (int e = t; e < nd * d; e += kSymWave) {
+    const int dir = e >= d, l = e - dir * d;
+    T* ho = out + 2 * ((size_t(dir ? gp : g) * 2 * D + 2 * mu + dir) * d);
+    st_cx(ho + 2 * l, res[e]);
   }
 }
 
