@@ -214,6 +214,7 @@ SymBuild build_sym(const RepInfo& r) {
 SymProgram sym_program(const SymBuild& s) {
   SymProgram p;
   const int N = s.N, k1 = s.k1, k2 = s.k2;
+  if (N > 3) return p;  // an entry has three sources and the link's columns 0..2 (3 is the copy column)
   auto sk = [&](int k) { return s.lvl_off[k + 1] - s.lvl_off[k]; };
   // level k lives in Ca when k1 - k is even, so that C_k1 ends in Ca
   int ca = 0, cb = 0;

@@ -63,6 +63,6 @@ struct SymProgram {
   bool ok = false;                 // the program, run on the host, reproduces D(U) and D(U)^dag (1e-10)
   double max_err = 0.0;
 };
-SymProgram sym_program(const SymBuild& s);
+SymProgram sym_program(const SymBuild& s);  // ok = false for N > 3
 
 }  // namespace ghx
