@@ -269,7 +269,7 @@ __global__ __launch_bounds__(kBigBlock) void big_matter_kernel(BigArgs<T> a, Big
   const int nmu = (a.flags & F_NO_HOP) ? 0 : D;
   for (int mu = 0; mu < nmu; ++mu) {
     // forward: psi = D(U_mu(x)) phi_{x+mu}
-    const unsigned gpm = g + nb.up[mu];
+    const unsigned gpm = g + nb.upv(rd, mu);
     for (int l = threadIdx.x; l < d; l += kBigBlock)
       vin[l] = {R(a.phi_in[size_t(2 * l) * Vp + gpm]), R(a.phi_in[size_t(2 * l + 1) * Vp + gpm])};
     const Mat<R, N> Ux = load_link<T, R, N>(a.U, Vp, mu, g);
@@ -304,7 +304,7 @@ __global__ __launch_bounds__(kBigBlock) void big_matter_kernel(BigArgs<T> a, Big
     for (int l = threadIdx.x; l < d; l += kBigBlock) { fph[l].r += k2 * psi[l].r; fph[l].i += k2 * psi[l].i; }
     __syncthreads();
     // backward: chi = D(U_mu(x-mu))^dag phi_{x-mu}
-    const unsigned gmm = g + nb.dn[mu];
+    const unsigned gmm = g + nb.dnv(rd, mu);
     for (int l = threadIdx.x; l < d; l += kBigBlock)
       vin[l] = {R(a.phi_in[size_t(2 * l) * Vp + gmm]), R(a.phi_in[size_t(2 * l + 1) * Vp + gmm])};
     const Mat<R, N> Ub = load_link<T, R, N>(a.U, Vp, mu, gmm);
@@ -412,10 +412,10 @@ __global__ __launch_bounds__(kRedBlock) void big_action_kernel(ActArgs a, const 
     const Nbr<D> nb(rd, s);
     for (int mu = 0; mu < D; ++mu) {
       const Mat<R, N> Ux = load_link<T, R, N>(U, Vp, mu, g);
-      const unsigned gpm = g + nb.up[mu];
+      const unsigned gpm = g + nb.upv(rd, mu);
       for (int nu = mu + 1; nu < D; ++nu) {
         const Mat<R, N> A = mul_nn(Ux, load_link<T, R, N>(U, Vp, nu, gpm));
-        const Mat<R, N> B = mul_nn(load_link<T, R, N>(U, Vp, nu, g), load_link<T, R, N>(U, Vp, mu, g + nb.up[nu]));
+        const Mat<R, N> B = mul_nn(load_link<T, R, N>(U, Vp, nu, g), load_link<T, R, N>(U, Vp, mu, g + nb.upv(rd, nu)));
         q[0] += retr_nd(A, B);
       }
       for (int c = 0; c < NGN; ++c) { const double v = double(P[size_t(mu * NGN + c) * Vp + g]); q[4] += v * v; }

@@ -238,31 +238,22 @@ GHX_DEV unsigned walk_block(unsigned p, const BlockWalk& w) {
 // direction 0 fastest). up[m]/dn[m] are signed offsets to add to the site.
 template <int D>
 struct Nbr {
-  int up[D], dn[D];
-  int x[D];
+  // Neighbour offsets of one site. They are kept as one mask of boundary flags
+  // and computed per direction from the lattice's strides (scalar loads): an
+  // array indexed with a runtime direction would live in scratch memory.
+  unsigned wrap = 0;  // bit m: x_m = L_m - 1; bit D + m: x_m = 0
   GHX_DEV Nbr(const RepDesc& r, unsigned s) {
 #pragma unroll
     for (int m = 0; m < D; ++m) {
       const unsigned xm = (s / r.stride[m]) % unsigned(r.L[m]);
-      x[m] = int(xm);
-      const int st = int(r.stride[m]);
-      up[m] = (int(xm) == r.L[m] - 1) ? -(r.L[m] - 1) * st : st;
-      dn[m] = (xm == 0) ? (r.L[m] - 1) * st : -st;
+      wrap |= (int(xm) == r.L[m] - 1 ? 1u << m : 0u) | (xm == 0 ? 1u << (D + m) : 0u);
     }
   }
-  // up[m] / dn[m] for a wave-uniform runtime m, from one mask of boundary
-  // flags and the lattice's strides (scalar loads): indexing the arrays with a
-  // runtime m would place them in scratch memory
-  unsigned wrap = 0;  // bit m: x_m = L_m - 1; bit D + m: x_m = 0
-  GHX_DEV void set_wrap(const RepDesc& r) {
-#pragma unroll
-    for (int m = 0; m < D; ++m) wrap |= (x[m] == r.L[m] - 1 ? 1u << m : 0u) | (x[m] == 0 ? 1u << (D + m) : 0u);
-  }
-  GHX_DEV int upv(const RepDesc& r, int m) const {
+  GHX_DEV int upv(const RepDesc& r, int m) const {  // offset of x + m
     const int st = int(r.stride[m]);
     return (wrap >> m & 1u) ? -(r.L[m] - 1) * st : st;
   }
-  GHX_DEV int dnv(const RepDesc& r, int m) const {
+  GHX_DEV int dnv(const RepDesc& r, int m) const {  // offset of x - m
     const int st = int(r.stride[m]);
     return (wrap >> (D + m) & 1u) ? (r.L[m] - 1) * st : -st;
   }
@@ -525,8 +516,7 @@ template <typename T, int N, int D, class REP>
 GHX_DEV void site_update(const StepArgs<T>& a, const typename REP::template Params<T>& rp, const RepDesc& rd, unsigned s) {
   constexpr int DR = REP::DR;
   const unsigned g = rd.offset + s;
-  Nbr<D> nb(rd, s);
-  nb.set_wrap(rd);
+  const Nbr<D> nb(rd, s);
   cx<T> phx[DR > 0 ? DR : 1], fph[DR > 0 ? DR : 1];
   if constexpr (DR > 0) site_phi_begin<T, DR>(a, rd, g, phx, fph);
   T* pst = nullptr;
@@ -597,8 +587,7 @@ __global__ __launch_bounds__(kLinkSites * D) void md_link_kernel(StepArgs<T> a, 
 #pragma unroll
   for (int k = 0; k < (DR > 0 ? DR : 1); ++k) fph[k] = {T(0), T(0)};
   if (live) {
-    Nbr<D> nb(rd, s);
-    nb.set_wrap(rd);
+    const Nbr<D> nb(rd, s);
     if constexpr (DR > 0) bload_vec<T, T, DR>(Planes<T>(a.phi_in, a.Vp, 2 * DR), gsite, phx);
     link_update<T, N, D, REP>(a, rp, rd, gsite, nb, mu, phx, fph);
   }
@@ -848,11 +837,11 @@ __global__ __launch_bounds__(kRedBlock) void action_kernel(ActArgs a, const T* _
 #pragma unroll 1
     for (int mu = 0; mu < D; ++mu) {
       const Mat<R, N> Ux = load_link<T, R, N>(U, Vp, mu, g);
-      const unsigned gpm = g + nb.up[mu];
+      const unsigned gpm = g + nb.upv(rd, mu);
 #pragma unroll 1
       for (int nu = mu + 1; nu < D; ++nu) {
         const Mat<R, N> A = mul_nn(Ux, load_link<T, R, N>(U, Vp, nu, gpm));
-        const Mat<R, N> B = mul_nn(load_link<T, R, N>(U, Vp, nu, g), load_link<T, R, N>(U, Vp, mu, g + nb.up[nu]));
+        const Mat<R, N> B = mul_nn(load_link<T, R, N>(U, Vp, nu, g), load_link<T, R, N>(U, Vp, mu, g + nb.upv(rd, nu)));
         q[0] += retr_nd(A, B);  // Re Tr(U_mu U_nu(x+mu) (U_nu U_mu(x+nu))^dag)
       }
       const T* Pm = P + size_t(mu) * NGN * Vp + g;

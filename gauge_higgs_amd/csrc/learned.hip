@@ -102,15 +102,15 @@ __global__ __launch_bounds__(kBlock) void surr_feature_kernel(const RepDesc* rep
     float* x = X + row * kNF;
     float* b = B + row * kNB * NGN;
     const Mat<R, N> Ux = load_link<T, R, N>(U, Vp, mu, g);
-    const unsigned gpm = g + nb.up[mu];
+    const unsigned gpm = g + nb.upv(rd, mu);
     Mat<R, N> Sig = mzero<R, N>();
     int fi = 0;
     for (int nu = 0; nu < D; ++nu) {
       if (nu == mu) continue;
-      Mat<R, N> Sf = mul_nd(mul_nd(load_link<T, R, N>(U, Vp, nu, gpm), load_link<T, R, N>(U, Vp, mu, g + nb.up[nu])),
+      Mat<R, N> Sf = mul_nd(mul_nd(load_link<T, R, N>(U, Vp, nu, gpm), load_link<T, R, N>(U, Vp, mu, g + nb.upv(rd, nu))),
                             load_link<T, R, N>(U, Vp, nu, g));
-      const unsigned gmn = g + nb.dn[nu];
-      Mat<R, N> Sb = mul_nn(mul_dd(load_link<T, R, N>(U, Vp, nu, gpm + nb.dn[nu]), load_link<T, R, N>(U, Vp, mu, gmn)),
+      const unsigned gmn = g + nb.dnv(rd, nu);
+      Mat<R, N> Sb = mul_nn(mul_dd(load_link<T, R, N>(U, Vp, nu, gpm + nb.dnv(rd, nu)), load_link<T, R, N>(U, Vp, mu, gmn)),
                             load_link<T, R, N>(U, Vp, nu, gmn));
       x[fi++] = retr_nn(Ux, Sf) / R(N);
       x[fi++] = retr_nn(Ux, Sb) / R(N);

@@ -205,10 +205,7 @@ __global__ __launch_bounds__(kSymWave) void sym_link_kernel(BigArgs<T> a, BigRep
   if (s >= rd.vol) return;  // whole block exits together
   const int mu = int(blockIdx.y), t = threadIdx.x, d = tb.d;
   const Nbr<D> nb(rd, s);
-  int up = nb.up[0];  // nb.up[mu] without a private array
-#pragma unroll
-  for (int m = 1; m < D; ++m) up = (mu == m) ? nb.up[m] : up;
-  const unsigned gp = g + up;
+  const unsigned gp = g + nb.upv(rd, mu);
   sym_build<R, N>(st, load_link<T, R, N>(a.U, Vp, mu, g), L);
   const cx<R>* S1 = L + st.lds_S1;
   const cx<R>* S2 = L + st.lds_S2;
