@@ -22,8 +22,12 @@ constexpr int kNQ = 6;          // per-site reduction quantities
 #define GHX_NU_UNROLL 0  // unroll the staple loop (more loads in flight, more registers)
 #endif
 #ifndef GHX_MD_WAVES
-#define GHX_MD_WAVES 3  // minimum waves per SIMD requested for the MD-step kernel
+#define GHX_MD_WAVES 3  // minimum waves per SIMD requested for the MD-step kernel (FP32)
 #endif
+// FP64 matrices take twice the registers: two waves per SIMD (256 VGPRs) keep
+// the SU(3) step free of spills
+template <typename T>
+constexpr int md_waves() { return sizeof(T) == 8 ? 2 : GHX_MD_WAVES; }
 
 struct RepDesc {
   int L[4];
@@ -533,7 +537,7 @@ GHX_DEV void site_update(const StepArgs<T>& a, const typename REP::template Para
 
 // One site per thread; blocks renumbered XCD-aware.
 template <typename T, int N, int D, class REP>
-__global__ __launch_bounds__(kBlock, GHX_MD_WAVES) void md_step_kernel(StepArgs<T> a, typename REP::template Params<T> rp) {
+__global__ __launch_bounds__(kBlock, md_waves<T>()) void md_step_kernel(StepArgs<T> a, typename REP::template Params<T> rp) {
   const unsigned bid = walk_block(xcd_block(blockIdx.x, gridDim.x), a.walk);
   const int r = a.blk_rep[bid];
   const RepDesc& rd = a.reps[r];
@@ -549,7 +553,7 @@ __global__ __launch_bounds__(kBlock, GHX_MD_WAVES) void md_step_kernel(StepArgs<
 // being re-fetched from HBM. Block b handles cross-section tile (b % ntile),
 // segment (b / ntile); ntile % 8 == 0 keeps all segments of a tile on one XCD.
 template <typename T, int N, int D, class REP>
-__global__ __launch_bounds__(kBlock, GHX_MD_WAVES) void md_stream_kernel(StepArgs<T> a, typename REP::template Params<T> rp, unsigned ntile,
+__global__ __launch_bounds__(kBlock, md_waves<T>()) void md_stream_kernel(StepArgs<T> a, typename REP::template Params<T> rp, unsigned ntile,
                                                                        int seg) {
   const RepDesc& rd = a.reps[0];
   const unsigned V3 = rd.stride[D - 1];
