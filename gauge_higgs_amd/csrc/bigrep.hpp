@@ -44,7 +44,8 @@ struct BigRepTables {
     const double* Yc = nullptr;       // dimS x d (the lift)
     const float* Ycf = nullptr;
     void* lift = nullptr;          // Yc phi per site, site-major: [Vp][dimS] complex, storage type
-    void* hops = nullptr;          // the 2D hops of every site, site-major: [Vp][2D][d] complex, storage type
+    void* hops = nullptr;          // forward hops D(U_mu(x)) phi(x+mu), site-major: [Vp][D][d] complex, storage type
+    void* hopsL = nullptr;         // backward hops of x+mu before projection, conjugated: [Vp][D][4 ng] complex
     int lds_one = 0, lds_S1 = 0, lds_S2 = 0, lds_V = 0, lds_T1 = 0, lds_W = 0, lds_total = 0;  // cx units
     // g^a by lanes: generator a owns lanes [a lpg, (a+1) lpg); entry k of lane t
     // is (l | m << 16, T^a_lm) at [k][t], gK entries per lane (zero padded)

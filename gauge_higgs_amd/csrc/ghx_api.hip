@@ -171,8 +171,11 @@ int upload_tables(ghx_ctx* ctx) {
     HIPCHK(hipMalloc(&lift, size_t(2) * sb.dimS * ctx->Vp * ctx->tsize));
     ctx->dsym.push_back(lift);
     void* hops = nullptr;
-    HIPCHK(hipMalloc(&hops, size_t(2) * 2 * ctx->D * r.d * ctx->Vp * ctx->tsize));
+    HIPCHK(hipMalloc(&hops, size_t(2) * ctx->D * r.d * ctx->Vp * ctx->tsize));
     ctx->dsym.push_back(hops);
+    void* hopsL = nullptr;
+    HIPCHK(hipMalloc(&hopsL, size_t(2) * ctx->D * 4 * sp.ng * ctx->Vp * ctx->tsize));
+    ctx->dsym.push_back(hopsL);
     BigRepTables::Sym& st = ctx->tb.sym;
     st.s1 = sb.s1; st.s2 = sb.s2; st.dimS = sb.dimS; st.npass = sp.npass; st.ng = sp.ng;
     st.prog = (const int4*)dp[0];
@@ -181,6 +184,7 @@ int upload_tables(ghx_ctx* ctx) {
     st.Yc = (const double*)dp[5]; st.Ycf = (const float*)dp[6];
     st.lift = lift;
     st.hops = hops;
+    st.hopsL = hopsL;
     st.lds_one = sp.lds_one; st.lds_S1 = sp.lds_S1; st.lds_S2 = sp.lds_S2;
     st.lds_V = sp.lds_V; st.lds_T1 = sp.lds_T1; st.lds_W = sp.lds_W; st.lds_total = sp.lds_total;
     // the nonzeros of T^a_R dealt round-robin to the lanes of generator a

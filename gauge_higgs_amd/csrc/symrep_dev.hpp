@@ -223,12 +223,16 @@ __global__ __launch_bounds__(kSymWave) void sym_link_kernel(BigArgs<T> a, BigRep
   }
   wave_sync();
   sym_apply2<R>(st, S1, S2, V, T1, W);
+  // the forward hop is projected here (its g^a needs it alone); the backward
+  // hop only enters the scalar force of x+mu, summed over mu, so it is stored
+  // lifted and projected once per site by sym_finish_kernel
   const R* Yp = sizeof(R) == 4 ? reinterpret_cast<const R*>(st.Ypf) : reinterpret_cast<const R*>(st.Yp);
-  sym_project2<R>(Yp, st.ng, d, nd, W, res);
-  for (int e = t; e < nd * d; e += kSymWave) {
-    const int dir = e >= d, l = e - dir * d;
-    T* ho = out + 2 * ((size_t(dir ? gp : g) * 2 * D + 2 * mu + dir) * d);
-    st_cx(ho + 2 * l, res[e]);
+  sym_project2<R>(Yp, st.ng, d, 1, W, res);
+  T* ho = out + 2 * ((size_t(g) * D + mu) * d);
+  for (int l = t; l < d; l += kSymWave) st_cx(ho + 2 * l, res[l]);
+  if (both) {
+    T* hl = static_cast<T*>(st.hopsL) + 2 * ((size_t(gp) * D + mu) * P);
+    for (int q = t; q < P; q += kSymWave) st_cx(hl + 2 * q, W[P + q]);
   }
 }
 
@@ -250,7 +254,7 @@ template <typename T, int N, int D>
 __global__ __launch_bounds__(kSymWave) void sym_finish_kernel(BigArgs<T> a, BigRepTables tb, const T* __restrict__ hops) {
   using R = T;
   constexpr int NGN = N * N - 1;
-  __shared__ cx<R> psi[kSymWave], phs[kSymWave];
+  __shared__ cx<R> psi[kSymWave], phs[kSymWave], lsum[4 * kSymWave];
   const unsigned g = blockIdx.x, Vp = a.Vp;
   const int r = a.blk_rep[g / kBlock];
   const RepDesc& rd = a.reps[r];
@@ -258,14 +262,14 @@ __global__ __launch_bounds__(kSymWave) void sym_finish_kernel(BigArgs<T> a, BigR
   if (s >= rd.vol) return;  // whole block exits together
   const int d = tb.d, l = threadIdx.x;
   const bool on = l < d;
-  const T* hg = hops + 2 * (size_t(g) * 2 * D * d);
-  auto hop = [&](int slot) { return on ? ld_cx(hg + 2 * (slot * d + l)) : cx<R>{R(0), R(0)}; };
+  const T* hg = hops + 2 * (size_t(g) * D * d);
+  auto hop = [&](int mu) { return on ? ld_cx(hg + 2 * (mu * d + l)) : cx<R>{R(0), R(0)}; };
   const cx<R> ph = on ? cx<R>{R(a.phi_in[size_t(2 * l) * Vp + g]), R(a.phi_in[size_t(2 * l + 1) * Vp + g])} : cx<R>{R(0), R(0)};
   const int nmu = (a.flags & F_NO_HOP) ? 0 : D;
   if (a.mode == BIG_ACTION) {
     double q = 0.0;
     for (int mu = 0; mu < nmu; ++mu) {
-      const cx<R> ps = hop(2 * mu);
+      const cx<R> ps = hop(mu);
       q += double(ph.r) * double(ps.r) + double(ph.i) * double(ps.i);
     }
     q = wave_sum(q);
@@ -279,8 +283,33 @@ __global__ __launch_bounds__(kSymWave) void sym_finish_kernel(BigArgs<T> a, BigR
   const R* gval = (sizeof(R) == 4 ? reinterpret_cast<const R*>(st.gvalf) : reinterpret_cast<const R*>(st.gval)) + 2 * l;
   phs[l] = ph;
   cx<R> f = {R(0), R(0)};
+  if (nmu) {  // the backward hops: sum the D lifted vectors, project once
+    const int P = 4 * st.ng;
+    const T* hl = static_cast<const T*>(st.hopsL) + 2 * (size_t(g) * D * P);
+    for (int q = l; q < P; q += kSymWave) {
+      cx<R> v = ld_cx(hl + 2 * q);
+#pragma unroll
+      for (int mu = 1; mu < D; ++mu) v = cadd(v, ld_cx(hl + 2 * (mu * P + q)));
+      lsum[q] = v;
+    }
+    wave_sync();
+    if (on) {  // chi = conj(sum_q Yp[1][q][l] lsum[q]) (the lifted hops are carried conjugated)
+      const R* y = (sizeof(R) == 4 ? reinterpret_cast<const R*>(st.Ypf) : reinterpret_cast<const R*>(st.Yp)) +
+                   (size_t(st.ng) * d + l) * 4;
+      cx<R> acc = {R(0), R(0)};
+#pragma unroll 2
+      for (int gq = 0; gq < st.ng; ++gq, y += 4 * d) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          acc.r = fma(y[i], lsum[4 * gq + i].r, acc.r);
+          acc.i = fma(y[i], lsum[4 * gq + i].i, acc.i);
+        }
+      }
+      f = {k2 * acc.r, -k2 * acc.i};
+    }
+  }
   for (int mu = 0; mu < nmu; ++mu) {
-    const cx<R> ps = hop(2 * mu), ch = hop(2 * mu + 1);
+    const cx<R> ps = hop(mu);
     psi[l] = ps;
     wave_sync();
     // g^a = -2 Im sum_lm conj(phi_l) T^a_lm psi_m over the lanes of generator a
@@ -297,8 +326,6 @@ __global__ __launch_bounds__(kSymWave) void sym_finish_kernel(BigArgs<T> a, BigR
     if (l % lpg == 0 && l / lpg < NGN) a.G[size_t(mu * NGN + l / lpg) * Vp + g] = T(R(-rd.kappa) * (R(-2) * gp));
     f.r += k2 * ps.r;
     f.i += k2 * ps.i;
-    f.r += k2 * ch.r;
-    f.i += k2 * ch.i;
     wave_sync();
   }
   // on-site term -2 dV/dphi^* (scalar_higgs.hpp:111-125) or the plugged-in potential's planes
