@@ -128,3 +128,27 @@ def test_sym_engine_default_and_speed():
         ms.append(g.lat.time_md_steps(6) / 6)
     print(f"{{6,1}} 8^4 FP32 MD step: engine {ms[0]:.3f} ms, tensor {ms[1]:.3f} ms")
     assert ms[0] < ms[1]
+
+
+def test_engine_uneven_replicas_match_oracle():
+    """Two replicas of SU(3) {4,2} (d = 27, symmetric-power engine) with
+    different extents and couplings in one launch per MD step, against
+    independent oracle chains (the reference's tensor apply)."""
+    ext = [[3, 3, 3, 3], [4, 3, 3, 3]]
+    kap, bet = [0.2, 0.3], [5.7, 5.5]
+    rs = gha.ReplicaSet(4, 3, ext, "4,2", bet, kap, [0.5] * 2, seed0=5, nmd=4, precision=gha.FP64)
+    assert rs.lat.sym_engine() == 1
+    rs.enabled = False
+    ors = []
+    for r in range(2):
+        o = O.Oracle(4, 3, ext[r], "4,2", seed=5 + r * 1000003)
+        o.couplings(beta=bet[r], kappa=kap[r], lam=0.5, nmd=4)
+        o.hot(0.8)
+        o.phi_gaussian(12345, 0.3)
+        ors.append(o)
+    acc, dH = rs.lat.trajectory_all()
+    for r, o in enumerate(ors):
+        a, d, _, _ = o.trajectory()
+        assert abs(dH[r] - d) < 1e-8 and acc[r] == a
+        assert np.max(np.abs(rs.lat.get("U", r) - o.get("U"))) < 1e-10
+        assert np.max(np.abs(rs.lat.get("phi", r) - o.get("phi"))) < 1e-10
