@@ -41,8 +41,9 @@ struct BigRepTables {
     const double* Yp = nullptr;       // [2][ng][d][4] (q = 4 g + i): Yc F / c and Yc F^-1 / c
     int ng = 0;
     const float* Ypf = nullptr;
-    const double* Yc = nullptr;       // dimS x d (the lift)
-    const float* Ycf = nullptr;
+    const double* Yl = nullptr;       // the lift table Yc transposed: [d][dimSp], dimSp = dimS rounded up to 16, zero padded
+    const float* Ylf = nullptr;
+    int dimSp = 0;
     void* lift = nullptr;          // Yc phi per site, site-major: [Vp][dimS] complex, storage type
     void* hops = nullptr;          // forward hops D(U_mu(x)) phi(x+mu), site-major: [Vp][D][d] complex, storage type
     void* hopsL = nullptr;         // backward hops of x+mu before projection, conjugated: [Vp][D][4 ng] complex

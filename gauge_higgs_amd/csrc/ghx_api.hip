@@ -157,10 +157,14 @@ int upload_tables(ghx_ctx* ctx) {
   const SymProgram sp = sb.ok ? sym_program(sb) : SymProgram{};
   ctx->sym_valid = sb.ok && sp.ok;
   if (ctx->sym_valid) {
-    const std::vector<float> Ycf(sb.Yc.begin(), sb.Yc.end()), pref(sp.pre.begin(), sp.pre.end()), Ypf(sp.Yp.begin(), sp.Yp.end());
-    const void* ssrc[7] = {sp.prog.data(), sp.pre.data(), pref.data(), sp.Yp.data(), Ypf.data(), sb.Yc.data(), Ycf.data()};
+    const int dimSp = (sb.dimS + 15) / 16 * 16;
+    std::vector<double> Yl(size_t(r.d) * dimSp, 0.0);
+    for (int q = 0; q < sb.dimS; ++q)
+      for (int l = 0; l < r.d; ++l) Yl[size_t(l) * dimSp + q] = sb.Yc[size_t(q) * r.d + l];
+    const std::vector<float> Ylf(Yl.begin(), Yl.end()), pref(sp.pre.begin(), sp.pre.end()), Ypf(sp.Yp.begin(), sp.Yp.end());
+    const void* ssrc[7] = {sp.prog.data(), sp.pre.data(), pref.data(), sp.Yp.data(), Ypf.data(), Yl.data(), Ylf.data()};
     const size_t sbytes[7] = {sp.prog.size() * 4, sp.pre.size() * 8, pref.size() * 4, sp.Yp.size() * 8, Ypf.size() * 4,
-                              sb.Yc.size() * 8, Ycf.size() * 4};
+                              Yl.size() * 8, Ylf.size() * 4};
     void* dp[7] = {};
     for (int k = 0; k < 7; ++k) {
       HIPCHK(hipMalloc(&dp[k], std::max<size_t>(sbytes[k], 16)));
@@ -181,7 +185,7 @@ int upload_tables(ghx_ctx* ctx) {
     st.prog = (const int4*)dp[0];
     st.pre = (const double*)dp[1]; st.pref = (const float*)dp[2];
     st.Yp = (const double*)dp[3]; st.Ypf = (const float*)dp[4];
-    st.Yc = (const double*)dp[5]; st.Ycf = (const float*)dp[6];
+    st.Yl = (const double*)dp[5]; st.Ylf = (const float*)dp[6]; st.dimSp = dimSp;
     st.lift = lift;
     st.hops = hops;
     st.hopsL = hopsL;

@@ -16,8 +16,7 @@ struct BigLaunch {
     if (tb.sym.on) {  // symmetric-power engine: lift every site's scalar, one wave per link, then one per site
       T* hops = static_cast<T*>(tb.sym.hops);
       if (!(flags & F_NO_HOP)) {
-        const size_t lds = size_t(tb.d) * kSymWave * sizeof(cx<T>) + size_t(kSymWave) * (2 * kLiftChunk + 1) * sizeof(T);
-        hipLaunchKernelGGL(sym_lift_kernel<T>, dim3(c.Vp / kSymWave, (tb.sym.dimS + kLiftChunk - 1) / kLiftChunk), dim3(kSymWave), lds,
+        hipLaunchKernelGGL(sym_lift_kernel<T>, dim3(c.Vp / kSymWave, (tb.sym.dimS + kLiftChunk - 1) / kLiftChunk), dim3(kSymWave), 0,
                            c.stream, c.Vp, (const T*)phi_in, tb.d, tb.sym);
         hipLaunchKernelGGL((sym_link_kernel<T, N, D>), dim3(c.Vp, D), dim3(kSymWave), size_t(tb.sym.lds_total) * sizeof(cx<T>),
                            c.stream, a, tb, hops, mode == BIG_ACTION ? 0 : 1);  // the action needs the forward hops only

@@ -148,31 +148,36 @@ GHX_DEV void sym_project2(const R* __restrict__ Yp, int ng, int d, int nd, const
 
 // Yc phi of every site, site-major: lift[2 (g dimS + q) + (re, im)], so that
 // the wave of a link reads one site's lifted vector in one contiguous run.
-// Block (x, y) lifts components [y kLiftChunk, (y+1) kLiftChunk) of 64 sites:
-// phi staged in LDS, the results through an LDS tile and out in runs of
+// Block (x, y) lifts components [y kLiftChunk, (y+1) kLiftChunk) of 64 sites,
+// a lane per site with the chunk's accumulators in registers: phi_l is read
+// once per lane (coalesced over sites), the table row Yl[l][chunk] is uniform
+// (scalar loads). The results go out through an LDS tile in runs of
 // kLiftChunk complex values per site.
 template <typename T>
-__global__ __launch_bounds__(kSymWave) void sym_lift_kernel(unsigned Vp, const T* phi, int d, BigRepTables::Sym st) {
+__global__ __launch_bounds__(kSymWave) void sym_lift_kernel(unsigned Vp, const T* __restrict__ phi, int d, BigRepTables::Sym st) {
   using R = T;
   constexpr int ts = 2 * kLiftChunk + 1;  // tile row stride (odd: no bank conflicts)
-  extern __shared__ unsigned char smem_raw[];
-  cx<R>* ph = reinterpret_cast<cx<R>*>(smem_raw);  // [d][64]
-  R* tile = reinterpret_cast<R*>(ph + d * kSymWave);  // [64][ts]
+  __shared__ R tile[kSymWave * ts];
   const unsigned g0 = blockIdx.x * kSymWave;
   const int t = threadIdx.x, dimS = st.dimS, q0 = blockIdx.y * kLiftChunk;
   const int nq = min(kLiftChunk, dimS - q0);
+  const R* __restrict__ yl = (sizeof(R) == 4 ? reinterpret_cast<const R*>(st.Ylf) : reinterpret_cast<const R*>(st.Yl)) + q0;
+  cx<R> acc[kLiftChunk];
+#pragma unroll
+  for (int q = 0; q < kLiftChunk; ++q) acc[q] = {R(0), R(0)};
   // padding sites are lifted too (never read): the block stays whole
-  for (int l = 0; l < d; ++l) ph[l * kSymWave + t] = {R(phi[size_t(2 * l) * Vp + g0 + t]), R(phi[size_t(2 * l + 1) * Vp + g0 + t])};
-  __syncthreads();
-  for (int q = 0; q < nq; ++q) {
-    cx<R> acc = {R(0), R(0)};
-    for (int l = 0; l < d; ++l) {
-      const R y = sizeof(R) == 4 ? R(st.Ycf[(q0 + q) * d + l]) : R(st.Yc[(q0 + q) * d + l]);
-      acc.r = fma(y, ph[l * kSymWave + t].r, acc.r);
-      acc.i = fma(y, ph[l * kSymWave + t].i, acc.i);
+  for (int l = 0; l < d; ++l, yl += st.dimSp) {
+    const cx<R> p = {R(phi[size_t(2 * l) * Vp + g0 + t]), R(phi[size_t(2 * l + 1) * Vp + g0 + t])};
+#pragma unroll
+    for (int q = 0; q < kLiftChunk; ++q) {  // the table's zero padding covers q >= nq
+      acc[q].r = fma(yl[q], p.r, acc[q].r);
+      acc[q].i = fma(yl[q], p.i, acc[q].i);
     }
-    tile[t * ts + 2 * q] = acc.r;
-    tile[t * ts + 2 * q + 1] = acc.i;
+  }
+#pragma unroll
+  for (int q = 0; q < kLiftChunk; ++q) {
+    tile[t * ts + 2 * q] = acc[q].r;
+    tile[t * ts + 2 * q + 1] = acc[q].i;
   }
   __syncthreads();
   T* out = static_cast<T*>(st.lift) + size_t(g0) * 2 * dimS + 2 * q0;
@@ -184,10 +189,10 @@ __global__ __launch_bounds__(kSymWave) void sym_lift_kernel(unsigned Vp, const T
 }
 
 // One link per wave (blockIdx.y = mu): Sym(U_mu(x)) is built once and serves
-// both hops through the link, D(U) phi_{x+mu} (the forward hop of x, slot 2 mu)
-// and D(U)^dag phi_x (the backward hop of x+mu, slot 2 mu + 1; skipped unless
-// both). Results (d complex) go to the receiving site's hop record, site-major:
-// hops[2 ((g 2D + slot) d + l) + (re, im)].
+// both hops through the link: D(U) phi_{x+mu}, the forward hop of x, projected
+// into hops[2 ((g D + mu) d + l) + (re, im)]; and D(U)^dag phi_x, the backward
+// hop of x+mu (skipped unless both), left lifted and conjugated in
+// hopsL[2 ((g D + mu) 4 ng + q) + (re, im)].
 template <typename T, int N, int D>
 __global__ __launch_bounds__(kSymWave) void sym_link_kernel(BigArgs<T> a, BigRepTables tb, T* out, int both) {
   using R = T;
