@@ -5,9 +5,11 @@
 // takes one link: it runs the host-built LDS program that forms the
 // unnormalised C_k1 (and C_k2) of the link, applies C1 (x) C2 and its adjoint
 // to the lifted scalars at both ends as C1 V C2^T (V viewed as s1 x s2, scaled
-// by F^-1 or F on load) and projects back with the tables Yc F^+-1 / c.
-// sym_finish_kernel (one wave per site, a lane per component) turns a site's
-// 2D hops into the matter link force, the scalar force and the scalar's update.
+// by F^-1 or F on load), projects the forward hop back with the table Yc F / c
+// and leaves the backward hop lifted. sym_finish_kernel (one wave per site, a
+// lane per component) projects the sum of a site's lifted backward hops once,
+// and turns the hops into the matter link force, the scalar force and the
+// scalar's update.
 #pragma once
 #include "bigrep.hpp"
 #include "symrep.hpp"
@@ -41,8 +43,8 @@ GHX_DEV void st_cx(T* p, cx<T> v) {
 }
 
 // C_k1 (and C_k2) of the link U by the LDS program: one entry per lane per
-// pass, L[dst] = sum_i Us[i][j] L[s_i]. Slot 0 is zero, slot 1 takes idle lanes' writes, Us holds U
-// with a fourth column e_0 (j = 3 copies).
+// pass, L[dst] = sum_i Us[i][j] L[s_i]. Slot 0 is zero, slot 1 takes idle
+// lanes' writes, Us holds U with a fourth column e_0 (j = 3 copies).
 template <typename R, int N>
 GHX_DEV void sym_build(const BigRepTables::Sym& st, const Mat<R, N>& U, cx<R>* L) {
   const int t = threadIdx.x;
