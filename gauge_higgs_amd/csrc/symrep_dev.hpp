@@ -261,7 +261,7 @@ template <typename T, int N, int D>
 __global__ __launch_bounds__(kSymWave) void sym_finish_kernel(BigArgs<T> a, BigRepTables tb, const T* __restrict__ hops) {
   using R = T;
   constexpr int NGN = N * N - 1;
-  __shared__ cx<R> psi[kSymWave], phs[kSymWave], lsum[4 * kSymWave];
+  __shared__ cx<R> psi[D][kSymWave], phs[kSymWave], lsum[4 * kSymWave];
   const unsigned g = blockIdx.x, Vp = a.Vp;
   const int r = a.blk_rep[g / kBlock];
   const RepDesc& rd = a.reps[r];
@@ -315,25 +315,38 @@ __global__ __launch_bounds__(kSymWave) void sym_finish_kernel(BigArgs<T> a, BigR
       f = {k2 * acc.r, -k2 * acc.i};
     }
   }
-  for (int mu = 0; mu < nmu; ++mu) {
-    const cx<R> ps = hop(mu);
-    psi[l] = ps;
+  if (nmu) {
+    // the D forward hops at once, then g^a = -2 Im sum_lm conj(phi_l) T^a_lm psi_m
+    // for every direction over the lanes of generator a (each entry read once)
+#pragma unroll
+    for (int mu = 0; mu < D; ++mu) {
+      const cx<R> ps = hop(mu);
+      psi[mu][l] = ps;
+      f.r += k2 * ps.r;
+      f.i += k2 * ps.i;
+    }
     wave_sync();
-    // g^a = -2 Im sum_lm conj(phi_l) T^a_lm psi_m over the lanes of generator a
-    R gp = R(0);
-#pragma unroll 4
+    R gp[D];
+#pragma unroll
+    for (int mu = 0; mu < D; ++mu) gp[mu] = R(0);
+#pragma unroll 2
     for (int k = 0; k < st.gK; ++k) {
       const int e = glm[k * kSymWave];
       const cx<R> v = ld_cx(gval + 2 * k * kSymWave);
-      const cx<R> x = phs[e & 0xffff], y = psi[e >> 16];
-      const R zr = x.r * y.r + x.i * y.i, zi = x.r * y.i - x.i * y.r;
-      gp = fma(v.r, zi, fma(v.i, zr, gp));
+      const cx<R> x = phs[e & 0xffff];
+#pragma unroll
+      for (int mu = 0; mu < D; ++mu) {
+        const cx<R> y = psi[mu][e >> 16];
+        const R zr = x.r * y.r + x.i * y.i, zi = x.r * y.i - x.i * y.r;
+        gp[mu] = fma(v.r, zi, fma(v.i, zr, gp[mu]));
+      }
     }
-    for (int o = 1; o < lpg; o <<= 1) gp += __shfl_xor(gp, o);
-    if (l % lpg == 0 && l / lpg < NGN) a.G[size_t(mu * NGN + l / lpg) * Vp + g] = T(R(-rd.kappa) * (R(-2) * gp));
-    f.r += k2 * ps.r;
-    f.i += k2 * ps.i;
-    wave_sync();
+#pragma unroll
+    for (int mu = 0; mu < D; ++mu) {
+      R v = gp[mu];
+      for (int o = 1; o < lpg; o <<= 1) v += __shfl_xor(v, o);
+      if (l % lpg == 0 && l / lpg < NGN) a.G[size_t(mu * NGN + l / lpg) * Vp + g] = T(R(-rd.kappa) * (R(-2) * v));
+    }
   }
   // on-site term -2 dV/dphi^* (scalar_higgs.hpp:111-125) or the plugged-in potential's planes
   const R n2 = wave_sum(ph.r * ph.r + ph.i * ph.i);
