@@ -323,73 +323,93 @@ __global__ __launch_bounds__(kBigBlock) void big_matter_kernel(BigArgs<T> a, Big
 }
 
 // On-site part alone (inner steps of the multi-timescale split, F_NO_HOP):
-// -2 dV/dphi^* (scalar_higgs.hpp:111-125), pi kick and phi drift, one thread
-// per site. Same arithmetic and summation order as big_matter_kernel with no
-// hops, without a workgroup and the LDS tensor per site.
+// -2 dV/dphi^* (scalar_higgs.hpp:111-125), pi kick and phi drift; the same
+// arithmetic as big_matter_kernel with no hops. A block covers 64 sites with
+// kOnsiteGroups waves, wave w taking the components l = w, w + G, ... of every
+// site, so that a small lattice still fills the GPU; the sums over a site's
+// components (|phi|^2 and, for a frozen scalar, Re phi^dag pi and |pi|^2) meet
+// in LDS.
+constexpr int kOnsiteGroups = 4;
 template <typename T>
-__global__ __launch_bounds__(kBlock) void onsite_step_kernel(const RepDesc* reps, const int* blk_rep, unsigned Vp, const T* phi_in,
-                                                           T* phi_out, T* pi, int d, double ck, double cd, int flags,
-                                                           const T* Fon) {
+__global__ __launch_bounds__(64 * kOnsiteGroups) void onsite_step_kernel(const RepDesc* reps, const int* blk_rep, unsigned Vp,
+                                                                        const T* phi_in, T* phi_out, T* pi, int d, double ck, double cd,
+                                                                        int flags, const T* Fon) {
   using R = T;
-  const int r = blk_rep[blockIdx.x];
-  const RepDesc& rd = reps[r];
-  const unsigned s = blockIdx.x * kBlock + threadIdx.x - rd.offset;
-  if (s >= rd.vol) return;
-  const unsigned g = rd.offset + s;
+  constexpr int G = kOnsiteGroups;
+  __shared__ R red[G][64];
+  const int lane = int(threadIdx.x) % 64, grp = int(threadIdx.x) / 64;
+  const unsigned g = blockIdx.x * 64 + unsigned(lane);
+  const RepDesc& rd = reps[blk_rep[blockIdx.x * 64 / kBlock]];
+  const bool live = g - rd.offset < rd.vol;  // the block stays whole for its barriers
+  auto site_sum = [&](R v) {
+    red[grp][lane] = v;
+    __syncthreads();
+    R t = red[0][lane];
+#pragma unroll
+    for (int k = 1; k < G; ++k) t += red[k][lane];
+    __syncthreads();
+    return t;
+  };
   R n2 = R(0);
-  for (int l = 0; l < d; ++l) {
-    const R xr = R(phi_in[size_t(2 * l) * Vp + g]), xi = R(phi_in[size_t(2 * l + 1) * Vp + g]);
-    n2 += xr * xr + xi * xi;
-  }
+  if (live)
+    for (int l = grp; l < d; l += G) {
+      const R xr = R(phi_in[size_t(2 * l) * Vp + g]), xi = R(phi_in[size_t(2 * l + 1) * Vp + g]);
+      n2 += xr * xr + xi * xi;
+    }
+  n2 = site_sum(n2);
   const R c = (flags & F_NO_ONSITE) ? R(0) : R(-2) * (R(1) + R(2.0 * rd.lambda) * (n2 - R(1)));
   const bool abs_eps = flags & F_ABS_EPS;
   const R ek = R(abs_eps ? ck : ck * rd.eps), ed = R(abs_eps ? cd : cd * rd.eps);
   const bool frozen = flags & F_FROZEN;
   R rp = R(0);
-  for (int l = 0; l < d; ++l) {
-    const R xr = R(phi_in[size_t(2 * l) * Vp + g]), xi = R(phi_in[size_t(2 * l + 1) * Vp + g]);
-    R fr = fma(c, xr, R(0)), fi = fma(c, xi, R(0));
-    if (Fon) {
-      const bool on = !(flags & F_NO_ONSITE);
-      fr = on ? R(Fon[size_t(2 * l) * Vp + g]) : R(0);
-      fi = on ? R(Fon[size_t(2 * l + 1) * Vp + g]) : R(0);
+  if (live)
+    for (int l = grp; l < d; l += G) {
+      const R xr = R(phi_in[size_t(2 * l) * Vp + g]), xi = R(phi_in[size_t(2 * l + 1) * Vp + g]);
+      R fr = fma(c, xr, R(0)), fi = fma(c, xi, R(0));
+      if (Fon) {
+        const bool on = !(flags & F_NO_ONSITE);
+        fr = on ? R(Fon[size_t(2 * l) * Vp + g]) : R(0);
+        fi = on ? R(Fon[size_t(2 * l + 1) * Vp + g]) : R(0);
+      }
+      R pr = R(pi[size_t(2 * l) * Vp + g]), pim = R(pi[size_t(2 * l + 1) * Vp + g]);
+      if (flags & F_KICK) {
+        pr = fma(ek, fr, pr);
+        pim = fma(ek, fi, pim);
+        pi[size_t(2 * l) * Vp + g] = T(pr);
+        pi[size_t(2 * l + 1) * Vp + g] = T(pim);
+      }
+      if (frozen) {
+        rp += xr * pr + xi * pim;
+        continue;
+      }
+      if (flags & F_DRIFT) {
+        phi_out[size_t(2 * l) * Vp + g] = T(fma(ed, pr, xr));
+        phi_out[size_t(2 * l + 1) * Vp + g] = T(fma(ed, pim, xi));
+      }
     }
-    R pr = R(pi[size_t(2 * l) * Vp + g]), pim = R(pi[size_t(2 * l + 1) * Vp + g]);
-    if (flags & F_KICK) {
-      pr = fma(ek, fr, pr);
-      pim = fma(ek, fi, pim);
-      pi[size_t(2 * l) * Vp + g] = T(pr);
-      pi[size_t(2 * l + 1) * Vp + g] = T(pim);
-    }
-    if (frozen) {
-      rp += xr * pr + xi * pim;
-      continue;
-    }
-    if (flags & F_DRIFT) {
-      phi_out[size_t(2 * l) * Vp + g] = T(fma(ed, pr, xr));
-      phi_out[size_t(2 * l + 1) * Vp + g] = T(fma(ed, pim, xi));
-    }
-  }
-  if (!frozen) return;
+  if (!frozen) return;  // uniform over the block
   // |phi| = 1: tangent projection of the kicked pi, then the great-circle drift
   // (gauge_higgs_hmc.hpp:61-104), over the site's planes in pi (in place)
+  rp = site_sum(rp);
   R r2 = R(0);
-  for (int l = 0; l < d; ++l) {
-    R pr = R(pi[size_t(2 * l) * Vp + g]), pim = R(pi[size_t(2 * l + 1) * Vp + g]);
-    if (flags & F_KICK) {
-      pr -= rp * R(phi_in[size_t(2 * l) * Vp + g]);
-      pim -= rp * R(phi_in[size_t(2 * l + 1) * Vp + g]);
-      pi[size_t(2 * l) * Vp + g] = T(pr);
-      pi[size_t(2 * l + 1) * Vp + g] = T(pim);
+  if (live)
+    for (int l = grp; l < d; l += G) {
+      R pr = R(pi[size_t(2 * l) * Vp + g]), pim = R(pi[size_t(2 * l + 1) * Vp + g]);
+      if (flags & F_KICK) {
+        pr -= rp * R(phi_in[size_t(2 * l) * Vp + g]);
+        pim -= rp * R(phi_in[size_t(2 * l + 1) * Vp + g]);
+        pi[size_t(2 * l) * Vp + g] = T(pr);
+        pi[size_t(2 * l + 1) * Vp + g] = T(pim);
+      }
+      r2 += pr * pr + pim * pim;
     }
-    r2 += pr * pr + pim * pim;
-  }
-  if (!(flags & F_DRIFT)) return;
+  r2 = site_sum(r2);
+  if (!(flags & F_DRIFT) || !live) return;
   const R rr = sqrt(r2);
   const R rmin = sizeof(R) == 8 ? R(1e-300) : R(1e-30);
   const bool move = rr >= rmin;
   const R cs = move ? cos(rr * ed) : R(1), sn = move ? sin(rr * ed) : R(0), q = move ? sn / rr : R(0), w = -rr * sn;
-  for (int l = 0; l < d; ++l)
+  for (int l = grp; l < d; l += G)
     for (int k = 0; k < 2; ++k) {
       const R fv = R(phi_in[size_t(2 * l + k) * Vp + g]), pv = R(pi[size_t(2 * l + k) * Vp + g]);
       phi_out[size_t(2 * l + k) * Vp + g] = T(move ? cs * fv + q * pv : fv);

@@ -59,8 +59,8 @@ struct BigLaunch {
       if (!(flags & F_NO_ONSITE) && (flags & F_KICK)) multi(c, d, phi_in, c.Fon, nullptr);
       Fon = (const T*)c.Fon;
     }
-    hipLaunchKernelGGL(onsite_step_kernel<T>, dim3(c.nblk), dim3(kBlock), 0, c.stream, c.reps, c.blk_rep, c.Vp, (const T*)phi_in,
-                       (T*)phi_out, (T*)pi, d, ck, cd, flags, Fon);
+    hipLaunchKernelGGL(onsite_step_kernel<T>, dim3(c.Vp / 64), dim3(64 * kOnsiteGroups), 0, c.stream, c.reps, c.blk_rep, c.Vp,
+                       (const T*)phi_in, (T*)phi_out, (T*)pi, d, ck, cd, flags, Fon);
   }
 };
 template <typename T, int N, int D>
