@@ -1189,7 +1189,7 @@ int ghx_autotune(ghx_ctx* ctx, int steps, int* best_seg, double* best_ms) {
     ctx->block_order = order;
     if (ctx->block_walk().order == order) cand.push_back({0, order});
   }
-  if (!ctx->big) cand.push_back({0, -1});
+  cand.push_back({0, -1});
   if (ctx->nrep == 1)
     for (int sg : {1, 2, 4, 8, 16, 32, 64})
       if (sg <= ctx->hreps[0].L[ctx->D - 1]) cand.push_back({sg, 0});

@@ -278,7 +278,7 @@ def test_block_walk_falls_back_where_it_does_not_apply():
     assert g.lat.block_order() == 0
 
 
-LINK_CASES = [("fund", 2, 4), ("2", 3, 4), ("adj", 2, 3), ("", 2, 4), ("adj", 3, 4), ("1,1", 3, 3)]
+LINK_CASES = [("fund", 2, 4), ("2", 3, 4), ("adj", 2, 3), ("", 2, 4), ("adj", 3, 4), ("1,1", 3, 3), ("4,2", 3, 4), ("6,1", 3, 4)]
 
 
 @pytest.mark.parametrize("rep,N,D", LINK_CASES, ids=[f"{r or 'pure'}-N{n}-D{d}" for r, n, d in LINK_CASES])
