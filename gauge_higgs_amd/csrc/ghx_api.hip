@@ -162,7 +162,15 @@ int upload_tables(ghx_ctx* ctx) {
     for (int q = 0; q < sb.dimS; ++q)
       for (int l = 0; l < r.d; ++l) Yl[size_t(l) * dimSp + q] = sb.Yc[size_t(q) * r.d + l];
     const std::vector<float> Ylf(Yl.begin(), Yl.end()), pref(sp.pre.begin(), sp.pre.end()), Ypf(sp.Yp.begin(), sp.Yp.end());
-    const void* ssrc[7] = {sp.prog.data(), sp.pre.data(), pref.data(), sp.Yp.data(), Ypf.data(), Yl.data(), Ylf.data()};
+    // the program in LDS byte offsets of this precision's complex slots: sources
+    // as offsets, the destination's offset with the column j in its low bits
+    const int csz = 2 * int(ctx->tsize);
+    std::vector<int> progb(sp.prog.size());
+    for (size_t e = 0; e < sp.prog.size(); e += 4) {
+      for (int i = 0; i < 3; ++i) progb[e + i] = sp.prog[e + i] * csz;
+      progb[e + 3] = (sp.prog[e + 3] >> 2) * csz | (sp.prog[e + 3] & 3);
+    }
+    const void* ssrc[7] = {progb.data(), sp.pre.data(), pref.data(), sp.Yp.data(), Ypf.data(), Yl.data(), Ylf.data()};
     const size_t sbytes[7] = {sp.prog.size() * 4, sp.pre.size() * 8, pref.size() * 4, sp.Yp.size() * 8, Ypf.size() * 4,
                               Yl.size() * 8, Ylf.size() * 4};
     void* dp[7] = {};

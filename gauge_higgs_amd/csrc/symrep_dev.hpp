@@ -61,12 +61,14 @@ GHX_DEV void sym_build(const BigRepTables::Sym& st, const Mat<R, N>& U, cx<R>* L
   wave_sync();
   const int4* pg = st.prog + t;
   const int last = st.npass - 1;
+  unsigned char* Lb = reinterpret_cast<unsigned char*>(L);  // the program's offsets are in bytes
+  auto at = [&](int off) -> cx<R>& { return *reinterpret_cast<cx<R>*>(Lb + off); };
   auto run = [&](const int4 rc) {
     const cx<R>* u = L + kSymLdsUs + (rc.w & 3);
-    cx<R> acc = cmul(u[0], L[rc.x]);
-    cfma(acc, u[4], L[rc.y]);
-    if constexpr (N > 2) cfma(acc, u[8], L[rc.z]);
-    L[rc.w >> 2] = acc;
+    cx<R> acc = cmul(u[0], at(rc.x));
+    cfma(acc, u[4], at(rc.y));
+    if constexpr (N > 2) cfma(acc, u[8], at(rc.z));
+    at(rc.w & ~3) = acc;
     wave_sync();
   };
   // two register sets, each refilled right after use: a pass's entries load
@@ -116,19 +118,17 @@ GHX_DEV void sym_apply2(const BigRepTables::Sym& st, const cx<R>* S1, const cx<R
   wave_sync();
 }
 
-// out[dir][l] = sum_q Yp[dir][q][l] W[dir][q] (lanes over (dir, l); Yp has 1/c
-// folded in; direction 1 conjugated back). Yp is grouped [g][l][4]
+// The forward hop out[l] = sum_q Yp[0][q][l] W[0][q] (lanes over l; Yp has
+// 1/c folded in), stored straight to the hop record. Yp is grouped [g][l][4]
 // (q = 4 g + i), so a lane reads 4 consecutive q of its row in one 16- or
-// 32-byte access; W is zero from dimS to 4 ng in each direction.
-template <typename R>
-GHX_DEV void sym_project2(const R* __restrict__ Yp, int ng, int d, int nd, const cx<R>* W, cx<R>* out) {
-  for (int e = threadIdx.x; e < nd * d; e += kSymWave) {
-    const int dir = e >= d, l = e - dir * d;
-    const R* y = Yp + (size_t(dir) * ng * d + l) * 4;
-    const cx<R>* w = W + dir * 4 * ng;
+// 32-byte access; W is zero from dimS to 4 ng.
+template <typename T, typename R>
+GHX_DEV void sym_project_fwd(const R* __restrict__ Yp, int ng, int d, const cx<R>* W, T* out) {
+  for (int l = threadIdx.x; l < d; l += kSymWave) {
+    const R* y = Yp + size_t(l) * 4;
     cx<R> acc = {R(0), R(0)};
 #pragma unroll 2
-    for (int g = 0; g < ng; ++g, y += 4 * d, w += 4) {
+    for (int g = 0; g < ng; ++g, y += 4 * d) {
       R v[4];
       if constexpr (sizeof(R) == 4) {
         const float4 f = *reinterpret_cast<const float4*>(y);
@@ -139,13 +139,12 @@ GHX_DEV void sym_project2(const R* __restrict__ Yp, int ng, int d, int nd, const
       }
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        acc.r = fma(v[i], w[i].r, acc.r);
-        acc.i = fma(v[i], w[i].i, acc.i);
+        acc.r = fma(v[i], W[4 * g + i].r, acc.r);
+        acc.i = fma(v[i], W[4 * g + i].i, acc.i);
       }
     }
-    out[e] = dir ? cconj(acc) : acc;
+    st_cx(out + 2 * l, cx<T>{T(acc.r), T(acc.i)});
   }
-  wave_sync();
 }
 
 // Yc phi of every site, site-major: lift[2 (g dimS + q) + (re, im)], so that
@@ -204,7 +203,6 @@ __global__ __launch_bounds__(kSymWave) void sym_link_kernel(BigArgs<T> a, BigRep
   cx<R>* V = L + st.lds_V;
   cx<R>* T1 = L + st.lds_T1;
   cx<R>* W = L + st.lds_W;
-  __shared__ cx<R> res[2 * kBigMaxD];
   const unsigned g = blockIdx.x, Vp = a.Vp;
   const int r = a.blk_rep[g / kBlock];
   const RepDesc& rd = a.reps[r];
@@ -234,9 +232,7 @@ __global__ __launch_bounds__(kSymWave) void sym_link_kernel(BigArgs<T> a, BigRep
   // hop only enters the scalar force of x+mu, summed over mu, so it is stored
   // lifted and projected once per site by sym_finish_kernel
   const R* Yp = sizeof(R) == 4 ? reinterpret_cast<const R*>(st.Ypf) : reinterpret_cast<const R*>(st.Yp);
-  sym_project2<R>(Yp, st.ng, d, 1, W, res);
-  T* ho = out + 2 * ((size_t(g) * D + mu) * d);
-  for (int l = t; l < d; l += kSymWave) st_cx(ho + 2 * l, res[l]);
+  sym_project_fwd<T, R>(Yp, st.ng, d, W, out + 2 * ((size_t(g) * D + mu) * d));
   if (both) {
     T* hl = static_cast<T*>(st.hopsL) + 2 * ((size_t(gp) * D + mu) * P);
     for (int q = t; q < P; q += kSymWave) st_cx(hl + 2 * q, W[P + q]);
