@@ -252,7 +252,7 @@ def main():
         g.lat.set_block_order(args.block_order)
         tuned["block_order"] = g.lat.block_order()
     if not args.no_tune:
-        seg, tms = g.lat.autotune(steps=20)  # launch-shape autotuner (fields restored)
+        seg, tms = g.lat.autotune(steps=50)  # launch-shape autotuner (fields restored)
         tuned = {"stream_seg": seg, "block_order": g.lat.block_order(), "link_lanes": g.lat.link_lanes(),
                  "ms_per_step_at_tune": tms}
     if args.warmup:
