@@ -152,7 +152,7 @@ GHX_DEV void big_apply(const BigRepTables& tb, const Mat<R, N>& M, const cx<R>* 
 }
 
 // The site's scalar after its hop terms (fph, in LDS, shared by the whole
-// workgroup): the on-site term -2 dV/dphi^* (scalar_higgs.hpp:111-125, or the
+// workgroup): the on-site term -2 dV/dphi^* (scalar_higgs.hpp:25-39, or the
 // Fon planes of a plugged-in potential), then the F_FORCE output or the pi kick
 // and phi drift, with the tangent projection and great-circle drift of a
 // frozen scalar. vin is workgroup scratch of d entries.
@@ -161,7 +161,7 @@ GHX_DEV void big_phi_finish(const BigArgs<T>& a, const RepDesc& rd, unsigned g, 
                             cx<T>* vin) {
   using R = T;
   const unsigned Vp = a.Vp;
-  // on-site term -2 dV/dphi^* (scalar_higgs.hpp:111-125)
+  // on-site term -2 dV/dphi^* (scalar_higgs.hpp:25-39)
   __shared__ R n2s;
   if (threadIdx.x == 0) {
     R n2 = R(0);
@@ -197,7 +197,7 @@ GHX_DEV void big_phi_finish(const BigArgs<T>& a, const RepDesc& rd, unsigned g, 
   const bool frozen = a.flags & F_FROZEN;
   __syncthreads();
   if (frozen) {
-    // |phi| = 1 (gauge_higgs_hmc.hpp:61-104): project the kicked pi onto the
+    // |phi| = 1 (gauge_higgs_hmc.hpp:61-103): project the kicked pi onto the
     // tangent space, then move along the great circle
     __shared__ R red;
     if (a.flags & F_KICK) {
@@ -323,7 +323,7 @@ __global__ __launch_bounds__(kBigBlock) void big_matter_kernel(BigArgs<T> a, Big
 }
 
 // On-site part alone (inner steps of the multi-timescale split, F_NO_HOP):
-// -2 dV/dphi^* (scalar_higgs.hpp:111-125), pi kick and phi drift; the same
+// -2 dV/dphi^* (scalar_higgs.hpp:25-39), pi kick and phi drift; the same
 // arithmetic as big_matter_kernel with no hops. A block covers 64 sites with
 // kOnsiteGroups waves, wave w taking the components l = w, w + G, ... of every
 // site, so that a small lattice still fills the GPU; the sums over a site's
@@ -389,7 +389,7 @@ __global__ __launch_bounds__(64 * kOnsiteGroups) void onsite_step_kernel(const R
     }
   if (!frozen) return;  // uniform over the block
   // |phi| = 1: tangent projection of the kicked pi, then the great-circle drift
-  // (gauge_higgs_hmc.hpp:61-104), over the site's planes in pi (in place)
+  // (gauge_higgs_hmc.hpp:61-103), over the site's planes in pi (in place)
   rp = site_sum(rp);
   R r2 = R(0);
   if (live)

@@ -21,7 +21,7 @@
 namespace {
 constexpr double kOmelyanLambda = 0.1931833275037836;  // config.hpp:25
 constexpr unsigned long long kTagP_pure = 0xA1, kTagMC_pure = 0xB2;        // hmc.hpp:74,86
-constexpr unsigned long long kTagP = 0xC3, kTagPi = 0xD4, kTagMC = 0xE5;   // gauge_higgs_hmc.hpp:167-185
+constexpr unsigned long long kTagP = 0xC3, kTagPi = 0xD4, kTagMC = 0xE5;   // gauge_higgs_hmc.hpp:167-188
 }  // namespace
 
 namespace {
@@ -741,7 +741,7 @@ int ghx_refresh_momenta(ghx_ctx* ctx) {
   return GHX_OK;
 }
 
-// Frozen-length scalar |phi_x| = 1 (GaugeHiggsHMC::frozen_phi, gauge_higgs_hmc.hpp:38, 61-104).
+// Frozen-length scalar |phi_x| = 1 (GaugeHiggsHMC::frozen_phi, gauge_higgs_hmc.hpp:38, 61-103).
 int ghx_set_frozen_phi(ghx_ctx* ctx, int on) {
   int rc = check_ready(ctx);
   if (rc) return rc;
@@ -750,7 +750,7 @@ int ghx_set_frozen_phi(ghx_ctx* ctx, int on) {
   drop_graph(ctx);
   return GHX_OK;
 }
-// normalize_phi (gauge_higgs_hmc.hpp:74-84): phi_x /= |phi_x| on every replica.
+// normalize_phi (gauge_higgs_hmc.hpp:74-83): phi_x /= |phi_x| on every replica.
 int ghx_normalize_phi(ghx_ctx* ctx) {
   int rc = check_ready(ctx);
   if (rc) return rc;

@@ -48,7 +48,7 @@ struct RepDesc {
 // F_NO_GAUGE / F_NO_HOP / F_NO_ONSITE drop force parts (multi-timescale
 // splitting); F_SURR adds surr_sign * S (a position-dependent surrogate link
 // force) to the link force.
-// F_FROZEN: |phi_x| = 1 scalar (gauge_higgs_hmc.hpp:61-104): pi is projected
+// F_FROZEN: |phi_x| = 1 scalar (gauge_higgs_hmc.hpp:61-103): pi is projected
 // onto the tangent space after each kick and the phi drift is the geodesic.
 // F_EXT_ONSITE: the on-site force -2 dV/dphi^* comes from the Fon planes (a
 // plugged-in potential, e.g. the multi-invariant one) instead of the quartic.
@@ -302,7 +302,7 @@ GHX_DEV void geodesic_drift(cx<R>* ph, cx<R>* pv, R e) {
 // With F_FORCE the forces are written out instead (parity / FD tests).
 // Reads U_in/phi_in (all neighbours), writes U_out/phi_out (double buffer),
 // updates P and pi in place. Gauge force: gauge_wilson.hpp:12-27,71-83;
-// matter forces: scalar_higgs.hpp:156-213; drift: gauge_higgs_hmc.hpp:136-147.
+// matter forces: scalar_higgs.hpp:67-128; drift: gauge_higgs_hmc.hpp:136-147.
 // ---------------------------------------------------------------------------
 // The site's scalar: phi_x into phx and the on-site term -2 dV/dphi^* into fph
 // (the quartic, or the Fon planes of a plugged-in potential).
@@ -778,7 +778,7 @@ __global__ __launch_bounds__(kBlock) void phi_drift_kernel(const RepDesc* reps, 
     for (int c = 0; c < nplanes; ++c) phi_out[size_t(c) * Vp + g] = phi_in[size_t(c) * Vp + g] + e * pi[size_t(c) * Vp + g];
     return;
   }
-  // |phi| = 1: great-circle drift of phi and pi (gauge_higgs_hmc.hpp:86-104)
+  // |phi| = 1: great-circle drift of phi and pi (gauge_higgs_hmc.hpp:84-103)
   T r2 = T(0);
   for (int c = 0; c < nplanes; ++c) r2 += pi[size_t(c) * Vp + g] * pi[size_t(c) * Vp + g];
   const T rr = sqrt(r2);
@@ -826,8 +826,8 @@ struct ActArgs {
 
 // Per-site FP64 contributions to H and the observables:
 //  q0 sum_{mu<nu} Re Tr U_pl (gauge_wilson.hpp:32-65)
-//  q1 sum_mu Re phi_x^dag D(U_mu(x)) phi_{x+mu} (scalar_higgs.hpp:128-143, 217-229)
-//  q2 V(phi_x), q3 |phi_x|^2 (scalar_higgs.hpp:111-125, 232-239)
+//  q1 sum_mu Re phi_x^dag D(U_mu(x)) phi_{x+mu} (scalar_higgs.hpp:42-58, 130-144)
+//  q2 V(phi_x), q3 |phi_x|^2 (scalar_higgs.hpp:25-39, 146-154)
 //  q4 sum P^2 (fields.hpp:69-74), q5 sum |pi|^2 (scalar_field.hpp:53-57)
 template <typename T, int N, int D, class REPD>
 __global__ __launch_bounds__(kRedBlock) void action_kernel(ActArgs a, const T* __restrict__ U, const T* __restrict__ P,

@@ -79,7 +79,7 @@ std::vector<double> symmetric_eigenvalues(std::vector<double> A, int n) {
 
 // Distinct eigenvalues of the Hermitian Chat, through its real 2dd x 2dd
 // embedding [[Re, -Im], [Im, Re]] (every eigenvalue appears twice there),
-// merged within 1e-4 as scalar_invariants.hpp:175-176 does.
+// merged within 1e-4 as scalar_invariants.hpp:186 does.
 std::vector<double> casimir_channels(const RepInfo& r) {
   const int dd = r.d * r.d, n = 2 * dd;
   const std::vector<cplx> C = casimir_superop(r);

@@ -248,7 +248,7 @@ GHX_DEV R wave_sum(R v) {
 
 // A site's matter terms from its 2D hops (sym_link_kernel), one wave per site
 // with lane l holding component l (d <= 64): the matter link force
-// -kappa g^a, g^a = -2 Im phi^dag T^a_R psi_mu (scalar_higgs.hpp:198-213), and
+// -kappa g^a, g^a = -2 Im phi^dag T^a_R psi_mu (scalar_higgs.hpp:110-128), and
 // the scalar force 2 kappa sum(hops) - 2 dV/dphi^* with the pi kick, the
 // tangent projection and great-circle drift of a frozen scalar and the phi
 // drift (big_matter_kernel's contract), or q1 = sum_mu Re phi^dag psi_mu for
@@ -344,7 +344,7 @@ __global__ __launch_bounds__(kSymWave) void sym_finish_kernel(BigArgs<T> a, BigR
       if (l % lpg == 0 && l / lpg < NGN) a.G[size_t(mu * NGN + l / lpg) * Vp + g] = T(R(-rd.kappa) * (R(-2) * v));
     }
   }
-  // on-site term -2 dV/dphi^* (scalar_higgs.hpp:111-125) or the plugged-in potential's planes
+  // on-site term -2 dV/dphi^* (scalar_higgs.hpp:25-39) or the plugged-in potential's planes
   const R n2 = wave_sum(ph.r * ph.r + ph.i * ph.i);
   if (!a.Fon) {
     const R c = (a.flags & F_NO_ONSITE) ? R(0) : R(-2) * (R(1) + R(2.0 * rd.lambda) * (n2 - R(1)));
@@ -368,7 +368,7 @@ __global__ __launch_bounds__(kSymWave) void sym_finish_kernel(BigArgs<T> a, BigR
   if (a.flags & F_KICK) {
     pv.r = fma(ek, f.r, pv.r);
     pv.i = fma(ek, f.i, pv.i);
-    if (frozen) {  // |phi| = 1 (gauge_higgs_hmc.hpp:61-104): tangent projection
+    if (frozen) {  // |phi| = 1 (gauge_higgs_hmc.hpp:61-103): tangent projection
       const R rp = wave_sum(ph.r * pv.r + ph.i * pv.i);
       pv.r -= rp * ph.r;
       pv.i -= rp * ph.i;

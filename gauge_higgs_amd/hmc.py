@@ -1,7 +1,7 @@
 """Python mirror of the reference HMC interface over the ghx C ABI.
 
 `GaugeHMC` mirrors hmc.hpp:13-96 and `GaugeHiggsHMC` mirrors
-gauge_higgs_hmc.hpp:113-293: the same public couplings (beta, kappa, lambda,
+gauge_higgs_hmc.hpp:15-195: the same public couplings (beta, kappa, lambda,
 tau, nmd, integ, reunit_each_traj), counters (traj_count, accept_count,
 last_dH) and methods (trajectory, md_evolve, kick, drift, hamiltonian,
 refresh_momenta). Every call goes to the HIP kernels in libghx.so; there is no
@@ -90,7 +90,7 @@ class Lattices:
         self._check(self._lib.ghx_set_multi_potential(self.h, _dp(f), len(f), mu2))
 
     def set_frozen_phi(self, on=True):
-        """|phi_x| = 1 scalar: tangent momenta and geodesic drift (gauge_higgs_hmc.hpp:61-104)."""
+        """|phi_x| = 1 scalar: tangent momenta and geodesic drift (gauge_higgs_hmc.hpp:61-103)."""
         self._check(self._lib.ghx_set_frozen_phi(self.h, int(on)))
 
     def normalize_phi(self):
@@ -298,7 +298,7 @@ class Lattices:
 
 
 class GaugeHiggsHMC:
-    """Single-lattice gauge+Higgs HMC (gauge_higgs_hmc.hpp:113-293). rep='' gives GaugeHMC."""
+    """Single-lattice gauge+Higgs HMC (gauge_higgs_hmc.hpp:15-195). rep='' gives GaugeHMC."""
 
     def __init__(self, extents, rep, seed, D=4, N=2, precision=FP64):
         if np.isscalar(extents):

@@ -3,7 +3,7 @@
 Workload: 4D SU(3) + complex scalar in the Young-rows {6,1} irrep (d = 48),
 beta = 5.7, kappa = 0.05, lambda = 0.5, tau = 1, FP32 MD with the FP64
 Hamiltonian and Metropolis test. The exact chain is the reference's HMC
-(gauge_higgs_hmc.hpp:149-204, Omelyan, nmd = 16). The learned chain uses the
+(gauge_higgs_hmc.hpp:149-165, Omelyan, nmd = 16). The learned chain uses the
 Sexton-Weingarten split (theory_notes.md:129-134, u1.hpp:278-320 for U(1)):
 the expensive hopping forces (matter link force and the D(U) phi hops of the
 scalar force, 12 tensor applies per site in the reference formulation) on

@@ -87,7 +87,7 @@ def kappa_scan(seed0=1, nmd=20, precision=FP32MD, n_chains=32, Ls=(6, 8, 10, 12)
 
 def measure_scan(rs, n_therm=200, n_meas=1000, use_graph=True):
     """Run a replica set: n_therm cycles, then n_meas cycles recording every
-    chain's L_link (scalar_higgs.hpp:217-229). Returns per-chain kappa, L,
+    chain's L_link (scalar_higgs.hpp:130-144). Returns per-chain kappa, L,
     <L_link>, chi_link = V Var(L_link) (observables.hpp:104-145 Stats), the HMC
     and swap acceptances, and the time per cycle (one trajectory of every chain
     plus the swap passes, graph-replayed)."""

@@ -36,7 +36,7 @@ enum { GHX_COLD = 0, GHX_HOT = 1 };           /* start modes, fields.hpp:21-34 *
 
 /* Create a context on the current HIP device. L holds n_replicas*D extents.
  * Replaces the construction of GaugeHMC / GaugeHiggsHMC (hmc.hpp:29-30,
- * gauge_higgs_hmc.hpp:143-148) and of ReplicaTempering's replica set
+ * gauge_higgs_hmc.hpp:45-50) and of ReplicaTempering's replica set
  * (replica_tempering.hpp:97-118). */
 int ghx_create(ghx_ctx** out, int D, int N, int n_replicas, const int* L, int precision);
 int ghx_destroy(ghx_ctx* ctx);
@@ -67,7 +67,7 @@ int ghx_set_integrator(ghx_ctx* ctx, int nmd, int integrator, int reunit_each_tr
  * restores the single-scale integrator. */
 int ghx_set_nested(ghx_ctx* ctx, int n_outer, int m_inner);
 /* Frozen-length scalar |phi_x| = 1 (GaugeHiggsHMC::frozen_phi,
- * gauge_higgs_hmc.hpp:38, 61-104, 133, 144, 172): momenta are projected onto
+ * gauge_higgs_hmc.hpp:38, 61-103, 133, 144, 172): momenta are projected onto
  * the tangent space at refresh and after every kick, and the phi drift is the
  * exact great-circle flow. ghx_normalize_phi is normalize_phi() (:74-84).
  * Not available with the large-irrep kernels (GHX_E_UNSUPPORTED). */
@@ -82,7 +82,7 @@ int ghx_normalize_phi(ghx_ctx* ctx);
  * quartic potential. Not available with the large-irrep kernels. */
 int ghx_casimir_channels(int N, const char* spec, double* C2, int* n);
 int ghx_set_multi_potential(ghx_ctx* ctx, const double* f, int nf, double mu2);
-/* RNG seed of one replica (Rng ctor, rng.hpp:27). */
+/* RNG seed of one replica (Rng ctor, rng.hpp:21). */
 int ghx_set_seed(ghx_ctx* ctx, int replica, uint64_t seed);
 
 /* Start configuration of one replica: links cold or reference hot start with
@@ -91,7 +91,7 @@ int ghx_set_seed(ghx_ctx* ctx, int replica, uint64_t seed);
 int ghx_init_fields(ghx_ctx* ctx, int replica, int link_mode, double sigma_U, int phi_mode, uint64_t phi_stream,
                     double phi_sigma);
 
-/* Trajectory/accept counters (hmc.hpp:26, gauge_higgs_hmc.hpp:140-141). */
+/* Trajectory/accept counters (hmc.hpp:26, gauge_higgs_hmc.hpp:42-43). */
 int ghx_set_counters(ghx_ctx* ctx, int replica, uint64_t traj_count, uint64_t accept_count);
 int ghx_get_counters(const ghx_ctx* ctx, int replica, uint64_t* traj_count, uint64_t* accept_count, double* last_dH);
 
@@ -112,7 +112,7 @@ int ghx_trajectory(ghx_ctx* ctx, int* accepted, double* dH);
 /* n trajectories; when use_graph, the first is captured into a hipGraph and replayed. */
 int ghx_trajectories(ghx_ctx* ctx, int n, int use_graph);
 /* Observables per replica: [plaquette, S_g, S_H, L_link, L_phi, Polyakov] x n_replicas
- * (gauge_wilson.hpp:51-65, scalar_higgs.hpp:217-239, observables.hpp:18-34). */
+ * (gauge_wilson.hpp:51-65, scalar_higgs.hpp:130-154, observables.hpp:18-34). */
 int ghx_observables(ghx_ctx* ctx, double* out);
 /* Wilson loop traced in a probe irrep, per replica (observables.hpp:36-81, wilson_loop_fund /
  * wilson_loop_rep): W[r] = < (1/d_R') Re Tr D^(R')(U_loop) > over sites and planes mu < nu of the
@@ -144,7 +144,7 @@ int ghx_get_fields(ghx_ctx* ctx, int replica, double* U, double* P, double* phi,
 int ghx_set_fields(ghx_ctx* ctx, int replica, const double* U, const double* P, const double* phi, const double* pi);
 /* Link force F (gauge + matter, add_gauge_force + add_matter_link_force) and
  * scalar force F_phi (scalar_force) of one replica for the current fields
- * (gauge_wilson.hpp:71-83, scalar_higgs.hpp:156-213). */
+ * (gauge_wilson.hpp:71-83, scalar_higgs.hpp:67-128). */
 int ghx_forces(ghx_ctx* ctx, int replica, double* F_link, double* F_phi);
 
 /* Replica swap proposal between replicas i and j (same extents): exchanges U

@@ -822,8 +822,8 @@ struct Engine {
         for (int a = 0; a < ngen<N>(); ++a) Fa[a] += c * tr_prod(gens<N>()[a], Om).imag();
       }
   }
-  // scalar_higgs.hpp:111-125 (QuarticPotential)
-  // on-site potential: the quartic default (scalar_higgs.hpp:26-39) or the plugged-in
+  // scalar_higgs.hpp:25-39 (QuarticPotential)
+  // on-site potential: the quartic default (scalar_higgs.hpp:25-39) or the plugged-in
   // multi-invariant one (GaugeHiggsHMC::potential, gauge_higgs_hmc.hpp:27)
   const Channels<N>* multi = nullptr;
   std::vector<double> multi_f;
@@ -841,7 +841,7 @@ struct Engine {
     for (std::size_t k = 0; k < x.size(); ++k) gr[k] = cd(c, 0) * x[k];
     return gr;
   }
-  double scalar_action() const {  // scalar_higgs.hpp:128-143
+  double scalar_action() const {  // scalar_higgs.hpp:42-58
     double S = 0.0;
     for (std::int64_t s = 0; s < g.vol; ++s) {
       const DV px = site_phi(s);
@@ -853,7 +853,7 @@ struct Engine {
     }
     return S;
   }
-  void scalar_force() {  // scalar_higgs.hpp:156-185
+  void scalar_force() {  // scalar_higgs.hpp:67-99
 #pragma omp parallel for schedule(static)
     for (std::int64_t s = 0; s < g.vol; ++s) {
       const DV px = site_phi(s);
@@ -869,7 +869,7 @@ struct Engine {
       for (int k = 0; k < d; ++k) Fphi[s * d + k] = Fs[k];
     }
   }
-  void add_matter_link_force() {  // scalar_higgs.hpp:198-213
+  void add_matter_link_force() {  // scalar_higgs.hpp:110-128
 #pragma omp parallel for schedule(static)
     for (std::int64_t s = 0; s < g.vol; ++s)
       for (int mu = 0; mu < D; ++mu) {
@@ -878,7 +878,7 @@ struct Engine {
         for (int a = 0; a < ngen<N>(); ++a) Fa[a] += -kappa * gg[a];
       }
   }
-  double link_energy() const {  // scalar_higgs.hpp:217-229
+  double link_energy() const {  // scalar_higgs.hpp:130-144
     double sm = 0.0;
     for (std::int64_t x = 0; x < g.vol; ++x)
       for (int mu = 0; mu < D; ++mu) {
@@ -887,7 +887,7 @@ struct Engine {
       }
     return sm / (double(g.vol) * D);
   }
-  double higgs_length() const {  // scalar_higgs.hpp:232-239
+  double higgs_length() const {  // scalar_higgs.hpp:146-154
     double sm = 0.0;
     for (std::int64_t x = 0; x < g.vol; ++x) sm += vnorm2(site_phi(x));
     return sm / double(g.vol);
@@ -962,7 +962,7 @@ struct Engine {
     if (!rep) return kinetic_P() + gauge_action();
     return kinetic_P() + kinetic_pi() + gauge_action() + scalar_action();
   }
-  // Frozen-length scalar |phi_x| = 1 — gauge_higgs_hmc.hpp:61-104.
+  // Frozen-length scalar |phi_x| = 1 — gauge_higgs_hmc.hpp:61-103.
   // Tangent projection: pi -= Re(phi^dag pi) phi per site.
   void project_pi_tangent() {
 #pragma omp parallel for schedule(static)

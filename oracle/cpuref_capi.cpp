@@ -220,7 +220,7 @@ void orc_hot(void* h, double sigma) { static_cast<Base*>(h)->hot(sigma); }
 void orc_cold(void* h) { static_cast<Base*>(h)->cold(); }
 void orc_phi_gaussian(void* h, std::uint64_t stream, double sigma) { static_cast<Base*>(h)->phi_gaussian(stream, sigma); }
 void orc_phi_cold(void* h, double c) { static_cast<Base*>(h)->phi_cold(c); }
-// frozen-length scalar (gauge_higgs_hmc.hpp:38, 61-104)
+// frozen-length scalar (gauge_higgs_hmc.hpp:38, 61-103)
 void orc_set_frozen(void* h, int on) { static_cast<Base*>(h)->set_frozen(on); }
 void orc_normalize_phi(void* h) { static_cast<Base*>(h)->normalize_phi(); }
 // multi-invariant potential (scalar_invariants.hpp): returns the channel count (or -1)
