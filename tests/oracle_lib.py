@@ -121,7 +121,7 @@ class Oracle:
         lib().orc_phi_cold(self.h, c)
 
     def frozen(self, on=True):
-        """|phi_x| = 1 scalar: tangent projections and geodesic drift (gauge_higgs_hmc.hpp:61-104)."""
+        """|phi_x| = 1 scalar: tangent projections and geodesic drift (gauge_higgs_hmc.hpp:61-103)."""
         lib().orc_set_frozen(self.h, int(on))
 
     def normalize_phi(self):

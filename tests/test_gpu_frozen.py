@@ -1,5 +1,5 @@
 """Frozen-length scalar |phi_x| = 1 on the GPU (GaugeHiggsHMC::frozen_phi,
-gauge_higgs_hmc.hpp:61-104, 133, 144, 172) against the reference's own dumps:
+gauge_higgs_hmc.hpp:61-103, 133, 144, 172) against the reference's own dumps:
 normalisation, tangent-projected momenta, kick + projection, geodesic drift,
 and whole trajectories of both integrators."""
 import numpy as np

@@ -166,7 +166,7 @@ FROZEN = [("su2_fund_frozen", 4, 2, "fund", 2.3, 0.6, 20, 4), ("su2_adj_frozen",
 
 @pytest.mark.parametrize("tag,D,N,rep,beta,kappa,nmd,L", FROZEN, ids=[c[0] for c in FROZEN])
 def test_frozen_scalar_matches_reference(goldens, tag, D, N, rep, beta, kappa, nmd, L):
-    """|phi_x| = 1 scalar (gauge_higgs_hmc.hpp:61-104, 133, 144, 172): normalisation,
+    """|phi_x| = 1 scalar (gauge_higgs_hmc.hpp:61-103, 133, 144, 172): normalisation,
     tangent-projected momenta and kicks, geodesic drift, trajectories."""
     g, sc = goldens
     o = O.Oracle(D, N, L, rep, seed=1)
