@@ -67,6 +67,7 @@ def load():
         "ghx_load_checkpoint": [vp, C.c_char_p],
         "ghx_normalize_phi": [vp],
         "ghx_surrogate_init": [vp, u64],
+        "ghx_rng_gauss": [u64, u64, C.c_int64, dp],
         "ghx_surrogate_enable": [vp, i],
         "ghx_surrogate_nparams": [],
         "ghx_surrogate_params": [vp, i, C.POINTER(C.c_float)],

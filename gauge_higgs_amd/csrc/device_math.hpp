@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "gauss_cr.hpp"
+
 namespace ghx {
 
 #define GHX_DEV __device__ __forceinline__
@@ -503,10 +505,13 @@ __host__ __device__ inline double draw_uniform(uint64_t seed, uint64_t k) {
   const uint64_t h = mix64_hd(seed ^ mix64_hd(k));
   return (double(h >> 11) + 0.5) * (1.0 / 9007199254740992.0);
 }
+// Box-Muller with correctly rounded log, cos and sqrt (gauss_cr.hpp) and IEEE
+// products, as the reference's glibc/x86 arithmetic, so a draw equals the
+// reference's wherever the host libm's log and cos round correctly.
 GHX_DEV double draw_gauss(uint64_t seed, uint64_t k) {
   const double u1 = draw_uniform(seed, mix64(k * 2 + 1));
   const double u2 = draw_uniform(seed, mix64(k * 2 + 2));
-  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586476925286766559 * u2);
+  return gcr::sqrt_cr(-2.0 * gcr::log_cr(u1)) * gcr::cos_cr(6.283185307179586476925286766559 * u2);
 }
 
 }  // namespace ghx

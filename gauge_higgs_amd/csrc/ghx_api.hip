@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "ctx.hpp"
+#include "rng_kernels.hpp"
 #include "potential.hpp"
 #include "symrep.hpp"
 
@@ -240,6 +241,41 @@ int upload_tables(ghx_ctx* ctx) {
   return GHX_OK;
 }
 
+// LDS that the large-irrep path asks for at launch, against the device's
+// per-workgroup limit: the engine's LDS program, or the tensor-apply kernel's
+// dimT-entry tensor (dynamic) plus its static arrays. Irreps that fit neither
+// are refused here instead of failing at their first launch.
+size_t big_lds_need(const ghx_ctx* ctx, bool engine) {
+  const size_t cs = 2 * size_t(ctx->tsize);
+  if (engine) return size_t(ctx->tb.sym.lds_total) * cs;
+  const size_t st = ctx->bops.matter_static_lds ? ctx->bops.matter_static_lds() : size_t(-1);
+  return st == size_t(-1) ? st : size_t(ctx->rep.dimT) * cs + st;
+}
+int lds_limit(ghx_ctx* ctx, size_t* lim) {
+  int dev = 0, v = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  *lim = size_t(v);
+  return GHX_OK;
+}
+int check_big_lds(ghx_ctx* ctx) {
+  if (!ctx->big) return GHX_OK;
+  size_t lim = 0;
+  int rc = lds_limit(ctx, &lim);
+  if (rc) return rc;
+  if (ctx->tb.sym.on && big_lds_need(ctx, true) > lim) {  // the tensor path may still fit
+    ctx->sym_valid = false;
+    ctx->tb.sym.on = 0;
+  }
+  if (!ctx->tb.sym.on) {
+    const size_t need = big_lds_need(ctx, false);
+    if (need > lim)
+      return fail(ctx, GHX_E_UNSUPPORTED, "irrep " + ctx->rep.name + ": the tensor apply needs " + std::to_string(need) +
+                                               " bytes of LDS per workgroup, the device allows " + std::to_string(lim));
+  }
+  return GHX_OK;
+}
+
 int select_ops(ghx_ctx* ctx) {
   const int p = ctx->prec == GHX_FP32MD ? 1 : 0;
   const RepInfo& r = ctx->rep;
@@ -264,12 +300,30 @@ int select_ops(ghx_ctx* ctx) {
   if (!ok)
     return fail(ctx, GHX_E_UNSUPPORTED, "no compiled kernels for N=" + std::to_string(ctx->N) + " D=" + std::to_string(ctx->D) +
                                             " rep=" + r.name);
+  if (int rc = check_big_lds(ctx)) {
+    ctx->ops_ok = false;
+    return rc;
+  }
   return GHX_OK;
 }
 
 int check_ready(ghx_ctx* ctx) {
   if (!ctx) return GHX_E_INVALID;
   if (!ctx->ops_ok) return fail(ctx, GHX_E_INVALID, "representation not set (call ghx_set_rep)");
+  return GHX_OK;
+}
+
+// The swap rule leaves the on-site potential out of Delta, which holds only
+// while the two replicas of a pair share lambda. ghx_set_couplings may change
+// lambda after the ladders were set, so every entry point that swaps checks it.
+int check_ladders(ghx_ctx* ctx) {
+  if (!ctx->temper || ctx->rep.d == 0) return GHX_OK;
+  for (int p = 0; p < ctx->npairs; ++p) {
+    const int a = ctx->hpairs[2 * p], b = ctx->hpairs[2 * p + 1];
+    if (ctx->hreps[a].lambda != ctx->hreps[b].lambda)
+      return fail(ctx, GHX_E_INVALID, "replicas " + std::to_string(a) + " and " + std::to_string(b) +
+                                          " of one ladder have different lambda (swaps need equal lambda)");
+  }
   return GHX_OK;
 }
 
@@ -627,6 +681,12 @@ int ghx_set_rep(ghx_ctx* ctx, const char* spec, int* d_out) {
     return fail(ctx, GHX_E_REP, e.what());
   }
   drop_graph(ctx);
+  // State sized or meant for the previous irrep goes with it: the plugged-in
+  // potential (d^2 x d^2 superoperator, on-site planes), the surrogate and the
+  // frozen-length flag. The caller sets them again for the new irrep.
+  free_multi(ctx);
+  surrogate_free(ctx);
+  ctx->frozen = false;
   int rc = select_ops(ctx);
   if (rc) return rc;
   if ((rc = alloc_fields(ctx))) return rc;
@@ -829,6 +889,7 @@ int ghx_trajectory(ghx_ctx* ctx, int* accepted, double* dH) {
 int ghx_trajectories(ghx_ctx* ctx, int n, int use_graph) {
   int rc = check_ready(ctx);
   if (rc) return rc;
+  if ((rc = check_ladders(ctx))) return rc;
   if (ctx->cur != 0) launch_settle(ctx, false);
   if (!use_graph) {
     for (int i = 0; i < n; ++i) { launch_trajectory(ctx); launch_swaps(ctx); }
@@ -1115,6 +1176,7 @@ int ghx_set_tempering(ghx_ctx* ctx, int enabled) {
 int ghx_swap_step(ghx_ctx* ctx) {
   int rc = check_ready(ctx);
   if (rc) return rc;
+  if ((rc = check_ladders(ctx))) return rc;
   if (ctx->cur != 0) launch_settle(ctx, false);
   launch_swaps(ctx);
   HIPCHK(hipGetLastError());
@@ -1155,8 +1217,14 @@ int ghx_set_tuning(ghx_ctx* ctx, int key, int value) {
   }
   if (key == GHX_TUNE_SYM_ENGINE) {
     if (value < 0 || value > 1) return fail(ctx, GHX_E_INVALID, "symmetric-power engine must be 0 or 1");
+    const bool was = ctx->sym_enabled;
     ctx->sym_enabled = value != 0;
     ctx->tb.sym.on = (ctx->sym_valid && ctx->sym_enabled) ? 1 : 0;
+    if (int rc = check_big_lds(ctx)) {  // the tensor apply does not fit: keep the engine
+      ctx->sym_enabled = was;
+      ctx->tb.sym.on = (ctx->sym_valid && ctx->sym_enabled) ? 1 : 0;
+      return rc;
+    }
     drop_graph(ctx);
     return GHX_OK;
   }
@@ -1265,6 +1333,27 @@ int ghx_time_md_steps(ghx_ctx* ctx, int steps, double* ms_total) {
   return GHX_OK;
 }
 
+int ghx_rng_gauss(uint64_t seed, uint64_t key0, int64_t n, double* out) {
+  if (n < 0 || (n > 0 && !out)) return GHX_E_INVALID;
+  if (n == 0) return GHX_OK;
+  ghx_ctx* ctx = nullptr;  // HIPCHK's error slot
+  constexpr int64_t kChunk = int64_t(1) << 24;
+  double* d = nullptr;
+  const int64_t cap = n < kChunk ? n : kChunk;
+  HIPCHK(hipMalloc(&d, size_t(cap) * sizeof(double)));
+  int rc = GHX_OK;
+  for (int64_t o = 0; o < n && rc == GHX_OK; o += cap) {
+    const int64_t m = n - o < cap ? n - o : cap;
+    hipLaunchKernelGGL(rng_gauss_kernel, dim3(unsigned((m + kRngBlock - 1) / kRngBlock)), dim3(kRngBlock), 0, nullptr,
+                       (unsigned long long)seed,
+                       (unsigned long long)(key0 + uint64_t(o)), (long long)m, d);
+    if (hipGetLastError() != hipSuccess || hipMemcpy(out + o, d, size_t(m) * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+      rc = GHX_E_HIP;
+  }
+  (void)hipFree(d);
+  return rc;
+}
+
 int ghx_rep_query(int N, const char* spec, int* d, int* nality, int* real_rep, long* dimT, double* W, double* T) {
   if (N != 2 && N != 3) return GHX_E_INVALID;
   RepInfo r;
@@ -1345,14 +1434,18 @@ int64_t ghx_volume(const ghx_ctx* ctx) {
 
 
 // ---- checkpoint / resume (roadmap item of the reference, README.md:126-127) ----
-// Binary file, little-endian: magic "GHXCKPT1", then int32 {version, D, N,
-// n_replicas, precision, d, frozen}, int32 extents[n_replicas*D], per replica
-// {u64 seed, traj_count, accept_count; f64 last_dH, beta, kappa, lambda, tau},
-// u64 {swap seed, swap counter}, then per replica the links (reference AoS
-// [(s*D+mu)*N*N + i*N + j] complex, FP64) and phi ([s*d+k] complex, FP64).
-// Momenta are not stored: every trajectory refreshes them from its counter.
+// Binary file, little-endian: magic "GHXCKPT1", then int32 {version = 2, D, N,
+// n_replicas, precision, d, frozen, nmd, integrator, reunit, n_outer, m_inner},
+// int32 extents[n_replicas*D], per replica {u64 seed, traj_count,
+// accept_count; f64 last_dH, beta, kappa, lambda, tau}, u64 {swap seed, swap
+// counter}, int32 n_pairs, u64 attempts[n_pairs], u64 accepts[n_pairs], then
+// per replica the links (reference AoS [(s*D+mu)*N*N + i*N + j] complex, FP64)
+// and phi ([s*d+k] complex, FP64). Momenta are not stored: every trajectory
+// refreshes them from its counter. Loading requires a context with the same
+// geometry, irrep, frozen flag, integrator settings and replica-exchange pairs.
 namespace {
 constexpr char kCkptMagic[8] = {'G', 'H', 'X', 'C', 'K', 'P', 'T', '1'};
+constexpr int kCkptVersion = 2, kCkptHdr = 12;
 struct CkptRep {
   uint64_t seed, traj, acc;
   double last_dH, beta, kappa, lambda, tau;
@@ -1367,7 +1460,8 @@ int ghx_save_checkpoint(ghx_ctx* ctx, const char* path) {
   FILE* f = std::fopen(path, "wb");
   if (!f) return fail(ctx, GHX_E_INVALID, std::string("cannot open ") + path);
   bool ok = std::fwrite(kCkptMagic, 1, 8, f) == 8;
-  const int32_t hdr[7] = {1, ctx->D, ctx->N, ctx->nrep, ctx->prec, ctx->rep.d, ctx->frozen ? 1 : 0};
+  const int32_t hdr[kCkptHdr] = {kCkptVersion, ctx->D, ctx->N, ctx->nrep, ctx->prec, ctx->rep.d, ctx->frozen ? 1 : 0,
+                                 ctx->nmd, ctx->integ, ctx->reunit ? 1 : 0, ctx->n_outer, ctx->m_inner};
   ok = ok && std::fwrite(hdr, sizeof hdr, 1, f) == 1;
   for (int r = 0; r < ctx->nrep && ok; ++r) {
     int32_t L[4] = {0, 0, 0, 0};
@@ -1384,6 +1478,16 @@ int ghx_save_checkpoint(ghx_ctx* ctx, const char* path) {
   SwapState sw{0, 0};
   if (ctx->dswap) HIPCHK(hipMemcpy(&sw, ctx->dswap, sizeof sw, hipMemcpyDeviceToHost));
   ok = ok && std::fwrite(&sw, sizeof sw, 1, f) == 1;
+  // swap statistics per pair, after the pair count they belong to
+  const int32_t np = ctx->npairs;
+  std::vector<unsigned long long> att(size_t(std::max(np, 0)), 0), acc(att.size(), 0);
+  if (np > 0) {
+    HIPCHK(hipMemcpy(att.data(), ctx->dattempts, sizeof(unsigned long long) * np, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(acc.data(), ctx->daccepts, sizeof(unsigned long long) * np, hipMemcpyDeviceToHost));
+  }
+  ok = ok && std::fwrite(&np, sizeof np, 1, f) == 1;
+  if (np > 0) ok = ok && std::fwrite(att.data(), sizeof(unsigned long long), np, f) == size_t(np) &&
+                   std::fwrite(acc.data(), sizeof(unsigned long long), np, f) == size_t(np);
   for (int r = 0; r < ctx->nrep && ok; ++r) {
     const size_t nl = size_t(ctx->hreps[r].vol) * ctx->D * ctx->N * ctx->N * 2;
     std::vector<double> U(nl);
@@ -1408,11 +1512,15 @@ int ghx_load_checkpoint(ghx_ctx* ctx, const char* path) {
   if (!f) return fail(ctx, GHX_E_INVALID, std::string("cannot open ") + path);
   auto bad = [&](const std::string& why) { std::fclose(f); return fail(ctx, GHX_E_INVALID, "checkpoint " + std::string(path) + ": " + why); };
   char magic[8];
-  int32_t hdr[7];
+  int32_t hdr[kCkptHdr];
   if (std::fread(magic, 1, 8, f) != 8 || std::memcmp(magic, kCkptMagic, 8) != 0) return bad("not a ghx checkpoint");
-  if (std::fread(hdr, sizeof hdr, 1, f) != 1 || hdr[0] != 1) return bad("unknown version");
+  if (std::fread(hdr, sizeof hdr, 1, f) != 1 || hdr[0] != kCkptVersion) return bad("unknown version");
   if (hdr[1] != ctx->D || hdr[2] != ctx->N || hdr[3] != ctx->nrep || hdr[5] != ctx->rep.d)
     return bad("lattice dimension, colours, replica count or scalar dimension differ from this context");
+  if (hdr[6] != (ctx->frozen ? 1 : 0)) return bad("frozen-length flag differs from this context (set it before loading)");
+  if (hdr[7] != ctx->nmd || hdr[8] != ctx->integ || hdr[9] != (ctx->reunit ? 1 : 0) || hdr[10] != ctx->n_outer ||
+      hdr[11] != ctx->m_inner)
+    return bad("integrator settings (nmd, integrator, reunitarization, nested scales) differ from this context");
   for (int r = 0; r < ctx->nrep; ++r) {
     int32_t L[4] = {0, 0, 0, 0};
     if (std::fread(L, sizeof(int32_t), ctx->D, f) != size_t(ctx->D)) return bad("truncated");
@@ -1423,6 +1531,13 @@ int ghx_load_checkpoint(ghx_ctx* ctx, const char* path) {
   if (std::fread(reps.data(), sizeof(CkptRep), ctx->nrep, f) != size_t(ctx->nrep)) return bad("truncated");
   SwapState sw{0, 0};
   if (std::fread(&sw, sizeof sw, 1, f) != 1) return bad("truncated");
+  int32_t np = 0;
+  if (std::fread(&np, sizeof np, 1, f) != 1) return bad("truncated");
+  if (np != ctx->npairs) return bad("replica-exchange pairs differ from this context (set the ladders before loading)");
+  std::vector<unsigned long long> att(size_t(std::max(np, 0))), acc(att.size());
+  if (np > 0 && (std::fread(att.data(), sizeof(unsigned long long), np, f) != size_t(np) ||
+                 std::fread(acc.data(), sizeof(unsigned long long), np, f) != size_t(np)))
+    return bad("truncated");
   std::vector<std::vector<double>> Us(ctx->nrep), phs(ctx->nrep);
   for (int r = 0; r < ctx->nrep; ++r) {
     Us[r].resize(size_t(ctx->hreps[r].vol) * ctx->D * ctx->N * ctx->N * 2);
@@ -1447,7 +1562,10 @@ int ghx_load_checkpoint(ghx_ctx* ctx, const char* path) {
     if (has_phi(ctx) && (rc = set_planes(ctx, ctx->phi[ctx->cur], int(ctx->nphi()), r, phs[r].data()))) return rc;
   }
   if (ctx->dswap) HIPCHK(hipMemcpy(ctx->dswap, &sw, sizeof sw, hipMemcpyHostToDevice));
-  if (hdr[6] && !ctx->frozen && (rc = ghx_set_frozen_phi(ctx, 1))) return rc;
+  if (np > 0) {
+    HIPCHK(hipMemcpy(ctx->dattempts, att.data(), sizeof(unsigned long long) * np, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->daccepts, acc.data(), sizeof(unsigned long long) * np, hipMemcpyHostToDevice));
+  }
   drop_graph(ctx);
   return GHX_OK;
 }

@@ -64,12 +64,19 @@ struct BigLaunch {
   }
 };
 template <typename T, int N, int D>
+size_t matter_static_lds() {
+  hipFuncAttributes fa{};
+  if (hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&big_matter_kernel<T, N, D>)) != hipSuccess) return size_t(-1);
+  return fa.sharedSizeBytes;
+}
+template <typename T, int N, int D>
 void fill(Ops* o, BigOps* b) {
   *o = Launch<T, N, D, RepExt>::ops();
   b->matter = &BigLaunch<T, N, D>::matter;
   b->action = &BigLaunch<T, N, D>::action;
   b->phi_drift = &BigLaunch<T, N, D>::phi_drift;
   b->onsite = &BigLaunch<T, N, D>::onsite;
+  b->matter_static_lds = &matter_static_lds<T, N, D>;
 }
 bool ops_big(int prec, int N, int D, Ops* o, BigOps* b) {
   if (N == 3 && D == 4) { prec ? fill<float, 3, 4>(o, b) : fill<double, 3, 4>(o, b); return true; }

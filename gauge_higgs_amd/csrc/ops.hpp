@@ -55,6 +55,8 @@ struct BigOps {
   void (*action)(const LaunchCtx&, const void* U, const void* P, const void* phi, const void* pi, const double* q1, int d);
   void (*phi_drift)(const LaunchCtx&, const void* phi_in, void* phi_out, const void* pi, int nplanes, double cd, int abs_eps);
   void (*onsite)(const LaunchCtx&, const void* phi_in, void* phi_out, void* pi, int d, double ck, double cd, int flags);
+  // static LDS bytes of the tensor-apply matter kernel (its dynamic LDS holds the dimT-entry tensor)
+  size_t (*matter_static_lds)();
 };
 
 struct Ops {

@@ -25,7 +25,11 @@ def _dp(a):
 
 
 class GhxError(RuntimeError):
-    pass
+    """A negative ghx return code; `code` holds it (GHX_E_* of ghx.h)."""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
 
 
 class Lattices:
@@ -41,7 +45,7 @@ class Lattices:
         h = C.c_void_p()
         rc = lib.ghx_create(C.byref(h), D, N, self.nrep, flat, precision)
         if rc != 0:
-            raise GhxError(f"ghx_create failed ({rc})")
+            raise GhxError(f"ghx_create failed ({rc})", rc)
         self.h = h
         self.precision = precision
         d = C.c_int()
@@ -55,7 +59,8 @@ class Lattices:
 
     def _check(self, rc, exc=GhxError):
         if rc != 0:
-            raise exc(f"ghx error {rc}: {self._lib.ghx_last_error(self.h).decode()}")
+            msg = f"ghx error {rc}: {self._lib.ghx_last_error(self.h).decode()}"
+            raise exc(msg, rc) if exc is GhxError else exc(msg)
 
     def close(self):
         if getattr(self, "h", None):
@@ -448,3 +453,13 @@ def casimir_channels(N, spec):
 def GaugeHMC(extents, seed, D=4, N=2, precision=FP64):
     """Pure-gauge HMC (hmc.hpp:13-96): keys 0xA1 / 0xB2, H = KE_P + S_g."""
     return GaugeHiggsHMC(extents, "", seed, D=D, N=N, precision=precision)
+
+
+def rng_gauss(seed, key0, n):
+    """The device's Gaussian stream Rng(seed).gauss(key0 + i), i < n (rng.hpp:42-46)."""
+    lib = _lib.load()
+    out = np.zeros(int(n))
+    rc = lib.ghx_rng_gauss(seed, key0, int(n), _dp(out))
+    if rc != 0:
+        raise GhxError(f"ghx_rng_gauss failed ({rc})", rc)
+    return out

@@ -210,9 +210,11 @@ int ghx_surrogate_snapshot(ghx_ctx* ctx, int replica, float* X, float* B, double
 
 /* Checkpoint / resume of every replica (roadmap item, README.md:126-127): links
  * and phi in the reference AoS layout (FP64), couplings, seeds, trajectory and
- * accept counters, last dH, the swap-coin counter and the frozen flag, in a
- * versioned binary file. Loading checks D, N, the replica count, extents and d
- * against the context and returns GHX_E_INVALID on any mismatch. */
+ * accept counters, last dH, the swap-coin counter, the per-pair swap attempts
+ * and accepts, the frozen flag and the integrator settings, in a versioned
+ * binary file. Loading checks D, N, the replica count, extents, d, the frozen
+ * flag, the integrator settings and the number of swap pairs against the
+ * context and returns GHX_E_INVALID on any mismatch. */
 int ghx_save_checkpoint(ghx_ctx* ctx, const char* path);
 int ghx_load_checkpoint(ghx_ctx* ctx, const char* path);
 
@@ -226,6 +228,10 @@ int ghx_time_md_steps(ghx_ctx* ctx, int steps, double* ms_total);
 int ghx_rep_query(int N, const char* spec, int* d, int* nality, int* real_rep, long* dimT, double* W, double* T);
 /* Total sites over all replicas. */
 int64_t ghx_volume(const ghx_ctx* ctx);
+/* The device's Gaussian stream: out[i] = Rng(seed).gauss(key0 + i) for
+ * i < n (Rng::gauss, rng.hpp:42-46), drawn on the current HIP device by the
+ * code the momentum refresh and the hot starts use, copied to host memory. */
+int ghx_rng_gauss(uint64_t seed, uint64_t key0, int64_t n, double* out);
 
 #ifdef __cplusplus
 }

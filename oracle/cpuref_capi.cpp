@@ -4,6 +4,7 @@
 //   U   [s*D+mu][i][j] complex (re,im)      (fields.hpp:13-16, linalg.hpp:17)
 //   P,F [s*D+mu][a] real                     (fields.hpp:47-50)
 //   phi [s*d+k] complex                      (scalar_field.hpp:12-18)
+#include <quadmath.h>
 #include "cpuref.hpp"
 #include <chrono>
 #include <cstring>
@@ -267,6 +268,26 @@ double orc_uniform(std::uint64_t seed, std::uint64_t k) { return draw_uniform(se
 double orc_gauss(std::uint64_t seed, std::uint64_t k) { return draw_gauss(seed, k); }
 void orc_gauss_many(std::uint64_t seed, const std::uint64_t* keys, long n, double* out) {
   for (long i = 0; i < n; ++i) out[i] = draw_gauss(seed, keys[i]);
+}
+// The Box-Muller draw with log and cos evaluated in quad precision
+// (libquadmath) and rounded to double: the correctly rounded draw, against
+// which a disagreement between the device and glibc is attributed.
+static double gauss_quad(std::uint64_t seed, std::uint64_t k) {
+  const double u1 = draw_uniform(seed, mix64(k * 2 + 1));
+  const double u2 = draw_uniform(seed, mix64(k * 2 + 2));
+  const double l = double(logq(__float128(u1))), c = double(cosq(__float128(2.0 * kPi * u2)));
+  return std::sqrt(-2.0 * l) * c;
+}
+void orc_gauss_quad_many(std::uint64_t seed, const std::uint64_t* keys, long n, double* out) {
+  for (long i = 0; i < n; ++i) out[i] = gauss_quad(seed, keys[i]);
+}
+// keys key0 .. key0+n-1 (OpenMP): the glibc draw and the correctly rounded draw
+void orc_gauss_range(std::uint64_t seed, std::uint64_t key0, long n, double* glibc, double* quad) {
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < n; ++i) {
+    if (glibc) glibc[i] = draw_gauss(seed, key0 + std::uint64_t(i));
+    if (quad) quad[i] = gauss_quad(seed, key0 + std::uint64_t(i));
+  }
 }
 // exp(iH) for an N x N Hermitian H (reference algorithm: Taylor-18 or SU(2) closed form)
 void orc_expi(int N, const double* H, double* out) {

@@ -49,6 +49,8 @@ def lib():
             "orc_splitmix64": (u64, [u64]), "orc_key": (u64, [u64, u64, u64, u64, u64]),
             "orc_uniform": (d, [u64, u64]), "orc_gauss": (d, [u64, u64]),
             "orc_gauss_many": (None, [u64, C.POINTER(C.c_uint64), C.c_long, dp]),
+            "orc_gauss_quad_many": (None, [u64, C.POINTER(C.c_uint64), C.c_long, dp]),
+            "orc_gauss_range": (None, [u64, u64, C.c_long, dp, dp]),
             "orc_expi": (None, [i, dp, dp]), "orc_generators": (None, [i, dp]),
             "orc_time_md": (d, [vp, i]), "orc_time_kick_drift": (d, [vp, i, d]),
         }
@@ -265,6 +267,22 @@ def gauss_many(seed, keys):
     out = np.zeros(len(keys))
     lib().orc_gauss_many(seed, keys.ctypes.data_as(C.POINTER(C.c_uint64)), len(keys), _dp(out))
     return out
+
+
+def gauss_quad_many(seed, keys):
+    """The correctly rounded Box-Muller draws (log and cos in quad precision)."""
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    out = np.zeros(len(keys))
+    lib().orc_gauss_quad_many(seed, keys.ctypes.data_as(C.POINTER(C.c_uint64)), len(keys), _dp(out))
+    return out
+
+
+def gauss_range(seed, key0, n, glibc=True, quad=False):
+    """Draws of keys key0..key0+n-1: (glibc draws or None, correctly rounded draws or None)."""
+    g = np.zeros(n) if glibc else None
+    q = np.zeros(n) if quad else None
+    lib().orc_gauss_range(seed, key0, n, _dp(g) if glibc else None, _dp(q) if quad else None)
+    return g, q
 
 
 def expi(H):

@@ -45,6 +45,8 @@ def test_resume_replica_exchange(tmp_path):
     for k in range(3):
         assert a.lat.counters(k) == b.lat.counters(k)
         assert np.array_equal(a.lat.get("U", k), b.lat.get("U", k))
+    sa, sb = a.swap_stats(), b.swap_stats()
+    assert sa[0] == sb[0] and np.array_equal(sa[1], sb[1]) and np.array_equal(sa[2], sb[2])
 
 
 def test_mismatch_refused(tmp_path):

@@ -5,13 +5,15 @@ test_oracle_golden.py), so agreement here is agreement with the reference.
 Tolerances (FP64 mode): per-site forces, actions and single kick/drift steps
 to 1e-12 relative; fields after one MD trajectory to 1e-10 absolute; dH to
 1e-9 absolute (|H| ~ 1e3-1e4, so this is ~1e-13 of H). Integer RNG keys are
-exact; Gaussians use the device log/cos, compared at 1 ulp.
+exact; Gaussian draws (phi, P, pi) are equal except where glibc misrounds log
+or cos, where the device holds the correctly rounded draw (gauss_policy.py).
 """
 import numpy as np
 import pytest
 
 import gauge_higgs_amd as gha
 import oracle_lib as O
+from gauss_policy import TAG_P, TAG_P_PURE, TAG_PI, assert_equal_but_glibc_misrounds, momentum_keys, scalar_keys
 
 pytestmark = pytest.mark.gpu
 
@@ -56,7 +58,9 @@ def test_start_fields(case):
     g, o = make_pair(case)
     assert np.max(np.abs(g.get("U") - o.get("U"))) < 1e-14
     if case[3]:
-        assert np.max(np.abs(g.get("phi") - o.get("phi"))) < 1e-15
+        d = g.d
+        assert_equal_but_glibc_misrounds(g.get("phi").view(np.float64), o.get("phi").view(np.float64),
+                                         lambda i: scalar_keys(i, d, 12345), 1, 0.3, case[0] + " phi")
 
 
 @pytest.mark.parametrize("case", CASES, ids=IDS)
@@ -64,11 +68,14 @@ def test_momentum_refresh(case):
     g, o = make_pair(case)
     g.refresh_momenta()
     o.refresh()
-    P, Po = g.get("P"), o.get("P")
-    # integer keys/uniforms exact; Box-Muller through the device log/cos: <= 2 ulp
-    assert np.max(np.abs(P - Po) / np.maximum(np.abs(Po), 1e-300)) < 5e-16
-    if case[3]:
-        assert np.max(np.abs(g.get("pi") - o.get("pi"))) < 1e-15 * max(1, np.max(np.abs(o.get("pi"))))
+    _, D, N, rep = case[:4]
+    sP = O.key(TAG_P if rep else TAG_P_PURE, 0)
+    assert_equal_but_glibc_misrounds(g.get("P"), o.get("P"), lambda i: momentum_keys(i, D, N * N - 1, sP), 1,
+                                     what=case[0] + " P")
+    if rep:
+        d, sPi = g.d, O.key(TAG_PI, 0)
+        assert_equal_but_glibc_misrounds(g.get("pi").view(np.float64), o.get("pi").view(np.float64),
+                                         lambda i: scalar_keys(i, d, sPi), 1, what=case[0] + " pi")
 
 
 @pytest.mark.parametrize("case", CASES, ids=IDS)
