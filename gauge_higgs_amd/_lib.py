@@ -68,6 +68,7 @@ def load():
         "ghx_normalize_phi": [vp],
         "ghx_surrogate_init": [vp, u64],
         "ghx_rng_gauss": [u64, u64, C.c_int64, dp],
+        "ghx_rng_gauss_noalloc": [u64, u64, C.c_int64, dp],
         "ghx_surrogate_enable": [vp, i],
         "ghx_surrogate_nparams": [],
         "ghx_surrogate_params": [vp, i, C.POINTER(C.c_float)],

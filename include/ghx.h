@@ -232,6 +232,13 @@ int64_t ghx_volume(const ghx_ctx* ctx);
  * i < n (Rng::gauss, rng.hpp:42-46), drawn on the current HIP device by the
  * code the momentum refresh and the hot starts use, copied to host memory. */
 int ghx_rng_gauss(uint64_t seed, uint64_t key0, int64_t n, double* out);
+/* ghx_rng_gauss without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc); no counterpart in the reference. The draws are staged in
+ * a file-scope device array of 2^20 doubles (8 MiB per GPU, held until the
+ * process ends): n above 2^20 gives GHX_E_INVALID. Calls that overlap on one
+ * GPU share that array and race. */
+int ghx_rng_gauss_noalloc(uint64_t seed, uint64_t key0, int64_t n, double* out);
 
 #ifdef __cplusplus
 }
