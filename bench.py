@@ -81,21 +81,44 @@ def copy_calibration(mib):
     return 0
 
 
-def cpu_baseline(workload, sample_steps=16):
-    """Reference (compiled from its own sources by oracle/Makefile) on this host."""
+def physical_cores():
+    """Physical cores of this host from lscpu (cores per socket x sockets), or None."""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout
+        f = {k.strip(): v.strip() for k, v in (line.split(":", 1) for line in out.splitlines() if ":" in line)}
+        return int(f["Core(s) per socket"]) * int(f["Socket(s)"])
+    except Exception:
+        return None
+
+
+def cpu_baseline(workload, sample_steps=16, single_L=16, single_steps=8):
+    """The reference (compiled from its own sources by oracle/Makefile) on this
+    host's cores: OMP threads = the physical cores lscpu reports, capped by the
+    CPUs this process may run on and by OMP_NUM_THREADS when the environment
+    sets it; plus a one-thread sample on a smaller lattice."""
     D, N, rep, L, *_ = WORKLOADS[workload]
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
-    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close")
+    phys = physical_cores()
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env_cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(v for v in (phys, avail, env_cap or None) if v)
     harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
     if workload == "su3_sextet_L32_fp32" and os.access(harness, os.X_OK):
-        out = subprocess.run([harness, "bench", str(L), str(sample_steps)], capture_output=True, text=True, env=env, timeout=600)
-        r = json.loads(out.stdout.strip().splitlines()[-1])
+        def run(nthreads, lat, steps):
+            env = dict(os.environ, OMP_NUM_THREADS=str(nthreads), OMP_PROC_BIND="close")
+            out = subprocess.run([harness, "bench", str(lat), str(steps)], capture_output=True, text=True, env=env, timeout=600)
+            return json.loads(out.stdout.strip().splitlines()[-1])
+        r = run(threads, L, sample_steps)
+        r1 = run(1, single_L, single_steps)
         return {"value": r["site_updates_per_s"], "unit": "site-updates/s", "cores": threads, "kind": "reference",
+                "physical_cores_lscpu": phys, "cpus_available": avail, "omp_num_threads_env": env_cap or None,
+                "one_thread": {"value": r1["site_updates_per_s"], "sample": f"{single_L}^4, {single_steps} drift+kick MD steps "
+                                                                             f"({r1['seconds']:.1f} s), OMP_NUM_THREADS=1"},
                 "sample": f"reference GaugeHiggsHMC<4,3> + GeneralRep{{2}} FP64, {L}^4, {sample_steps} drift+kick MD steps "
-                          f"({r['seconds']:.1f} s), OMP_NUM_THREADS={threads}"}
+                          f"({r['seconds']:.1f} s), OMP_NUM_THREADS={threads} (lscpu physical cores {phys}, "
+                          f"{avail} CPUs available to the process)"}
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
-    os.environ["OMP_NUM_THREADS"] = str(threads)
+    os.environ["OMP_NUM_THREADS"] = str(threads)  # read when the oracle library first starts OpenMP
     o = O.Oracle(D, N, L, rep, seed=1)
     o.couplings(beta=WORKLOADS[workload][4], kappa=WORKLOADS[workload][5], lam=0.5, nmd=20)
     o.hot(0.8)
