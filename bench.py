@@ -239,7 +239,7 @@ def main():
                     help="force the block walk of the MD-step kernel (implies --no-tune)")
     ap.add_argument("--no-learned", action="store_true", help="skip the exact vs learned-force HMC comparison")
     ap.add_argument("--only-learned", action="store_true", help="run only the exact vs learned-force comparison")
-    ap.add_argument("--learned-meas", type=int, default=24, help="measured trajectories per chain in the comparison")
+    ap.add_argument("--learned-meas", type=int, default=200, help="measured trajectories per chain in the comparison")
     ap.add_argument("--no-graph", action="store_true", help="launch trajectories directly instead of replaying hipGraphs")
     ap.add_argument("--no-replicas", action="store_true", help="skip the 32-chain kappa-scan block")
     ap.add_argument("--no-precision", action="store_true", help="skip the FP32-vs-FP64 block (SU(2)+fund 16^4)")

@@ -36,16 +36,18 @@ GHX_DEV short to_bf16(float f) {  // round to nearest even
 }
 GHX_DEV f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
 GHX_DEV f32x4 zero4() { return f32x4{0.f, 0.f, 0.f, 0.f}; }
+// The weights are kept in BF16 (Pb, rounded once after every update of the FP32
+// master copy P), so the MFMA operands are loaded as they are used.
 // B operand from row-major W[K][ld]: lane (i, j) takes W[k0 + 4j + r][n0 + i]
-GHX_DEV bf16x4 bop(const float* W, int ld, int k0, int n0, int i, int j) {
+GHX_DEV bf16x4 bop(const short* W, int ld, int k0, int n0, int i, int j) {
   bf16x4 b;
-  for (int r = 0; r < 4; ++r) b[r] = to_bf16(W[(k0 + 4 * j + r) * ld + n0 + i]);
+  for (int r = 0; r < 4; ++r) b[r] = W[(k0 + 4 * j + r) * ld + n0 + i];
   return b;
 }
 // B operand from the transpose: lane (i, j) takes W[n0 + i][k0 + 4j + r]
-GHX_DEV bf16x4 bopT(const float* W, int ld, int k0, int n0, int i, int j) {
+GHX_DEV bf16x4 bopT(const short* W, int ld, int k0, int n0, int i, int j) {
   bf16x4 b;
-  for (int r = 0; r < 4; ++r) b[r] = to_bf16(W[(n0 + i) * ld + k0 + 4 * j + r]);
+  for (int r = 0; r < 4; ++r) b[r] = W[(n0 + i) * ld + k0 + 4 * j + r];
   return b;
 }
 
@@ -159,7 +161,8 @@ struct MlpTile {
 };
 
 // Forward of one 16-row tile starting at row0 (rows >= nrows read as zero).
-GHX_DEV void mlp_forward(const float* __restrict__ P, const float* __restrict__ X, size_t row0, size_t nrows, float* lds, MlpTile& t) {
+GHX_DEV void mlp_forward(const float* __restrict__ P, const short* __restrict__ Pb, const float* __restrict__ X, size_t row0,
+                         size_t nrows, float* lds, MlpTile& t) {
   const int lane = threadIdx.x & 63, i = lane & 15, j = lane >> 4;
   bf16x4 a1;
   {
@@ -167,7 +170,7 @@ GHX_DEV void mlp_forward(const float* __restrict__ P, const float* __restrict__ 
     for (int r = 0; r < 4; ++r) a1[r] = to_bf16(row < nrows ? X[row * kNF + 4 * j + r] : 0.f);
   }
   for (int n = 0; n < 4; ++n) {
-    f32x4 z = mfma(a1, bop(P + OFF_W1, kH, 0, 16 * n, i, j), zero4());
+    f32x4 z = mfma(a1, bop(Pb + OFF_W1, kH, 0, 16 * n, i, j), zero4());
     for (int r = 0; r < 4; ++r) t.h1[n][r] = tanhf(z[r] + P[OFF_B1 + 16 * n + i]);
   }
   // C layout -> A layout through the wave's LDS tile [16][64]
@@ -180,7 +183,7 @@ GHX_DEV void mlp_forward(const float* __restrict__ P, const float* __restrict__ 
   __syncthreads();
   for (int n = 0; n < 4; ++n) {
     f32x4 z = zero4();
-    for (int k = 0; k < 4; ++k) z = mfma(a2[k], bop(P + OFF_W2, kH, 16 * k, 16 * n, i, j), z);
+    for (int k = 0; k < 4; ++k) z = mfma(a2[k], bop(Pb + OFF_W2, kH, 16 * k, 16 * n, i, j), z);
     for (int r = 0; r < 4; ++r) t.h2[n][r] = tanhf(z[r] + P[OFF_B2 + 16 * n + i]);
   }
   for (int n = 0; n < 4; ++n)
@@ -191,14 +194,14 @@ GHX_DEV void mlp_forward(const float* __restrict__ P, const float* __restrict__ 
     for (int r = 0; r < 4; ++r) a3[k][r] = to_bf16(lds[i * kH + 16 * k + 4 * j + r]);
   __syncthreads();
   f32x4 z = zero4();
-  for (int k = 0; k < 4; ++k) z = mfma(a3[k], bop(P + OFF_W3, kNO, 16 * k, 0, i, j), z);
+  for (int k = 0; k < 4; ++k) z = mfma(a3[k], bop(Pb + OFF_W3, kNO, 16 * k, 0, i, j), z);
   for (int r = 0; r < 4; ++r) t.c[r] = z[r] + P[OFF_B3 + i];
 }
 
 // Inference: S planes (link force, storage type T) for all links.
 template <typename T, int NGN>
-__global__ __launch_bounds__(kMlpBlock) void surr_apply_kernel(const float* P, const float* X, const float* B, size_t nrows, unsigned Vp,
-                                                               T* S) {
+__global__ __launch_bounds__(kMlpBlock) void surr_apply_kernel(const float* P, const short* Pb, const float* X, const float* B,
+                                                               size_t nrows, unsigned Vp, T* S) {
   __shared__ float lds_all[4][16 * kH];
   __shared__ float cb[4][16][kNB];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, i = lane & 15, j = lane >> 4;
@@ -206,7 +209,7 @@ __global__ __launch_bounds__(kMlpBlock) void surr_apply_kernel(const float* P, c
   for (size_t base = size_t(blockIdx.x) * 4; base < ntiles; base += size_t(gridDim.x) * 4) {
     const size_t tile = base + w;
     MlpTile t;
-    mlp_forward(P, X, tile * 16, tile < ntiles ? nrows : 0, lds_all[w], t);
+    mlp_forward(P, Pb, X, tile * 16, tile < ntiles ? nrows : 0, lds_all[w], t);
     if (i < kNB)
       for (int r = 0; r < 4; ++r) cb[w][4 * j + r][i] = t.c[r];
     __syncthreads();
@@ -230,8 +233,9 @@ __global__ __launch_bounds__(kMlpBlock) void surr_apply_kernel(const float* P, c
 //   L = (1/nvalid) sum_rows sum_a (sum_k c_k B_k^a - F^a)^2
 // over its tiles, then write the partial gradient (NPARAM floats) and loss.
 template <typename T, int NGN>
-__global__ __launch_bounds__(kMlpBlock) void surr_grad_kernel(const float* P, const float* X, const float* B, const T* F, size_t nrows,
-                                                              unsigned Vp, double inv_n, float* part, double* lpart) {
+__global__ __launch_bounds__(kMlpBlock) void surr_grad_kernel(const float* P, const short* Pb, const float* X, const float* B,
+                                                              const T* F, size_t nrows, unsigned Vp, double inv_n, float* part,
+                                                              double* lpart) {
   __shared__ float lds_all[4][16 * kH];
   __shared__ float g3s[4][16][16];
   __shared__ float cb[4][16][kNB];
@@ -247,7 +251,7 @@ __global__ __launch_bounds__(kMlpBlock) void surr_grad_kernel(const float* P, co
     const size_t row0 = tile * 16;
     const size_t nr = tile < ntiles ? nrows : 0;
     MlpTile t;
-    mlp_forward(P, X, row0, nr, lds, t);
+    mlp_forward(P, Pb, X, row0, nr, lds, t);
     if (i < kNB)
       for (int r = 0; r < 4; ++r) cb[w][4 * j + r][i] = t.c[r];
     __syncthreads();
@@ -285,7 +289,7 @@ __global__ __launch_bounds__(kMlpBlock) void surr_grad_kernel(const float* P, co
     // dH2 = g3 W3^T ; dZ2 = dH2 (1 - H2^2)
     f32x4 dz2[4];
     for (int n = 0; n < 4; ++n) {
-      f32x4 z = mfma(g3a, bopT(P + OFF_W3, kNO, 0, 16 * n, i, j), zero4());
+      f32x4 z = mfma(g3a, bopT(Pb + OFF_W3, kNO, 0, 16 * n, i, j), zero4());
       for (int r = 0; r < 4; ++r) { dz2[n][r] = z[r] * (1.f - t.h2[n][r] * t.h2[n][r]); db2[n] += dz2[n][r]; }
     }
     // dW2 += H1^T dZ2
@@ -308,7 +312,7 @@ __global__ __launch_bounds__(kMlpBlock) void surr_grad_kernel(const float* P, co
     __syncthreads();
     for (int n = 0; n < 4; ++n) {
       f32x4 z = zero4();
-      for (int k = 0; k < 4; ++k) z = mfma(a2[k], bopT(P + OFF_W2, kH, 16 * k, 16 * n, i, j), z);
+      for (int k = 0; k < 4; ++k) z = mfma(a2[k], bopT(Pb + OFF_W2, kH, 16 * k, 16 * n, i, j), z);
       f32x4 dz1;
       for (int r = 0; r < 4; ++r) { dz1[r] = z[r] * (1.f - t.h1[n][r] * t.h1[n][r]); db1[n] += dz1[r]; }
       // dW1 += X^T dZ1 (X^T in A layout: lane (i, j) takes X[row 4j + r][i])
@@ -345,35 +349,65 @@ __global__ __launch_bounds__(kMlpBlock) void surr_grad_kernel(const float* P, co
   if (lane == 0) lpart[wid] = loss;
 }
 
-__global__ void surr_reduce_kernel(const float* part, const double* lpart, int nw, float* grad, double* loss) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+// Sum of the per-wave partial gradients in two passes with a fixed order (the
+// result does not depend on timing): pass 1, grid (parameter chunks x
+// kRedSplit), sums a contiguous range of waves per parameter; pass 2 sums the
+// kRedSplit partial sums per parameter, and block 0 reduces the loss partials
+// as a tree in LDS.
+constexpr int kRedSplit = 32;
+__global__ __launch_bounds__(256) void surr_reduce1_kernel(const float* part, int nw, float* part2) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  const int per = (nw + kRedSplit - 1) / kRedSplit, w0 = blockIdx.y * per, w1 = min(nw, w0 + per);
+  if (p >= NPARAM) return;
+  float s = 0.f;
+  for (int w = w0; w < w1; ++w) s += part[size_t(w) * NPARAM + p];
+  part2[size_t(blockIdx.y) * NPARAM + p] = s;
+}
+__global__ __launch_bounds__(256) void surr_reduce2_kernel(const float* part2, const double* lpart, int nw, float* grad, double* loss) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
   if (p < NPARAM) {
     float s = 0.f;
-    for (int w = 0; w < nw; ++w) s += part[size_t(w) * NPARAM + p];
+    for (int k = 0; k < kRedSplit; ++k) s += part2[size_t(k) * NPARAM + p];
     grad[p] = s;
   }
-  if (p == 0) {
+  if (blockIdx.x == 0) {
+    __shared__ double red[256];
     double l = 0.0;
-    for (int w = 0; w < nw; ++w) l += lpart[w];
-    *loss = l;
+    for (int w = threadIdx.x; w < nw; w += 256) l += lpart[w];
+    red[threadIdx.x] = l;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if (int(threadIdx.x) < o) red[threadIdx.x] += red[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) *loss = red[0];
   }
 }
 
-__global__ void surr_adam_kernel(float* P, float* m, float* v, const float* grad, float lr, float b1t, float b2t) {
+// BF16 copy of the weights (biases stay FP32 in P).
+__global__ void surr_bf16_kernel(const float* P, short* Pb) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < NPARAM) Pb[p] = to_bf16(P[p]);
+}
+
+__global__ void surr_adam_kernel(float* P, short* Pb, float* m, float* v, const float* grad, float lr, float b1t, float b2t) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= NPARAM) return;
   const float g = grad[p];
   const float mm = 0.9f * m[p] + 0.1f * g, vv = 0.999f * v[p] + 0.001f * g * g;
   m[p] = mm;
   v[p] = vv;
-  P[p] -= lr * (mm / (1.f - b1t)) / (sqrtf(vv / (1.f - b2t)) + 1e-8f);
+  const float np = P[p] - lr * (mm / (1.f - b1t)) / (sqrtf(vv / (1.f - b2t)) + 1e-8f);
+  P[p] = np;
+  Pb[p] = to_bf16(np);
 }
 
 }  // namespace ghx
 
 // ---------------------------------------------------------------- host side
 struct SurrogateState {
-  float *P = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr, *part = nullptr;
+  float *P = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr, *part = nullptr, *part2 = nullptr;
+  short* Pb = nullptr;
   double *lpart = nullptr, *loss = nullptr;
   float *X = nullptr, *B = nullptr, *Q = nullptr;
   TEntry* te = nullptr;
@@ -407,9 +441,11 @@ template <typename T>
 void apply_T(ghx_ctx* c) {
   SurrogateState* st = c->surr_state;
   if (c->N == 3)
-    hipLaunchKernelGGL((surr_apply_kernel<T, 8>), dim3(st->nblocks), dim3(kMlpBlock), 0, c->stream, st->P, st->X, st->B, st->nrows, c->Vp, (T*)c->S);
+    hipLaunchKernelGGL((surr_apply_kernel<T, 8>), dim3(st->nblocks), dim3(kMlpBlock), 0, c->stream, st->P, st->Pb, st->X, st->B, st->nrows, c->Vp,
+                       (T*)c->S);
   else
-    hipLaunchKernelGGL((surr_apply_kernel<T, 3>), dim3(st->nblocks), dim3(kMlpBlock), 0, c->stream, st->P, st->X, st->B, st->nrows, c->Vp, (T*)c->S);
+    hipLaunchKernelGGL((surr_apply_kernel<T, 3>), dim3(st->nblocks), dim3(kMlpBlock), 0, c->stream, st->P, st->Pb, st->X, st->B, st->nrows, c->Vp,
+                       (T*)c->S);
 }
 
 // S planes from the fields of buffer b (used before inner/outer kicks).
@@ -422,13 +458,15 @@ template <typename T>
 void grad_T(ghx_ctx* c, double inv_n) {
   SurrogateState* st = c->surr_state;
   if (c->N == 3)
-    hipLaunchKernelGGL((surr_grad_kernel<T, 8>), dim3(st->nblocks), dim3(kMlpBlock), 0, c->stream, st->P, st->X, st->B, (const T*)c->F,
+    hipLaunchKernelGGL((surr_grad_kernel<T, 8>), dim3(st->nblocks), dim3(kMlpBlock), 0, c->stream, st->P, st->Pb, st->X, st->B, (const T*)c->F,
                        st->nrows, c->Vp, inv_n, st->part, st->lpart);
   else
-    hipLaunchKernelGGL((surr_grad_kernel<T, 3>), dim3(st->nblocks), dim3(kMlpBlock), 0, c->stream, st->P, st->X, st->B, (const T*)c->F,
+    hipLaunchKernelGGL((surr_grad_kernel<T, 3>), dim3(st->nblocks), dim3(kMlpBlock), 0, c->stream, st->P, st->Pb, st->X, st->B, (const T*)c->F,
                        st->nrows, c->Vp, inv_n, st->part, st->lpart);
-  hipLaunchKernelGGL(surr_reduce_kernel, dim3((NPARAM + 255) / 256), dim3(256), 0, c->stream, st->part, st->lpart, st->nwaves, st->grad,
-                     st->loss);
+  hipLaunchKernelGGL(surr_reduce1_kernel, dim3((NPARAM + 255) / 256, kRedSplit), dim3(256), 0, c->stream, st->part, st->nwaves,
+                     st->part2);
+  hipLaunchKernelGGL(surr_reduce2_kernel, dim3((NPARAM + 255) / 256), dim3(256), 0, c->stream, st->part2, st->lpart, st->nwaves,
+                     st->grad, st->loss);
 }
 
 // Targets (exact matter link force into ctx->F) and features of the current fields.
@@ -449,7 +487,8 @@ namespace ghx {
 void surrogate_free(ghx_ctx* c) {
   SurrogateState* st = c->surr_state;
   if (!st) return;
-  for (void* p : {(void*)st->P, (void*)st->m, (void*)st->v, (void*)st->grad, (void*)st->part, (void*)st->lpart, (void*)st->loss,
+  for (void* p : {(void*)st->P, (void*)st->Pb, (void*)st->part2, (void*)st->m, (void*)st->v, (void*)st->grad, (void*)st->part,
+                  (void*)st->lpart, (void*)st->loss,
                   (void*)st->X, (void*)st->B, (void*)st->Q, (void*)st->te, (void*)st->te_off})
     if (p) (void)hipFree(p);
   if (c->S) { (void)hipFree(c->S); c->S = nullptr; }
@@ -485,6 +524,8 @@ int ghx_surrogate_init(ghx_ctx* ctx, uint64_t seed) {
   SCHK(hipMalloc((void**)&st->v, NPARAM * 4));
   SCHK(hipMalloc((void**)&st->grad, NPARAM * 4));
   SCHK(hipMalloc((void**)&st->part, size_t(st->nwaves) * NPARAM * 4));
+  SCHK(hipMalloc((void**)&st->part2, size_t(kRedSplit) * NPARAM * 4));
+  SCHK(hipMalloc((void**)&st->Pb, NPARAM * 2));
   SCHK(hipMalloc((void**)&st->lpart, size_t(st->nwaves) * 8));
   SCHK(hipMalloc((void**)&st->loss, 8));
   SCHK(hipMalloc((void**)&st->X, st->nrows * kNF * 4));
@@ -525,6 +566,9 @@ int ghx_surrogate_init(ghx_ctx* ctx, uint64_t seed) {
   for (int r = 0; r < kH; ++r)
     for (int o = 0; o < kNB; ++o) P[OFF_W3 + r * kNO + o] = float(0.01 * gauss(k++) / std::sqrt(double(kH)));
   SCHK(hipMemcpy(st->P, P.data(), NPARAM * 4, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(surr_bf16_kernel, dim3((NPARAM + 255) / 256), dim3(256), 0, ctx->stream, st->P, st->Pb);
+  SCHK(hipGetLastError());
+  SCHK(hipStreamSynchronize(ctx->stream));
   ctx->surr_fill = &surr_fill;
   return GHX_OK;
 }
@@ -544,7 +588,12 @@ int ghx_surrogate_params(ghx_ctx* ctx, int get, float* params) {
   SurrogateState* st = ctx->surr_state;
   SCHK(hipStreamSynchronize(ctx->stream));
   if (get) SCHK(hipMemcpy(params, st->P, NPARAM * 4, hipMemcpyDeviceToHost));
-  else SCHK(hipMemcpy(st->P, params, NPARAM * 4, hipMemcpyHostToDevice));
+  else {
+    SCHK(hipMemcpy(st->P, params, NPARAM * 4, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(surr_bf16_kernel, dim3((NPARAM + 255) / 256), dim3(256), 0, ctx->stream, st->P, st->Pb);
+    SCHK(hipGetLastError());
+    SCHK(hipStreamSynchronize(ctx->stream));
+  }
   return GHX_OK;
 }
 
@@ -576,8 +625,8 @@ int ghx_surrogate_train(ghx_ctx* ctx, int iters, double lr, double* loss_first, 
     if (it == 0 && loss_first) { SCHK(hipStreamSynchronize(ctx->stream)); SCHK(hipMemcpy(loss_first, st->loss, 8, hipMemcpyDeviceToHost)); }
     ++st->step;
     const float b1t = float(std::pow(0.9, double(st->step))), b2t = float(std::pow(0.999, double(st->step)));
-    hipLaunchKernelGGL(surr_adam_kernel, dim3((NPARAM + 255) / 256), dim3(256), 0, ctx->stream, st->P, st->m, st->v, st->grad, float(lr),
-                       b1t, b2t);
+    hipLaunchKernelGGL(surr_adam_kernel, dim3((NPARAM + 255) / 256), dim3(256), 0, ctx->stream, st->P, st->Pb, st->m, st->v, st->grad,
+                       float(lr), b1t, b2t);
   }
   if (loss_last) {
     if (ctx->tsize == 4) grad_T<float>(ctx, inv_n); else grad_T<double>(ctx, inv_n);

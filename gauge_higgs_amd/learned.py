@@ -42,7 +42,8 @@ def _chain(g, n, record=True, use_graph=True):
             o = g.observables()
             obs.append((o["plaq"], o["Llink"]))
     dhs = np.array(dhs)
-    out = {"trajectories": n, "gpu_s_per_traj": t / n, "acceptance": acc / n,
+    a = acc / n
+    out = {"trajectories": n, "gpu_s_per_traj": t / n, "acceptance": a, "acceptance_err": float(np.sqrt(max(a * (1 - a), 1.0 / n) / n)),
            "exp_minus_dH": float(np.mean(np.exp(-np.clip(dhs, -50, 50)))),
            "exp_minus_dH_err": float(np.std(np.exp(-np.clip(dhs, -50, 50))) / np.sqrt(n)),
            "mean_abs_dH": float(np.mean(np.abs(dhs)))}
@@ -51,14 +52,45 @@ def _chain(g, n, record=True, use_graph=True):
         out.update(plaq=float(ob[:, 0].mean()), plaq_err=float(ob[:, 0].std() / np.sqrt(n)),
                    Llink=float(ob[:, 1].mean()), Llink_err=float(ob[:, 1].std() / np.sqrt(n)))
     out["gpu_s_per_accepted"] = out["gpu_s_per_traj"] / max(out["acceptance"], 1e-9)
+    out["gpu_s_per_accepted_err"] = out["gpu_s_per_accepted"] * out["acceptance_err"] / max(out["acceptance"], 1e-9)
+    return out
+
+
+def _ratio(a, b):
+    """a / b for two chain summaries' GPU-s per accepted trajectory, with the
+    error from both (independent chains, relative errors in quadrature)."""
+    x, y = a["gpu_s_per_accepted"], b["gpu_s_per_accepted"]
+    r = x / y
+    return {"ratio": r, "err": r * float(np.hypot(a["gpu_s_per_accepted_err"] / x, b["gpu_s_per_accepted_err"] / y))}
+
+
+def basis_bound(B, F, subsets=((0,), (0, 1), (0, 1, 2), (0, 1, 2, 3))):
+    """Fraction of |F|^2 left unexplained when every link's force F is replaced
+    by its least-squares projection onto the span of its own basis vectors
+    B[link, k, :] (k in the subset): the residual of a network that picked the
+    best coefficients for every link, a lower bound for any surrogate over these
+    bases. Also the residual of one global coefficient per basis."""
+    F = np.asarray(F, dtype=np.float64)
+    out = {}
+    fn = float(np.sum(F ** 2))
+    for sub in subsets:
+        Bs = np.asarray(B, dtype=np.float64)[:, list(sub), :]          # [links, k, a]
+        A = np.transpose(Bs, (0, 2, 1))                                # [links, a, k]
+        G = np.einsum("lak,lam->lkm", A, A) + 1e-30 * np.eye(len(sub))
+        c = np.linalg.solve(G, np.einsum("lak,la->lk", A, F)[..., None])[..., 0]
+        r_link = float(np.sum((F - np.einsum("lak,lk->la", A, c)) ** 2)) / fn
+        Ag = A.reshape(-1, len(sub))
+        cg, *_ = np.linalg.lstsq(Ag, F.reshape(-1), rcond=None)
+        r_glob = float(np.sum((F.reshape(-1) - Ag @ cg) ** 2)) / fn
+        out["B" + "".join(str(k) for k in sub)] = {"per_link_optimum": r_link, "global_coefficients": r_glob}
     return out
 
 
 RESIDUAL_GATE = 0.9  # use S only if it explains > 10 % of |F|^2 on a held-out configuration
 
 
-def compare(L=12, rep="6,1", beta=5.7, kappa=0.05, lam=0.5, nmd=16, exact_nmds=(16, 32, 48), n_therm=40, n_meas=24,
-            splits=((5, 10), (8, 6)), train_configs=8, train_iters=200, lr=2e-3, seed=1, precision=_h.FP32MD,
+def compare(L=12, rep="6,1", beta=5.7, kappa=0.05, lam=0.5, nmd=16, exact_nmds=(16, 20, 24, 28, 32, 40), n_therm=40, n_meas=200,
+            splits=((5, 10),), train_configs=8, train_iters=200, lr=2e-3, seed=1, precision=_h.FP32MD,
             therm_split=(12, 6), warm=(0.4, 0.27), use_graph=True):
     """Cost per accepted trajectory of exact-force HMC and of the multi-timescale
     chain with and without the learned surrogate, from one thermalised
@@ -99,8 +131,9 @@ def compare(L=12, rep="6,1", beta=5.7, kappa=0.05, lam=0.5, nmd=16, exact_nmds=(
     train_s = time.perf_counter() - t0
     g.lat.trajectories(1, use_graph=use_graph)
     # held-out residual: fraction of |F|^2 the surrogate leaves unexplained on a new configuration
-    _, _, S, F = g.lat.surrogate_snapshot()
+    _, Bh, S, F = g.lat.surrogate_snapshot()
     resid = float(np.sum((S - F) ** 2) / np.sum(F ** 2))
+    bound = basis_bound(Bh.reshape(F.shape[0], 4, -1), F)
     U0, phi0 = g.get("U"), g.get("phi")
 
     def restart():
@@ -131,15 +164,15 @@ def compare(L=12, rep="6,1", beta=5.7, kappa=0.05, lam=0.5, nmd=16, exact_nmds=(
     g.lat.close()
     out["training"] = {"configs": train_configs, "pairs": train_configs * 4 * L ** 4, "adam_steps": train_configs * train_iters,
                        "seconds": train_s, "loss_first": losses[0][0], "loss_last": losses[-1][1],
-                       "heldout_residual_fraction": resid}
+                       "heldout_residual_fraction": resid, "basis_bound": bound}
     use_s = resid < RESIDUAL_GATE
+    key = "omelyan_outer%d_inner%d" % splits[0]
     out["learned"] = {"surrogate_used": use_s, "gate": f"held-out residual {resid:.3f} vs {RESIDUAL_GATE}",
-                      "chains": "surrogate_on" if use_s else "split_no_surrogate"}
-    learned = out["surrogate_on" if use_s else "split_no_surrogate"]
-    cost = lambda d: min(v["gpu_s_per_accepted"] for v in d.values())
-    ref = out["exact"].get(f"omelyan_nmd{nmd}", {}).get("gpu_s_per_accepted")
-    out["learned"]["gpu_s_per_accepted"] = cost(learned)
-    out["speedup_vs_reference_nmd"] = ref / cost(learned) if ref else None
-    out["speedup_vs_best_exact"] = cost(out["exact"]) / cost(learned)
-    out["surrogate_gain"] = cost(out["split_no_surrogate"]) / cost(out["surrogate_on"])
+                      "chain": ("surrogate_on" if use_s else "split_no_surrogate") + "/" + key}
+    learned = out["surrogate_on" if use_s else "split_no_surrogate"][key]
+    out["learned"]["gpu_s_per_accepted"] = learned["gpu_s_per_accepted"]
+    out["learned"]["gpu_s_per_accepted_err"] = learned["gpu_s_per_accepted_err"]
+    # every exact chain against the learned one; no selection of the best chain
+    out["exact_over_learned"] = {k: _ratio(v, learned) for k, v in out["exact"].items()}
+    out["surrogate_gain"] = _ratio(out["split_no_surrogate"][key], out["surrogate_on"][key])
     return out

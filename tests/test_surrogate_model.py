@@ -28,3 +28,28 @@ def test_numpy_gradient_by_finite_differences():
         e[p] = 1e-6
         fd = (model(P + e, X, B, F, ident)[1] - model(P - e, X, B, F, ident)[1]) / 2e-6
         assert abs(fd - grad[p]) < 1e-6 * max(1.0, abs(fd)), p
+
+
+def test_basis_bound_limits():
+    """The per-link projection bound: 0 when every force lies in the span of
+    its link's bases, 1 when it is orthogonal to them, and between the two
+    (never above the global-coefficient residual) otherwise."""
+    from gauge_higgs_amd.learned import basis_bound
+    rng = np.random.default_rng(3)
+    B = rng.normal(size=(500, 4, 8))
+    c = rng.normal(size=(500, 4))
+    F_in = np.einsum("lk,lka->la", c, B)
+    r = basis_bound(B, F_in)
+    assert r["B0123"]["per_link_optimum"] < 1e-20
+    assert r["B0"]["per_link_optimum"] > 0.1
+    # orthogonal complement of each link's span
+    F_out = rng.normal(size=(500, 8))
+    for l in range(500):
+        q, _ = np.linalg.qr(B[l].T)
+        F_out[l] -= q @ (q.T @ F_out[l])
+    r = basis_bound(B, F_out)
+    assert abs(r["B0123"]["per_link_optimum"] - 1.0) < 1e-12
+    F = F_in + F_out
+    r = basis_bound(B, F)
+    for v in r.values():
+        assert 0.0 <= v["per_link_optimum"] <= v["global_coefficients"] + 1e-12
