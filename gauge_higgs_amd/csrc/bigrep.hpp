@@ -35,6 +35,7 @@ struct BigRepTables {
   struct Sym {
     int on = 0;           // 1: the engine replaces big_matter_kernel
     int s1 = 0, s2 = 1, dimS = 0, npass = 0;
+    int s1p = 0, s2p = 1;                 // LDS row strides of C1 and C2
     const int4* prog = nullptr;       // LDS program: npass x 64 entries {s0, s1, s2, dst | j}, LDS byte offsets
     const double* pre = nullptr;      // [2][dimS]: 1/F (forward hop) and F (adjoint), FP64 and FP32 copies
     const float* pref = nullptr;

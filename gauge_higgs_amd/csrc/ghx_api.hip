@@ -191,6 +191,7 @@ int upload_tables(ghx_ctx* ctx) {
     ctx->dsym.push_back(hopsL);
     BigRepTables::Sym& st = ctx->tb.sym;
     st.s1 = sb.s1; st.s2 = sb.s2; st.dimS = sb.dimS; st.npass = sp.npass; st.ng = sp.ng;
+    st.s1p = sp.s1p; st.s2p = sp.s2p;
     st.prog = (const int4*)dp[0];
     st.pre = (const double*)dp[1]; st.pref = (const float*)dp[2];
     st.Yp = (const double*)dp[3]; st.Ypf = (const float*)dp[4];
@@ -432,15 +433,26 @@ void launch_md(ghx_ctx* c) {
 }
 
 // Bring the current buffer back to buffer 0 (reunitarizing if requested).
+// Device-to-device copy of `bytes` (a multiple of 16: every field is whole
+// planes of Vp = 128 k sites) as a kernel, so that a captured trajectory graph
+// holds kernel nodes only (no memcpy nodes).
+__global__ __launch_bounds__(256) void copy16_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+  for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) dst[i] = src[i];
+}
+void stream_copy(ghx_ctx* c, void* dst, const void* src, size_t bytes) {
+  const size_t n = bytes / 16;
+  const unsigned nb = unsigned(std::min<size_t>((n + 255) / 256, 4096));
+  if (n) hipLaunchKernelGGL(copy16_kernel, dim3(nb), dim3(256), 0, c->stream, static_cast<const uint4*>(src), static_cast<uint4*>(dst), n);
+}
+
 void launch_settle(ghx_ctx* c, bool reunit) {
   const LaunchCtx l = c->lc();
   if (reunit) {
     c->ops.reunit(l, c->U[c->cur], c->U[0]);
   } else if (c->cur != 0) {
-    (void)hipMemcpyAsync(c->U[0], c->U[1], c->nU() * c->Vp * c->tsize, hipMemcpyDeviceToDevice, c->stream);
+    stream_copy(c, c->U[0], c->U[1], c->nU() * c->Vp * c->tsize);
   }
-  if (c->cur != 0 && has_phi(c))
-    (void)hipMemcpyAsync(c->phi[0], c->phi[1], c->nphi() * c->Vp * c->tsize, hipMemcpyDeviceToDevice, c->stream);
+  if (c->cur != 0 && has_phi(c)) stream_copy(c, c->phi[0], c->phi[1], c->nphi() * c->Vp * c->tsize);
   c->cur = 0;
 }
 
@@ -470,8 +482,8 @@ void launch_trajectory(ghx_ctx* c) {
   const bool hp = has_phi(c);
   c->ops.refresh(l, c->P, c->pi, c->mc, hp ? kTagP : kTagP_pure, kTagPi);
   if (c->frozen) c->ops.frozen_site(l, c->phi[c->cur], c->pi, 0);  // tangent momenta (gauge_higgs_hmc.hpp:172)
-  (void)hipMemcpyAsync(c->Usave, c->U[0], c->nU() * c->Vp * c->tsize, hipMemcpyDeviceToDevice, c->stream);
-  if (hp) (void)hipMemcpyAsync(c->phisave, c->phi[0], c->nphi() * c->Vp * c->tsize, hipMemcpyDeviceToDevice, c->stream);
+  stream_copy(c, c->Usave, c->U[0], c->nU() * c->Vp * c->tsize);
+  if (hp) stream_copy(c, c->phisave, c->phi[0], c->nphi() * c->Vp * c->tsize);
   launch_hamiltonian(c, 0, c->Hi);
   launch_md(c);
   launch_settle(c, c->reunit);

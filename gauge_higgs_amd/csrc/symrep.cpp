@@ -217,13 +217,16 @@ SymProgram sym_program(const SymBuild& s) {
   if (N > 3) return p;  // an entry has three sources and the link's columns 0..2 (3 is the copy column)
   auto sk = [&](int k) { return s.lvl_off[k + 1] - s.lvl_off[k]; };
   // level k lives in Ca when k1 - k is even, so that C_k1 ends in Ca
+  p.s1p = s.s1 | 1;  // odd row stride of C_k1
+  auto stride = [&](int k) { return k == k1 ? p.s1p : sk(k); };
   int ca = 0, cb = 0;
-  for (int k = 0; k <= k1; ++k) ((k1 - k) % 2 == 0 ? ca : cb) = std::max((k1 - k) % 2 == 0 ? ca : cb, sk(k) * sk(k));
+  for (int k = 0; k <= k1; ++k) ((k1 - k) % 2 == 0 ? ca : cb) = std::max((k1 - k) % 2 == 0 ? ca : cb, sk(k) * stride(k));
   const int Ca = kSymLdsBase, Cb = Ca + ca;
   auto base = [&](int k) { return (k1 - k) % 2 == 0 ? Ca : Cb; };
   const bool copy2 = k2 > 0 && k2 < k1;
   p.lds_S1 = Ca;
   p.lds_S2 = copy2 ? Cb + cb : (k2 == k1 ? Ca : 0);
+  p.s2p = (k2 == k1) ? p.s1p : s.s2;
   const int after = Cb + cb + (copy2 ? s.s2 * s.s2 : 0);
   // V, T1 and W hold both hops of a link, P = 4 ng entries per direction; W
   // takes V's place once C1 V is formed (or is T1 itself for one-row irreps)
@@ -251,7 +254,7 @@ SymProgram sym_program(const SymBuild& s) {
           const int ri = s.rowsrc[size_t(off + m) * N + i];
           if (ri >= 0) src[i] = base(k - 1) + ri * sp + np;
         }
-        pass.insert(pass.end(), {src[0], src[1], src[2], ((base(k) + m * n_k + n) << 2) | s.colj[off + n]});
+        pass.insert(pass.end(), {src[0], src[1], src[2], ((base(k) + m * stride(k) + n) << 2) | s.colj[off + n]});
         if (pass.size() == size_t(4 * 64)) flush(pass);
       }
     if (!pass.empty()) flush(pass);
@@ -316,9 +319,9 @@ SymProgram sym_program(const SymBuild& s) {
           for (int b = 0; b < s.s2; ++b)
             for (int a2 = 0; a2 < s.s1; ++a2)
               for (int b2 = 0; b2 < s.s2; ++b2) {
-                const cplx c1 = dir ? std::conj(L[size_t(p.lds_S1 + a2 * s.s1 + a)]) : L[size_t(p.lds_S1 + a * s.s1 + a2)];
+                const cplx c1 = dir ? std::conj(L[size_t(p.lds_S1 + a2 * p.s1p + a)]) : L[size_t(p.lds_S1 + a * p.s1p + a2)];
                 const cplx c2 = k2 == 0 ? cplx(1, 0)
-                                        : (dir ? std::conj(L[size_t(p.lds_S2 + b2 * s.s2 + b)]) : L[size_t(p.lds_S2 + b * s.s2 + b2)]);
+                                        : (dir ? std::conj(L[size_t(p.lds_S2 + b2 * p.s2p + b)]) : L[size_t(p.lds_S2 + b * p.s2p + b2)]);
                 W[size_t(a * s.s2 + b)] += c1 * c2 * V[size_t(a2 * s.s2 + b2)];
                 const cplx r1 = dir ? std::conj(S1[size_t(a2) * s.s1 + a]) : S1[size_t(a) * s.s1 + a2];
                 const cplx r2 = dir ? std::conj(S2[size_t(b2) * s.s2 + b]) : S2[size_t(b) * s.s2 + b2];

@@ -57,6 +57,10 @@ struct SymProgram {
   int npass = 0;                   // even (padded with an idle pass)
   int lds_one = 0;                 // level 0's entry (1)
   int lds_S1 = 0, lds_S2 = 0, lds_V = 0, lds_T1 = 0, lds_W = 0, lds_total = 0;
+  // row strides of C1 and C2 in LDS: C_k1 is written with an odd row stride
+  // (s1 + 1 when s1 is even), so that lanes reading one column of its rows hit
+  // different LDS banks; C2 shares it when k2 == k1 and keeps s2 when copied
+  int s1p = 0, s2p = 0;
   std::vector<double> pre;         // [2][dimS]: 1/F (forward), F (adjoint)
   std::vector<double> Yp;          // [2][ng][d][4], q = 4 g + i: Yc F / c (forward), Yc F^-1 / c (adjoint); zero for q >= dimS
   int ng = 0;                      // groups of 4 components (dimS rounded up)

@@ -90,14 +90,61 @@ GHX_DEV void sym_build(const BigRepTables::Sym& st, const Mat<R, N>& U, cx<R>* L
 //   conj(C1^dag V1) = C1^T conj(V1),   conj(T conj(C2)) = conj(T) C2,
 // and the projection conjugates its result back (Yp is real). Regions V, T1
 // and W hold the two directions P = 4 ng entries apart (V, W: s1 x s2).
+// With s2 <= 4 a lane takes one row a of one direction: the s2 entries of
+// C1 V (or C1^T V) for that row stay in registers, so C1 is read once per lane
+// and V's row a2 is the same for all lanes of a direction (an LDS broadcast),
+// and C2 is applied in registers before W is written. Larger s2 use the
+// two-stage form below (a lane per entry, C1 V through LDS).
+constexpr int kSymRowS2 = 4;
+// W (separate from V: lanes of a later pass may still read V) gets zeros in
+// its padding entries [dimS, P) of both directions for the projection.
 template <typename R>
-GHX_DEV void sym_apply2(const BigRepTables::Sym& st, const cx<R>* S1, const cx<R>* S2, const cx<R>* V, cx<R>* T1, cx<R>* W) {
-  const int t = threadIdx.x, s1 = st.s1, s2 = st.s2, dimS = st.dimS, P = 4 * st.ng;
+GHX_DEV void sym_apply2_rows(const BigRepTables::Sym& st, const cx<R>* S1, const cx<R>* S2, const cx<R>* V, cx<R>* W) {
+  const int t = threadIdx.x, s1 = st.s1, s2 = st.s2, P = 4 * st.ng, dimS = st.dimS, s1p = st.s1p, s2p = st.s2p;
+  for (int e = t; e < 2 * P; e += kSymWave)
+    if (e % P >= dimS) W[e] = {R(0), R(0)};
+  for (int e = t; e < 2 * s1; e += kSymWave) {
+    const int dir = e >= s1, a = e - dir * s1;
+    const cx<R>* c = S1 + (dir ? a : a * s1p);  // row a of C1, or column a
+    const int cs = dir ? s1p : 1;
+    const cx<R>* v = V + dir * P;
+    cx<R> acc[kSymRowS2];
+#pragma unroll
+    for (int b = 0; b < kSymRowS2; ++b) acc[b] = {R(0), R(0)};
+    for (int a2 = 0; a2 < s1; ++a2, c += cs, v += s2) {
+      const cx<R> cv = *c;
+#pragma unroll
+      for (int b = 0; b < kSymRowS2; ++b)
+        if (b < s2) cfma(acc[b], cv, v[b]);
+    }
+    cx<R>* w = W + dir * P + a * s2;
+    if (s2 == 1) { w[0] = acc[0]; continue; }
+#pragma unroll
+    for (int b = 0; b < kSymRowS2; ++b) {
+      if (b >= s2) break;
+      cx<R> z = {R(0), R(0)};
+#pragma unroll
+      for (int b2 = 0; b2 < kSymRowS2; ++b2)
+        if (b2 < s2) cfma(z, dir ? S2[b2 * s2p + b] : S2[b * s2p + b2], acc[b2]);
+      w[b] = z;
+    }
+  }
+  wave_sync();
+}
+
+// Returns where the result lies: W, or T1 on the row path.
+template <typename R>
+GHX_DEV const cx<R>* sym_apply2(const BigRepTables::Sym& st, const cx<R>* S1, const cx<R>* S2, const cx<R>* V, cx<R>* T1, cx<R>* W) {
+  const int t = threadIdx.x, s1 = st.s1, s2 = st.s2, dimS = st.dimS, P = 4 * st.ng, s1p = st.s1p, s2p = st.s2p;
+  if (s2 <= kSymRowS2) {
+    sym_apply2_rows<R>(st, S1, S2, V, T1);
+    return T1;
+  }
   cx<R>* out1 = s2 == 1 ? W : T1;
   for (int e = t; e < 2 * dimS; e += kSymWave) {
     const int dir = e >= dimS, q = e - dir * dimS, a = q / s2, b = q - a * s2;
-    const cx<R>* c = S1 + (dir ? a : a * s1);  // row a of C1, or column a
-    const int cs = dir ? s1 : 1;
+    const cx<R>* c = S1 + (dir ? a : a * s1p);  // row a of C1, or column a
+    const int cs = dir ? s1p : 1;
     const cx<R>* v = V + dir * P + b;
     cx<R> acc = {R(0), R(0)};
 #pragma unroll 4
@@ -105,17 +152,18 @@ GHX_DEV void sym_apply2(const BigRepTables::Sym& st, const cx<R>* S1, const cx<R
     out1[dir * P + q] = acc;
   }
   wave_sync();
-  if (s2 == 1) return;
+  if (s2 == 1) return W;
   for (int e = t; e < 2 * dimS; e += kSymWave) {
     const int dir = e >= dimS, q = e - dir * dimS, a = q / s2, b = q - a * s2;
-    const cx<R>* c = S2 + (dir ? b : b * s2);  // row b of C2, or column b
-    const int cs = dir ? s2 : 1;
+    const cx<R>* c = S2 + (dir ? b : b * s2p);  // row b of C2, or column b
+    const int cs = dir ? s2p : 1;
     const cx<R>* u = T1 + dir * P + a * s2;
     cx<R> acc = {R(0), R(0)};
     for (int b2 = 0; b2 < s2; ++b2, c += cs) cfma(acc, *c, u[b2]);
     W[dir * P + q] = acc;
   }
   wave_sync();
+  return W;
 }
 
 // The forward hop out[l] = sum_q Yp[0][q][l] W[0][q] (lanes over l; Yp has
@@ -227,15 +275,15 @@ __global__ __launch_bounds__(kSymWave) void sym_link_kernel(BigArgs<T> a, BigRep
     V[dir * P + q] = dir ? cconj(v) : v;
   }
   wave_sync();
-  sym_apply2<R>(st, S1, S2, V, T1, W);
+  const cx<R>* Wr = sym_apply2<R>(st, S1, S2, V, T1, W);
   // the forward hop is projected here (its g^a needs it alone); the backward
   // hop only enters the scalar force of x+mu, summed over mu, so it is stored
   // lifted and projected once per site by sym_finish_kernel
   const R* Yp = sizeof(R) == 4 ? reinterpret_cast<const R*>(st.Ypf) : reinterpret_cast<const R*>(st.Yp);
-  sym_project_fwd<T, R>(Yp, st.ng, d, W, out + 2 * ((size_t(g) * D + mu) * d));
+  sym_project_fwd<T, R>(Yp, st.ng, d, Wr, out + 2 * ((size_t(g) * D + mu) * d));
   if (both) {
     T* hl = static_cast<T*>(st.hopsL) + 2 * ((size_t(gp) * D + mu) * P);
-    for (int q = t; q < P; q += kSymWave) st_cx(hl + 2 * q, W[P + q]);
+    for (int q = t; q < P; q += kSymWave) st_cx(hl + 2 * q, Wr[P + q]);
   }
 }
 
