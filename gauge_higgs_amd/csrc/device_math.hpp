@@ -513,5 +513,18 @@ GHX_DEV double draw_gauss(uint64_t seed, uint64_t k) {
   const double u2 = draw_uniform(seed, mix64(k * 2 + 2));
   return gcr::sqrt_cr(-2.0 * gcr::log_cr(u1)) * gcr::cos_cr(6.283185307179586476925286766559 * u2);
 }
+// The same draw rounded to float, for FP32 storage. A float only needs the
+// draw to the nearest float: the device's own double log, cos and sqrt (a few
+// double ulp) give it unless the draw lies within 2^-46 (relative) of a float
+// rounding boundary, about 2 draws in 10^7; there the correctly rounded draw
+// decides. The result is float(draw_gauss(seed, k)) either way.
+GHX_DEV float draw_gauss_f32(uint64_t seed, uint64_t k) {
+  const double u1 = draw_uniform(seed, mix64(k * 2 + 1));
+  const double u2 = draw_uniform(seed, mix64(k * 2 + 2));
+  const double g = sqrt(-2.0 * log(u1)) * cos(6.283185307179586476925286766559 * u2);
+  const float lo = float(g * (1.0 - 0x1p-46)), hi = float(g * (1.0 + 0x1p-46));
+  if (lo == hi) return lo;
+  return float(draw_gauss(seed, k));
+}
 
 }  // namespace ghx

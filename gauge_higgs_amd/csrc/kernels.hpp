@@ -1008,14 +1008,19 @@ __global__ __launch_bounds__(kBlock) void refresh_kernel(const RepDesc* reps, co
   const unsigned g = rd.offset + s;
   const unsigned long long t = st[r].traj_count;
   const unsigned long long sP = stream_key(tagP, t);
+  // FP32 storage takes the float draw directly (equal to T(draw_gauss), cheaper)
+  auto draw = [&](unsigned long long key) -> T {
+    if constexpr (sizeof(T) == 4) return draw_gauss_f32(rd.seed, key);
+    else return draw_gauss(rd.seed, key);
+  };
   for (int mu = 0; mu < D; ++mu)
     for (int c = 0; c < NGN; ++c)
-      P[size_t(mu * NGN + c) * Vp + g] = T(draw_gauss(rd.seed, stream_key(sP, s, mu, c)));
+      P[size_t(mu * NGN + c) * Vp + g] = draw(stream_key(sP, s, mu, c));
   if (d > 0) {
     const unsigned long long sPi = stream_key(tagPi, t);
     for (int k = 0; k < d; ++k) {
-      pi[size_t(2 * k) * Vp + g] = T(draw_gauss(rd.seed, stream_key(sPi, s, 2 * k)));
-      pi[size_t(2 * k + 1) * Vp + g] = real_rep ? T(0) : T(draw_gauss(rd.seed, stream_key(sPi, s, 2 * k + 1)));
+      pi[size_t(2 * k) * Vp + g] = draw(stream_key(sPi, s, 2 * k));
+      pi[size_t(2 * k + 1) * Vp + g] = real_rep ? T(0) : draw(stream_key(sPi, s, 2 * k + 1));
     }
   }
 }

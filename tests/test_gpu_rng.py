@@ -77,3 +77,21 @@ def test_start_fields_and_momenta_equal_reference_dumps(goldens, case):
         exc["pi0"] = assert_equal_but_glibc_misrounds(h.get("pi").view(np.float64), g[tag + "_pi0"],
                                                       lambda i: scalar_keys(i, d, sPi), 1, what=tag + " pi0")
     print(f"{tag}: glibc misroundings (the listed exceptions) {exc}")
+
+
+@pytest.mark.parametrize("case", [("su3_sym2", 4, 3, "2", 4), ("su2_fund", 4, 2, "fund", (4, 3, 5, 6))], ids=["su3_sym2", "su2_fund"])
+def test_fp32_refresh_is_the_rounded_draw(case):
+    """FP32 contexts refresh with draw_gauss_f32: the device's fast double draw,
+    rounded to float, with the correctly rounded draw deciding near float
+    rounding boundaries. The stored momenta must be exactly float(FP64 draw)."""
+    _, D, N, rep, L = case
+    out = {}
+    for prec in (gha.FP64, gha.FP32MD):
+        h = gha.GaugeHiggsHMC(L, rep, seed=11, D=D, N=N, precision=prec)
+        h.hot(0.8, 12345, 0.3)
+        h.refresh_momenta()
+        out[prec] = (h.get("P"), h.get("pi"))
+    P64, pi64 = out[gha.FP64]
+    P32, pi32 = out[gha.FP32MD]
+    assert np.array_equal(P32, P64.astype(np.float32).astype(np.float64))
+    assert np.array_equal(pi32.view(np.float64), pi64.view(np.float64).astype(np.float32).astype(np.float64))
