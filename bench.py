@@ -38,6 +38,8 @@ WORKLOADS = {
     "su2_fund_L16_fp32": (4, 2, "fund", 16, 2.3, 0.265, 0.5, "fp32", 2),
     "su2_fund_L16_fp64": (4, 2, "fund", 16, 2.3, 0.265, 0.5, "fp64", 2),
     "su2_pure_L8_fp64": (4, 2, "", 8, 2.3, 0.0, 0.0, "fp64", 0),
+    # the large irrep of the learned-force comparison (symmetric-power engine)
+    "su3_61_L12_fp32": (4, 3, "6,1", 12, 5.7, 0.05, 0.5, "fp32", 48),
 }
 
 
@@ -96,7 +98,7 @@ def cpu_baseline(workload, sample_steps=16, single_L=16, single_steps=8):
     host's cores: OMP threads = the physical cores lscpu reports, capped by the
     CPUs this process may run on and by OMP_NUM_THREADS when the environment
     sets it; plus a one-thread sample on a smaller lattice."""
-    D, N, rep, L, *_ = WORKLOADS[workload]
+    D, N, rep, L, *_, d = WORKLOADS[workload]
     phys = physical_cores()
     avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     env_cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
@@ -125,6 +127,8 @@ def cpu_baseline(workload, sample_steps=16, single_L=16, single_steps=8):
     if rep:
         o.phi_gaussian(12345, 1.0)
     o.refresh()
+    if d > 6:  # the tensor apply of a large irrep costs ~10^3 times a small one per site
+        sample_steps = 1
     sec = O.lib().orc_time_kick_drift(o.h, sample_steps, 0.05)
     return {"value": o.V * sample_steps / sec, "unit": "site-updates/s", "cores": threads, "kind": "port",
             "sample": f"FP64 oracle restatement, {L}^4, {sample_steps} kick+drift MD steps ({sec:.1f} s)"}
@@ -300,6 +304,8 @@ def main():
     launch_s = sec / args.steps
     achieved = V * bpsite / launch_s / 1e9
     kernel = "md_stream_kernel" if tuned.get("stream_seg") else ("md_link_kernel" if tuned.get("link_lanes") else "md_step_kernel")
+    if d > 6:  # a large irrep: the matter part runs in the symmetric-power engine before the step kernel
+        kernel = "sym_lift_kernel + sym_link_kernel + sym_finish_kernel + " + kernel
     traffic = load_traffic(args.workload, kernel, tuned.get("block_order", 0))
     line = {
         "metric": "MD site-updates/s; GPU-s per accepted traj, learned vs exact force (1 GPU)",
