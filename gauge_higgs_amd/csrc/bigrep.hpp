@@ -49,6 +49,9 @@ struct BigRepTables {
     void* hops = nullptr;          // forward hops D(U_mu(x)) phi(x+mu), site-major: [Vp][D][d] complex, storage type
     void* hopsL = nullptr;         // backward hops of x+mu before projection, conjugated: [Vp][D][4 ng] complex
     int lds_one = 0, lds_S1 = 0, lds_S2 = 0, lds_V = 0, lds_T1 = 0, lds_W = 0, lds_total = 0;  // cx units
+    int k1 = 0, k2 = 0, lds_Cb = 0;   // Young rows; the level buffer that alternates with lds_S1
+    int pair = 0;                     // 1: sym_link_pair_kernel (SU(3), k1 <= 6), two LDS regions per workgroup
+    int pair_S2 = 0, pair_V = 0, pair_T1 = 0, pair_total = 0;  // its per-link region (C at 0), cx units
     // g^a by lanes: generator a owns lanes [a lpg, (a+1) lpg); entry k of lane t
     // is (l | m << 16, T^a_lm) at [k][t], gK entries per lane (zero padded)
     const int* glm = nullptr;

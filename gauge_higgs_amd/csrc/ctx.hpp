@@ -73,6 +73,7 @@ struct ghx_ctx {
   std::vector<void*> dsym;   // symmetric-power engine tables and lifted scalar
   bool sym_valid = false;    // the engine applies to this irrep (host-checked)
   bool sym_enabled = true;   // GHX_TUNE_SYM_ENGINE
+  bool sym_pair_enabled = true;  // GHX_TUNE_SYM_PAIRS
   void* G = nullptr;
   double* q1 = nullptr;
   // replica exchange

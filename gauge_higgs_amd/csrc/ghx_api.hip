@@ -201,6 +201,8 @@ int upload_tables(ghx_ctx* ctx) {
     st.hopsL = hopsL;
     st.lds_one = sp.lds_one; st.lds_S1 = sp.lds_S1; st.lds_S2 = sp.lds_S2;
     st.lds_V = sp.lds_V; st.lds_T1 = sp.lds_T1; st.lds_W = sp.lds_W; st.lds_total = sp.lds_total;
+    st.k1 = sb.k1; st.k2 = sb.k2; st.lds_Cb = sp.lds_Cb;
+    st.pair_S2 = sp.pair_S2; st.pair_V = sp.pair_V; st.pair_T1 = sp.pair_T1; st.pair_total = sp.pair_total;
     // the nonzeros of T^a_R dealt round-robin to the lanes of generator a
     const int lpg = ngn <= 4 ? 16 : 8;
     std::vector<std::vector<int>> lm(64);
@@ -246,9 +248,12 @@ int upload_tables(ghx_ctx* ctx) {
 // per-workgroup limit: the engine's LDS program, or the tensor-apply kernel's
 // dimT-entry tensor (dynamic) plus its static arrays. Irreps that fit neither
 // are refused here instead of failing at their first launch.
+bool sym_pair_applies(const ghx_ctx* ctx) {
+  return ctx->sym_valid && ctx->sym_pair_enabled && ctx->N == 3 && ctx->tb.sym.k1 >= 2 && ctx->tb.sym.k1 <= kSymPairMaxK;
+}
 size_t big_lds_need(const ghx_ctx* ctx, bool engine) {
   const size_t cs = 2 * size_t(ctx->tsize);
-  if (engine) return size_t(ctx->tb.sym.lds_total) * cs;
+  if (engine) return ctx->tb.sym.pair ? 2 * size_t(ctx->tb.sym.pair_total) * cs : size_t(ctx->tb.sym.lds_total) * cs;
   const size_t st = ctx->bops.matter_static_lds ? ctx->bops.matter_static_lds() : size_t(-1);
   return st == size_t(-1) ? st : size_t(ctx->rep.dimT) * cs + st;
 }
@@ -261,9 +266,11 @@ int lds_limit(ghx_ctx* ctx, size_t* lim) {
 }
 int check_big_lds(ghx_ctx* ctx) {
   if (!ctx->big) return GHX_OK;
+  ctx->tb.sym.pair = sym_pair_applies(ctx) ? 1 : 0;
   size_t lim = 0;
   int rc = lds_limit(ctx, &lim);
   if (rc) return rc;
+  if (ctx->tb.sym.on && ctx->tb.sym.pair && big_lds_need(ctx, true) > lim) ctx->tb.sym.pair = 0;  // one link per workgroup
   if (ctx->tb.sym.on && big_lds_need(ctx, true) > lim) {  // the tensor path may still fit
     ctx->sym_valid = false;
     ctx->tb.sym.on = 0;
@@ -1240,6 +1247,18 @@ int ghx_set_tuning(ghx_ctx* ctx, int key, int value) {
     drop_graph(ctx);
     return GHX_OK;
   }
+  if (key == GHX_TUNE_SYM_PAIRS) {
+    if (value < 0 || value > 1) return fail(ctx, GHX_E_INVALID, "paired symmetric-power links must be 0 or 1");
+    const bool was = ctx->sym_pair_enabled;
+    ctx->sym_pair_enabled = value != 0;
+    if (int rc = check_big_lds(ctx)) {
+      ctx->sym_pair_enabled = was;
+      check_big_lds(ctx);
+      return rc;
+    }
+    drop_graph(ctx);
+    return GHX_OK;
+  }
   return fail(ctx, GHX_E_INVALID, "unknown tuning key");
 }
 
@@ -1249,6 +1268,7 @@ int ghx_get_tuning(const ghx_ctx* ctx, int key, int* value) {
   if (key == GHX_TUNE_BLOCK_ORDER) { *value = ctx->block_walk().order; return GHX_OK; }
   if (key == GHX_TUNE_LINK_LANES) { *value = ctx->link_lanes; return GHX_OK; }
   if (key == GHX_TUNE_SYM_ENGINE) { *value = ctx->tb.sym.on; return GHX_OK; }
+  if (key == GHX_TUNE_SYM_PAIRS) { *value = ctx->tb.sym.on && ctx->tb.sym.pair; return GHX_OK; }
   return GHX_E_INVALID;
 }
 

@@ -5,6 +5,27 @@
 namespace ghx {
 template <typename T, int N, int D>
 struct BigLaunch {
+  // SU(3), first Young row 2..kSymPairMaxK: two links per wave (sym_link_pair_kernel)
+  template <int K>
+  static void pair_launch(const LaunchCtx& c, const BigRepTables& tb, const BigArgs<T>& a, T* hops, int both) {
+    hipLaunchKernelGGL((sym_link_pair_kernel<T, D, K>), dim3(c.Vp, (D + 1) / 2), dim3(kSymWave),
+                       2 * size_t(tb.sym.pair_total) * sizeof(cx<T>), c.stream, a, tb, hops, both);
+  }
+  static bool link_pair(const LaunchCtx& c, const BigRepTables& tb, const BigArgs<T>& a, T* hops, int both) {
+    if constexpr (N != 3) {
+      return false;
+    } else {
+      if (!tb.sym.pair) return false;
+      switch (tb.sym.k1) {
+        case 2: pair_launch<2>(c, tb, a, hops, both); return true;
+        case 3: pair_launch<3>(c, tb, a, hops, both); return true;
+        case 4: pair_launch<4>(c, tb, a, hops, both); return true;
+        case 5: pair_launch<5>(c, tb, a, hops, both); return true;
+        case 6: pair_launch<6>(c, tb, a, hops, both); return true;
+        default: return false;
+      }
+    }
+  }
   static void matter(const LaunchCtx& c, const BigRepTables& tb, const void* U, const void* phi_in, void* phi_out, void* pi, void* G,
                      void* Fphi, double* q1, double ck, double cd, int flags, int mode) {
     if (c.frozen) flags |= F_FROZEN;
@@ -18,8 +39,10 @@ struct BigLaunch {
       if (!(flags & F_NO_HOP)) {
         hipLaunchKernelGGL(sym_lift_kernel<T>, dim3(c.Vp / kSymWave, (tb.sym.dimS + kLiftChunk - 1) / kLiftChunk), dim3(kSymWave), 0,
                            c.stream, c.Vp, (const T*)phi_in, tb.d, tb.sym);
-        hipLaunchKernelGGL((sym_link_kernel<T, N, D>), dim3(c.Vp, D), dim3(kSymWave), size_t(tb.sym.lds_total) * sizeof(cx<T>),
-                           c.stream, a, tb, hops, mode == BIG_ACTION ? 0 : 1);  // the action needs the forward hops only
+        const int both = mode == BIG_ACTION ? 0 : 1;  // the action needs the forward hops only
+        if (!link_pair(c, tb, a, hops, both))
+          hipLaunchKernelGGL((sym_link_kernel<T, N, D>), dim3(c.Vp, D), dim3(kSymWave), size_t(tb.sym.lds_total) * sizeof(cx<T>),
+                             c.stream, a, tb, hops, both);
       }
       hipLaunchKernelGGL((sym_finish_kernel<T, N, D>), dim3(c.Vp), dim3(kSymWave), 0, c.stream, a, tb, (const T*)hops);
       return;

@@ -114,7 +114,7 @@ SymBuild build_sym(const RepInfo& r) {
   s.s1 = int(lv[s.k1].size());
   s.s2 = s.k2 > 0 ? int(lv[s.k2].size()) : 1;
   s.dimS = s.s1 * s.s2;
-  if (s.dimS > kSymMaxDimS) return s;
+  if (s.dimS > kSymMaxDimS || d > kSymMaxD) return s;
   s.lvl_off.assign(s.k1 + 2, 0);
   for (int k = 0; k <= s.k1; ++k) s.lvl_off[k + 1] = s.lvl_off[k] + int(lv[k].size());
   const int tot = s.lvl_off[s.k1 + 1];
@@ -222,6 +222,7 @@ SymProgram sym_program(const SymBuild& s) {
   int ca = 0, cb = 0;
   for (int k = 0; k <= k1; ++k) ((k1 - k) % 2 == 0 ? ca : cb) = std::max((k1 - k) % 2 == 0 ? ca : cb, sk(k) * stride(k));
   const int Ca = kSymLdsBase, Cb = Ca + ca;
+  p.lds_Cb = Cb;
   auto base = [&](int k) { return (k1 - k) % 2 == 0 ? Ca : Cb; };
   const bool copy2 = k2 > 0 && k2 < k1;
   p.lds_S1 = Ca;
@@ -237,6 +238,11 @@ SymProgram sym_program(const SymBuild& s) {
   p.lds_W = s.s2 == 1 ? p.lds_T1 : p.lds_V;
   p.lds_total = std::max(after, p.lds_T1 + 2 * P);
   p.lds_one = base(0);
+  p.pair_C = 0;
+  p.pair_S2 = copy2 ? s.s1 * p.s1p : 0;
+  p.pair_V = s.s1 * p.s1p + (copy2 ? s.s2 * s.s2 : 0);
+  p.pair_T1 = s.s2 <= kSymRowS2 ? p.pair_V : p.pair_V + 2 * P;
+  p.pair_total = p.pair_T1 + 2 * P;
   auto flush = [&](std::vector<int>& pass) {  // pad a pass to 64 entries (idle lanes write slot 1)
     while (pass.size() < size_t(4 * 64)) { pass.push_back(0); pass.push_back(0); pass.push_back(0); pass.push_back((1 << 2) | 3); }
     p.prog.insert(p.prog.end(), pass.begin(), pass.end());

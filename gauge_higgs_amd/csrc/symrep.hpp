@@ -22,6 +22,7 @@ namespace ghx {
 
 constexpr int kSymMaxS = 36;      // largest dim Sym^k of the engine (SU(3): k <= 7)
 constexpr int kSymMaxDimS = 256;  // largest dim S
+constexpr int kSymMaxD = 64;      // largest irrep: the per-site kernel holds one component per lane of a wave
 
 struct SymBuild {
   bool ok = false;
@@ -50,6 +51,11 @@ SymBuild build_sym(const RepInfo& r);
 //   S1 (x) S2 = F (C1 (x) C2) F^-1,  F = diag(sqrt(a!) sqrt(b!)) over q = (a, b),
 // so the forward hop lifts V = F^-1 Yc phi and projects with Yc F / c, the
 // adjoint lifts F Yc phi and projects with Yc F^-1 / c.
+// SU(3) irreps with k1 <= kSymPairMaxK (dim Sym^k1 <= 28) may run two links
+// per wave, C_k built a column per lane (sym_link_pair_kernel).
+constexpr int kSymPairMaxK = 6;
+// The apply keeps a row of C1 V in registers when dim Sym^k2 <= kSymRowS2.
+constexpr int kSymRowS2 = 4;
 constexpr int kSymLdsUs = 4;
 constexpr int kSymLdsBase = 16;
 struct SymProgram {
@@ -57,6 +63,12 @@ struct SymProgram {
   int npass = 0;                   // even (padded with an idle pass)
   int lds_one = 0;                 // level 0's entry (1)
   int lds_S1 = 0, lds_S2 = 0, lds_V = 0, lds_T1 = 0, lds_W = 0, lds_total = 0;
+  int lds_Cb = 0;                  // the second level buffer: level k lives at lds_S1 when k1 - k is even, else here
+  // layout of one link's region in the paired kernel (cx units): every level
+  // in place in C (the last with row stride s1p), the saved C_k2, then V, which
+  // the row-blocked apply overwrites with its result (T1 = V); the two-stage
+  // apply keeps T1 apart
+  int pair_C = 0, pair_S2 = 0, pair_V = 0, pair_T1 = 0, pair_total = 0;
   // row strides of C1 and C2 in LDS: C_k1 is written with an odd row stride
   // (s1 + 1 when s1 is even), so that lanes reading one column of its rows hit
   // different LDS banks; C2 shares it when k2 == k1 and keeps s2 when copied

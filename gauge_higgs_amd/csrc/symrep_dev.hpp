@@ -95,15 +95,15 @@ GHX_DEV void sym_build(const BigRepTables::Sym& st, const Mat<R, N>& U, cx<R>* L
 // and V's row a2 is the same for all lanes of a direction (an LDS broadcast),
 // and C2 is applied in registers before W is written. Larger s2 use the
 // two-stage form below (a lane per entry, C1 V through LDS).
-constexpr int kSymRowS2 = 4;
 // W (separate from V: lanes of a later pass may still read V) gets zeros in
 // its padding entries [dimS, P) of both directions for the projection.
+// t and nl: this lane's index among the nl lanes that serve the link.
 template <typename R>
-GHX_DEV void sym_apply2_rows(const BigRepTables::Sym& st, const cx<R>* S1, const cx<R>* S2, const cx<R>* V, cx<R>* W) {
-  const int t = threadIdx.x, s1 = st.s1, s2 = st.s2, P = 4 * st.ng, dimS = st.dimS, s1p = st.s1p, s2p = st.s2p;
-  for (int e = t; e < 2 * P; e += kSymWave)
+GHX_DEV void sym_apply2_rows(const BigRepTables::Sym& st, const cx<R>* S1, const cx<R>* S2, const cx<R>* V, cx<R>* W, int t, int nl) {
+  const int s1 = st.s1, s2 = st.s2, P = 4 * st.ng, dimS = st.dimS, s1p = st.s1p, s2p = st.s2p;
+  for (int e = t; e < 2 * P; e += nl)
     if (e % P >= dimS) W[e] = {R(0), R(0)};
-  for (int e = t; e < 2 * s1; e += kSymWave) {
+  for (int e = t; e < 2 * s1; e += nl) {
     const int dir = e >= s1, a = e - dir * s1;
     const cx<R>* c = S1 + (dir ? a : a * s1p);  // row a of C1, or column a
     const int cs = dir ? s1p : 1;
@@ -111,6 +111,7 @@ GHX_DEV void sym_apply2_rows(const BigRepTables::Sym& st, const cx<R>* S1, const
     cx<R> acc[kSymRowS2];
 #pragma unroll
     for (int b = 0; b < kSymRowS2; ++b) acc[b] = {R(0), R(0)};
+#pragma unroll 4
     for (int a2 = 0; a2 < s1; ++a2, c += cs, v += s2) {
       const cx<R> cv = *c;
 #pragma unroll
@@ -134,14 +135,15 @@ GHX_DEV void sym_apply2_rows(const BigRepTables::Sym& st, const cx<R>* S1, const
 
 // Returns where the result lies: W, or T1 on the row path.
 template <typename R>
-GHX_DEV const cx<R>* sym_apply2(const BigRepTables::Sym& st, const cx<R>* S1, const cx<R>* S2, const cx<R>* V, cx<R>* T1, cx<R>* W) {
-  const int t = threadIdx.x, s1 = st.s1, s2 = st.s2, dimS = st.dimS, P = 4 * st.ng, s1p = st.s1p, s2p = st.s2p;
+GHX_DEV const cx<R>* sym_apply2(const BigRepTables::Sym& st, const cx<R>* S1, const cx<R>* S2, const cx<R>* V, cx<R>* T1, cx<R>* W,
+                                int t, int nl) {
+  const int s1 = st.s1, s2 = st.s2, dimS = st.dimS, P = 4 * st.ng, s1p = st.s1p, s2p = st.s2p;
   if (s2 <= kSymRowS2) {
-    sym_apply2_rows<R>(st, S1, S2, V, T1);
+    sym_apply2_rows<R>(st, S1, S2, V, T1, t, nl);
     return T1;
   }
   cx<R>* out1 = s2 == 1 ? W : T1;
-  for (int e = t; e < 2 * dimS; e += kSymWave) {
+  for (int e = t; e < 2 * dimS; e += nl) {
     const int dir = e >= dimS, q = e - dir * dimS, a = q / s2, b = q - a * s2;
     const cx<R>* c = S1 + (dir ? a : a * s1p);  // row a of C1, or column a
     const int cs = dir ? s1p : 1;
@@ -153,7 +155,7 @@ GHX_DEV const cx<R>* sym_apply2(const BigRepTables::Sym& st, const cx<R>* S1, co
   }
   wave_sync();
   if (s2 == 1) return W;
-  for (int e = t; e < 2 * dimS; e += kSymWave) {
+  for (int e = t; e < 2 * dimS; e += nl) {
     const int dir = e >= dimS, q = e - dir * dimS, a = q / s2, b = q - a * s2;
     const cx<R>* c = S2 + (dir ? b : b * s2p);  // row b of C2, or column b
     const int cs = dir ? s2p : 1;
@@ -171,11 +173,11 @@ GHX_DEV const cx<R>* sym_apply2(const BigRepTables::Sym& st, const cx<R>* S1, co
 // (q = 4 g + i), so a lane reads 4 consecutive q of its row in one 16- or
 // 32-byte access; W is zero from dimS to 4 ng.
 template <typename T, typename R>
-GHX_DEV void sym_project_fwd(const R* __restrict__ Yp, int ng, int d, const cx<R>* W, T* out) {
-  for (int l = threadIdx.x; l < d; l += kSymWave) {
+GHX_DEV void sym_project_fwd(const R* __restrict__ Yp, int ng, int d, const cx<R>* W, T* out, int t, int nl) {
+  for (int l = t; l < d; l += nl) {
     const R* y = Yp + size_t(l) * 4;
     cx<R> acc = {R(0), R(0)};
-#pragma unroll 2
+#pragma unroll 4
     for (int g = 0; g < ng; ++g, y += 4 * d) {
       R v[4];
       if constexpr (sizeof(R) == 4) {
@@ -275,15 +277,272 @@ __global__ __launch_bounds__(kSymWave) void sym_link_kernel(BigArgs<T> a, BigRep
     V[dir * P + q] = dir ? cconj(v) : v;
   }
   wave_sync();
-  const cx<R>* Wr = sym_apply2<R>(st, S1, S2, V, T1, W);
+  const cx<R>* Wr = sym_apply2<R>(st, S1, S2, V, T1, W, t, kSymWave);
   // the forward hop is projected here (its g^a needs it alone); the backward
   // hop only enters the scalar force of x+mu, summed over mu, so it is stored
   // lifted and projected once per site by sym_finish_kernel
   const R* Yp = sizeof(R) == 4 ? reinterpret_cast<const R*>(st.Ypf) : reinterpret_cast<const R*>(st.Yp);
-  sym_project_fwd<T, R>(Yp, st.ng, d, Wr, out + 2 * ((size_t(g) * D + mu) * d));
+  sym_project_fwd<T, R>(Yp, st.ng, d, Wr, out + 2 * ((size_t(g) * D + mu) * d), t, kSymWave);
   if (both) {
     T* hl = static_cast<T*>(st.hopsL) + 2 * ((size_t(gp) * D + mu) * P);
     for (int q = t; q < P; q += kSymWave) st_cx(hl + 2 * q, Wr[P + q]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// SU(3) with dim Sym^k1 <= 32 (k1 <= 6): two links per wave, C_k built a column
+// per lane. Lanes 0-31 take link mu = 2 blockIdx.y of site blockIdx.x, lanes
+// 32-63 link mu + 1, each half in its own LDS region laid out as for the
+// per-entry program. Column n of level k follows from column n - e_j of level
+// k - 1 (j = the first colour of n):
+//   C_k[m][n] = sum_i U[i][j] C_(k-1)[m - e_i][n - e_j],
+// so a lane reads that one column of the previous level into registers,
+// forms every entry of its column from them with the lane's fixed U[i][j],
+// and writes it: the build moves one read of level k-1 and one write of level
+// k through LDS, against three reads of C_(k-1), three of U and one write per
+// entry in the program. Level 1 is U itself and stays in registers.
+// Monomials of degree k over the three colours are ordered as on the host:
+// first exponent descending, then the second.
+namespace symc {
+__host__ __device__ constexpr int sdim(int k) { return (k + 1) * (k + 2) / 2; }
+__host__ __device__ constexpr int tri(int r) { return r * (r + 1) / 2; }
+// the index of (a0, a1, k - a0 - a1) among the monomials of degree k
+__host__ __device__ constexpr int midx(int k, int a0, int a1) { return tri(k - a0) + (k - a0 - a1); }
+struct Mono {
+  int a[3];
+};
+// the exponents of monomial m of degree k
+__host__ __device__ constexpr Mono mono(int k, int m) {
+  int r = 0;
+  while (tri(r + 1) <= m) ++r;
+  const int a0 = k - r, a1 = r - (m - tri(r));
+  return Mono{{a0, a1, k - a0 - a1}};
+}
+// the row of level k - 1 that row m of level k draws on through colour i, or -1
+__host__ __device__ constexpr int rowsrc(int k, int m, int i) {
+  Mono x = mono(k, m);
+  if (x.a[i] == 0) return -1;
+  x.a[i] -= 1;
+  return midx(k - 1, x.a[0], x.a[1]);
+}
+}  // namespace symc
+
+constexpr int kSymPairLanes = kSymWave / 2;  // dim Sym^6 = 28 columns fit a half-wave; Sym^7 (36) does not
+
+template <int B, int E, class F>
+GHX_DEV void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+// row[j] for a per-lane j in {0, 1, 2} as a one-hot weighted sum (exact for
+// finite entries). A chain of selects would be folded into an indexed load,
+// which puts the whole matrix in scratch memory.
+template <typename R>
+GHX_DEV cx<R> sel3(const cx<R>* row, int j) {
+  const R w0 = R(j == 0), w1 = R(j == 1), w2 = R(j == 2);
+  return {fma(w0, row[0].r, fma(w1, row[1].r, w2 * row[2].r)), fma(w0, row[0].i, fma(w1, row[1].i, w2 * row[2].i))};
+}
+
+// Level K (>= 2) of the build for column n = t of this lane, written with row
+// stride SC into Lc, and copied into S2c (row stride s2p) when that is set.
+template <typename R, int K, int SC>
+GHX_DEV void sym_level(const Mat<R, 3>& U, const cx<R>* Lp, cx<R>* Lc, cx<R>* S2c, int s2p, int t) {
+  constexpr int SK = symc::sdim(K), SP = symc::sdim(K - 1);
+  if (t < SK) {
+    const symc::Mono x = symc::mono(K, t);
+    const int j = x.a[0] > 0 ? 0 : (x.a[1] > 0 ? 1 : 2);
+    const int np = j == 0 ? symc::midx(K - 1, x.a[0] - 1, x.a[1]) : (j == 1 ? symc::midx(K - 1, 0, x.a[1] - 1) : symc::midx(K - 1, 0, 0));
+    cx<R> col[SP];
+    if constexpr (K == 2) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) col[i] = sel3(U.e[i], np);
+    } else {
+      static_for<0, SP>([&](auto mp) { col[int(mp)] = Lp[int(mp) * SP + np]; });
+    }
+    cx<R> u[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) u[i] = sel3(U.e[i], j);
+    static_for<0, SK>([&](auto mc) {
+      constexpr int m = mc;
+      constexpr int r0 = symc::rowsrc(K, m, 0), r1 = symc::rowsrc(K, m, 1), r2 = symc::rowsrc(K, m, 2);
+      cx<R> acc = {R(0), R(0)};
+      if constexpr (r0 >= 0) cfma(acc, u[0], col[r0]);
+      if constexpr (r1 >= 0) cfma(acc, u[1], col[r1]);
+      if constexpr (r2 >= 0) cfma(acc, u[2], col[r2]);
+      Lc[m * SC + t] = acc;
+      if (S2c) S2c[m * s2p + t] = acc;
+    });
+  }
+  wave_sync();
+}
+
+// The row-blocked apply for a half-wave (s2 <= kSymRowS2, s1 <= 32): lane a
+// takes row a of C1 for the forward hop and column a for the adjoint at once,
+// two independent accumulations that share each step's loop and hide each
+// other's LDS latency. Same arithmetic as sym_apply2_rows, with the row length
+// S1 = dim Sym^k1 known at compile time; the result lands in W (both
+// directions, P apart, padding zeroed). W may be V itself: every lane's reads
+// of V precede its writes in the wave's instruction stream.
+template <typename R, int S1N>
+GHX_DEV void sym_apply_pair_rows(const BigRepTables::Sym& st, const cx<R>* S1, const cx<R>* S2, const cx<R>* V, cx<R>* W, int t) {
+  constexpr int s1 = S1N, s1p = S1N | 1;
+  const int s2 = st.s2, P = 4 * st.ng, dimS = st.dimS, s2p = st.s2p;
+  for (int e = t; e < 2 * P; e += kSymPairLanes)
+    if (e % P >= dimS) W[e] = {R(0), R(0)};
+  if (t < s1) {
+    const int a = t;
+    const cx<R>* c0 = S1 + a * s1p;  // row a (forward)
+    const cx<R>* c1 = S1 + a;        // column a (adjoint)
+    const cx<R>* v0 = V;
+    const cx<R>* v1 = V + P;
+    cx<R> f[kSymRowS2], b[kSymRowS2];
+#pragma unroll
+    for (int k = 0; k < kSymRowS2; ++k) f[k] = b[k] = {R(0), R(0)};
+#pragma unroll 7
+    for (int a2 = 0; a2 < s1; ++a2, v0 += s2, v1 += s2) {
+      const cx<R> x0 = c0[a2], x1 = c1[a2 * s1p];
+#pragma unroll
+      for (int k = 0; k < kSymRowS2; ++k)
+        if (k < s2) {
+          cfma(f[k], x0, v0[k]);
+          cfma(b[k], x1, v1[k]);
+        }
+    }
+    cx<R>* w0 = W + a * s2;
+    cx<R>* w1 = W + P + a * s2;
+    if (s2 == 1) {
+      w0[0] = f[0];
+      w1[0] = b[0];
+    } else {
+#pragma unroll
+      for (int k = 0; k < kSymRowS2; ++k) {
+        if (k >= s2) break;
+        cx<R> z0 = {R(0), R(0)}, z1 = {R(0), R(0)};
+#pragma unroll
+        for (int k2 = 0; k2 < kSymRowS2; ++k2)
+          if (k2 < s2) {
+            cfma(z0, S2[k * s2p + k2], f[k2]);
+            cfma(z1, S2[k2 * s2p + k], b[k2]);
+          }
+        w0[k] = z0;
+        w1[k] = z1;
+      }
+    }
+  }
+  wave_sync();
+}
+
+// The forward projection for a half-wave: lane t forms components t and t + 32
+// (d <= 64) from one read of each W entry.
+template <typename T, typename R>
+GHX_DEV void sym_project_pair(const R* __restrict__ Yp, int ng, int d, const cx<R>* W, T* out, int t) {
+  if (t >= d) return;
+  const int l1 = t + kSymPairLanes;
+  const bool two = l1 < d;
+  const R* y0 = Yp + size_t(t) * 4;
+  const R* y1 = Yp + size_t(two ? l1 : t) * 4;  // a valid row when the second component is absent
+  cx<R> a0 = {R(0), R(0)}, a1 = {R(0), R(0)};
+#pragma unroll 4
+  for (int g = 0; g < ng; ++g, y0 += 4 * d, y1 += 4 * d) {
+    R u[4], v[4];
+    if constexpr (sizeof(R) == 4) {
+      const float4 p = *reinterpret_cast<const float4*>(y0), q = *reinterpret_cast<const float4*>(y1);
+      u[0] = p.x; u[1] = p.y; u[2] = p.z; u[3] = p.w;
+      v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+    } else {
+      const double2 p0 = *reinterpret_cast<const double2*>(y0), p1 = *reinterpret_cast<const double2*>(y0 + 2);
+      const double2 q0 = *reinterpret_cast<const double2*>(y1), q1 = *reinterpret_cast<const double2*>(y1 + 2);
+      u[0] = p0.x; u[1] = p0.y; u[2] = p1.x; u[3] = p1.y;
+      v[0] = q0.x; v[1] = q0.y; v[2] = q1.x; v[3] = q1.y;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const cx<R> w = W[4 * g + i];
+      a0.r = fma(u[i], w.r, a0.r);
+      a0.i = fma(u[i], w.i, a0.i);
+      a1.r = fma(v[i], w.r, a1.r);
+      a1.i = fma(v[i], w.i, a1.i);
+    }
+  }
+  st_cx(out + 2 * t, cx<T>{T(a0.r), T(a0.i)});
+  if (two) st_cx(out + 2 * l1, cx<T>{T(a1.r), T(a1.i)});
+}
+
+template <typename T, int D, int K1>
+__global__ __launch_bounds__(kSymWave) void sym_link_pair_kernel(BigArgs<T> a, BigRepTables tb, T* out, int both) {
+  using R = T;
+  constexpr int N = 3, NL = kSymPairLanes;
+  constexpr int kPre = 2 * kSymMaxDimS / NL;  // lifted entries a lane loads (both directions)
+  const BigRepTables::Sym& st = tb.sym;
+  extern __shared__ unsigned char smem_raw[];
+  const int lane = threadIdx.x, h = lane / NL, t = lane % NL;
+  cx<R>* L = reinterpret_cast<cx<R>*>(smem_raw) + h * st.pair_total;
+  const unsigned g = blockIdx.x, Vp = a.Vp;
+  const int r = a.blk_rep[g / kBlock];
+  const RepDesc& rd = a.reps[r];
+  const unsigned s = g - rd.offset;
+  if (s >= rd.vol) return;  // whole block exits together
+  const int mu = 2 * int(blockIdx.y) + h;
+  const bool act = mu < D;
+  const int ml = act ? mu : 0;  // an idle half (odd D) repeats link 0 and stores nothing
+  const Nbr<D> nb(rd, s);
+  const unsigned gp = g + nb.upv(rd, ml);
+  const Mat<R, N> U = load_link<T, R, N>(a.U, Vp, ml, g);
+  const int P = 4 * st.ng, dimS = st.dimS, nd = both + 1, k2 = st.k2;
+  // the lifted scalars of both ends, loaded now so that the build hides their latency
+  const T* lift = static_cast<const T*>(st.lift);
+  cx<R> pv[kPre];
+#pragma unroll
+  for (int c = 0; c < kPre; ++c) {
+    const int e = t + c * NL;
+    if (e < nd * dimS) {
+      const int dir = e >= dimS, q = e - dir * dimS;
+      pv[c] = ld_cx(lift + size_t(dir ? g : gp) * 2 * dimS + 2 * q);
+    }
+  }
+  // every level in place in C: a level's reads all precede its writes in the
+  // wave's instruction stream, and a wave's LDS operations execute in order
+  cx<R>* C = L;
+  cx<R>* S2 = L + st.pair_S2;
+  const bool copy2 = k2 > 0 && k2 < K1;
+  if (copy2 && k2 == 1 && t < N) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) S2[i * st.s2p + t] = sel3(U.e[i], t);
+  }
+  static_for<2, K1 + 1>([&](auto kc) {
+    constexpr int k = kc;
+    sym_level<R, k, (k == K1 ? (symc::sdim(k) | 1) : symc::sdim(k))>(U, C, C, (copy2 && k == k2) ? S2 : nullptr, st.s2p, t);
+  });
+  cx<R>* V = L + st.pair_V;
+  cx<R>* T1 = L + st.pair_T1;
+  cx<R>* W = V;
+  for (int e = t; e < 2 * P; e += NL)  // the projection's padding
+    if (e % P >= dimS) W[e] = {R(0), R(0)};
+  // V0 = F^-1 Yc phi(x+mu) for the forward hop of x, V1 = conj(F Yc phi(x)) for
+  // the adjoint hop of x+mu
+  const R* pre = sizeof(R) == 4 ? reinterpret_cast<const R*>(st.pref) : reinterpret_cast<const R*>(st.pre);
+#pragma unroll
+  for (int c = 0; c < kPre; ++c) {
+    const int e = t + c * NL;
+    if (e < nd * dimS) {
+      const int dir = e >= dimS, q = e - dir * dimS;
+      const cx<R> v = cscale(pv[c], pre[e]);
+      V[dir * P + q] = dir ? cconj(v) : v;
+    }
+  }
+  wave_sync();
+  const cx<R>* Wr = T1;
+  if (st.s2 <= kSymRowS2) sym_apply_pair_rows<R, symc::sdim(K1)>(st, C, S2, V, T1, t);  // T1 = V here
+  else Wr = sym_apply2<R>(st, C, S2, V, T1, W, t, NL);
+  const R* Yp = sizeof(R) == 4 ? reinterpret_cast<const R*>(st.Ypf) : reinterpret_cast<const R*>(st.Yp);
+  if (!act) return;
+  sym_project_pair<T, R>(Yp, st.ng, tb.d, Wr, out + 2 * ((size_t(g) * D + mu) * tb.d), t);
+  if (both) {
+    T* hl = static_cast<T*>(st.hopsL) + 2 * ((size_t(gp) * D + mu) * P);
+    for (int q = t; q < P; q += NL) st_cx(hl + 2 * q, Wr[P + q]);
   }
 }
 

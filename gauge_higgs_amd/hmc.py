@@ -221,6 +221,17 @@ class Lattices:
         self._check(self._lib.ghx_get_tuning(self.h, 3, C.byref(v)))
         return v.value
 
+    def set_sym_pairs(self, on):
+        """Symmetric-power engine, SU(3) with first Young row <= 6: two links per
+        wave with Sym^k(U) built a column per lane (1), or one link per wave
+        with the per-entry LDS program (0)."""
+        self._check(self._lib.ghx_set_tuning(self.h, 4, int(on)))
+
+    def sym_pairs(self):
+        v = C.c_int()
+        self._check(self._lib.ghx_get_tuning(self.h, 4, C.byref(v)))
+        return v.value
+
     def autotune(self, steps=10):
         """Pick the fastest MD-step launch shape on the live fields (fields restored)."""
         best, ms = C.c_int(), C.c_double()

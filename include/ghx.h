@@ -179,10 +179,14 @@ int ghx_swap_stats(ghx_ctx* ctx, int* npairs, int* pairs, uint64_t* attempts, ui
  * thread per site, for lattices too small to fill the GPU; GHX_TUNE_SYM_ENGINE
  * = 1 (default where it applies) applies a large tensor irrep through
  * symmetric powers of the link, D(U) = Yc^T (Sym^k1(U) x Sym^k2(U)) Yc / c, instead of
- * slot by slot on the N^n tensor (get_tuning reports whether it is in use). The
- * autotuner times every candidate for `steps` MD steps on the live fields,
- * restores the fields, and keeps the fastest. */
-enum { GHX_TUNE_STREAM_SEG = 0, GHX_TUNE_BLOCK_ORDER = 1, GHX_TUNE_LINK_LANES = 2, GHX_TUNE_SYM_ENGINE = 3 };
+ * slot by slot on the N^n tensor (get_tuning reports whether it is in use);
+ * GHX_TUNE_SYM_PAIRS = 1 (default where it applies: SU(3), first Young row
+ * <= 6) runs the engine two links per wave with Sym^k(U) built a column per
+ * lane, 0 one link per wave with the per-entry LDS program (get_tuning reports
+ * whether the paired kernel is in use). The autotuner times every candidate
+ * for `steps` MD steps on the live fields, restores the fields, and keeps the
+ * fastest. */
+enum { GHX_TUNE_STREAM_SEG = 0, GHX_TUNE_BLOCK_ORDER = 1, GHX_TUNE_LINK_LANES = 2, GHX_TUNE_SYM_ENGINE = 3, GHX_TUNE_SYM_PAIRS = 4 };
 int ghx_set_tuning(ghx_ctx* ctx, int key, int value);
 int ghx_get_tuning(const ghx_ctx* ctx, int key, int* value);
 int ghx_autotune(ghx_ctx* ctx, int steps, int* best_seg, double* best_ms_per_step);

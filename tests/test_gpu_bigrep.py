@@ -115,6 +115,41 @@ def test_sym_engine_equals_tensor_apply(rep, N, D, L, prec, tol):
     assert relmax(b[4], a[4]) < 1e2 * tol and relmax(b[5], a[5]) < 1e2 * tol
 
 
+@pytest.mark.parametrize("rep,N,D,L", ENGINE_CASES, ids=[c[0] for c in ENGINE_CASES])
+@pytest.mark.parametrize("prec,tol", [(gha.FP64, 1e-12), (gha.FP32MD, 1e-5)], ids=["fp64", "fp32"])
+def test_paired_link_kernel_equals_program_kernel(rep, N, D, L, prec, tol):
+    """The paired engine kernel (two links per wave, Sym^k(U) built a column per
+    lane from the previous level's column) against the one-link-per-wave LDS
+    program on the same fields: link and scalar forces and the action agree to
+    rounding. It runs exactly where it applies: SU(3) with first Young row <= 6
+    (so not SU(2) spin 6 nor {7,1}); with D = 3 one half of the last wave idles."""
+    k1 = int(rep.split(",")[0])
+    out = []
+    for pairs in (0, 1):
+        g = gha.GaugeHiggsHMC(L, rep, seed=3, D=D, N=N, precision=prec)
+        g.configure(beta=5.7 if N == 3 else 2.3, kappa=0.3, lam=0.5, nmd=4)
+        g.lat.set_sym_pairs(pairs)
+        assert g.lat.sym_engine() == 1
+        assert g.lat.sym_pairs() == int(pairs == 1 and N == 3 and k1 <= 6)
+        g.hot(0.8, 12345, 0.4)
+        F, Fp = g.forces()
+        out.append((F, Fp, g.observables()["SH"]))
+    a, b = out
+    assert relmax(b[0], a[0]) < tol and relmax(b[1], a[1]) < tol
+    assert abs(b[2] - a[2]) < tol * max(1.0, abs(a[2]))
+
+
+def test_irrep_beyond_one_component_per_lane_is_refused():
+    """The large-irrep kernels hold one component per lane of a 64-wide wave
+    (d <= 64). SU(3) {7,2} (d = 81, dim S = 216) would fit the engine's tables,
+    but it is refused at ghx_set_rep with GHX_E_UNSUPPORTED, in both
+    precisions, instead of running with components dropped."""
+    for prec in (gha.FP64, gha.FP32MD):
+        with pytest.raises(gha.GhxError) as e:
+            gha.GaugeHiggsHMC(4, "7,2", seed=1, D=3, N=3, precision=prec)
+        assert e.value.code == -4 and "no compiled kernels" in str(e.value)
+
+
 def test_sym_engine_default_and_speed():
     """{6,1} uses the engine by default; one MD step is faster than with the
     tensor apply."""
