@@ -44,7 +44,8 @@ struct BigLaunch {
           hipLaunchKernelGGL((sym_link_kernel<T, N, D>), dim3(c.Vp, D), dim3(kSymWave), size_t(tb.sym.lds_total) * sizeof(cx<T>),
                              c.stream, a, tb, hops, both);
       }
-      hipLaunchKernelGGL((sym_finish_kernel<T, N, D>), dim3(c.Vp), dim3(kSymWave), 0, c.stream, a, tb, (const T*)hops);
+      hipLaunchKernelGGL((sym_finish_kernel<T, N, D>), dim3(c.Vp / kFinishSites), dim3(kSymWave, kFinishSites), 0, c.stream, a, tb,
+                         (const T*)hops);
       return;
     }
     const size_t lds = size_t(tb.dimT) * sizeof(cx<T>);

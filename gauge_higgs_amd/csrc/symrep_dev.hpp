@@ -17,6 +17,7 @@
 namespace ghx {
 
 constexpr int kSymWave = 64;
+constexpr int kFinishSites = 4;  // sites (waves) per workgroup of sym_finish_kernel
 constexpr int kLiftChunk = 16;  // lifted components per block of sym_lift_kernel
 
 // The link kernel runs one wave per workgroup. LDS instructions of a wave
@@ -554,6 +555,7 @@ GHX_DEV R wave_sum(R v) {
 }
 
 // A site's matter terms from its 2D hops (sym_link_kernel), one wave per site
+// (kFinishSites waves per workgroup, each with its own LDS slice)
 // with lane l holding component l (d <= 64): the matter link force
 // -kappa g^a, g^a = -2 Im phi^dag T^a_R psi_mu (scalar_higgs.hpp:110-128), and
 // the scalar force 2 kappa sum(hops) - 2 dV/dphi^* with the pi kick, the
@@ -561,15 +563,18 @@ GHX_DEV R wave_sum(R v) {
 // drift (big_matter_kernel's contract), or q1 = sum_mu Re phi^dag psi_mu for
 // the action. Sums over components are wave reductions.
 template <typename T, int N, int D>
-__global__ __launch_bounds__(kSymWave) void sym_finish_kernel(BigArgs<T> a, BigRepTables tb, const T* __restrict__ hops) {
+__global__ __launch_bounds__(kSymWave* kFinishSites) void sym_finish_kernel(BigArgs<T> a, BigRepTables tb, const T* __restrict__ hops) {
   using R = T;
   constexpr int NGN = N * N - 1;
-  __shared__ cx<R> psi[D][kSymWave], phs[kSymWave], lsum[4 * kSymWave];
-  const unsigned g = blockIdx.x, Vp = a.Vp;
+  __shared__ cx<R> psi_s[kFinishSites][D][kSymWave], phs_s[kFinishSites][kSymWave], lsum_s[kFinishSites][4 * kSymWave];
+  cx<R>(&psi)[D][kSymWave] = psi_s[threadIdx.y];
+  cx<R>(&phs)[kSymWave] = phs_s[threadIdx.y];
+  cx<R>(&lsum)[4 * kSymWave] = lsum_s[threadIdx.y];
+  const unsigned g = blockIdx.x * kFinishSites + threadIdx.y, Vp = a.Vp;
   const int r = a.blk_rep[g / kBlock];
   const RepDesc& rd = a.reps[r];
   const unsigned s = g - rd.offset;
-  if (s >= rd.vol) return;  // whole block exits together
+  if (s >= rd.vol) return;  // the wave exits (no workgroup barrier follows)
   const int d = tb.d, l = threadIdx.x;
   const bool on = l < d;
   const T* hg = hops + 2 * (size_t(g) * D * d);
