@@ -241,6 +241,8 @@ def main():
     ap.add_argument("--stream-seg", type=int, default=None, help="force a streaming segment (implies --no-tune)")
     ap.add_argument("--block-order", type=int, default=None, choices=(0, 1, 2),
                     help="force the block walk of the MD-step kernel (implies --no-tune)")
+    ap.add_argument("--sym-pairs", type=int, default=None, choices=(0, 1),
+                    help="large irreps: two links per wave in the symmetric-power engine (1, the default where it applies) or one (0)")
     ap.add_argument("--no-learned", action="store_true", help="skip the exact vs learned-force HMC comparison")
     ap.add_argument("--only-learned", action="store_true", help="run only the exact vs learned-force comparison")
     ap.add_argument("--learned-meas", type=int, default=200, help="measured trajectories per chain in the comparison")
@@ -277,6 +279,8 @@ def main():
     g.hot(0.8, 12345, 1.0)
     g.refresh_momenta()
     V = g.lat.vols[0]
+    if args.sym_pairs is not None:
+        g.lat.set_sym_pairs(args.sym_pairs)
     tuned = {"stream_seg": args.stream_seg or 0, "block_order": args.block_order or 0}
     if args.stream_seg is not None:
         g.lat.set_stream_segment(args.stream_seg)
@@ -305,7 +309,9 @@ def main():
     achieved = V * bpsite / launch_s / 1e9
     kernel = "md_stream_kernel" if tuned.get("stream_seg") else ("md_link_kernel" if tuned.get("link_lanes") else "md_step_kernel")
     if d > 6:  # a large irrep: the matter part runs in the symmetric-power engine before the step kernel
-        kernel = "sym_lift_kernel + sym_link_kernel + sym_finish_kernel + " + kernel
+        link = "sym_link_pair_kernel" if g.lat.sym_pairs() else "sym_link_kernel"
+        kernel = f"sym_lift_kernel + {link} + sym_finish_kernel + " + kernel
+        tuned["sym_pairs"] = g.lat.sym_pairs()
     traffic = load_traffic(args.workload, kernel, tuned.get("block_order", 0))
     line = {
         "metric": "MD site-updates/s; GPU-s per accepted traj, learned vs exact force (1 GPU)",
