@@ -239,7 +239,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
     ap.add_argument("--stream-seg", type=int, default=None, help="force a streaming segment (implies --no-tune)")
-    ap.add_argument("--block-order", type=int, default=None, choices=(0, 1, 2),
+    ap.add_argument("--block-order", type=int, default=None, choices=(0, 1, 2, 3, 4, 5, 6),
                     help="force the block walk of the MD-step kernel (implies --no-tune)")
     ap.add_argument("--sym-pairs", type=int, default=None, choices=(0, 1),
                     help="large irreps: two links per wave in the symmetric-power engine (1, the default where it applies) or one (0)")

@@ -98,6 +98,18 @@ struct ghx_ctx {
     BlockWalk w;
     if (block_order == 0 || nrep != 1) return w;
     const RepDesc& r = hreps[0];
+    if (block_order >= 3) {  // 4D tiles of kBlock sites
+      static const unsigned shape[4][3] = {{16, 4, 2}, {8, 4, 2}, {16, 2, 2}, {32, 2, 2}};
+      const unsigned* sh = shape[block_order - 3];
+      const unsigned tt = unsigned(kBlock) / (sh[0] * sh[1] * sh[2]);
+      if (D != 4 || nblk * unsigned(kBlock) != unsigned(r.vol)) return w;
+      if (unsigned(r.L[0]) % sh[0] || unsigned(r.L[1]) % sh[1] || unsigned(r.L[2]) % sh[2] || unsigned(r.L[3]) % tt) return w;
+      w.order = block_order;
+      w.tx = sh[0]; w.ty = sh[1]; w.tz = sh[2];
+      w.nbx = unsigned(r.L[0]) / sh[0]; w.nby = unsigned(r.L[1]) / sh[1]; w.nbz = unsigned(r.L[2]) / sh[2];
+      w.L0 = unsigned(r.L[0]); w.L1 = unsigned(r.L[1]); w.L2 = unsigned(r.L[2]);
+      return w;
+    }
     const unsigned Lt = unsigned(r.L[D - 1]), V3 = r.stride[D - 1], plane = r.stride[D - 2];
     if (V3 % kBlock || plane % kBlock || Lt % 8 || nblk != V3 / kBlock * Lt) return w;
     w.order = block_order;

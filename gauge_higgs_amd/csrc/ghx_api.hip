@@ -1223,7 +1223,7 @@ int ghx_set_tuning(ghx_ctx* ctx, int key, int value) {
     return GHX_OK;
   }
   if (key == GHX_TUNE_BLOCK_ORDER) {
-    if (value < 0 || value > 2) return fail(ctx, GHX_E_INVALID, "block order must be 0, 1 or 2");
+    if (value < 0 || value > 6) return fail(ctx, GHX_E_INVALID, "block order must be 0 to 6");
     ctx->block_order = value;
     drop_graph(ctx);
     return GHX_OK;

@@ -543,6 +543,15 @@ GHX_DEV void site_update(const StepArgs<T>& a, const typename REP::template Para
 // One site per thread; blocks renumbered XCD-aware.
 template <typename T, int N, int D, class REP>
 __global__ __launch_bounds__(kBlock, (md_waves<T, REP>())) void md_step_kernel(StepArgs<T> a, typename REP::template Params<T> rp) {
+  if (a.walk.order >= 3) {  // one lattice in 4D tiles (BlockWalk)
+    const BlockWalk& w = a.walk;
+    unsigned p = xcd_block(blockIdx.x, gridDim.x), i = threadIdx.x;
+    const unsigned bx = p % w.nbx, by = (p / w.nbx) % w.nby, bz = (p / (w.nbx * w.nby)) % w.nbz, bt = p / (w.nbx * w.nby * w.nbz);
+    const unsigned x = bx * w.tx + i % w.tx, y = by * w.ty + (i / w.tx) % w.ty, z = bz * w.tz + (i / (w.tx * w.ty)) % w.tz;
+    const unsigned t = bt * (kBlock / (w.tx * w.ty * w.tz)) + i / (w.tx * w.ty * w.tz);
+    site_update<T, N, D, REP>(a, rp, a.reps[0], x + w.L0 * (y + w.L1 * (z + w.L2 * t)));
+    return;
+  }
   const unsigned bid = walk_block(xcd_block(blockIdx.x, gridDim.x), a.walk);
   const int r = a.blk_rep[bid];
   const RepDesc& rd = a.reps[r];

@@ -17,9 +17,13 @@ struct McState;
 //     a whole slice of blocks apart;
 //  2: as 1, and inside a slice direction D-2 runs before the lower directions.
 // nbt = blocks per slice, tt = Lt/8, ngy = blocks per (D-2)-plane, lz = L_{D-2}.
+//  3-6: a workgroup takes a 4D tile of kBlock sites (tx x ty x tz x tt: 3 16x4x2x1,
+//  4 8x4x2x2, 5 16x2x2x2, 6 32x2x2x1), tiles numbered x fastest and dealt to
+//  the XCDs in contiguous chunks; nb* = tiles per direction, L* = extents.
 struct BlockWalk {
   int order = 0;
   unsigned nbt = 1, tt = 1, ngy = 1, lz = 1;
+  unsigned tx = 1, ty = 1, tz = 1, nbx = 1, nby = 1, nbz = 1, L0 = 1, L1 = 1, L2 = 1;
 };
 
 struct LaunchCtx {

@@ -194,7 +194,8 @@ class Lattices:
         return v.value
 
     def set_block_order(self, order):
-        """Block walk of the one-site-per-thread MD kernel (0, 1, 2; see ghx.h)."""
+        """Block walk of the one-site-per-thread MD kernel: 0 storage order, 1 and 2
+        XCD slices, 3-6 4D tiles of 128 sites (see ghx.h)."""
         self._check(self._lib.ghx_set_tuning(self.h, 1, order))
 
     def block_order(self):

@@ -174,7 +174,9 @@ int ghx_swap_stats(ghx_ctx* ctx, int* npairs, int* pairs, uint64_t* attempts, ui
  * GHX_TUNE_BLOCK_ORDER = order in which the one-site-per-thread kernel walks the
  * blocks of a single lattice (0 storage order; 1 each XCD's share of slices
  * walked with the last direction fastest; 2 as 1 with direction D-2 next;
- * get_tuning reports 0 where the order does not apply); GHX_TUNE_LINK_LANES = 1
+ * 3-6 a 4D tile of 128 sites per workgroup, 16x4x2x1, 8x4x2x2, 16x2x2x2 or
+ * 32x2x2x1, for D = 4 lattices the tile divides; get_tuning reports 0 where the
+ * order does not apply); GHX_TUNE_LINK_LANES = 1
  * runs the MD step with one lane per link (four per site) instead of one
  * thread per site, for lattices too small to fill the GPU; GHX_TUNE_SYM_ENGINE
  * = 1 (default where it applies) applies a large tensor irrep through

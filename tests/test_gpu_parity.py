@@ -258,16 +258,18 @@ def test_streaming_kernel_equals_site_kernel(seg, prec):
         assert np.array_equal(out[0][k], out[1][k])
 
 
-@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("order", [1, 2, 3, 4, 5, 6])
 @pytest.mark.parametrize("prec", [gha.FP64, gha.FP32MD])
 def test_block_walk_equals_storage_order(order, prec):
-    """Walking the blocks in another order changes only the schedule: every
-    site update reads the step's input fields and writes its own site, so the
-    trajectory is bitwise equal. 16x8x4x8 is the smallest shape the walk
-    applies to (whole-block slices and planes, Lt divisible by 8)."""
+    """Walking the blocks in another order, or covering 4D tiles of 128 sites
+    (orders 3-6: 16x4x2x1, 8x4x2x2, 16x2x2x2, 32x2x2x1), changes only the
+    schedule: every site update reads the step's input fields and writes its
+    own site, so the trajectory is bitwise equal. 32x8x4x8 admits every walk
+    (whole-block slices and planes, Lt divisible by 8, extents divisible by
+    every tile)."""
     out = []
     for o in (0, order):
-        g = gha.GaugeHiggsHMC([16, 8, 4, 8], "2", seed=9, D=4, N=3, precision=prec).configure(beta=5.7, kappa=0.1, lam=0.5, nmd=4)
+        g = gha.GaugeHiggsHMC([32, 8, 4, 8], "2", seed=9, D=4, N=3, precision=prec).configure(beta=5.7, kappa=0.1, lam=0.5, nmd=4)
         g.lat.set_block_order(o)
         assert g.lat.block_order() == o
         g.hot(0.8, 12345, 0.5)
