@@ -37,8 +37,12 @@ struct BigLaunch {
     if (tb.sym.on) {  // symmetric-power engine: lift every site's scalar, one wave per link, then one per site
       T* hops = static_cast<T*>(tb.sym.hops);
       if (!(flags & F_NO_HOP)) {
-        hipLaunchKernelGGL(sym_lift_kernel<T>, dim3(c.Vp / kSymWave, (tb.sym.dimS + kLiftChunk - 1) / kLiftChunk), dim3(kSymWave), 0,
-                           c.stream, c.Vp, (const T*)phi_in, tb.d, tb.sym);
+        if constexpr (sizeof(T) == 4)
+          hipLaunchKernelGGL(sym_lift_mfma_kernel, dim3(c.Vp / (kLiftMfmaWaves * kLiftMfmaSites)), dim3(kSymWave * kLiftMfmaWaves), 0,
+                             c.stream, c.Vp, (const float*)phi_in, tb.d, tb.sym);
+        else
+          hipLaunchKernelGGL(sym_lift_kernel<T>, dim3(c.Vp / kSymWave, (tb.sym.dimS + kLiftChunk - 1) / kLiftChunk), dim3(kSymWave), 0,
+                             c.stream, c.Vp, (const T*)phi_in, tb.d, tb.sym);
         const int both = mode == BIG_ACTION ? 0 : 1;  // the action needs the forward hops only
         if (!link_pair(c, tb, a, hops, both))
           hipLaunchKernelGGL((sym_link_kernel<T, N, D>), dim3(c.Vp, D), dim3(kSymWave), size_t(tb.sym.lds_total) * sizeof(cx<T>),

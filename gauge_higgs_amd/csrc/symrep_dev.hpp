@@ -240,6 +240,62 @@ __global__ __launch_bounds__(kSymWave) void sym_lift_kernel(unsigned Vp, const T
   }
 }
 
+// FP32: the same lift on the matrix cores. Over all sites it is one GEMM,
+// lift[site][q] = sum_l phi_l(site) Yc[q][l]: A = the scalars (M = two rows per
+// site, re and im; K = d in steps of 4, zero beyond d), B = Yc^T (N = dim S in
+// 16-column tiles, the table's zero padding), v_mfma_f32_16x16x4f32 with lane
+// t holding A[t % 16][t / 16] and B[t / 16][t % 16] and receiving
+// D[4 (t / 16) + r][t % 16] in accumulator r. Accumulators r = 0, 1 (and 2, 3)
+// are the re and im of one site at one q, so a lane stores them as one pair
+// and 16 lanes store 16 consecutive q of a site. A wave takes 8 sites (one
+// 16-row tile), four waves a workgroup of 32 sites.
+constexpr int kLiftMfmaWaves = 4;
+constexpr int kLiftMfmaSites = 8;  // per wave
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(kSymWave* kLiftMfmaWaves) void sym_lift_mfma_kernel(unsigned Vp, const float* __restrict__ phi, int d,
+                                                                                  BigRepTables::Sym st) {
+  constexpr int kMaxK = 16;                     // d <= 64
+  constexpr int kTiles = kLiftMfmaSites / 8;  // 16-row tiles per wave
+  const int lane = int(threadIdx.x) % kSymWave, wave = int(threadIdx.x) / kSymWave;
+  const int col = lane % 16, kk = lane / 16;
+  const unsigned site0 = (blockIdx.x * kLiftMfmaWaves + unsigned(wave)) * kLiftMfmaSites;
+  const int dimS = st.dimS, dimSp = st.dimSp, ks = (d + 3) / 4, nt = dimSp / 16;
+  const float* __restrict__ yl = st.Ylf;
+  float* lift = static_cast<float*>(st.lift);
+  float a[kTiles][kMaxK];  // A rows: site site0 + 8 m + col / 2, re (col even) or im
+#pragma unroll
+  for (int m = 0; m < kTiles; ++m)
+#pragma unroll
+    for (int s2 = 0; s2 < kMaxK; ++s2) {
+      const int l = 4 * s2 + kk;
+      a[m][s2] = (s2 < ks && l < d) ? phi[size_t(2 * l + (col & 1)) * Vp + site0 + unsigned(8 * m + col / 2)] : 0.f;
+    }
+#pragma unroll 2
+  for (int t = 0; t < nt; ++t) {
+    float b[kMaxK];
+#pragma unroll
+    for (int s2 = 0; s2 < kMaxK; ++s2) {
+      const int l = 4 * s2 + kk;
+      b[s2] = (s2 < ks && l < d) ? yl[size_t(l) * dimSp + 16 * t + col] : 0.f;
+    }
+    const int q = 16 * t + col;
+#pragma unroll
+    for (int m = 0; m < kTiles; ++m) {
+      mfma_f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int s2 = 0; s2 < kMaxK; ++s2) {
+        if (s2 >= ks) break;
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[m][s2], b[s2], acc, 0, 0, 0);
+      }
+      if (q < dimS) {  // rows 4 kk + r: sites 8 m + 2 kk (r = 0, 1) and 8 m + 2 kk + 1 (r = 2, 3)
+        const unsigned sa = site0 + unsigned(8 * m + 2 * kk);
+        *reinterpret_cast<float2*>(lift + (size_t(sa) * dimS + q) * 2) = float2{acc[0], acc[1]};
+        *reinterpret_cast<float2*>(lift + (size_t(sa + 1) * dimS + q) * 2) = float2{acc[2], acc[3]};
+      }
+    }
+  }
+}
+
 // One link per wave (blockIdx.y = mu): Sym(U_mu(x)) is built once and serves
 // both hops through the link: D(U) phi_{x+mu}, the forward hop of x, projected
 // into hops[2 ((g D + mu) d + l) + (re, im)]; and D(U)^dag phi_x, the backward
@@ -591,6 +647,12 @@ __global__ __launch_bounds__(kSymWave* kFinishSites) void sym_finish_kernel(BigA
     if (l == 0) a.q1[g] = q;
     return;
   }
+  // loads that do not depend on the backward hops, issued before them
+  cx<R> psf[D];
+#pragma unroll
+  for (int mu = 0; mu < D; ++mu) psf[mu] = mu < nmu ? hop(mu) : cx<R>{R(0), R(0)};
+  const cx<R> pv0 = (on && a.mode == BIG_STEP) ? cx<R>{R(a.pi[size_t(2 * l) * Vp + g]), R(a.pi[size_t(2 * l + 1) * Vp + g])}
+                                                 : cx<R>{R(0), R(0)};
   const R k2 = R(2.0 * rd.kappa);
   const BigRepTables::Sym& st = tb.sym;
   const int lpg = st.glpg;
@@ -612,7 +674,7 @@ __global__ __launch_bounds__(kSymWave* kFinishSites) void sym_finish_kernel(BigA
       const R* y = (sizeof(R) == 4 ? reinterpret_cast<const R*>(st.Ypf) : reinterpret_cast<const R*>(st.Yp)) +
                    (size_t(st.ng) * d + l) * 4;
       cx<R> acc = {R(0), R(0)};
-#pragma unroll 2
+#pragma unroll 4
       for (int gq = 0; gq < st.ng; ++gq, y += 4 * d) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -628,7 +690,7 @@ __global__ __launch_bounds__(kSymWave* kFinishSites) void sym_finish_kernel(BigA
     // for every direction over the lanes of generator a (each entry read once)
 #pragma unroll
     for (int mu = 0; mu < D; ++mu) {
-      const cx<R> ps = hop(mu);
+      const cx<R> ps = psf[mu];
       psi[mu][l] = ps;
       f.r += k2 * ps.r;
       f.i += k2 * ps.i;
@@ -637,7 +699,7 @@ __global__ __launch_bounds__(kSymWave* kFinishSites) void sym_finish_kernel(BigA
     R gp[D];
 #pragma unroll
     for (int mu = 0; mu < D; ++mu) gp[mu] = R(0);
-#pragma unroll 2
+#pragma unroll 4
     for (int k = 0; k < st.gK; ++k) {
       const int e = glm[k * kSymWave];
       const cx<R> v = ld_cx(gval + 2 * k * kSymWave);
@@ -676,7 +738,7 @@ __global__ __launch_bounds__(kSymWave* kFinishSites) void sym_finish_kernel(BigA
   const bool abs_eps = a.flags & F_ABS_EPS;
   const R ek = R(abs_eps ? a.ck : a.ck * rd.eps), ed = R(abs_eps ? a.cd : a.cd * rd.eps);
   const bool frozen = a.flags & F_FROZEN;
-  cx<R> pv = on ? cx<R>{R(a.pi[size_t(2 * l) * Vp + g]), R(a.pi[size_t(2 * l + 1) * Vp + g])} : cx<R>{R(0), R(0)};
+  cx<R> pv = pv0;
   if (a.flags & F_KICK) {
     pv.r = fma(ek, f.r, pv.r);
     pv.i = fma(ek, f.i, pv.i);
