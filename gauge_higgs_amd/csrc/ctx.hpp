@@ -51,6 +51,7 @@ struct ghx_ctx {
   int stream_seg = 0;  // streaming MD kernel (0 = one site per thread)
   int block_order = 0;  // BlockWalk order of the MD-step kernel (single lattice)
   int link_lanes = 0;   // link-parallel MD step (one lane per link) for small lattices
+  bool xlane_enabled = true;  // GHX_TUNE_X_LANES
   // multi-timescale integrator (0 = single scale)
   int n_outer = 0, m_inner = 1;
   // surrogate link force (learned); S planes are filled by surr_fill before a kick
@@ -119,6 +120,20 @@ struct ghx_ctx {
     w.lz = unsigned(r.L[D - 2]);
     return w;
   }
+  // Every wave of the one-site-per-thread MD step holds whole x-lines and is
+  // either wholly live or wholly past the end of its lattice: SU(3) with two
+  // stored rows, L0 a power of two dividing 64, replica offsets and volumes (and
+  // the cross-section of a single lattice, for the streaming kernel) multiples
+  // of 64 sites.
+  bool xlane_ok() const {
+    if (tsize != 4 || N != 3 || link_rows() != 2) return false;
+    for (int r = 0; r < nrep; ++r) {
+      const RepDesc& h = hreps[r];
+      const unsigned l0 = unsigned(h.L[0]);
+      if (l0 < 2 || l0 > 64 || (l0 & (l0 - 1)) || h.offset % 64 || h.vol % 64) return false;
+    }
+    return nrep != 1 || hreps[0].stride[D - 1] % 64 == 0;
+  }
   LaunchCtx lc() const {
     LaunchCtx c;
     c.stream = stream; c.nblk = nblk; c.nred = nred; c.Vp = Vp;
@@ -138,6 +153,7 @@ struct ghx_ctx {
     c.Lt = unsigned(hreps[0].L[D - 1]);
     c.walk = block_walk();
     c.link_lanes = link_lanes;
+    c.xlane = (xlane_enabled && xlane_ok()) ? 1 : 0;
     return c;
   }
 };

@@ -1259,6 +1259,12 @@ int ghx_set_tuning(ghx_ctx* ctx, int key, int value) {
     drop_graph(ctx);
     return GHX_OK;
   }
+  if (key == GHX_TUNE_X_LANES) {
+    if (value < 0 || value > 1) return fail(ctx, GHX_E_INVALID, "x lanes must be 0 or 1");
+    ctx->xlane_enabled = value != 0;
+    drop_graph(ctx);
+    return GHX_OK;
+  }
   return fail(ctx, GHX_E_INVALID, "unknown tuning key");
 }
 
@@ -1269,6 +1275,7 @@ int ghx_get_tuning(const ghx_ctx* ctx, int key, int* value) {
   if (key == GHX_TUNE_LINK_LANES) { *value = ctx->link_lanes; return GHX_OK; }
   if (key == GHX_TUNE_SYM_ENGINE) { *value = ctx->tb.sym.on; return GHX_OK; }
   if (key == GHX_TUNE_SYM_PAIRS) { *value = ctx->tb.sym.on && ctx->tb.sym.pair; return GHX_OK; }
+  if (key == GHX_TUNE_X_LANES) { *value = ctx->xlane_enabled && ctx->xlane_ok(); return GHX_OK; }
   return GHX_E_INVALID;
 }
 

@@ -244,6 +244,14 @@ GHX_DEV unsigned walk_block(unsigned p, const BlockWalk& w) {
   return t * w.nbt + j;
 }
 
+// Orders LDS accesses across the lanes of one wave: a wave's LDS operations
+// complete in issue order, so a wave-scope fence (no s_barrier) suffices.
+GHX_DEV void wave_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Site -> neighbour offsets for a periodic lattice (geometry.hpp:20-38 order:
 // direction 0 fastest). up[m]/dn[m] are signed offsets to add to the site.
 template <int D>
@@ -335,9 +343,23 @@ GHX_DEV void site_phi_begin(const StepArgs<T>& a, const RepDesc& rd, unsigned g,
 // same loop as P^dag, the term -(beta/N) im_tr(P), stashed in slot nu. Link mu
 // then skips its forward staples for nu < mu: 6 plaquettes per site instead of
 // 12 forward staples, and 18 fewer link loads.
-template <typename T, int N, int D, class REP>
+//
+// XL (x-lines in lanes: every wave holds whole lines of direction 0, so the
+// site x - 0 sits in lane xsrc / 4 of the same wave; links run in the order
+// 1, ..., D-1, 0): link mu >= 1 forms the plaquette of plane (0, mu),
+// P = a b c^dag d^dag with a = U_0(x), b = U_mu(x+0), c = U_0(x+mu), d = U_mu(x),
+// as a (b (d c)^dag). Its lower staple (d c)^dag a is the backward staple of
+// link mu at x+0 in direction 0, so every lane forms it once for its x+0
+// neighbour, parks rows 0, 1 in its LDS column lst, and at the end of the
+// staple sum reads its own from the column of lane x-0 (the thread xsrc of the
+// block) instead of forming it from three more links. A wave's LDS operations
+// complete in issue order, so a wave-scope fence orders the lanes' writes
+// before the reads. Link 0, updated last, has all of its forward plaquette
+// terms in the stash.
+template <typename T, int N, int D, class REP, bool XL = false>
 GHX_DEV void link_update(const StepArgs<T>& a, const typename REP::template Params<T>& rp, const RepDesc& rd, unsigned g,
-                         const Nbr<D>& nb, int mu, const cx<T>* phx, cx<T>* fph, T* pst = nullptr) {
+                         const Nbr<D>& nb, int mu, const cx<T>* phx, cx<T>* fph, T* pst = nullptr, T* lst = nullptr,
+                         int xsrc = 0) {
   using R = T;
   constexpr int NC = link_reals<T, N>();
   constexpr int NGN = N * N - 1;
@@ -393,6 +415,35 @@ GHX_DEV void link_update(const StepArgs<T>& a, const typename REP::template Para
   constexpr bool kTopRows = (N == 3 || N == 2) && link_rows<T, N>() == N - 1;
   constexpr bool kQuat = N == 2 && kTopRows;
   constexpr bool kShare = N == 3 && kTopRows;
+  constexpr bool kXL = XL && kShare;
+  if constexpr (kXL) {
+    if (do_gauge && mu != 0) {  // plane (0, mu), see above
+      const Mat<R, N> A0 = bload_link<T, R, N>(Uin, 0, g);
+      Mat<R, N> DC = mul_nn_top(Ux, bload_link<T, R, N>(Uin, 0, gpm));
+      rebuild_last_row<R, N>(DC);
+      {
+        const Mat<R, N> Lo = mul_dn_top(DC, A0);  // (d c)^dag a, for the x+0 neighbour
+#pragma unroll
+        for (int i = 0; i < N - 1; ++i)
+#pragma unroll
+          for (int j = 0; j < N; ++j) {
+            lst[(2 * (i * N + j)) * kBlock] = Lo.e[i][j].r;
+            lst[(2 * (i * N + j) + 1) * kBlock] = Lo.e[i][j].i;
+          }
+      }
+      Mat<R, N> BT = mul_nd_top(bload_link<T, R, N, false>(Uin, mu, g + nb.upv(rd, 0)), DC);
+      rebuild_last_row<R, N>(BT);
+      Mat<R, N> Pl = mul_nn_top(A0, BT);
+      rebuild_last_row<R, N>(Pl);
+      R fp[NGN];
+      im_tr_gen<R, N>(Pl, fp);
+#pragma unroll
+      for (int c = 0; c < NGN; ++c) {
+        fa[c] = fma(-bN, fp[c], fa[c]);
+        pst[c * kBlock] += bN * fp[c];
+      }
+    }
+  }
 #if GHX_NU_UNROLL
 #pragma unroll
 #else
@@ -400,8 +451,9 @@ GHX_DEV void link_update(const StepArgs<T>& a, const typename REP::template Para
 #endif
   for (int nu = 0; nu < (do_gauge ? D : 0); ++nu) {
     if (nu == mu) continue;
+    if (kXL && mu != 0 && nu == 0) continue;  // plane (0, mu) done above
     if (kShare && pst) {
-      if (nu > mu) {  // the plaquette of plane (mu, nu), for both of its links at x
+      if (nu > mu && !(kXL && mu == 0)) {  // the plaquette of plane (mu, nu), for both of its links at x
         Mat<R, N> AB = mul_nd_top(bload_link<T, R, N, false>(Uin, nu, gpm), bload_link<T, R, N>(Uin, mu, g + nb.upv(rd, nu)));
         rebuild_last_row<R, N>(AB);
         Mat<R, N> S = mul_nd_top(AB, bload_link<T, R, N>(Uin, nu, g));
@@ -439,6 +491,26 @@ GHX_DEV void link_update(const StepArgs<T>& a, const typename REP::template Para
         const Mat<R, N> AB = mul_dd(bload_link<T, R, N>(Uin, nu, gpm + nb.dnv(rd, nu)), bload_link<T, R, N>(Uin, mu, gmn));
         mac_nn(Sig, AB, bload_link<T, R, N>(Uin, nu, gmn));
       }
+    }
+  }
+  if constexpr (kXL) {
+    if (do_gauge && mu != 0) {  // this link's backward staple in direction 0, from lane x-0
+      wave_fence();
+      const T* src = lst - int(threadIdx.x) + xsrc;
+      Mat<R, N> Lb;
+#pragma unroll
+      for (int i = 0; i < N - 1; ++i)
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+          Lb.e[i][j].r = src[(2 * (i * N + j)) * kBlock];
+          Lb.e[i][j].i = src[(2 * (i * N + j) + 1) * kBlock];
+        }
+      wave_fence();
+      rebuild_last_row<R, N>(Lb);
+#pragma unroll
+      for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = 0; j < N; ++j) Sig.e[i][j] = cadd(Sig.e[i][j], Lb.e[i][j]);
     }
   }
   Mat<R, N> X;
@@ -521,7 +593,9 @@ GHX_DEV void site_phi_end(const StepArgs<T>& a, const RepDesc& rd, unsigned g, c
   if ((a.flags & F_KICK) || (frozen && (a.flags & F_DRIFT))) bstore_vec<T, R, DR>(Pi, g, pv);
 }
 
-template <typename T, int N, int D, class REP>
+// XL: the caller guarantees that every wave holds whole x-lines (L0 divides 64,
+// site = lane mod 64) and that all 64 lanes of a wave are live.
+template <typename T, int N, int D, class REP, bool XL = false>
 GHX_DEV void site_update(const StepArgs<T>& a, const typename REP::template Params<T>& rp, const RepDesc& rd, unsigned s) {
   constexpr int DR = REP::DR;
   const unsigned g = rd.offset + s;
@@ -535,29 +609,43 @@ GHX_DEV void site_update(const StepArgs<T>& a, const typename REP::template Para
 #pragma unroll
     for (int c = 0; c < D * (N * N - 1); ++c) pst[c * kBlock] = T(0);
   }
+  T* lst = nullptr;
+  int xsrc = 0;  // the thread of this block that holds x - 0
+  if constexpr (XL && N == 3 && link_rows<T, N>() == 2) {
+    __shared__ T lower[2 * (N - 1) * N * kBlock];
+    lst = lower + threadIdx.x;
+    const int lane = int(threadIdx.x) & 63, l0 = rd.L[0];
+    xsrc = (int(threadIdx.x) & ~63) | (lane & ~(l0 - 1)) | ((lane + l0 - 1) & (l0 - 1));
+  }
 #pragma unroll 1
-  for (int mu = 0; mu < D; ++mu) link_update<T, N, D, REP>(a, rp, rd, g, nb, mu, phx, fph, pst);
+  for (int i = 0; i < D; ++i) {
+    const int mu = XL ? (i + 1 < D ? i + 1 : 0) : i;
+    link_update<T, N, D, REP, XL>(a, rp, rd, g, nb, mu, phx, fph, pst, lst, xsrc);
+  }
   if constexpr (DR > 0) site_phi_end<T, DR>(a, rd, g, phx, fph);
 }
 
-// One site per thread; blocks renumbered XCD-aware.
-template <typename T, int N, int D, class REP>
+// One site per thread; blocks renumbered XCD-aware. XL: launched only in
+// storage-order or XCD-slice walks of lattices whose x-lines fill waves.
+template <typename T, int N, int D, class REP, bool XL = false>
 __global__ __launch_bounds__(kBlock, (md_waves<T, REP>())) void md_step_kernel(StepArgs<T> a, typename REP::template Params<T> rp) {
-  if (a.walk.order >= 3) {  // one lattice in 4D tiles (BlockWalk)
-    const BlockWalk& w = a.walk;
-    unsigned p = xcd_block(blockIdx.x, gridDim.x), i = threadIdx.x;
-    const unsigned bx = p % w.nbx, by = (p / w.nbx) % w.nby, bz = (p / (w.nbx * w.nby)) % w.nbz, bt = p / (w.nbx * w.nby * w.nbz);
-    const unsigned x = bx * w.tx + i % w.tx, y = by * w.ty + (i / w.tx) % w.ty, z = bz * w.tz + (i / (w.tx * w.ty)) % w.tz;
-    const unsigned t = bt * (kBlock / (w.tx * w.ty * w.tz)) + i / (w.tx * w.ty * w.tz);
-    site_update<T, N, D, REP>(a, rp, a.reps[0], x + w.L0 * (y + w.L1 * (z + w.L2 * t)));
-    return;
+  if constexpr (!XL) {
+    if (a.walk.order >= 3) {  // one lattice in 4D tiles (BlockWalk)
+      const BlockWalk& w = a.walk;
+      unsigned p = xcd_block(blockIdx.x, gridDim.x), i = threadIdx.x;
+      const unsigned bx = p % w.nbx, by = (p / w.nbx) % w.nby, bz = (p / (w.nbx * w.nby)) % w.nbz, bt = p / (w.nbx * w.nby * w.nbz);
+      const unsigned x = bx * w.tx + i % w.tx, y = by * w.ty + (i / w.tx) % w.ty, z = bz * w.tz + (i / (w.tx * w.ty)) % w.tz;
+      const unsigned t = bt * (kBlock / (w.tx * w.ty * w.tz)) + i / (w.tx * w.ty * w.tz);
+      site_update<T, N, D, REP>(a, rp, a.reps[0], x + w.L0 * (y + w.L1 * (z + w.L2 * t)));
+      return;
+    }
   }
   const unsigned bid = walk_block(xcd_block(blockIdx.x, gridDim.x), a.walk);
   const int r = a.blk_rep[bid];
   const RepDesc& rd = a.reps[r];
   const unsigned s = bid * kBlock + threadIdx.x - rd.offset;
   if (s >= rd.vol) return;
-  site_update<T, N, D, REP>(a, rp, rd, s);
+  site_update<T, N, D, REP, XL>(a, rp, rd, s);
 }
 
 // Streaming variant for a single lattice: a block owns kBlock consecutive
@@ -566,7 +654,7 @@ __global__ __launch_bounds__(kBlock, (md_waves<T, REP>())) void md_step_kernel(S
 // by the same block one or two iterations earlier (cache-resident) instead of
 // being re-fetched from HBM. Block b handles cross-section tile (b % ntile),
 // segment (b / ntile); ntile % 8 == 0 keeps all segments of a tile on one XCD.
-template <typename T, int N, int D, class REP>
+template <typename T, int N, int D, class REP, bool XL = false>
 __global__ __launch_bounds__(kBlock, md_waves<T>()) void md_stream_kernel(StepArgs<T> a, typename REP::template Params<T> rp, unsigned ntile,
                                                                        int seg) {
   const RepDesc& rd = a.reps[0];
@@ -578,7 +666,7 @@ __global__ __launch_bounds__(kBlock, md_waves<T>()) void md_stream_kernel(StepAr
   const int t0 = int(sg) * seg;
   const int t1 = t0 + seg < Lt ? t0 + seg : Lt;
 #pragma unroll 1
-  for (int t = t0; t < t1; ++t) site_update<T, N, D, REP>(a, rp, rd, s3 + unsigned(t) * V3);
+  for (int t = t0; t < t1; ++t) site_update<T, N, D, REP, XL>(a, rp, rd, s3 + unsigned(t) * V3);
 }
 
 // Link-parallel variant for lattices too small to fill the GPU with one thread

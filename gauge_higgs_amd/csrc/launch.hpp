@@ -35,17 +35,32 @@ struct Launch {
     }
     StepArgs<T> a{c.reps, c.blk_rep, c.Vp, (const T*)U_in, (T*)U_out, (T*)P, (const T*)phi_in, (T*)phi_out, (T*)pi, (T*)F, (T*)Fphi,
                   ck, cd, flags, (const T*)c.G, (const T*)c.S, c.surr_sign, (const T*)c.Fon, c.walk};
+    // x-lines in lanes (link_update): SU(3) with two stored rows, and only where
+    // the context found every wave to hold whole x-lines
+    constexpr bool kXLable = N == 3 && link_rows<T, N>() == 2;
+    const bool xl = kXLable && c.xlane && !c.link_lanes && c.walk.order < 3;
     if (c.stream_seg > 0) {
       const unsigned V3 = c.V3, Lt = c.Lt;
       unsigned ntile = (V3 + kBlock - 1) / kBlock;
       ntile = (ntile + 7) / 8 * 8;
       const unsigned nseg = (Lt + unsigned(c.stream_seg) - 1) / unsigned(c.stream_seg);
+      if constexpr (kXLable)
+        if (xl) {
+          hipLaunchKernelGGL((md_stream_kernel<T, N, D, REP, true>), dim3(ntile * nseg), dim3(kBlock), 0, c.stream, a,
+                             ParamFill<REP, T>::make(c), ntile, c.stream_seg);
+          return;
+        }
       hipLaunchKernelGGL((md_stream_kernel<T, N, D, REP>), dim3(ntile * nseg), dim3(kBlock), 0, c.stream, a, ParamFill<REP, T>::make(c),
                          ntile, c.stream_seg);
     } else if (c.link_lanes) {
       hipLaunchKernelGGL((md_link_kernel<T, N, D, REP>), dim3(c.nblk * (kBlock / kLinkSites)), dim3(kLinkSites * D), 0, c.stream, a,
                          ParamFill<REP, T>::make(c));
     } else {
+      if constexpr (kXLable)
+        if (xl) {
+          hipLaunchKernelGGL((md_step_kernel<T, N, D, REP, true>), dim3(c.nblk), dim3(kBlock), 0, c.stream, a, ParamFill<REP, T>::make(c));
+          return;
+        }
       hipLaunchKernelGGL((md_step_kernel<T, N, D, REP>), dim3(c.nblk), dim3(kBlock), 0, c.stream, a, ParamFill<REP, T>::make(c));
     }
   }

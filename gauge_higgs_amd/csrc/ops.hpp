@@ -41,6 +41,7 @@ struct LaunchCtx {
   unsigned V3, Lt;  // cross-section volume and last extent of replica 0 (streaming kernel)
   BlockWalk walk;   // block order of the MD-step kernel
   int link_lanes;   // 1: link-parallel MD step (one lane per link)
+  int xlane;        // 1: every wave of the MD step holds whole x-lines (x-neighbour lanes share products)
   const void* S;    // surrogate link force planes
   double surr_sign;
   int frozen;       // |phi_x| = 1 scalar: adds F_FROZEN to every MD launch

@@ -24,11 +24,7 @@ constexpr int kLiftChunk = 16;  // lifted components per block of sym_lift_kerne
 // complete in issue order, so ordering one step's LDS writes before the next
 // step's reads needs only a wave-scope fence (no s_barrier, and no wait for the
 // global load prefetched for the next step).
-GHX_DEV void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
+GHX_DEV void wave_sync() { wave_fence(); }
 
 // One complex value as a single 8- or 16-byte access (p at an even index).
 template <typename T>

@@ -233,6 +233,17 @@ class Lattices:
         self._check(self._lib.ghx_get_tuning(self.h, 4, C.byref(v)))
         return v.value
 
+    def set_x_lanes(self, on):
+        """SU(3) FP32 MD where waves hold whole x-lines: each site hands its lower
+        staple of plane (0, mu) to the x+0 neighbour lane (1, default) or every
+        site forms its own backward staples (0)."""
+        self._check(self._lib.ghx_set_tuning(self.h, 5, int(on)))
+
+    def x_lanes(self):
+        v = C.c_int()
+        self._check(self._lib.ghx_get_tuning(self.h, 5, C.byref(v)))
+        return v.value
+
     def autotune(self, steps=10):
         """Pick the fastest MD-step launch shape on the live fields (fields restored)."""
         best, ms = C.c_int(), C.c_double()

@@ -185,10 +185,16 @@ int ghx_swap_stats(ghx_ctx* ctx, int* npairs, int* pairs, uint64_t* attempts, ui
  * GHX_TUNE_SYM_PAIRS = 1 (default where it applies: SU(3), first Young row
  * <= 6) runs the engine two links per wave with Sym^k(U) built a column per
  * lane, 0 one link per wave with the per-entry LDS program (get_tuning reports
- * whether the paired kernel is in use). The autotuner times every candidate
- * for `steps` MD steps on the live fields, restores the fields, and keeps the
- * fastest. */
-enum { GHX_TUNE_STREAM_SEG = 0, GHX_TUNE_BLOCK_ORDER = 1, GHX_TUNE_LINK_LANES = 2, GHX_TUNE_SYM_ENGINE = 3, GHX_TUNE_SYM_PAIRS = 4 };
+ * whether the paired kernel is in use); GHX_TUNE_X_LANES = 1 (default where it
+ * applies: SU(3) in FP32 MD, L0 a power of two dividing 64, volumes and
+ * cross-sections multiples of 64 sites) lets the one-site-per-thread and
+ * streaming MD steps hand each site's lower staple in plane (0, mu) to its
+ * x + 0 neighbour lane instead of forming it there (same physics, another
+ * rounding; not with the 4D-tile walks; get_tuning reports whether it
+ * applies). The autotuner times every candidate for `steps` MD steps on the
+ * live fields, restores the fields, and keeps the fastest. */
+enum { GHX_TUNE_STREAM_SEG = 0, GHX_TUNE_BLOCK_ORDER = 1, GHX_TUNE_LINK_LANES = 2, GHX_TUNE_SYM_ENGINE = 3, GHX_TUNE_SYM_PAIRS = 4,
+       GHX_TUNE_X_LANES = 5 };
 int ghx_set_tuning(ghx_ctx* ctx, int key, int value);
 int ghx_get_tuning(const ghx_ctx* ctx, int key, int* value);
 int ghx_autotune(ghx_ctx* ctx, int steps, int* best_seg, double* best_ms_per_step);

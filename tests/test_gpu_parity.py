@@ -266,10 +266,13 @@ def test_block_walk_equals_storage_order(order, prec):
     schedule: every site update reads the step's input fields and writes its
     own site, so the trajectory is bitwise equal. 32x8x4x8 admits every walk
     (whole-block slices and planes, Lt divisible by 8, extents divisible by
-    every tile)."""
+    every tile). The 4D tiles do not hold whole x-lines per wave, so there
+    the x-lane hand-over of lower staples is switched off on both sides."""
     out = []
     for o in (0, order):
         g = gha.GaugeHiggsHMC([32, 8, 4, 8], "2", seed=9, D=4, N=3, precision=prec).configure(beta=5.7, kappa=0.1, lam=0.5, nmd=4)
+        if order >= 3:
+            g.lat.set_x_lanes(0)
         g.lat.set_block_order(o)
         assert g.lat.block_order() == o
         g.hot(0.8, 12345, 0.5)
@@ -279,6 +282,48 @@ def test_block_walk_equals_storage_order(order, prec):
     assert out[0][0] == out[1][0]
     for k in (1, 2, 3):
         assert np.array_equal(out[0][k], out[1][k])
+
+
+XL_CASES = [[4, 4, 4, 4], [8, 4, 2, 4], [2, 4, 8, 4], [16, 2, 2, 4], [32, 2, 2, 2], [64, 2, 2, 2]]
+
+
+@pytest.mark.parametrize("dims", XL_CASES, ids=["x".join(map(str, d)) for d in XL_CASES])
+@pytest.mark.parametrize("rep", ["2", "fund", ""])
+def test_x_lanes_equal_own_staples(dims, rep):
+    """SU(3) FP32 MD with the lower staple of plane (0, mu) handed from lane x-0
+    to lane x (x-lines in lanes, links updated in the order 1, 2, 3, 0) against
+    every site forming its own backward staples: forces, the one-site-per-thread
+    step and the streaming step agree to FP32 rounding, for each L0 that fills
+    waves with whole x-lines (2 to 64)."""
+    out = []
+    for xl, seg in ((0, 0), (1, 0), (1, 2)):
+        g = gha.GaugeHiggsHMC(dims, rep, seed=9, D=4, N=3, precision=gha.FP32MD).configure(beta=5.7, kappa=0.2, lam=0.5, nmd=6)
+        g.lat.set_x_lanes(xl)
+        assert g.lat.x_lanes() == xl
+        g.lat.set_stream_segment(seg)
+        g.hot(0.8, 12345, 0.5)
+        F, Fp = g.forces()
+        g.trajectory()
+        out.append((F, Fp, g.last_dH, g.get("U"), g.get("phi") if rep else None, g.get("P")))
+    base = out[0]
+    for b in out[1:]:
+        assert relmax(b[0], base[0]) < 2e-5
+        if rep:
+            assert relmax(b[1], base[1]) < 2e-5
+            assert relmax(b[4], base[4]) < 2e-3
+        assert abs(b[2] - base[2]) < 2e-2
+        for k in (3, 5):
+            assert relmax(b[k], base[k]) < 2e-3
+    # the one-site-per-thread and the streaming kernel run the same arithmetic
+    assert out[1][2] == out[2][2]
+    for k in (3, 5):
+        assert np.array_equal(out[1][k], out[2][k])
+
+
+@pytest.mark.parametrize("dims,prec", [([6, 4, 4, 4], gha.FP32MD), ([4, 6, 4, 10], gha.FP32MD), ([4, 4, 4, 4], gha.FP64)])
+def test_x_lanes_off_where_waves_do_not_hold_x_lines(dims, prec):
+    g = gha.GaugeHiggsHMC(dims, "2", seed=9, D=4, N=3, precision=prec)
+    assert g.lat.x_lanes() == 0
 
 
 def test_block_walk_falls_back_where_it_does_not_apply():
