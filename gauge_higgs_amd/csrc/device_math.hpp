@@ -190,19 +190,6 @@ template <typename R, int N> GHX_DEV Mat<R, N> mul_dn_top(const Mat<R, N>& A, co
     }
   return C;
 }
-// Rows 0..N-2 of another lane's matrix (ds_bpermute; src_byte = 4 * source
-// lane). Every lane of the wave must execute it.
-template <int N> GHX_DEV Mat<float, N> lane_take_top(const Mat<float, N>& m, int src_byte) {
-  Mat<float, N> o;
-#pragma unroll
-  for (int i = 0; i < N - 1; ++i)
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-      o.e[i][j].r = __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(src_byte, __builtin_bit_cast(int, m.e[i][j].r)));
-      o.e[i][j].i = __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(src_byte, __builtin_bit_cast(int, m.e[i][j].i)));
-    }
-  return o;
-}
 // Rows 0..N-2 of Acc += A B^dag and of Acc += A^dag B.
 template <typename R, int N> GHX_DEV void mac_nd_top(Mat<R, N>& Acc, const Mat<R, N>& A, const Mat<R, N>& B) {
 #pragma unroll

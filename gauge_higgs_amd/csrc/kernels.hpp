@@ -345,21 +345,43 @@ GHX_DEV void site_phi_begin(const StepArgs<T>& a, const RepDesc& rd, unsigned g,
 // 12 forward staples, and 18 fewer link loads.
 //
 // XL (x-lines in lanes: every wave holds whole lines of direction 0, so the
-// site x - 0 sits in lane xsrc / 4 of the same wave; links run in the order
-// 1, ..., D-1, 0): link mu >= 1 forms the plaquette of plane (0, mu),
-// P = a b c^dag d^dag with a = U_0(x), b = U_mu(x+0), c = U_0(x+mu), d = U_mu(x),
-// as a (b (d c)^dag). Its lower staple (d c)^dag a is the backward staple of
-// link mu at x+0 in direction 0, so every lane forms it once for its x+0
-// neighbour, parks rows 0, 1 in its LDS column lst, and at the end of the
-// staple sum reads its own from the column of lane x-0 (the thread xsrc of the
-// block) instead of forming it from three more links. A wave's LDS operations
-// complete in issue order, so a wave-scope fence orders the lanes' writes
-// before the reads. Link 0, updated last, has all of its forward plaquette
-// terms in the stash.
+// sites x - 0 and x + 0 sit in lanes xl.src and xl.dst of the same wave; links
+// run in the order 1, ..., D-1, 0): link mu >= 1 forms the plaquette of plane
+// (0, mu), P = a b c^dag d^dag with a = U_0(x), b = U_mu(x+0) (rows 0, 1 taken
+// from lane x+0, whose own link it is), c = U_0(x+mu), d = U_mu(x), as
+// a (b (d c)^dag). Its lower staple (d c)^dag a is the backward staple of link
+// mu at x+0 in direction 0, so every lane forms it once for its x+0 neighbour,
+// parks rows 0, 1 in its LDS column (the stash slot that link 1 does not use
+// in this order, plus four words), and at the end of the staple sum reads its
+// own from the column of lane x-0 instead of forming it from three more links.
+// A wave's LDS operations complete in issue order, so a wave-scope fence
+// orders the lanes' writes before the reads. Link 0, updated last, has all of
+// its forward plaquette terms in the stash; its hop terms take phi(x+0) from
+// lane x+0 and the backward hop D(U_0(x-0))^dag phi(x-0) as lane x-0 formed it
+// from its own link and scalar.
+struct XLanes {
+  int src = 0, dst = 0;  // lanes of x - 0 and x + 0 (byte addresses for ds_bpermute)
+  int src_thread = 0;    // thread of this block that holds x - 0
+};
+GHX_DEV float lane_take(float v, int src_byte) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(src_byte, __builtin_bit_cast(int, v)));
+}
+GHX_DEV double lane_take(double v, int) { return v; }  // XL is FP32 only
+template <typename R, int DR>
+GHX_DEV void lane_take_vec(const cx<R>* v, int src_byte, cx<R>* o) {
+#pragma unroll
+  for (int k = 0; k < DR; ++k) o[k] = {lane_take(v[k].r, src_byte), lane_take(v[k].i, src_byte)};
+}
+// Stash word of element k (0..11) of the parked lower staple: slot 1, then four
+// words past the D slots.
+template <int D, int NGN>
+constexpr int xl_word(int k) { return k < NGN ? NGN + k : D * NGN + k - NGN; }
+template <typename T, int N>
+constexpr bool xl_able() { return N == 3 && link_rows<T, N>() == 2; }
+
 template <typename T, int N, int D, class REP, bool XL = false>
 GHX_DEV void link_update(const StepArgs<T>& a, const typename REP::template Params<T>& rp, const RepDesc& rd, unsigned g,
-                         const Nbr<D>& nb, int mu, const cx<T>* phx, cx<T>* fph, T* pst = nullptr, T* lst = nullptr,
-                         int xsrc = 0) {
+                         const Nbr<D>& nb, int mu, const cx<T>* phx, cx<T>* fph, T* pst = nullptr, const XLanes& xl = {}) {
   using R = T;
   constexpr int NC = link_reals<T, N>();
   constexpr int NGN = N * N - 1;
@@ -381,11 +403,13 @@ GHX_DEV void link_update(const StepArgs<T>& a, const typename REP::template Para
   R fa[NGN];
 #pragma unroll
   for (int c = 0; c < NGN; ++c) fa[c] = R(0);
+  constexpr bool kXL = XL && xl_able<T, N>();
   if constexpr (DR > 0) if (do_hop) {
     const Planes<T> Ph(a.phi_in, Vp, 2 * DR);
     {
       cx<R> phy[DR], psi[DR];
-      bload_vec<T, R, DR>(Ph, gpm, phy);
+      if (kXL && mu == 0) lane_take_vec<R, DR>(phx, xl.dst, phy);  // phi(x+0) of lane x+0
+      else bload_vec<T, R, DR>(Ph, gpm, phy);
       Mat<R, N> M;
       REP::fwd_M(Ux, phy, phx, psi, M, rp);
       R gm[NGN];
@@ -396,11 +420,18 @@ GHX_DEV void link_update(const StepArgs<T>& a, const typename REP::template Para
       for (int k = 0; k < DR; ++k) { fph[k].r = fma(k2, psi[k].r, fph[k].r); fph[k].i = fma(k2, psi[k].i, fph[k].i); }
     }
     {
-      const unsigned gmm = g + nb.dnv(rd, mu);
-      cx<R> phb[DR], chi[DR];
-      bload_vec<T, R, DR>(Ph, gmm, phb);
-      const Mat<R, N> Ub = bload_link<T, R, N>(Uin, mu, gmm);
-      REP::apply_dag(Ub, phb, chi, rp);
+      cx<R> chi[DR];
+      if (kXL && mu == 0) {  // D(U_0(x))^dag phi_x is the backward hop of x+0: lane x-0 formed this one
+        cx<R> own[DR];
+        REP::apply_dag(Ux, phx, own, rp);
+        lane_take_vec<R, DR>(own, xl.src, chi);
+      } else {
+        const unsigned gmm = g + nb.dnv(rd, mu);
+        cx<R> phb[DR];
+        bload_vec<T, R, DR>(Ph, gmm, phb);
+        const Mat<R, N> Ub = bload_link<T, R, N>(Uin, mu, gmm);
+        REP::apply_dag(Ub, phb, chi, rp);
+      }
 #pragma unroll
       for (int k = 0; k < DR; ++k) { fph[k].r = fma(k2, chi[k].r, fph[k].r); fph[k].i = fma(k2, chi[k].i, fph[k].i); }
     }
@@ -415,7 +446,6 @@ GHX_DEV void link_update(const StepArgs<T>& a, const typename REP::template Para
   constexpr bool kTopRows = (N == 3 || N == 2) && link_rows<T, N>() == N - 1;
   constexpr bool kQuat = N == 2 && kTopRows;
   constexpr bool kShare = N == 3 && kTopRows;
-  constexpr bool kXL = XL && kShare;
   if constexpr (kXL) {
     if (do_gauge && mu != 0) {  // plane (0, mu), see above
       const Mat<R, N> A0 = bload_link<T, R, N>(Uin, 0, g);
@@ -427,11 +457,16 @@ GHX_DEV void link_update(const StepArgs<T>& a, const typename REP::template Para
         for (int i = 0; i < N - 1; ++i)
 #pragma unroll
           for (int j = 0; j < N; ++j) {
-            lst[(2 * (i * N + j)) * kBlock] = Lo.e[i][j].r;
-            lst[(2 * (i * N + j) + 1) * kBlock] = Lo.e[i][j].i;
+            pst[xl_word<D, NGN>(2 * (i * N + j)) * kBlock] = Lo.e[i][j].r;
+            pst[xl_word<D, NGN>(2 * (i * N + j) + 1) * kBlock] = Lo.e[i][j].i;
           }
       }
-      Mat<R, N> BT = mul_nd_top(bload_link<T, R, N, false>(Uin, mu, g + nb.upv(rd, 0)), DC);
+      Mat<R, N> Bx;  // rows 0, 1 of U_mu(x+0), lane x+0's own link
+#pragma unroll
+      for (int i = 0; i < N - 1; ++i)
+#pragma unroll
+        for (int j = 0; j < N; ++j) Bx.e[i][j] = {lane_take(Ux.e[i][j].r, xl.dst), lane_take(Ux.e[i][j].i, xl.dst)};
+      Mat<R, N> BT = mul_nd_top(Bx, DC);
       rebuild_last_row<R, N>(BT);
       Mat<R, N> Pl = mul_nn_top(A0, BT);
       rebuild_last_row<R, N>(Pl);
@@ -496,14 +531,14 @@ GHX_DEV void link_update(const StepArgs<T>& a, const typename REP::template Para
   if constexpr (kXL) {
     if (do_gauge && mu != 0) {  // this link's backward staple in direction 0, from lane x-0
       wave_fence();
-      const T* src = lst - int(threadIdx.x) + xsrc;
+      const T* src = pst - int(threadIdx.x) + xl.src_thread;
       Mat<R, N> Lb;
 #pragma unroll
       for (int i = 0; i < N - 1; ++i)
 #pragma unroll
         for (int j = 0; j < N; ++j) {
-          Lb.e[i][j].r = src[(2 * (i * N + j)) * kBlock];
-          Lb.e[i][j].i = src[(2 * (i * N + j) + 1) * kBlock];
+          Lb.e[i][j].r = src[xl_word<D, NGN>(2 * (i * N + j)) * kBlock];
+          Lb.e[i][j].i = src[xl_word<D, NGN>(2 * (i * N + j) + 1) * kBlock];
         }
       wave_fence();
       rebuild_last_row<R, N>(Lb);
@@ -525,7 +560,7 @@ GHX_DEV void link_update(const StepArgs<T>& a, const typename REP::template Para
   im_tr_gen<R, N>(X, f);
 #pragma unroll
   for (int c = 0; c < NGN; ++c) f[c] = fma(bN, f[c], fa[c]);
-  if (kShare && pst) {
+  if (kShare && pst && !(kXL && mu == 1)) {  // (with XL, slot 1 holds the parked lower staple)
 #pragma unroll
     for (int c = 0; c < NGN; ++c) f[c] += pst[(mu * NGN + c) * kBlock];
   }
@@ -603,24 +638,25 @@ GHX_DEV void site_update(const StepArgs<T>& a, const typename REP::template Para
   cx<T> phx[DR > 0 ? DR : 1], fph[DR > 0 ? DR : 1];
   if constexpr (DR > 0) site_phi_begin<T, DR>(a, rd, g, phx, fph);
   T* pst = nullptr;
-  if constexpr (N == 3 && link_rows<T, N>() == 2) {  // shared plaquettes (link_update)
-    __shared__ T stash[D * (N * N - 1) * kBlock];
+  if constexpr (xl_able<T, N>()) {  // shared plaquettes (link_update); XL parks a lower staple in slot 1 and 4 more words
+    constexpr int kWords = D * (N * N - 1) + (XL ? 2 * (N - 1) * N - (N * N - 1) : 0);
+    __shared__ T stash[kWords * kBlock];
     pst = stash + threadIdx.x;
 #pragma unroll
     for (int c = 0; c < D * (N * N - 1); ++c) pst[c * kBlock] = T(0);
   }
-  T* lst = nullptr;
-  int xsrc = 0;  // the thread of this block that holds x - 0
-  if constexpr (XL && N == 3 && link_rows<T, N>() == 2) {
-    __shared__ T lower[2 * (N - 1) * N * kBlock];
-    lst = lower + threadIdx.x;
-    const int lane = int(threadIdx.x) & 63, l0 = rd.L[0];
-    xsrc = (int(threadIdx.x) & ~63) | (lane & ~(l0 - 1)) | ((lane + l0 - 1) & (l0 - 1));
+  XLanes xl;
+  if constexpr (XL && xl_able<T, N>()) {
+    const int lane = int(threadIdx.x) & 63, l0 = rd.L[0], line = lane & ~(l0 - 1);
+    const int src = line | ((lane + l0 - 1) & (l0 - 1)), dst = line | ((lane + 1) & (l0 - 1));
+    xl.src = 4 * src;
+    xl.dst = 4 * dst;
+    xl.src_thread = (int(threadIdx.x) & ~63) | src;
   }
 #pragma unroll 1
   for (int i = 0; i < D; ++i) {
     const int mu = XL ? (i + 1 < D ? i + 1 : 0) : i;
-    link_update<T, N, D, REP, XL>(a, rp, rd, g, nb, mu, phx, fph, pst, lst, xsrc);
+    link_update<T, N, D, REP, XL>(a, rp, rd, g, nb, mu, phx, fph, pst, xl);
   }
   if constexpr (DR > 0) site_phi_end<T, DR>(a, rd, g, phx, fph);
 }
