@@ -1308,29 +1308,60 @@ int ghx_autotune(ghx_ctx* ctx, int steps, int* best_seg, double* best_ms) {
   if (ctx->nrep == 1)
     for (int sg : {1, 2, 4, 8, 16, 32, 64})
       if (sg <= ctx->hreps[0].L[ctx->D - 1]) cand.push_back({sg, 0});
-  int best = ctx->stream_seg, best_order = 0;
-  double bms = 1e300;
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  for (const auto& [sg, order] : cand) {
-    ctx->stream_seg = sg;
-    ctx->block_order = order < 0 ? 0 : order;
-    ctx->link_lanes = order < 0 ? 1 : 0;
-    launch_step(ctx, 1.0, 1.0, true);  // warm
+  auto use = [&](const std::pair<int, int>& c) {
+    ctx->stream_seg = c.first;
+    ctx->block_order = c.second < 0 ? 0 : c.second;
+    ctx->link_lanes = c.second < 0 ? 1 : 0;
+  };
+  auto time_it = [&](int n, double& out) -> int {  // GHX_OK or an error code
+    launch_step(ctx, 1.0, 1.0, true);  // the shape's first launch stays out of the timing
     HIPCHK(hipEventRecord(e0, ctx->stream));
-    for (int i = 0; i < steps; ++i) launch_step(ctx, 1.0, 1.0, true);
+    for (int i = 0; i < n; ++i) launch_step(ctx, 1.0, 1.0, true);
     HIPCHK(hipEventRecord(e1, ctx->stream));
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-    if (ms < bms) { bms = ms; best = sg; best_order = order; }
+    out = ms;
+    return GHX_OK;
+  };
+  // The GPU runs a newly started kernel slower for its first few tens of
+  // milliseconds, so the first candidate runs untimed first, the two fastest
+  // are timed again, and the winner runs last: the shape is chosen on warm
+  // timings and the caller's next steps start warm.
+  use(cand[0]);
+  for (int i = 0; i < steps; ++i) launch_step(ctx, 1.0, 1.0, true);
+  std::vector<double> ms(cand.size());
+  for (size_t k = 0; k < cand.size() && rc == GHX_OK; ++k) {
+    use(cand[k]);
+    rc = time_it(steps, ms[k]);
   }
+  size_t b0 = 0, b1 = cand.size() > 1 ? 1 : 0;
+  for (size_t k = 0; k < cand.size(); ++k)
+    if (ms[k] < ms[b0]) b0 = k;
+  for (size_t k = 0; k < cand.size(); ++k)
+    if (k != b0 && (b1 == b0 || ms[k] < ms[b1])) b1 = k;
+  if (rc == GHX_OK && b1 != b0) {  // runner-up again, then the leader again; keep the better second timing
+    double m0 = 0.0, m1 = 0.0;
+    use(cand[b1]);
+    rc = time_it(steps, m1);
+    use(cand[b0]);
+    if (rc == GHX_OK) rc = time_it(steps, m0);
+    if (rc == GHX_OK && m1 < m0) std::swap(b0, b1);
+  }
+  use(cand[b0]);
+  double bms = 0.0;
+  if (rc == GHX_OK) rc = time_it(steps, bms);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  ctx->stream_seg = best;
-  ctx->block_order = best_order < 0 ? 0 : best_order;
-  ctx->link_lanes = best_order < 0 ? 1 : 0;
+  if (rc != GHX_OK) {  // the fields stay as the failed launch left them
+    (void)hipFree(Ps);
+    if (pis) (void)hipFree(pis);
+    return rc;
+  }
+  const int best = cand[b0].first;
   ctx->cur = 0;
   HIPCHK(hipMemcpyAsync(ctx->U[0], ctx->Usave, nUb, hipMemcpyDeviceToDevice, ctx->stream));
   HIPCHK(hipMemcpyAsync(ctx->P, Ps, nPb, hipMemcpyDeviceToDevice, ctx->stream));
