@@ -267,6 +267,28 @@ template <typename R, int N> GHX_DEV Mat<R, N> alg_to_mat(const R* p, R eps) {
   }
   return H;
 }
+// f[a] = 2 Im Tr(T^a X) for all generators: the factor 1/2 of T^a left to the
+// caller's coefficient (halving is exact, so c * im_tr_gen(X)[a] and
+// (c / 2) * im_tr_gen2(X)[a] are the same number)
+template <typename R, int N> GHX_DEV void im_tr_gen2(const Mat<R, N>& X, R* f) {
+  int a = 0;
+#pragma unroll
+  for (int j = 0; j < N; ++j)
+#pragma unroll
+    for (int k = j + 1; k < N; ++k) {
+      f[a] = X.e[k][j].i + X.e[j][k].i;
+      f[a + NPAIR<N>] = X.e[j][k].r - X.e[k][j].r;
+      ++a;
+    }
+#pragma unroll
+  for (int l = 1; l < N; ++l) {
+    R s = R(0);
+#pragma unroll
+    for (int j = 0; j < l; ++j) s += X.e[j][j].i;
+    s -= R(l) * X.e[l][l].i;
+    f[2 * NPAIR<N> + l - 1] = R(sqrt(2.0 / (double(l) * (l + 1)))) * s;
+  }
+}
 // f[a] = Im Tr(T^a X) for all generators
 template <typename R, int N> GHX_DEV void im_tr_gen(const Mat<R, N>& X, R* f) {
   int a = 0;

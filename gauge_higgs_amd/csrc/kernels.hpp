@@ -97,11 +97,21 @@ GHX_DEV void rebuild_last_row(Mat<R, N>& m) {
     m.e[1][1] = {m.e[0][0].r, -m.e[0][0].i};
   } else {
     static_assert(N == 3, "compressed links are defined for SU(2) and SU(3)");
+    // conj(a_i b_j - a_j b_i), a = row 0, b = row 1: one product and three
+    // fused multiply-adds per real part (the rebuild runs on every link load)
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       const int i = (k + 1) % 3, j = (k + 2) % 3;
-      const cx<R> z = csub(cmul(m.e[0][i], m.e[1][j]), cmul(m.e[0][j], m.e[1][i]));
-      m.e[2][k] = cconj(z);
+      const cx<R> ai = m.e[0][i], aj = m.e[0][j], bi = m.e[1][i], bj = m.e[1][j];
+      R re = ai.r * bj.r;
+      re = fma(-ai.i, bj.i, re);
+      re = fma(-aj.r, bi.r, re);
+      re = fma(aj.i, bi.i, re);
+      R im = aj.r * bi.i;
+      im = fma(aj.i, bi.r, im);
+      im = fma(-ai.r, bj.i, im);
+      im = fma(-ai.i, bj.r, im);
+      m.e[2][k] = {re, im};
     }
   }
 }
@@ -392,6 +402,7 @@ GHX_DEV void link_update(const StepArgs<T>& a, const typename REP::template Para
   const R ed = R(abs_eps ? a.cd : a.cd * rd.eps);
   const R bN = R(rd.beta / N);
   const R k2 = R(2.0 * rd.kappa);
+  const R hbN = R(0.5) * bN, hk2 = R(0.5) * k2;  // coefficients of im_tr_gen2 (exact halves)
   const Planes<T> Uin(a.U_in, Vp, D * NC), Uout(a.U_out, Vp, D * NC), Pp(a.P, Vp, D * NGN);
   const bool do_gauge = !(a.flags & F_NO_GAUGE), do_hop = !(a.flags & F_NO_HOP);
   const Mat<R, N> Ux = bload_link<T, R, N>(Uin, mu, g);
@@ -413,9 +424,9 @@ GHX_DEV void link_update(const StepArgs<T>& a, const typename REP::template Para
       Mat<R, N> M;
       REP::fwd_M(Ux, phy, phx, psi, M, rp);
       R gm[NGN];
-      im_tr_gen<R, N>(M, gm);
+      im_tr_gen2<R, N>(M, gm);
 #pragma unroll
-      for (int c = 0; c < NGN; ++c) fa[c] = k2 * gm[c];
+      for (int c = 0; c < NGN; ++c) fa[c] = hk2 * gm[c];
 #pragma unroll
       for (int k = 0; k < DR; ++k) { fph[k].r = fma(k2, psi[k].r, fph[k].r); fph[k].i = fma(k2, psi[k].i, fph[k].i); }
     }
@@ -471,11 +482,11 @@ GHX_DEV void link_update(const StepArgs<T>& a, const typename REP::template Para
       Mat<R, N> Pl = mul_nn_top(A0, BT);
       rebuild_last_row<R, N>(Pl);
       R fp[NGN];
-      im_tr_gen<R, N>(Pl, fp);
+      im_tr_gen2<R, N>(Pl, fp);
 #pragma unroll
       for (int c = 0; c < NGN; ++c) {
-        fa[c] = fma(-bN, fp[c], fa[c]);
-        pst[c * kBlock] += bN * fp[c];
+        fa[c] = fma(-hbN, fp[c], fa[c]);
+        pst[c * kBlock] += hbN * fp[c];
       }
     }
   }
@@ -496,11 +507,11 @@ GHX_DEV void link_update(const StepArgs<T>& a, const typename REP::template Para
         Mat<R, N> Pl = mul_nn_top(Ux, S);
         rebuild_last_row<R, N>(Pl);
         R fp[NGN];
-        im_tr_gen<R, N>(Pl, fp);
+        im_tr_gen2<R, N>(Pl, fp);
 #pragma unroll
         for (int c = 0; c < NGN; ++c) {
-          fa[c] = fma(bN, fp[c], fa[c]);
-          pst[(nu * NGN + c) * kBlock] -= bN * fp[c];
+          fa[c] = fma(hbN, fp[c], fa[c]);
+          pst[(nu * NGN + c) * kBlock] -= hbN * fp[c];
         }
       }
     } else {  // forward staple U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag
@@ -557,9 +568,9 @@ GHX_DEV void link_update(const StepArgs<T>& a, const typename REP::template Para
     X = mul_nn(Ux, Sig);
   }
   R f[NGN];
-  im_tr_gen<R, N>(X, f);
+  im_tr_gen2<R, N>(X, f);
 #pragma unroll
-  for (int c = 0; c < NGN; ++c) f[c] = fma(bN, f[c], fa[c]);
+  for (int c = 0; c < NGN; ++c) f[c] = fma(hbN, f[c], fa[c]);
   if (kShare && pst && !(kXL && mu == 1)) {  // (with XL, slot 1 holds the parked lower staple)
 #pragma unroll
     for (int c = 0; c < NGN; ++c) f[c] += pst[(mu * NGN + c) * kBlock];
