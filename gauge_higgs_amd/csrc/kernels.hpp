@@ -10,6 +10,8 @@
 // Replicas are laid back to back; each starts on a block boundary so a block
 // never straddles two lattices (blk_rep maps block -> replica).
 #pragma once
+#include <type_traits>
+
 #include "ops.hpp"
 #include "reps.hpp"
 
@@ -354,6 +356,12 @@ GHX_DEV void site_phi_begin(const StepArgs<T>& a, const RepDesc& rd, unsigned g,
 // then skips its forward staples for nu < mu: 6 plaquettes per site instead of
 // 12 forward staples, and 18 fewer link loads.
 //
+// The exact factor a representation policy leaves on the M of fwd_M (reps.hpp).
+template <class REP, class = void>
+struct m_scale_of { static constexpr int value = 1; };
+template <class REP>
+struct m_scale_of<REP, std::void_t<decltype(REP::M_SCALE)>> { static constexpr int value = REP::M_SCALE; };
+
 // XL (x-lines in lanes: every wave holds whole lines of direction 0, so the
 // sites x - 0 and x + 0 sit in lanes xl.src and xl.dst of the same wave; links
 // run in the order 1, ..., D-1, 0): link mu >= 1 forms the plaquette of plane
@@ -426,7 +434,7 @@ GHX_DEV void link_update(const StepArgs<T>& a, const typename REP::template Para
       R gm[NGN];
       im_tr_gen2<R, N>(M, gm);
 #pragma unroll
-      for (int c = 0; c < NGN; ++c) fa[c] = hk2 * gm[c];
+      for (int c = 0; c < NGN; ++c) fa[c] = (hk2 * R(m_scale_of<REP>::value)) * gm[c];
 #pragma unroll
       for (int k = 0; k < DR; ++k) { fph[k].r = fma(k2, psi[k].r, fph[k].r); fph[k].i = fma(k2, psi[k].i, fph[k].i); }
     }

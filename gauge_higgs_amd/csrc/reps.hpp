@@ -3,7 +3,9 @@
 //   apply_dag(U, phi, out)   out = D^(R)(U)^dag phi
 //   fwd_M(U, phy, phx, psi, M)
 //       psi = D^(R)(U) phy and the N x N matrix M with
-//       phx^dag T^a_R psi = Tr(T^a M) for every fundamental generator T^a,
+//       phx^dag T^a_R psi = M_SCALE Tr(T^a M) for every fundamental generator
+//       T^a (M_SCALE an optional integer member, 1 when absent: a policy whose
+//       M carries an exact factor leaves it to the caller's coefficient),
 //   so the matter back-reaction g^a = -2 Im phx^dag T^a_R D(U) phy
 //   (representation.hpp:36-46) is -2 Im Tr(T^a M), and the total link force is
 //   one TA projection of (beta/N) U Sigma + 2 kappa M (theory_notes 3.3).
@@ -113,8 +115,9 @@ struct RepAdj {
 #pragma unroll
     for (int i = 0; i < N; ++i)
 #pragma unroll
-      for (int j = 0; j < N; ++j) M.e[i][j] = cscale(csub(A.e[i][j], B.e[i][j]), R(-2));
+      for (int j = 0; j < N; ++j) M.e[i][j] = csub(A.e[i][j], B.e[i][j]);
   }
+  static constexpr int M_SCALE = -2;
 };
 
 // ---------------------------------------------------------------- small tensor irreps
@@ -299,11 +302,8 @@ struct RepSym2 {
     project(S, psi);
     const Mat<R, N> X = lift(px);
     M = mul_nd(S, X);  // Psi Phx^dag (Phx symmetric)
-#pragma unroll
-    for (int i = 0; i < N; ++i)
-#pragma unroll
-      for (int j = 0; j < N; ++j) M.e[i][j] = cscale(M.e[i][j], R(2));
   }
+  static constexpr int M_SCALE = 2;
 };
 
 }  // namespace ghx
