@@ -407,12 +407,14 @@ template <typename R, int N, int ORDER> GHX_DEV Mat<R, N> expi_taylor(const Mat<
 // For traceless Hermitian Q, Q^3 = c1 Q + c0 1 with c1 = Tr(Q^2)/2, c0 = det Q,
 // so Q^n = a_n 1 + b_n Q + c_n Q^2 with (a,b,c)_{n+1} = (c0 c_n, a_n + c1 c_n, b_n)
 // and exp(iQ) = f0 1 + f1 Q + f2 Q^2, f_j = sum_n i^n/n! (a,b,c)_n. After scaling
-// to ||Q||_F <= 1 the series is cut at n = 12 (1/13! < float rounding); the
-// result is squared back. Costs one Hermitian Q^2 instead of a Taylor chain.
+// to ||Q||_F <= 1/2 the series is cut at n = 8 (remainder <= 2^-9 / 9! = 5e-9,
+// below float rounding); the result is squared back. An MD step's eps P has
+// ||.||_F near 0.1, so the squaring is rare. Costs one Hermitian Q^2 instead
+// of a Taylor chain.
 template <typename R> GHX_DEV Mat<R, 3> expi_su3_ch(const Mat<R, 3>& H) {
   R n2 = fnorm2(H), sc = R(1);
   int s = 0;
-  while (n2 * sc * sc > R(1) && s < 64) { sc *= R(0.5); ++s; }  // bounded: non-finite input cannot spin
+  while (n2 * sc * sc > R(0.25) && s < 64) { sc *= R(0.5); ++s; }  // bounded: non-finite input cannot spin
   Mat<R, 3> Q;
 #pragma unroll
   for (int i = 0; i < 3; ++i)
@@ -441,7 +443,7 @@ template <typename R> GHX_DEV Mat<R, 3> expi_su3_ch(const Mat<R, 3>& H) {
   R f0r = R(1), f0i = R(0), f1r = R(0), f1i = R(0), f2r = R(0), f2i = R(0);
   R coef = R(1);
 #pragma unroll
-  for (int n = 1; n <= 12; ++n) {
+  for (int n = 1; n <= 8; ++n) {
     const R an = c0 * c, bn = fma(c1, c, a), cn = b;
     a = an; b = bn; c = cn;
     coef *= R(1.0 / n);
