@@ -384,7 +384,12 @@ struct XLanes {
 GHX_DEV float lane_take(float v, int src_byte) {
   return __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(src_byte, __builtin_bit_cast(int, v)));
 }
-GHX_DEV double lane_take(double v, int) { return v; }  // XL is FP32 only
+GHX_DEV double lane_take(double v, int src_byte) {  // two 32-bit halves (XL runs in FP32; this keeps any caller correct)
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = unsigned(__builtin_amdgcn_ds_bpermute(src_byte, int(unsigned(u))));
+  const unsigned hi = unsigned(__builtin_amdgcn_ds_bpermute(src_byte, int(unsigned(u >> 32))));
+  return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+}
 template <typename R, int DR>
 GHX_DEV void lane_take_vec(const cx<R>* v, int src_byte, cx<R>* o) {
 #pragma unroll
