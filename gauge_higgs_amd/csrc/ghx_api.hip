@@ -1275,7 +1275,10 @@ int ghx_get_tuning(const ghx_ctx* ctx, int key, int* value) {
   if (key == GHX_TUNE_LINK_LANES) { *value = ctx->link_lanes; return GHX_OK; }
   if (key == GHX_TUNE_SYM_ENGINE) { *value = ctx->tb.sym.on; return GHX_OK; }
   if (key == GHX_TUNE_SYM_PAIRS) { *value = ctx->tb.sym.on && ctx->tb.sym.pair; return GHX_OK; }
-  if (key == GHX_TUNE_X_LANES) { *value = ctx->xlane_enabled && ctx->xlane_ok(); return GHX_OK; }
+  if (key == GHX_TUNE_X_LANES) {  // whether the MD step uses it (not with link lanes or the 4D-tile walks)
+    *value = ctx->xlane_enabled && ctx->xlane_ok() && !ctx->link_lanes && ctx->block_walk().order < 3;
+    return GHX_OK;
+  }
   return GHX_E_INVALID;
 }
 

@@ -190,8 +190,8 @@ int ghx_swap_stats(ghx_ctx* ctx, int* npairs, int* pairs, uint64_t* attempts, ui
  * cross-sections multiples of 64 sites) lets the one-site-per-thread and
  * streaming MD steps hand each site's lower staple in plane (0, mu) to its
  * x + 0 neighbour lane instead of forming it there (same physics, another
- * rounding; not with the 4D-tile walks; get_tuning reports whether it
- * applies). The autotuner times every candidate for `steps` MD steps on the
+ * rounding; not with the 4D-tile walks or link lanes; get_tuning reports
+ * whether the MD step uses it). The autotuner times every candidate for `steps` MD steps on the
  * live fields, restores the fields, and keeps the fastest. */
 enum { GHX_TUNE_STREAM_SEG = 0, GHX_TUNE_BLOCK_ORDER = 1, GHX_TUNE_LINK_LANES = 2, GHX_TUNE_SYM_ENGINE = 3, GHX_TUNE_SYM_PAIRS = 4,
        GHX_TUNE_X_LANES = 5 };

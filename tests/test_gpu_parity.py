@@ -303,8 +303,10 @@ def test_x_lanes_equal_own_staples(dims, rep):
         g.lat.set_stream_segment(seg)
         g.hot(0.8, 12345, 0.5)
         F, Fp = g.forces()
-        g.trajectory()
-        out.append((F, Fp, g.last_dH, g.get("U"), g.get("phi") if rep else None, g.get("P")))
+        g.refresh_momenta()  # MD without accept/reject: the comparison cannot hinge on a Metropolis decision
+        h0 = g.hamiltonian()
+        g.md_evolve()
+        out.append((F, Fp, g.hamiltonian() - h0, g.get("U"), g.get("phi") if rep else None, g.get("P")))
     base = out[0]
     for b in out[1:]:
         assert relmax(b[0], base[0]) < 2e-5
@@ -324,6 +326,40 @@ def test_x_lanes_equal_own_staples(dims, rep):
 def test_x_lanes_off_where_waves_do_not_hold_x_lines(dims, prec):
     g = gha.GaugeHiggsHMC(dims, "2", seed=9, D=4, N=3, precision=prec)
     assert g.lat.x_lanes() == 0
+
+
+def test_x_lanes_report_the_launch_shape():
+    g = gha.GaugeHiggsHMC([32, 8, 4, 8], "2", seed=9, D=4, N=3, precision=gha.FP32MD)
+    assert g.lat.x_lanes() == 1
+    g.lat.set_link_lanes(1)
+    assert g.lat.x_lanes() == 0
+    g.lat.set_link_lanes(0)
+    g.lat.set_block_order(3)  # 4D tiles of 16 x 4 x 2 x 1 sites
+    assert g.lat.block_order() == 3 and g.lat.x_lanes() == 0
+    g.lat.set_block_order(1)
+    assert g.lat.x_lanes() == 1
+
+
+def test_x_lanes_with_uneven_replicas():
+    """Three SU(3) + {2} replicas of different extents (and x-lines of 8, 4 and
+    16 sites) in one launch per MD step, FP32 MD: the MD evolution (refresh, 6
+    Omelyan steps; no accept/reject, so the comparison cannot hinge on a
+    Metropolis decision) with the x-lane hand-over agrees with every site
+    forming its own staples, replica by replica."""
+    from gauge_higgs_amd.replicas import ReplicaSet
+    ext = [[8, 4, 4, 4], [4, 4, 4, 4], [16, 2, 2, 4]]
+    out = []
+    for xl in (0, 1):
+        rs = ReplicaSet(4, 3, ext, "2", [5.7] * 3, [0.1, 0.15, 0.2], [0.5] * 3, nmd=6, precision=gha.FP32MD)
+        rs.lat.set_x_lanes(xl)
+        assert rs.lat.x_lanes() == xl
+        rs.lat.refresh_momenta()
+        rs.lat.md_evolve()
+        out.append([(rs.lat.get("U", r), rs.lat.get("phi", r), rs.lat.get("P", r)) for r in range(3)])
+        rs.lat.close()
+    for r in range(3):
+        for k in range(3):
+            assert relmax(out[1][r][k], out[0][r][k]) < 2e-3
 
 
 def test_block_walk_falls_back_where_it_does_not_apply():
