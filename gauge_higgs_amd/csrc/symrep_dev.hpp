@@ -17,7 +17,7 @@
 namespace ghx {
 
 constexpr int kSymWave = 64;
-constexpr int kFinishSites = 4;  // sites (waves) per workgroup of sym_finish_kernel
+constexpr int kFinishSites = 16;  // sites (waves) per workgroup of sym_finish_kernel
 constexpr int kLiftChunk = 16;  // lifted components per block of sym_lift_kernel
 
 // The link kernel runs one wave per workgroup. LDS instructions of a wave
@@ -606,49 +606,27 @@ GHX_DEV R wave_sum(R v) {
   return v;
 }
 
-// A site's matter terms from its 2D hops (sym_link_kernel), one wave per site
-// (kFinishSites waves per workgroup, each with its own LDS slice)
-// with lane l holding component l (d <= 64): the matter link force
-// -kappa g^a, g^a = -2 Im phi^dag T^a_R psi_mu (scalar_higgs.hpp:110-128), and
-// the scalar force 2 kappa sum(hops) - 2 dV/dphi^* with the pi kick, the
-// tangent projection and great-circle drift of a frozen scalar and the phi
-// drift (big_matter_kernel's contract), or q1 = sum_mu Re phi^dag psi_mu for
-// the action. Sums over components are wave reductions.
-template <typename T, int N, int D>
-__global__ __launch_bounds__(kSymWave* kFinishSites) void sym_finish_kernel(BigArgs<T> a, BigRepTables tb, const T* __restrict__ hops) {
+template <typename T, int D>
+GHX_DEV const T* hl_of(const BigRepTables& tb, unsigned g) {
+  return static_cast<const T*>(tb.sym.hopsL) + 2 * (size_t(g) * D * (4 * tb.sym.ng));
+}
+
+// The part of a site's finish after its scalar and momentum are loaded: the
+// backward hops projected once, g^a and the forward hops, the on-site term,
+// then the kick and drift into the site's staged rows (or its scalar force).
+template <typename T, int N, int D, class HOP>
+GHX_DEV void finish_site_body(const BigArgs<T>& a, const BigRepTables& tb, const RepDesc& rd, const T* __restrict__ hl, HOP&& hop,
+                              const cx<T> ph, const cx<T> pv0, cx<T>* lsum, cx<T> (&psi)[D][kSymWave], cx<T>* phs, T* rph, T* rpi,
+                              T* io_g, int si, unsigned g, int nmu) {
   using R = T;
   constexpr int NGN = N * N - 1;
-  __shared__ cx<R> psi_s[kFinishSites][D][kSymWave], phs_s[kFinishSites][kSymWave], lsum_s[kFinishSites][4 * kSymWave];
-  cx<R>(&psi)[D][kSymWave] = psi_s[threadIdx.y];
-  cx<R>(&phs)[kSymWave] = phs_s[threadIdx.y];
-  cx<R>(&lsum)[4 * kSymWave] = lsum_s[threadIdx.y];
-  const unsigned g = blockIdx.x * kFinishSites + threadIdx.y, Vp = a.Vp;
-  const int r = a.blk_rep[g / kBlock];
-  const RepDesc& rd = a.reps[r];
-  const unsigned s = g - rd.offset;
-  if (s >= rd.vol) return;  // the wave exits (no workgroup barrier follows)
+  const unsigned Vp = a.Vp;
   const int d = tb.d, l = threadIdx.x;
   const bool on = l < d;
-  const T* hg = hops + 2 * (size_t(g) * D * d);
-  auto hop = [&](int mu) { return on ? ld_cx(hg + 2 * (mu * d + l)) : cx<R>{R(0), R(0)}; };
-  const cx<R> ph = on ? cx<R>{R(a.phi_in[size_t(2 * l) * Vp + g]), R(a.phi_in[size_t(2 * l + 1) * Vp + g])} : cx<R>{R(0), R(0)};
-  const int nmu = (a.flags & F_NO_HOP) ? 0 : D;
-  if (a.mode == BIG_ACTION) {
-    double q = 0.0;
-    for (int mu = 0; mu < nmu; ++mu) {
-      const cx<R> ps = hop(mu);
-      q += double(ph.r) * double(ps.r) + double(ph.i) * double(ps.i);
-    }
-    q = wave_sum(q);
-    if (l == 0) a.q1[g] = q;
-    return;
-  }
   // loads that do not depend on the backward hops, issued before them
   cx<R> psf[D];
 #pragma unroll
   for (int mu = 0; mu < D; ++mu) psf[mu] = mu < nmu ? hop(mu) : cx<R>{R(0), R(0)};
-  const cx<R> pv0 = (on && a.mode == BIG_STEP) ? cx<R>{R(a.pi[size_t(2 * l) * Vp + g]), R(a.pi[size_t(2 * l + 1) * Vp + g])}
-                                                 : cx<R>{R(0), R(0)};
   const R k2 = R(2.0 * rd.kappa);
   const BigRepTables::Sym& st = tb.sym;
   const int lpg = st.glpg;
@@ -658,7 +636,6 @@ __global__ __launch_bounds__(kSymWave* kFinishSites) void sym_finish_kernel(BigA
   cx<R> f = {R(0), R(0)};
   if (nmu) {  // the backward hops: sum the D lifted vectors, project once
     const int P = 4 * st.ng;
-    const T* hl = static_cast<const T*>(st.hopsL) + 2 * (size_t(g) * D * P);
     for (int q = l; q < P; q += kSymWave) {
       cx<R> v = ld_cx(hl + 2 * q);
 #pragma unroll
@@ -680,8 +657,7 @@ __global__ __launch_bounds__(kSymWave* kFinishSites) void sym_finish_kernel(BigA
       }
       f = {k2 * acc.r, -k2 * acc.i};
     }
-  }
-  if (nmu) {
+    wave_sync();  // lsum and psi share the wave's LDS slice
     // the D forward hops at once, then g^a = -2 Im sum_lm conj(phi_l) T^a_lm psi_m
     // for every direction over the lanes of generator a (each entry read once)
 #pragma unroll
@@ -711,7 +687,7 @@ __global__ __launch_bounds__(kSymWave* kFinishSites) void sym_finish_kernel(BigA
     for (int mu = 0; mu < D; ++mu) {
       R v = gp[mu];
       for (int o = 1; o < lpg; o <<= 1) v += __shfl_xor(v, o);
-      if (l % lpg == 0 && l / lpg < NGN) a.G[size_t(mu * NGN + l / lpg) * Vp + g] = T(R(-rd.kappa) * (R(-2) * v));
+      if (l % lpg == 0 && l / lpg < NGN) io_g[(mu * NGN + l / lpg) * kFinishSites + si] = T(R(-rd.kappa) * (R(-2) * v));
     }
   }
   // on-site term -2 dV/dphi^* (scalar_higgs.hpp:25-39) or the plugged-in potential's planes
@@ -725,10 +701,7 @@ __global__ __launch_bounds__(kSymWave* kFinishSites) void sym_finish_kernel(BigA
     f.i += R(a.Fon[size_t(2 * l + 1) * Vp + g]);
   }
   if (a.mode == BIG_FORCE) {
-    if (on) {
-      a.Fphi[size_t(2 * l) * Vp + g] = T(f.r);
-      a.Fphi[size_t(2 * l + 1) * Vp + g] = T(f.i);
-    }
+    if (on) st_cx(rpi + 2 * l, cx<T>{T(f.r), T(f.i)});
     return;
   }
   const bool abs_eps = a.flags & F_ABS_EPS;
@@ -759,14 +732,96 @@ __global__ __launch_bounds__(kSymWave* kFinishSites) void sym_finish_kernel(BigA
     }
   }
   if (!on) return;
-  if ((a.flags & F_KICK) || (frozen && (a.flags & F_DRIFT))) {
-    a.pi[size_t(2 * l) * Vp + g] = T(pv.r);
-    a.pi[size_t(2 * l + 1) * Vp + g] = T(pv.i);
+  if ((a.flags & F_KICK) || (frozen && (a.flags & F_DRIFT))) st_cx(rpi + 2 * l, cx<T>{T(pv.r), T(pv.i)});
+  if (a.flags & F_DRIFT) st_cx(rph + 2 * l, cx<T>{T(pn.r), T(pn.i)});
+}
+
+// A site's matter terms from its 2D hops (sym_link_kernel), one wave per site
+// with lane l holding component l (d <= 64): the matter link force
+// -kappa g^a, g^a = -2 Im phi^dag T^a_R psi_mu (scalar_higgs.hpp:110-128), and
+// the scalar force 2 kappa sum(hops) - 2 dV/dphi^* with the pi kick, the
+// tangent projection and great-circle drift of a frozen scalar and the phi
+// drift (big_matter_kernel's contract), or q1 = sum_mu Re phi^dag psi_mu for
+// the action. Sums over components are wave reductions.
+//
+// phi, pi, G and the scalar force live in planes (one per real component), so
+// a wave that holds one site's components would touch a different cache line
+// in every lane. A workgroup therefore holds kFinishSites consecutive sites,
+// one wave each: all its threads read the sites' plane rows (kFinishSites
+// values each) into LDS, every wave finishes its site from LDS, and the
+// results leave through LDS as plane rows again.
+constexpr int kFinishRow = 2 * kSymWave + 2;  // LDS words per staged site (2d <= 128, even for 8-byte access)
+
+template <typename T, int N, int D>
+__global__ __launch_bounds__(kSymWave* kFinishSites) void sym_finish_kernel(BigArgs<T> a, BigRepTables tb, const T* __restrict__ hops) {
+  using R = T;
+  constexpr int NGN = N * N - 1, S = kFinishSites, NT = kSymWave * kFinishSites;
+  constexpr int kScr = (D > 4 ? D : 4) * kSymWave;  // the lifted backward hops (4 ng <= 4 * 64), then the forward hops
+  static_assert(kBlock % S == 0, "a workgroup's sites lie in one replica block");
+  __shared__ cx<R> scr_s[S][kScr], phs_s[S][kSymWave];
+  __shared__ __align__(16) T io_ph[S * kFinishRow];
+  __shared__ __align__(16) T io_pi[S * kFinishRow];
+  __shared__ T io_g[D * NGN * S];
+  const unsigned g0 = blockIdx.x * S, Vp = a.Vp;
+  const int r = a.blk_rep[g0 / kBlock];
+  const RepDesc& rd = a.reps[r];
+  const int tid = int(threadIdx.y) * kSymWave + int(threadIdx.x);
+  const int np = 2 * tb.d;
+  const bool stepm = a.mode == BIG_STEP;
+  auto live = [&](int si) { return g0 + unsigned(si) - rd.offset < rd.vol; };
+  // plane rows in: component plane p of the S sites, S consecutive values
+  for (int e = tid; e < np * S; e += NT) {
+    const int p = e / S, si = e % S;
+    const bool lv = live(si);
+    io_ph[si * kFinishRow + p] = lv ? a.phi_in[size_t(p) * Vp + g0 + si] : T(0);
+    if (stepm) io_pi[si * kFinishRow + p] = lv ? a.pi[size_t(p) * Vp + g0 + si] : T(0);
   }
-  if (a.flags & F_DRIFT) {
-    a.phi_out[size_t(2 * l) * Vp + g] = T(pn.r);
-    a.phi_out[size_t(2 * l + 1) * Vp + g] = T(pn.i);
+  __syncthreads();
+  const int si = int(threadIdx.y);
+  if (live(si)) {
+    const unsigned g = g0 + unsigned(si);
+    cx<R>* lsum = scr_s[si];
+    cx<R>(&psi)[D][kSymWave] = *reinterpret_cast<cx<R>(*)[D][kSymWave]>(scr_s[si]);
+    cx<R>(&phs)[kSymWave] = phs_s[si];
+    T* rph = io_ph + si * kFinishRow;  // this site's staged phi (then phi_out)
+    T* rpi = io_pi + si * kFinishRow;  // its staged pi (then pi, or the scalar force of BIG_FORCE)
+    const int d = tb.d, l = threadIdx.x;
+    const bool on = l < d;
+    const T* hg = hops + 2 * (size_t(g) * D * d);
+    auto hop = [&](int mu) { return on ? ld_cx(hg + 2 * (mu * d + l)) : cx<R>{R(0), R(0)}; };
+    const cx<R> ph = on ? ld_cx(rph + 2 * l) : cx<R>{R(0), R(0)};
+    const int nmu = (a.flags & F_NO_HOP) ? 0 : D;
+    if (a.mode == BIG_ACTION) {
+      double q = 0.0;
+      for (int mu = 0; mu < nmu; ++mu) {
+        const cx<R> ps = hop(mu);
+        q += double(ph.r) * double(ps.r) + double(ph.i) * double(ps.i);
+      }
+      q = wave_sum(q);
+      if (l == 0) a.q1[g] = q;
+    } else {
+      finish_site_body<T, N, D>(a, tb, rd, hl_of<T, D>(tb, g), hop, ph, on ? (stepm ? ld_cx(rpi + 2 * l) : cx<R>{R(0), R(0)}) : cx<R>{R(0), R(0)},
+                                lsum, psi, phs, rph, rpi, io_g, si, g, nmu);
+    }
   }
+  if (a.mode == BIG_ACTION) return;
+  __syncthreads();
+  // plane rows out
+  const bool put_pi = stepm && ((a.flags & F_KICK) || ((a.flags & F_FROZEN) && (a.flags & F_DRIFT)));
+  const bool put_ph = stepm && (a.flags & F_DRIFT);
+  for (int e = tid; e < np * S; e += NT) {
+    const int p = e / S, sj = e % S;
+    if (!live(sj)) continue;
+    const size_t o = size_t(p) * Vp + g0 + sj;
+    if (a.mode == BIG_FORCE) a.Fphi[o] = io_pi[sj * kFinishRow + p];
+    if (put_pi) a.pi[o] = io_pi[sj * kFinishRow + p];
+    if (put_ph) a.phi_out[o] = io_ph[sj * kFinishRow + p];
+  }
+  if (!(a.flags & F_NO_HOP))
+    for (int e = tid; e < D * NGN * S; e += NT) {
+      const int p = e / S, sj = e % S;
+      if (live(sj)) a.G[size_t(p) * Vp + g0 + sj] = io_g[e];
+    }
 }
 
 }  // namespace ghx
