@@ -322,6 +322,26 @@ def test_x_lanes_equal_own_staples(dims, rep):
         assert np.array_equal(out[1][k], out[2][k])
 
 
+def test_x_lanes_with_the_symmetric_power_engine():
+    """SU(3) + {6,1} (d = 48) on an 8 x 4 x 2 x 4 lattice: the gauge part of the
+    step runs with x-lanes and adds the engine's matter link force G. Forces
+    and the MD evolution agree with the step that forms every staple itself."""
+    out = []
+    for xl in (0, 1):
+        g = gha.GaugeHiggsHMC([8, 4, 2, 4], "6,1", seed=9, D=4, N=3, precision=gha.FP32MD).configure(beta=5.7, kappa=0.05, lam=0.5,
+                                                                                                       nmd=4)
+        g.lat.set_x_lanes(xl)
+        assert g.lat.x_lanes() == xl and g.lat.sym_engine() == 1
+        g.hot(0.4, 12345, 0.27)
+        F, Fp = g.forces()
+        g.refresh_momenta()
+        g.md_evolve()
+        out.append((F, Fp, g.get("U"), g.get("phi"), g.get("P")))
+    assert relmax(out[1][0], out[0][0]) < 2e-5 and relmax(out[1][1], out[0][1]) < 2e-5
+    for k in (2, 3, 4):
+        assert relmax(out[1][k], out[0][k]) < 2e-3
+
+
 @pytest.mark.parametrize("dims,prec", [([6, 4, 4, 4], gha.FP32MD), ([4, 6, 4, 10], gha.FP32MD), ([4, 4, 4, 4], gha.FP64)])
 def test_x_lanes_off_where_waves_do_not_hold_x_lines(dims, prec):
     g = gha.GaugeHiggsHMC(dims, "2", seed=9, D=4, N=3, precision=prec)
