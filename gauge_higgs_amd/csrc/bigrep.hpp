@@ -335,15 +335,37 @@ __global__ __launch_bounds__(kBigBlock) void big_matter_kernel(BigArgs<T> a, Big
 // in LDS.
 constexpr int kOnsiteGroups = 4;
 template <typename T>
-__global__ __launch_bounds__(64 * kOnsiteGroups) void onsite_step_kernel(const RepDesc* reps, const int* blk_rep, unsigned Vp,
-                                                                        const T* phi_in, T* phi_out, T* pi, int d, double ck, double cd,
-                                                                        int flags, const T* Fon) {
+struct OnsiteArgs {
+  const RepDesc* reps;
+  const int* blk_rep;
+  unsigned Vp;
+  const T* phi_in;
+  T* phi_out;
+  T* pi;
+  int d;
+  double ck, cd;
+  int flags;
+  const T* Fon;
+};
+// One workgroup's part: the 64 sites of block blk.
+template <typename T>
+GHX_DEV void onsite_step_block(const OnsiteArgs<T>& o, unsigned blk) {
   using R = T;
   constexpr int G = kOnsiteGroups;
+  const RepDesc* reps = o.reps;
+  const int* blk_rep = o.blk_rep;
+  const unsigned Vp = o.Vp;
+  const T* phi_in = o.phi_in;
+  T* phi_out = o.phi_out;
+  T* pi = o.pi;
+  const int d = o.d;
+  const double ck = o.ck, cd = o.cd;
+  const int flags = o.flags;
+  const T* Fon = o.Fon;
   __shared__ R red[G][64];
   const int lane = int(threadIdx.x) % 64, grp = int(threadIdx.x) / 64;
-  const unsigned g = blockIdx.x * 64 + unsigned(lane);
-  const RepDesc& rd = reps[blk_rep[blockIdx.x * 64 / kBlock]];
+  const unsigned g = blk * 64 + unsigned(lane);
+  const RepDesc& rd = reps[blk_rep[blk * 64 / kBlock]];
   const bool live = g - rd.offset < rd.vol;  // the block stays whole for its barriers
   auto site_sum = [&](R v) {
     red[grp][lane] = v;
@@ -419,6 +441,22 @@ __global__ __launch_bounds__(64 * kOnsiteGroups) void onsite_step_kernel(const R
       phi_out[size_t(2 * l + k) * Vp + g] = T(move ? cs * fv + q * pv : fv);
       pi[size_t(2 * l + k) * Vp + g] = T(move ? w * fv + cs * pv : pv);
     }
+}
+template <typename T>
+__global__ __launch_bounds__(64 * kOnsiteGroups) void onsite_step_kernel(OnsiteArgs<T> o) {
+  onsite_step_block<T>(o, blockIdx.x);
+}
+
+// The inner-scale step of a large irrep in one launch: workgroups [0, nlink)
+// run the link-parallel gauge update (U and P), the rest the on-site update of
+// the scalar (phi and pi). The two parts read and write disjoint fields, so
+// their latencies overlap instead of adding up in two launches of a few waves
+// per CU each; the arithmetic is that of the two separate kernels.
+template <typename T, int N, int D>
+__global__ __launch_bounds__(kLinkSites * D) void md_link_onsite_kernel(StepArgs<T> a, unsigned nlink, OnsiteArgs<T> o) {
+  static_assert(kLinkSites * D == 64 * kOnsiteGroups, "both parts run workgroups of the same size");
+  if (blockIdx.x < nlink) md_link_block<T, N, D, RepExt<T, N>>(a, NoParams{}, xcd_block(blockIdx.x, nlink));
+  else onsite_step_block<T>(o, blockIdx.x - nlink);
 }
 
 // Per-site FP64 contributions for H with a large rep: q1 from big_matter_kernel

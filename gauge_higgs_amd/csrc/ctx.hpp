@@ -52,6 +52,7 @@ struct ghx_ctx {
   int block_order = 0;  // BlockWalk order of the MD-step kernel (single lattice)
   int link_lanes = 0;   // link-parallel MD step (one lane per link) for small lattices
   bool xlane_enabled = true;  // GHX_TUNE_X_LANES
+  bool fuse_inner = true;     // GHX_TUNE_FUSE_INNER
   // multi-timescale integrator (0 = single scale)
   int n_outer = 0, m_inner = 1;
   // surrogate link force (learned); S planes are filled by surr_fill before a kick
@@ -154,6 +155,7 @@ struct ghx_ctx {
     c.walk = block_walk();
     c.link_lanes = link_lanes;
     c.xlane = (xlane_enabled && xlane_ok()) ? 1 : 0;
+    c.fuse_inner = fuse_inner ? 1 : 0;
     return c;
   }
 };

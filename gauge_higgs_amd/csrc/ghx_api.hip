@@ -365,10 +365,12 @@ void launch_part_step(ghx_ctx* c, double ck, double cd, bool drift, int parts, d
     flags |= F_SURR;
     l.surr_sign = surr_sign;
   }
-  if (c->big) {
-    if (flags & F_NO_HOP) c->bops.onsite(l, c->phi[in], c->phi[out], c->pi, c->rep.d, ck, cd, flags);
-    else c->bops.matter(l, c->tb, c->U[in], c->phi[in], c->phi[out], c->pi, c->G, nullptr, nullptr, ck, cd, flags, BIG_STEP);
+  if (c->big && (flags & F_NO_HOP)) {  // the inner scale of a large irrep: on-site and gauge updates
+    c->bops.onsite_gauge(l, c->U[in], c->U[out], c->P, c->phi[in], c->phi[out], c->pi, c->rep.d, ck, cd, flags);
+    if (drift) c->cur = out;
+    return;
   }
+  if (c->big) c->bops.matter(l, c->tb, c->U[in], c->phi[in], c->phi[out], c->pi, c->G, nullptr, nullptr, ck, cd, flags, BIG_STEP);
   c->ops.md_step(l, c->U[in], c->U[out], c->P, c->phi[in], c->phi[out], c->pi, c->F, c->Fphi, ck, cd, flags);
   if (drift) c->cur = out;
 }
@@ -1265,6 +1267,12 @@ int ghx_set_tuning(ghx_ctx* ctx, int key, int value) {
     drop_graph(ctx);
     return GHX_OK;
   }
+  if (key == GHX_TUNE_FUSE_INNER) {
+    if (value < 0 || value > 1) return fail(ctx, GHX_E_INVALID, "fused inner step must be 0 or 1");
+    ctx->fuse_inner = value != 0;
+    drop_graph(ctx);
+    return GHX_OK;
+  }
   return fail(ctx, GHX_E_INVALID, "unknown tuning key");
 }
 
@@ -1275,6 +1283,10 @@ int ghx_get_tuning(const ghx_ctx* ctx, int key, int* value) {
   if (key == GHX_TUNE_LINK_LANES) { *value = ctx->link_lanes; return GHX_OK; }
   if (key == GHX_TUNE_SYM_ENGINE) { *value = ctx->tb.sym.on; return GHX_OK; }
   if (key == GHX_TUNE_SYM_PAIRS) { *value = ctx->tb.sym.on && ctx->tb.sym.pair; return GHX_OK; }
+  if (key == GHX_TUNE_FUSE_INNER) {  // whether a large irrep's inner-scale step runs as one launch
+    *value = ctx->fuse_inner && ctx->big && ctx->D == 4 && ctx->link_lanes && !(ctx->nrep == 1 && ctx->stream_seg > 0);
+    return GHX_OK;
+  }
   if (key == GHX_TUNE_X_LANES) {  // whether the MD step uses it (not with link lanes or the 4D-tile walks)
     *value = ctx->xlane_enabled && ctx->xlane_ok() && !ctx->link_lanes && ctx->block_walk().order < 3;
     return GHX_OK;

@@ -80,15 +80,38 @@ struct BigLaunch {
     hipLaunchKernelGGL(phi_drift_kernel<T>, dim3(c.nblk), dim3(kBlock), 0, c.stream, c.reps, c.blk_rep, c.Vp, (const T*)phi_in,
                        (T*)phi_out, (T*)const_cast<void*>(pi), nplanes, cd, abs_eps, c.frozen);
   }
-  static void onsite(const LaunchCtx& c, const void* phi_in, void* phi_out, void* pi, int d, double ck, double cd, int flags) {
+  static OnsiteArgs<T> onsite_args(const LaunchCtx& c, const void* phi_in, void* phi_out, void* pi, int d, double ck, double cd,
+                                   int flags) {
     if (c.frozen) flags |= F_FROZEN;
     const T* Fon = nullptr;
     if (c.multi) {
       if (!(flags & F_NO_ONSITE) && (flags & F_KICK)) multi(c, d, phi_in, c.Fon, nullptr);
       Fon = (const T*)c.Fon;
     }
-    hipLaunchKernelGGL(onsite_step_kernel<T>, dim3(c.Vp / 64), dim3(64 * kOnsiteGroups), 0, c.stream, c.reps, c.blk_rep, c.Vp,
-                       (const T*)phi_in, (T*)phi_out, (T*)pi, d, ck, cd, flags, Fon);
+    return OnsiteArgs<T>{c.reps, c.blk_rep, c.Vp, (const T*)phi_in, (T*)phi_out, (T*)pi, d, ck, cd, flags, Fon};
+  }
+  static void onsite(const LaunchCtx& c, const void* phi_in, void* phi_out, void* pi, int d, double ck, double cd, int flags) {
+    const OnsiteArgs<T> o = onsite_args(c, phi_in, phi_out, pi, d, ck, cd, flags);
+    hipLaunchKernelGGL(onsite_step_kernel<T>, dim3(c.Vp / 64), dim3(64 * kOnsiteGroups), 0, c.stream, o);
+  }
+  // The inner-scale step (F_NO_HOP): the on-site update of the scalar and the
+  // gauge update of the links, in one launch where the links run
+  // link-parallel (D = 4), else as the two launches it replaces.
+  static void onsite_gauge(const LaunchCtx& c, const void* U_in, void* U_out, void* P, const void* phi_in, void* phi_out, void* pi,
+                           int d, double ck, double cd, int flags) {
+    if constexpr (kLinkSites * D == 64 * kOnsiteGroups) {
+      if (c.fuse_inner && c.link_lanes && c.stream_seg == 0) {
+        const OnsiteArgs<T> o = onsite_args(c, phi_in, phi_out, pi, d, ck, cd, flags);
+        const int gf = c.frozen ? flags | F_FROZEN : flags;
+        const StepArgs<T> a{c.reps, c.blk_rep, c.Vp, (const T*)U_in, (T*)U_out, (T*)P, (const T*)phi_in, (T*)phi_out, (T*)pi, nullptr,
+                            nullptr, ck, cd, gf, (const T*)c.G, (const T*)c.S, c.surr_sign, (const T*)c.Fon, c.walk};
+        const unsigned nlink = c.nblk * unsigned(kBlock / kLinkSites);
+        hipLaunchKernelGGL((md_link_onsite_kernel<T, N, D>), dim3(nlink + c.Vp / 64), dim3(kLinkSites * D), 0, c.stream, a, nlink, o);
+        return;
+      }
+    }
+    onsite(c, phi_in, phi_out, pi, d, ck, cd, flags);
+    Launch<T, N, D, RepExt>::md_step(c, U_in, U_out, P, phi_in, phi_out, pi, nullptr, nullptr, ck, cd, flags);
   }
 };
 template <typename T, int N, int D>
@@ -104,6 +127,7 @@ void fill(Ops* o, BigOps* b) {
   b->action = &BigLaunch<T, N, D>::action;
   b->phi_drift = &BigLaunch<T, N, D>::phi_drift;
   b->onsite = &BigLaunch<T, N, D>::onsite;
+  b->onsite_gauge = &BigLaunch<T, N, D>::onsite_gauge;
   b->matter_static_lds = &matter_static_lds<T, N, D>;
 }
 bool ops_big(int prec, int N, int D, Ops* o, BigOps* b) {

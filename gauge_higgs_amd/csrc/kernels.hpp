@@ -738,10 +738,10 @@ __global__ __launch_bounds__(kBlock, md_waves<T>()) void md_stream_kernel(StepAr
 // load latency; the same arithmetic except for the order in which the scalar
 // force terms are summed.
 constexpr int kLinkSites = 64;  // sites per workgroup (one wave per direction)
+// One workgroup's part: the kLinkSites sites of block bid (storage order).
 template <typename T, int N, int D, class REP>
-__global__ __launch_bounds__(kLinkSites * D) void md_link_kernel(StepArgs<T> a, typename REP::template Params<T> rp) {
+GHX_DEV void md_link_block(const StepArgs<T>& a, const typename REP::template Params<T>& rp, unsigned bid) {
   constexpr int DR = REP::DR;
-  const unsigned bid = xcd_block(blockIdx.x, gridDim.x);
   const int mu = __builtin_amdgcn_readfirstlane(int(threadIdx.x) / kLinkSites);
   const int lane = int(threadIdx.x) % kLinkSites;
   const unsigned gsite = bid * kLinkSites + unsigned(lane);
@@ -774,6 +774,10 @@ __global__ __launch_bounds__(kLinkSites * D) void md_link_kernel(StepArgs<T> a, 
     }
     site_phi_end<T, DR>(a, rd, gsite, phx, fph);
   }
+}
+template <typename T, int N, int D, class REP>
+__global__ __launch_bounds__(kLinkSites * D) void md_link_kernel(StepArgs<T> a, typename REP::template Params<T> rp) {
+  md_link_block<T, N, D, REP>(a, rp, xcd_block(blockIdx.x, gridDim.x));
 }
 
 // Drift alone (U' = exp(i eps P) U, phi' = phi + eps pi) — for the kick/drift API.

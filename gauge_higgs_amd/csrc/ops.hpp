@@ -42,6 +42,7 @@ struct LaunchCtx {
   BlockWalk walk;   // block order of the MD-step kernel
   int link_lanes;   // 1: link-parallel MD step (one lane per link)
   int xlane;        // 1: every wave of the MD step holds whole x-lines (x-neighbour lanes share products)
+  int fuse_inner;   // 1: a large irrep's inner-scale step runs its on-site and gauge updates in one launch
   const void* S;    // surrogate link force planes
   double surr_sign;
   int frozen;       // |phi_x| = 1 scalar: adds F_FROZEN to every MD launch
@@ -60,6 +61,9 @@ struct BigOps {
   void (*action)(const LaunchCtx&, const void* U, const void* P, const void* phi, const void* pi, const double* q1, int d);
   void (*phi_drift)(const LaunchCtx&, const void* phi_in, void* phi_out, const void* pi, int nplanes, double cd, int abs_eps);
   void (*onsite)(const LaunchCtx&, const void* phi_in, void* phi_out, void* pi, int d, double ck, double cd, int flags);
+  // the inner-scale step (F_NO_HOP): on-site update and gauge update, fused where the links run link-parallel
+  void (*onsite_gauge)(const LaunchCtx&, const void* U_in, void* U_out, void* P, const void* phi_in, void* phi_out, void* pi, int d,
+                       double ck, double cd, int flags);
   // static LDS bytes of the tensor-apply matter kernel (its dynamic LDS holds the dimT-entry tensor)
   size_t (*matter_static_lds)();
 };

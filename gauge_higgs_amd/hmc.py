@@ -244,6 +244,16 @@ class Lattices:
         self._check(self._lib.ghx_get_tuning(self.h, 5, C.byref(v)))
         return v.value
 
+    def set_fuse_inner(self, on):
+        """Large irreps under the multi-timescale integrator: the inner-scale
+        on-site and gauge updates in one launch (1, default) or two (0)."""
+        self._check(self._lib.ghx_set_tuning(self.h, 6, int(on)))
+
+    def fuse_inner(self):
+        v = C.c_int()
+        self._check(self._lib.ghx_get_tuning(self.h, 6, C.byref(v)))
+        return v.value
+
     def autotune(self, steps=10):
         """Pick the fastest MD-step launch shape on the live fields (fields restored)."""
         best, ms = C.c_int(), C.c_double()

@@ -191,10 +191,15 @@ int ghx_swap_stats(ghx_ctx* ctx, int* npairs, int* pairs, uint64_t* attempts, ui
  * streaming MD steps hand each site's lower staple in plane (0, mu) to its
  * x + 0 neighbour lane instead of forming it there (same physics, another
  * rounding; not with the 4D-tile walks or link lanes; get_tuning reports
- * whether the MD step uses it). The autotuner times every candidate for `steps` MD steps on the
+ * whether the MD step uses it); GHX_TUNE_FUSE_INNER = 1 (default) runs the
+ * inner-scale step of a large irrep under the multi-timescale integrator (the
+ * scalar's on-site update and the links' gauge update, which touch disjoint
+ * fields) as one launch where the links run link-parallel in D = 4, with the
+ * arithmetic of the two separate launches (get_tuning reports whether it
+ * applies). The autotuner times every candidate for `steps` MD steps on the
  * live fields, restores the fields, and keeps the fastest. */
 enum { GHX_TUNE_STREAM_SEG = 0, GHX_TUNE_BLOCK_ORDER = 1, GHX_TUNE_LINK_LANES = 2, GHX_TUNE_SYM_ENGINE = 3, GHX_TUNE_SYM_PAIRS = 4,
-       GHX_TUNE_X_LANES = 5 };
+       GHX_TUNE_X_LANES = 5, GHX_TUNE_FUSE_INNER = 6 };
 int ghx_set_tuning(ghx_ctx* ctx, int key, int value);
 int ghx_get_tuning(const ghx_ctx* ctx, int key, int* value);
 int ghx_autotune(ghx_ctx* ctx, int steps, int* best_seg, double* best_ms_per_step);

@@ -289,3 +289,39 @@ def test_nested_m1_is_leapfrog_bigrep():
     b.md_evolve()
     for k in ("U", "P", "phi", "pi"):
         assert np.max(np.abs(a.get(k) - b.get(k))) < 1e-11, k
+
+
+@pytest.mark.parametrize("prec", [gha.FP64, gha.FP32MD])
+@pytest.mark.parametrize("surr", [False, True])
+def test_fused_inner_step_equals_two_launches(prec, surr):
+    """The inner scale of a large irrep ({6,1}) under the multi-timescale
+    integrator, with the links link-parallel: the on-site and gauge updates in
+    one launch (md_link_onsite_kernel) give the fields of the two separate
+    launches bit for bit, with and without the surrogate force."""
+    out = []
+    for fuse in (0, 1):
+        g = gha.GaugeHiggsHMC(4, "6,1", seed=1, D=4, N=3, precision=prec).configure(beta=5.7, kappa=0.05, lam=0.5, nmd=4,
+                                                                                    reunit_each_traj=False)
+        g.hot(0.8, 12345, 0.3)
+        g.lat.set_link_lanes(1)
+        g.lat.set_fuse_inner(fuse)
+        g.lat.set_nested(3, 4)
+        assert g.lat.fuse_inner() == fuse
+        if surr:
+            g.lat.surrogate_init(seed=2)
+            g.lat.surrogate_enable(True)
+        g.refresh_momenta()
+        g.md_evolve()
+        out.append({k: g.get(k) for k in ("U", "P", "phi", "pi")})
+    for k in out[0]:
+        assert np.array_equal(out[0][k], out[1][k]), k
+
+
+def test_fused_inner_step_reported_only_where_it_runs():
+    g = gha.GaugeHiggsHMC(4, "6,1", seed=1, D=4, N=3, precision=gha.FP32MD)
+    assert g.lat.fuse_inner() == 0  # one thread per site: two launches
+    g.lat.set_link_lanes(1)
+    assert g.lat.fuse_inner() == 1
+    h = gha.GaugeHiggsHMC(4, "2", seed=1, D=4, N=3, precision=gha.FP32MD)
+    h.lat.set_link_lanes(1)
+    assert h.lat.fuse_inner() == 0  # small irreps fuse the whole step already
