@@ -164,12 +164,14 @@ using rsrc_t = __amdgpu_buffer_rsrc_t;
 GHX_DEV rsrc_t make_rsrc(const void* p, unsigned bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
 }
-template <typename T> GHX_DEV T bload(rsrc_t r, unsigned voff, unsigned soff);
-template <> GHX_DEV float bload<float>(rsrc_t r, unsigned voff, unsigned soff) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
-}
-template <> GHX_DEV double bload<double>(rsrc_t r, unsigned voff, unsigned soff) {
-  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+// POL is the cache policy of the load: 0, or kLoadNT for data a step reads
+// once (a site's own momenta, the external and surrogate link forces): such
+// loads stream through L2 and leave it to the neighbour links, which the
+// staples read several times.
+constexpr int kLoadNT = 2;
+template <typename T, int POL = 0> GHX_DEV T bload(rsrc_t r, unsigned voff, unsigned soff) {
+  if constexpr (sizeof(T) == 4) return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, POL));
+  else return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, POL));
 }
 // Stores of the MD step are streamed (non-temporal, cache policy bit NT = 2):
 // nothing written in a step is read again in that step, so they should not
@@ -189,6 +191,7 @@ struct Planes {
   unsigned pstride;  // Vp * sizeof(T)
   GHX_DEV Planes(const T* p, unsigned Vp, unsigned nplanes) : r(make_rsrc(p, nplanes * Vp * unsigned(sizeof(T)))), pstride(Vp * unsigned(sizeof(T))) {}
   GHX_DEV T ld(unsigned g, unsigned c) const { return bload<T>(r, g * unsigned(sizeof(T)), c * pstride); }
+  GHX_DEV T ld_once(unsigned g, unsigned c) const { return bload<T, kLoadNT>(r, g * unsigned(sizeof(T)), c * pstride); }
   GHX_DEV void st(unsigned g, unsigned c, T v) const { bstore(v, r, g * unsigned(sizeof(T)), c * pstride); }
 };
 // FULL = false leaves the last row of a compressed link unset (for a left
@@ -592,14 +595,14 @@ GHX_DEV void link_update(const StepArgs<T>& a, const typename REP::template Para
     if (do_hop) {
       const Planes<T> Gp(a.G, Vp, D * NGN);
 #pragma unroll
-      for (int c = 0; c < NGN; ++c) f[c] += R(Gp.ld(g, unsigned(mu * NGN + c)));
+      for (int c = 0; c < NGN; ++c) f[c] += R(Gp.ld_once(g, unsigned(mu * NGN + c)));
     }
   }
   if (a.flags & F_SURR) {
     const Planes<T> Sp(a.S, Vp, D * NGN);
     const R sg = R(a.surr_sign);
 #pragma unroll
-    for (int c = 0; c < NGN; ++c) f[c] = fma(sg, R(Sp.ld(g, unsigned(mu * NGN + c))), f[c]);
+    for (int c = 0; c < NGN; ++c) f[c] = fma(sg, R(Sp.ld_once(g, unsigned(mu * NGN + c))), f[c]);
   }
   if (a.flags & F_FORCE) {
     const Planes<T> Fp(a.F_out, Vp, D * NGN);
@@ -609,7 +612,7 @@ GHX_DEV void link_update(const StepArgs<T>& a, const typename REP::template Para
   }
   R p[NGN];
 #pragma unroll
-  for (int c = 0; c < NGN; ++c) p[c] = R(Pp.ld(g, unsigned(mu * NGN + c)));
+  for (int c = 0; c < NGN; ++c) p[c] = R(Pp.ld_once(g, unsigned(mu * NGN + c)));
   if (a.flags & F_KICK) {
 #pragma unroll
     for (int c = 0; c < NGN; ++c) { p[c] = fma(-ek, f[c], p[c]); Pp.st(g, unsigned(mu * NGN + c), T(p[c])); }
@@ -633,7 +636,8 @@ GHX_DEV void site_phi_end(const StepArgs<T>& a, const RepDesc& rd, unsigned g, c
   if (a.flags & F_FORCE) { bstore_vec<T, R, DR>(Planes<T>(a.Fphi_out, Vp, 2 * DR), g, fph); return; }
   const Planes<T> Pi(a.pi, Vp, 2 * DR);
   cx<R> pv[DR];
-  bload_vec<T, R, DR>(Pi, g, pv);
+#pragma unroll
+  for (int k = 0; k < DR; ++k) { pv[k].r = R(Pi.ld_once(g, 2 * k)); pv[k].i = R(Pi.ld_once(g, 2 * k + 1)); }
   const bool frozen = a.flags & F_FROZEN;
   if (a.flags & F_KICK) {
 #pragma unroll
