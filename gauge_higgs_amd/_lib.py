@@ -30,6 +30,7 @@ def load():
         "ghx_set_integrator": [vp, i, i, i],
         "ghx_set_seed": [vp, i, u64],
         "ghx_set_nested": [vp, i, i],
+        "ghx_set_nested_inner": [vp, i],
         "ghx_init_fields": [vp, i, i, d, i, u64, d],
         "ghx_set_counters": [vp, i, u64, u64],
         "ghx_get_counters": [vp, i, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), dp],

@@ -54,7 +54,7 @@ struct ghx_ctx {
   bool xlane_enabled = true;  // GHX_TUNE_X_LANES
   bool fuse_inner = true;     // GHX_TUNE_FUSE_INNER
   // multi-timescale integrator (0 = single scale)
-  int n_outer = 0, m_inner = 1;
+  int n_outer = 0, m_inner = 1, inner_integ = GHX_LEAPFROG;
   // surrogate link force (learned); S planes are filled by surr_fill before a kick
   void* S = nullptr;
   bool use_surr = false;

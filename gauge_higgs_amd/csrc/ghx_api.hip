@@ -395,11 +395,25 @@ void launch_step(ghx_ctx* c, double ck, double cd, bool drift) {
 // Adjacent inner half kicks are merged and fused with the drifts. With the
 // Omelyan integrator selected the outer scale is the minimum-norm scheme
 //   [K_out(l eps) I(eps/2) K_out((1-2l) eps) I(eps/2) K_out(l eps)]^n_outer
-// (adjacent K_out(l eps) merged), I(t) = m inner leapfrog steps over time t.
+// (adjacent K_out(l eps) merged), I(t) = m inner steps over time t: leapfrog,
+// or with GHX_OMELYAN chosen for the inner scale the minimum-norm step
+//   K_in(l h) D(h/2) K_in((1-2l) h) D(h/2) K_in(l h),  h = t/m,
+// adjacent K_in(l h) merged (the inner_fast scheme of u1.hpp:300-317): 2m+1
+// fused launches against m+1 for leapfrog, with a far smaller error constant.
 constexpr int kInParts = F_NO_HOP, kOutParts = F_NO_GAUGE | F_NO_ONSITE;
 void launch_inner(ghx_ctx* c, double t) {  // t in units of eps
   const int m = c->m_inner;
   const double h = t / m;
+  if (c->inner_integ == GHX_OMELYAN) {
+    const double lam = kOmelyanLambda;
+    launch_part_step(c, lam * h, 0.5 * h, true, kInParts, +1.0);
+    for (int k = 0; k < m; ++k) {
+      launch_part_step(c, (1.0 - 2.0 * lam) * h, 0.5 * h, true, kInParts, +1.0);
+      if (k != m - 1) launch_part_step(c, 2.0 * lam * h, 0.5 * h, true, kInParts, +1.0);
+    }
+    launch_part_step(c, lam * h, 0.0, false, kInParts, +1.0);
+    return;
+  }
   launch_part_step(c, 0.5 * h, h, true, kInParts, +1.0);
   for (int k = 1; k < m; ++k) launch_part_step(c, h, h, true, kInParts, +1.0);
   launch_part_step(c, 0.5 * h, 0.0, false, kInParts, +1.0);
@@ -760,6 +774,13 @@ int ghx_set_nested(ghx_ctx* ctx, int n_outer, int m_inner) {
   for (auto& rd : ctx->hreps) rd.eps = rd.tau / (n_outer > 0 ? n_outer : ctx->nmd);
   drop_graph(ctx);
   return upload_reps(ctx);
+}
+
+int ghx_set_nested_inner(ghx_ctx* ctx, int integrator) {
+  if (!ctx || (integrator != GHX_LEAPFROG && integrator != GHX_OMELYAN)) return GHX_E_INVALID;
+  ctx->inner_integ = integrator;
+  drop_graph(ctx);
+  return GHX_OK;
 }
 
 int ghx_set_seed(ghx_ctx* ctx, int replica, uint64_t seed) {
@@ -1519,8 +1540,10 @@ int64_t ghx_volume(const ghx_ctx* ctx) {
 
 
 // ---- checkpoint / resume (roadmap item of the reference, README.md:126-127) ----
-// Binary file, little-endian: magic "GHXCKPT1", then int32 {version = 2, D, N,
-// n_replicas, precision, d, frozen, nmd, integrator, reunit, n_outer, m_inner},
+// Binary file, little-endian: magic "GHXCKPT1", then int32 {version = 3, D, N,
+// n_replicas, precision, d, frozen, nmd, integrator, reunit, n_outer, m_inner,
+// inner integrator} (version 2 files end the header after m_inner and load as
+// a leapfrog inner scale),
 // int32 extents[n_replicas*D], per replica {u64 seed, traj_count,
 // accept_count; f64 last_dH, beta, kappa, lambda, tau}, u64 {swap seed, swap
 // counter}, int32 n_pairs, u64 attempts[n_pairs], u64 accepts[n_pairs], then
@@ -1530,7 +1553,7 @@ int64_t ghx_volume(const ghx_ctx* ctx) {
 // geometry, irrep, frozen flag, integrator settings and replica-exchange pairs.
 namespace {
 constexpr char kCkptMagic[8] = {'G', 'H', 'X', 'C', 'K', 'P', 'T', '1'};
-constexpr int kCkptVersion = 2, kCkptHdr = 12;
+constexpr int kCkptVersion = 3, kCkptHdr = 13;
 struct CkptRep {
   uint64_t seed, traj, acc;
   double last_dH, beta, kappa, lambda, tau;
@@ -1546,7 +1569,7 @@ int ghx_save_checkpoint(ghx_ctx* ctx, const char* path) {
   if (!f) return fail(ctx, GHX_E_INVALID, std::string("cannot open ") + path);
   bool ok = std::fwrite(kCkptMagic, 1, 8, f) == 8;
   const int32_t hdr[kCkptHdr] = {kCkptVersion, ctx->D, ctx->N, ctx->nrep, ctx->prec, ctx->rep.d, ctx->frozen ? 1 : 0,
-                                 ctx->nmd, ctx->integ, ctx->reunit ? 1 : 0, ctx->n_outer, ctx->m_inner};
+                                 ctx->nmd, ctx->integ, ctx->reunit ? 1 : 0, ctx->n_outer, ctx->m_inner, ctx->inner_integ};
   ok = ok && std::fwrite(hdr, sizeof hdr, 1, f) == 1;
   for (int r = 0; r < ctx->nrep && ok; ++r) {
     int32_t L[4] = {0, 0, 0, 0};
@@ -1599,12 +1622,15 @@ int ghx_load_checkpoint(ghx_ctx* ctx, const char* path) {
   char magic[8];
   int32_t hdr[kCkptHdr];
   if (std::fread(magic, 1, 8, f) != 8 || std::memcmp(magic, kCkptMagic, 8) != 0) return bad("not a ghx checkpoint");
-  if (std::fread(hdr, sizeof hdr, 1, f) != 1 || hdr[0] != kCkptVersion) return bad("unknown version");
+  if (std::fread(hdr, sizeof(int32_t), kCkptHdr - 1, f) != size_t(kCkptHdr - 1) || (hdr[0] != 2 && hdr[0] != kCkptVersion))
+    return bad("unknown version");
+  hdr[kCkptHdr - 1] = GHX_LEAPFROG;
+  if (hdr[0] == kCkptVersion && std::fread(hdr + kCkptHdr - 1, sizeof(int32_t), 1, f) != 1) return bad("truncated");
   if (hdr[1] != ctx->D || hdr[2] != ctx->N || hdr[3] != ctx->nrep || hdr[5] != ctx->rep.d)
     return bad("lattice dimension, colours, replica count or scalar dimension differ from this context");
   if (hdr[6] != (ctx->frozen ? 1 : 0)) return bad("frozen-length flag differs from this context (set it before loading)");
   if (hdr[7] != ctx->nmd || hdr[8] != ctx->integ || hdr[9] != (ctx->reunit ? 1 : 0) || hdr[10] != ctx->n_outer ||
-      hdr[11] != ctx->m_inner)
+      hdr[11] != ctx->m_inner || (ctx->n_outer > 0 && hdr[12] != ctx->inner_integ))
     return bad("integrator settings (nmd, integrator, reunitarization, nested scales) differ from this context");
   for (int r = 0; r < ctx->nrep; ++r) {
     int32_t L[4] = {0, 0, 0, 0};

@@ -81,9 +81,11 @@ class Lattices:
         self.nmd, self.integ, self.reunit = nmd, integ, reunit
         self._check(self._lib.ghx_set_integrator(self.h, nmd, integ, int(reunit)))
 
-    def set_nested(self, n_outer, m_inner=1):
-        """Multi-timescale leapfrog: n_outer hopping-force steps, m_inner gauge/on-site steps each (0 = off)."""
+    def set_nested(self, n_outer, m_inner=1, inner=LEAPFROG):
+        """Multi-timescale integrator: n_outer hopping-force steps (integrator of set_integrator), m_inner
+        gauge/on-site steps each, leapfrog or Omelyan (inner); n_outer = 0 turns it off."""
         self._check(self._lib.ghx_set_nested(self.h, n_outer, m_inner))
+        self._check(self._lib.ghx_set_nested_inner(self.h, inner))
 
     def set_multi_potential(self, f, mu2):
         """Multi-invariant quartic potential V = -mu2 |phi|^2 + sum_c f_c V_c (scalar_invariants.hpp);

@@ -86,11 +86,17 @@ def basis_bound(B, F, subsets=((0,), (0, 1), (0, 1, 2), (0, 1, 2, 3))):
     return out
 
 
+def split_key(split):
+    """Result key of a split (n_outer, m_inner[, inner integrator])."""
+    inner = split[2] if len(split) > 2 else _h.LEAPFROG
+    return "omelyan_outer%d_%sinner%d" % (split[0], "omelyan_" if inner == _h.OMELYAN else "", split[1])
+
+
 RESIDUAL_GATE = 0.9  # use S only if it explains > 10 % of |F|^2 on a held-out configuration
 
 
 def compare(L=12, rep="6,1", beta=5.7, kappa=0.05, lam=0.5, nmd=16, exact_nmds=(16, 20, 24, 28, 32, 40), n_therm=40, n_meas=200,
-            splits=((5, 10),), train_configs=8, train_iters=200, lr=2e-3, seed=1, precision=_h.FP32MD,
+            splits=((5, 4, _h.OMELYAN), (5, 10)), train_configs=8, train_iters=200, lr=2e-3, seed=1, precision=_h.FP32MD,
             therm_split=(12, 6), warm=(0.4, 0.27), use_graph=True):
     """Cost per accepted trajectory of exact-force HMC and of the multi-timescale
     chain with and without the learned surrogate, from one thermalised
@@ -103,8 +109,12 @@ def compare(L=12, rep="6,1", beta=5.7, kappa=0.05, lam=0.5, nmd=16, exact_nmds=(
     from equilibrium). The exact chains run Omelyan at each nmd in exact_nmds
     (the reference setting is nmd = 16); the surrogate is trained on
     (configuration, exact matter link force) pairs from the thermalised chain;
-    each split (n_outer, m_inner) runs with the Omelyan outer scale, with the
-    surrogate forced on ("surrogate_on") and without it ("split_no_surrogate").
+    each split (n_outer, m_inner[, inner integrator], leapfrog inner by default)
+    runs with the Omelyan outer scale, with the surrogate forced on
+    ("surrogate_on") and without it ("split_no_surrogate"). The learned chain is
+    the first split; the default pairs 5 outer steps of 4 inner Omelyan steps
+    (9 inner launches per half step) with the same outer scale over 10 inner
+    leapfrog steps (11 launches), the cheapest leapfrog-inner split measured.
 
     The learned chain keeps the exact force as fallback and arbiter: it uses the
     surrogate only when its held-out residual |S - F|^2 / |F|^2 is below
@@ -152,11 +162,11 @@ def compare(L=12, rep="6,1", beta=5.7, kappa=0.05, lam=0.5, nmd=16, exact_nmds=(
         g.configure(nmd=n)
         out["exact"][f"omelyan_nmd{n}"] = _chain(g, n_meas, use_graph=use_graph)
     g.configure(nmd=nmd)
-    for (no, mi) in splits:
-        key = f"omelyan_outer{no}_inner{mi}"
+    for sp in splits:
+        key = split_key(sp)
         for name, surr in (("surrogate_on", True), ("split_no_surrogate", False)):
             restart()
-            g.lat.set_nested(no, mi)
+            g.lat.set_nested(*sp)
             g.lat.surrogate_enable(surr)
             out[name][key] = _chain(g, n_meas, use_graph=use_graph)
     g.lat.surrogate_enable(False)
@@ -166,7 +176,7 @@ def compare(L=12, rep="6,1", beta=5.7, kappa=0.05, lam=0.5, nmd=16, exact_nmds=(
                        "seconds": train_s, "loss_first": losses[0][0], "loss_last": losses[-1][1],
                        "heldout_residual_fraction": resid, "basis_bound": bound}
     use_s = resid < RESIDUAL_GATE
-    key = "omelyan_outer%d_inner%d" % splits[0]
+    key = split_key(splits[0])
     out["learned"] = {"surrogate_used": use_s, "gate": f"held-out residual {resid:.3f} vs {RESIDUAL_GATE}",
                       "chain": ("surrogate_on" if use_s else "split_no_surrogate") + "/" + key}
     learned = out["surrogate_on" if use_s else "split_no_surrogate"][key]

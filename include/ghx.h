@@ -66,6 +66,11 @@ int ghx_set_integrator(ghx_ctx* ctx, int nmd, int integrator, int reunit_each_tr
  * steps that carry the gauge force and the on-site scalar force. n_outer = 0
  * restores the single-scale integrator. */
 int ghx_set_nested(ghx_ctx* ctx, int n_outer, int m_inner);
+/* Integrator of the inner scale of ghx_set_nested: GHX_LEAPFROG (default) or
+ * GHX_OMELYAN, the minimum-norm step the reference's U(1) nested scheme uses
+ * on both scales (inner_fast, u1.hpp:300-317). m_inner Omelyan steps cost
+ * 2*m_inner+1 fused inner launches, m_inner leapfrog steps m_inner+1. */
+int ghx_set_nested_inner(ghx_ctx* ctx, int integrator);
 /* Frozen-length scalar |phi_x| = 1 (GaugeHiggsHMC::frozen_phi,
  * gauge_higgs_hmc.hpp:38, 61-103, 133, 144, 172): momenta are projected onto
  * the tangent space at refresh and after every kick, and the phi drift is the

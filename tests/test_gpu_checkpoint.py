@@ -59,3 +59,40 @@ def test_mismatch_refused(tmp_path):
     c = gha.GaugeHiggsHMC(4, "adj", seed=5, D=4, N=2)
     with pytest.raises(gha.GhxError):
         c.lat.load_checkpoint(path)
+
+
+def test_nested_inner_integrator_in_checkpoint(tmp_path):
+    # the inner integrator of the multi-timescale scheme is part of the
+    # integrator settings a checkpoint must match
+    path = str(tmp_path / "n.ghx")
+    a = chain(gha.FP64)
+    a.lat.set_nested(2, 3, inner=gha.OMELYAN)
+    a.lat.trajectories(2, use_graph=True)
+    a.lat.save_checkpoint(path)
+    a.lat.trajectories(2, use_graph=True)
+    b = chain(gha.FP64)
+    b.lat.set_nested(2, 3, inner=gha.LEAPFROG)
+    with pytest.raises(gha.GhxError):
+        b.lat.load_checkpoint(path)
+    b.lat.set_nested(2, 3, inner=gha.OMELYAN)
+    b.lat.load_checkpoint(path)
+    b.lat.trajectories(2, use_graph=True)
+    assert a.lat.counters() == b.lat.counters()
+    assert np.array_equal(a.get("U"), b.get("U")) and np.array_equal(a.get("phi"), b.get("phi"))
+
+
+def test_version2_checkpoint_loads(tmp_path):
+    # a version-2 file (header without the inner integrator word) still loads
+    p3, p2 = str(tmp_path / "v3.ghx"), str(tmp_path / "v2.ghx")
+    a = chain(gha.FP64)
+    a.lat.trajectories(2, use_graph=True)
+    a.lat.save_checkpoint(p3)
+    raw = bytearray(open(p3, "rb").read())
+    assert raw[:8] == b"GHXCKPT1" and int.from_bytes(raw[8:12], "little") == 3
+    raw[8:12] = (2).to_bytes(4, "little")
+    del raw[8 + 4 * 12:8 + 4 * 13]
+    open(p2, "wb").write(bytes(raw))
+    b = chain(gha.FP64)
+    b.lat.load_checkpoint(p2)
+    assert a.lat.counters() == b.lat.counters()
+    assert np.array_equal(a.get("U"), b.get("U")) and np.array_equal(a.get("phi"), b.get("phi"))

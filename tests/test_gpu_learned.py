@@ -272,6 +272,73 @@ def test_nested_omelyan_outer():
     assert 2.5 < dh[0] / dh[1] < 6.0, dh
 
 
+@pytest.mark.parametrize("inner", [gha.LEAPFROG, gha.OMELYAN])
+def test_nested_inner_schedule_at_kappa0(inner):
+    # at kappa = 0 the hopping forces vanish and the outer kicks do nothing:
+    # n outer steps of m inner ones are then the single-scale integrator with
+    # n*m (leapfrog outer) or 2*n*m (Omelyan outer) steps, up to the rounding
+    # of merged kicks. Pins the inner step coefficients of both schemes.
+    n, m = 2, 3
+    a, b = make(kappa=0.0), make(kappa=0.0)
+    for g in (a, b):
+        g.configure(integ=inner)
+    a.configure(nmd=n * m * (2 if inner == gha.OMELYAN else 1))
+    b.lat.set_nested(n, m, inner=inner)
+    a.refresh_momenta()
+    b.set("P", a.get("P"))
+    b.set("pi", a.get("pi"))
+    a.md_evolve()
+    b.md_evolve()
+    for k in ("U", "P", "phi", "pi"):
+        assert np.max(np.abs(a.get(k) - b.get(k))) < 1e-11, k
+
+
+@pytest.mark.parametrize("fuse", [0, 1])
+def test_nested_omelyan_inner_bigrep_at_kappa0(fuse):
+    # the same pin for the {6,1} irrep, whose inner steps run the on-site
+    # kernel (fused with the gauge update or not)
+    def mk():
+        g = gha.GaugeHiggsHMC(4, "6,1", seed=1, D=4, N=3).configure(beta=5.7, kappa=0.0, lam=0.5, nmd=8, integ=gha.OMELYAN,
+                                                                    reunit_each_traj=False)
+        g.hot(0.8, 12345, 0.3)
+        g.lat.set_link_lanes(1)
+        g.lat.set_fuse_inner(fuse)
+        return g
+    a, b = mk(), mk()
+    b.lat.set_nested(2, 2, inner=gha.OMELYAN)
+    a.refresh_momenta()
+    b.set("P", a.get("P"))
+    b.set("pi", a.get("pi"))
+    a.md_evolve()
+    b.md_evolve()
+    for k in ("U", "P", "phi", "pi"):
+        assert np.max(np.abs(a.get(k) - b.get(k))) < 1e-10, k
+
+
+def test_nested_omelyan_inner_reversible_second_order():
+    g = make()
+    g.configure(integ=gha.OMELYAN)
+    g.lat.set_nested(3, 2, inner=gha.OMELYAN)
+    g.refresh_momenta()
+    U0, phi0 = g.get("U"), g.get("phi")
+    g.md_evolve()
+    g.set("P", -g.get("P"))
+    g.set("pi", -g.get("pi"))
+    g.md_evolve()
+    assert np.max(np.abs(g.get("U") - U0)) < 1e-10
+    assert np.max(np.abs(g.get("phi") - phi0)) < 1e-10
+    dh = []
+    for n in (6, 12):
+        c = make()
+        c.configure(integ=gha.OMELYAN)
+        c.lat.set_nested(n, 2, inner=gha.OMELYAN)
+        c.refresh_momenta()
+        h0 = c.hamiltonian()
+        c.md_evolve()
+        dh.append(abs(c.hamiltonian() - h0))
+    assert 2.5 < dh[0] / dh[1] < 6.0, dh
+
+
 def test_nested_m1_is_leapfrog_bigrep():
     # inner steps of a large irrep run the per-site on-site kernel; with one inner
     # step the split must still be the reference leapfrog up to rounding
