@@ -39,15 +39,19 @@ struct BigRepTables {
     const int4* prog = nullptr;       // LDS program: npass x 64 entries {s0, s1, s2, dst | j}, LDS byte offsets
     const double* pre = nullptr;      // [2][dimS]: 1/F (forward hop) and F (adjoint), FP64 and FP32 copies
     const float* pref = nullptr;
-    const double* Yp = nullptr;       // [2][ng][d][4] (q = 4 g + i): Yc F / c and Yc F^-1 / c
-    int ng = 0;
-    const float* Ypf = nullptr;
+    // the projections Yc F / c (forward hop) and Yc F^-1 / c (adjoint) as sparse
+    // rows: entry k of component l of direction dir at [(dir ynz + k) d + l]
+    const int* ysq = nullptr;         // the lifted index q
+    const double* ysv = nullptr;      // the value, FP64 and FP32 copies
+    const float* ysvf = nullptr;
+    int ynz = 0;
+    int ng = 0;                       // groups of 4 lifted components (dim S rounded up): V and W regions are 4 ng apart
     const double* Yl = nullptr;       // the lift table Yc transposed: [d][dimSp], dimSp = dimS rounded up to 16, zero padded
     const float* Ylf = nullptr;
     int dimSp = 0;
     void* lift = nullptr;          // Yc phi per site, site-major: [Vp][dimS] complex, storage type
     void* hops = nullptr;          // forward hops D(U_mu(x)) phi(x+mu), site-major: [Vp][D][d] complex, storage type
-    void* hopsL = nullptr;         // backward hops of x+mu before projection, conjugated: [Vp][D][4 ng] complex
+    void* hopsL = nullptr;         // backward hops D(U_mu(x))^dag phi(x) of x+mu, conjugated: [Vp][D][d] complex
     int lds_one = 0, lds_S1 = 0, lds_S2 = 0, lds_V = 0, lds_T1 = 0, lds_W = 0, lds_total = 0;  // cx units
     int k1 = 0, k2 = 0, lds_Cb = 0;   // Young rows; the level buffer that alternates with lds_S1
     int pair = 0;                     // 1: sym_link_pair_kernel (SU(3), k1 <= 6), two LDS regions per workgroup

@@ -162,7 +162,7 @@ int upload_tables(ghx_ctx* ctx) {
     std::vector<double> Yl(size_t(r.d) * dimSp, 0.0);
     for (int q = 0; q < sb.dimS; ++q)
       for (int l = 0; l < r.d; ++l) Yl[size_t(l) * dimSp + q] = sb.Yc[size_t(q) * r.d + l];
-    const std::vector<float> Ylf(Yl.begin(), Yl.end()), pref(sp.pre.begin(), sp.pre.end()), Ypf(sp.Yp.begin(), sp.Yp.end());
+    const std::vector<float> Ylf(Yl.begin(), Yl.end()), pref(sp.pre.begin(), sp.pre.end()), ysvf(sp.ysv.begin(), sp.ysv.end());
     // the program in LDS byte offsets of this precision's complex slots: sources
     // as offsets, the destination's offset with the column j in its low bits
     const int csz = 2 * int(ctx->tsize);
@@ -171,11 +171,11 @@ int upload_tables(ghx_ctx* ctx) {
       for (int i = 0; i < 3; ++i) progb[e + i] = sp.prog[e + i] * csz;
       progb[e + 3] = (sp.prog[e + 3] >> 2) * csz | (sp.prog[e + 3] & 3);
     }
-    const void* ssrc[7] = {progb.data(), sp.pre.data(), pref.data(), sp.Yp.data(), Ypf.data(), Yl.data(), Ylf.data()};
-    const size_t sbytes[7] = {sp.prog.size() * 4, sp.pre.size() * 8, pref.size() * 4, sp.Yp.size() * 8, Ypf.size() * 4,
-                              Yl.size() * 8, Ylf.size() * 4};
-    void* dp[7] = {};
-    for (int k = 0; k < 7; ++k) {
+    const void* ssrc[8] = {progb.data(), sp.pre.data(), pref.data(), sp.ysv.data(), ysvf.data(), Yl.data(), Ylf.data(), sp.ysq.data()};
+    const size_t sbytes[8] = {sp.prog.size() * 4, sp.pre.size() * 8, pref.size() * 4, sp.ysv.size() * 8, ysvf.size() * 4,
+                              Yl.size() * 8, Ylf.size() * 4, sp.ysq.size() * 4};
+    void* dp[8] = {};
+    for (int k = 0; k < 8; ++k) {
       HIPCHK(hipMalloc(&dp[k], std::max<size_t>(sbytes[k], 16)));
       ctx->dsym.push_back(dp[k]);
       HIPCHK(hipMemcpy(dp[k], ssrc[k], sbytes[k], hipMemcpyHostToDevice));
@@ -187,14 +187,14 @@ int upload_tables(ghx_ctx* ctx) {
     HIPCHK(hipMalloc(&hops, size_t(2) * ctx->D * r.d * ctx->Vp * ctx->tsize));
     ctx->dsym.push_back(hops);
     void* hopsL = nullptr;
-    HIPCHK(hipMalloc(&hopsL, size_t(2) * ctx->D * 4 * sp.ng * ctx->Vp * ctx->tsize));
+    HIPCHK(hipMalloc(&hopsL, size_t(2) * ctx->D * r.d * ctx->Vp * ctx->tsize));
     ctx->dsym.push_back(hopsL);
     BigRepTables::Sym& st = ctx->tb.sym;
     st.s1 = sb.s1; st.s2 = sb.s2; st.dimS = sb.dimS; st.npass = sp.npass; st.ng = sp.ng;
     st.s1p = sp.s1p; st.s2p = sp.s2p;
     st.prog = (const int4*)dp[0];
     st.pre = (const double*)dp[1]; st.pref = (const float*)dp[2];
-    st.Yp = (const double*)dp[3]; st.Ypf = (const float*)dp[4];
+    st.ysv = (const double*)dp[3]; st.ysvf = (const float*)dp[4]; st.ysq = (const int*)dp[7]; st.ynz = sp.ynz;
     st.Yl = (const double*)dp[5]; st.Ylf = (const float*)dp[6]; st.dimSp = dimSp;
     st.lift = lift;
     st.hops = hops;

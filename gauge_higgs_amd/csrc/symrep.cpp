@@ -290,6 +290,26 @@ SymProgram sym_program(const SymBuild& s) {
       yp(1, q, l) = y / F[q];
     }
   }
+  // the same tables as sparse rows (exact zeros dropped: entries of another
+  // colour content are products with zero)
+  for (int dir = 0; dir < 2; ++dir)
+    for (int l = 0; l < s.d; ++l) {
+      int nz = 0;
+      for (int q = 0; q < s.dimS; ++q) nz += yp(dir, q, l) != 0.0;
+      p.ynz = std::max(p.ynz, nz);
+    }
+  p.ysq.assign(size_t(2) * p.ynz * s.d, 0);
+  p.ysv.assign(p.ysq.size(), 0.0);
+  for (int dir = 0; dir < 2; ++dir)
+    for (int l = 0; l < s.d; ++l) {
+      int k = 0;
+      for (int q = 0; q < s.dimS; ++q)
+        if (yp(dir, q, l) != 0.0) {
+          const size_t e = (size_t(dir) * p.ynz + k++) * s.d + l;
+          p.ysq[e] = q;
+          p.ysv[e] = yp(dir, q, l);
+        }
+    }
   // run the program on the host, as the device does, against the engine's
   // matrices (already checked against the tensor apply): every column of
   // D(U) and of D(U)^dag on two generic links
@@ -318,7 +338,7 @@ SymProgram sym_program(const SymBuild& s) {
     const std::vector<cplx> S2 = k2 > 0 ? sym_matrix(s, U, k2) : std::vector<cplx>(1, cplx(1, 0));
     for (int dir = 0; dir < 2; ++dir)
       for (int l = 0; l < s.d; ++l) {
-        // device path: V = pre (Yc e_l), W = (C1 x C2) V (or its adjoint), out = Yp^T W
+        // device path: V = pre (Yc e_l), W = (C1 x C2) V (or its adjoint), out = the sparse rows of Yp^T applied to W
         std::vector<cplx> V(s.dimS), W(s.dimS, cplx(0, 0)), ref(s.dimS, cplx(0, 0));
         for (int q = 0; q < s.dimS; ++q) V[size_t(q)] = p.pre[size_t(dir) * s.dimS + q] * s.Yc[size_t(q) * s.d + l];
         for (int a = 0; a < s.s1; ++a)
@@ -335,10 +355,11 @@ SymProgram sym_program(const SymBuild& s) {
               }
         for (int k = 0; k < s.d; ++k) {
           cplx dev(0, 0), eng(0, 0);
-          for (int q = 0; q < s.dimS; ++q) {
-            dev += yp(dir, q, k) * W[size_t(q)];
-            eng += s.Yc[size_t(q) * s.d + k] * s.invc * ref[size_t(q)];
+          for (int z = 0; z < p.ynz; ++z) {
+            const size_t e = (size_t(dir) * p.ynz + z) * s.d + k;
+            dev += p.ysv[e] * W[size_t(p.ysq[e])];
           }
+          for (int q = 0; q < s.dimS; ++q) eng += s.Yc[size_t(q) * s.d + k] * s.invc * ref[size_t(q)];
           worst = std::max(worst, std::abs(dev - eng));
         }
       }

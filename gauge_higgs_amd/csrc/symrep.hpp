@@ -76,6 +76,16 @@ struct SymProgram {
   std::vector<double> pre;         // [2][dimS]: 1/F (forward), F (adjoint)
   std::vector<double> Yp;          // [2][ng][d][4], q = 4 g + i: Yc F / c (forward), Yc F^-1 / c (adjoint); zero for q >= dimS
   int ng = 0;                      // groups of 4 components (dimS rounded up)
+  // The projections as sparse rows. Every basis vector of the irrep is a weight
+  // vector (the Young symmetrizer keeps a tuple's colour content, and vectors of
+  // different content are orthogonal, so Gram-Schmidt never mixes them), and a
+  // monomial pair q = (a, b) has the content a + b: row l of Yc is nonzero only
+  // on the q that split the content of l, at most dim Sym^k2 of them ({6,1}:
+  // 111 nonzeros of 48 x 84). Entry k of row l of direction dir is
+  // (ysq, ysv)[(dir ynz + k) d + l], rows padded with (0, 0.0) to ynz entries.
+  std::vector<int> ysq;
+  std::vector<double> ysv;
+  int ynz = 0;
   bool ok = false;                 // the program, run on the host, reproduces D(U) and D(U)^dag (1e-10)
   double max_err = 0.0;
 };

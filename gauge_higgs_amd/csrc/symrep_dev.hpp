@@ -5,11 +5,10 @@
 // takes one link: it runs the host-built LDS program that forms the
 // unnormalised C_k1 (and C_k2) of the link, applies C1 (x) C2 and its adjoint
 // to the lifted scalars at both ends as C1 V C2^T (V viewed as s1 x s2, scaled
-// by F^-1 or F on load), projects the forward hop back with the table Yc F / c
-// and leaves the backward hop lifted. sym_finish_kernel (one wave per site, a
-// lane per component) projects the sum of a site's lifted backward hops once,
-// and turns the hops into the matter link force, the scalar force and the
-// scalar's update.
+// by F^-1 or F on load), and projects both hops back with the sparse rows of
+// Yc F / c and Yc F^-1 / c. sym_finish_kernel (one wave per site, a lane per
+// component) turns a site's hops into the matter link force, the scalar force
+// and the scalar's update.
 #pragma once
 #include "bigrep.hpp"
 #include "symrep.hpp"
@@ -92,18 +91,15 @@ GHX_DEV void sym_build(const BigRepTables::Sym& st, const Mat<R, N>& U, cx<R>* L
 // and V's row a2 is the same for all lanes of a direction (an LDS broadcast),
 // and C2 is applied in registers before W is written. Larger s2 use the
 // two-stage form below (a lane per entry, C1 V through LDS).
-// W (separate from V: lanes of a later pass may still read V) gets zeros in
-// its padding entries [dimS, P) of both directions for the projection.
+// W is separate from V: lanes of a later pass may still read V.
 // t and nl: this lane's index among the nl lanes that serve the link.
 template <typename R>
 GHX_DEV void sym_apply2_rows(const BigRepTables::Sym& st, const cx<R>* S1, const cx<R>* S2, const cx<R>* V, cx<R>* W, int t, int nl) {
-  const int s1 = st.s1, s2 = st.s2, P = 4 * st.ng, dimS = st.dimS, s1p = st.s1p, s2p = st.s2p;
-  for (int e = t; e < 2 * P; e += nl)
-    if (e % P >= dimS) W[e] = {R(0), R(0)};
+  const int s1 = st.s1, s2 = st.s2, P = 4 * st.ng, s2p = st.s2p;
   for (int e = t; e < 2 * s1; e += nl) {
     const int dir = e >= s1, a = e - dir * s1;
-    const cx<R>* c = S1 + (dir ? a : a * s1p);  // row a of C1, or column a
-    const int cs = dir ? s1p : 1;
+    const cx<R>* c = S1 + (dir ? a : a * st.s1p);  // row a of C1, or column a
+    const int cs = dir ? st.s1p : 1;
     const cx<R>* v = V + dir * P;
     cx<R> acc[kSymRowS2];
 #pragma unroll
@@ -165,30 +161,23 @@ GHX_DEV const cx<R>* sym_apply2(const BigRepTables::Sym& st, const cx<R>* S1, co
   return W;
 }
 
-// The forward hop out[l] = sum_q Yp[0][q][l] W[0][q] (lanes over l; Yp has
-// 1/c folded in), stored straight to the hop record. Yp is grouped [g][l][4]
-// (q = 4 g + i), so a lane reads 4 consecutive q of its row in one 16- or
-// 32-byte access; W is zero from dimS to 4 ng.
+// A hop projected back to the irrep, out[l] = sum_k ysv[dir][k][l] W[q_k] with
+// q_k = ysq[dir][k][l] (lanes over l; 1/c and F or F^-1 folded into the
+// values), stored straight to the hop record. The rows hold a few nonzeros
+// each (the q that split the colour content of component l).
 template <typename T, typename R>
-GHX_DEV void sym_project_fwd(const R* __restrict__ Yp, int ng, int d, const cx<R>* W, T* out, int t, int nl) {
+GHX_DEV void sym_project(const BigRepTables::Sym& st, int dir, int d, const cx<R>* W, T* out, int t, int nl) {
+  const int nz = st.ynz;
+  const int* __restrict__ qs = st.ysq + size_t(dir) * nz * d;
+  const R* __restrict__ vs =
+      (sizeof(R) == 4 ? reinterpret_cast<const R*>(st.ysvf) : reinterpret_cast<const R*>(st.ysv)) + size_t(dir) * nz * d;
   for (int l = t; l < d; l += nl) {
-    const R* y = Yp + size_t(l) * 4;
     cx<R> acc = {R(0), R(0)};
-#pragma unroll 4
-    for (int g = 0; g < ng; ++g, y += 4 * d) {
-      R v[4];
-      if constexpr (sizeof(R) == 4) {
-        const float4 f = *reinterpret_cast<const float4*>(y);
-        v[0] = f.x; v[1] = f.y; v[2] = f.z; v[3] = f.w;
-      } else {
-        const double2 f0 = *reinterpret_cast<const double2*>(y), f1 = *reinterpret_cast<const double2*>(y + 2);
-        v[0] = f0.x; v[1] = f0.y; v[2] = f1.x; v[3] = f1.y;
-      }
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        acc.r = fma(v[i], W[4 * g + i].r, acc.r);
-        acc.i = fma(v[i], W[4 * g + i].i, acc.i);
-      }
+    for (int k = 0; k < nz; ++k) {
+      const R v = vs[k * d + l];
+      const cx<R> w = W[qs[k * d + l]];
+      acc.r = fma(v, w.r, acc.r);
+      acc.i = fma(v, w.i, acc.i);
     }
     st_cx(out + 2 * l, cx<T>{T(acc.r), T(acc.i)});
   }
@@ -293,10 +282,10 @@ __global__ __launch_bounds__(kSymWave* kLiftMfmaWaves) void sym_lift_mfma_kernel
 }
 
 // One link per wave (blockIdx.y = mu): Sym(U_mu(x)) is built once and serves
-// both hops through the link: D(U) phi_{x+mu}, the forward hop of x, projected
-// into hops[2 ((g D + mu) d + l) + (re, im)]; and D(U)^dag phi_x, the backward
-// hop of x+mu (skipped unless both), left lifted and conjugated in
-// hopsL[2 ((g D + mu) 4 ng + q) + (re, im)].
+// both hops through the link: D(U) phi_{x+mu}, the forward hop of x, into
+// hops[2 ((g D + mu) d + l) + (re, im)]; and D(U)^dag phi_x, the backward hop
+// of x+mu (skipped unless both), conjugated, into
+// hopsL[2 ((g' D + mu) d + l) + (re, im)] with g' = x+mu.
 template <typename T, int N, int D>
 __global__ __launch_bounds__(kSymWave) void sym_link_kernel(BigArgs<T> a, BigRepTables tb, T* out, int both) {
   using R = T;
@@ -318,8 +307,6 @@ __global__ __launch_bounds__(kSymWave) void sym_link_kernel(BigArgs<T> a, BigRep
   const cx<R>* S1 = L + st.lds_S1;
   const cx<R>* S2 = L + st.lds_S2;
   const int P = 4 * st.ng, dimS = st.dimS, nd = both + 1;
-  for (int e = t; e < 2 * P; e += kSymWave)  // the projection's padding
-    if (e % P >= dimS) W[e] = {R(0), R(0)};
   // V0 = F^-1 Yc phi(x+mu) for the forward hop of x, V1 = conj(F Yc phi(x)) for
   // the adjoint hop of x+mu
   const T* lift = static_cast<const T*>(st.lift);
@@ -331,15 +318,8 @@ __global__ __launch_bounds__(kSymWave) void sym_link_kernel(BigArgs<T> a, BigRep
   }
   wave_sync();
   const cx<R>* Wr = sym_apply2<R>(st, S1, S2, V, T1, W, t, kSymWave);
-  // the forward hop is projected here (its g^a needs it alone); the backward
-  // hop only enters the scalar force of x+mu, summed over mu, so it is stored
-  // lifted and projected once per site by sym_finish_kernel
-  const R* Yp = sizeof(R) == 4 ? reinterpret_cast<const R*>(st.Ypf) : reinterpret_cast<const R*>(st.Yp);
-  sym_project_fwd<T, R>(Yp, st.ng, d, Wr, out + 2 * ((size_t(g) * D + mu) * d), t, kSymWave);
-  if (both) {
-    T* hl = static_cast<T*>(st.hopsL) + 2 * ((size_t(gp) * D + mu) * P);
-    for (int q = t; q < P; q += kSymWave) st_cx(hl + 2 * q, Wr[P + q]);
-  }
+  sym_project<T, R>(st, 0, d, Wr, out + 2 * ((size_t(g) * D + mu) * d), t, kSymWave);
+  if (both) sym_project<T, R>(st, 1, d, Wr + P, static_cast<T*>(st.hopsL) + 2 * ((size_t(gp) * D + mu) * d), t, kSymWave);
 }
 
 // ---------------------------------------------------------------------------
@@ -437,14 +417,12 @@ GHX_DEV void sym_level(const Mat<R, 3>& U, const cx<R>* Lp, cx<R>* Lc, cx<R>* S2
 // two independent accumulations that share each step's loop and hide each
 // other's LDS latency. Same arithmetic as sym_apply2_rows, with the row length
 // S1 = dim Sym^k1 known at compile time; the result lands in W (both
-// directions, P apart, padding zeroed). W may be V itself: every lane's reads
+// directions, P apart). W may be V itself: every lane's reads
 // of V precede its writes in the wave's instruction stream.
 template <typename R, int S1N>
 GHX_DEV void sym_apply_pair_rows(const BigRepTables::Sym& st, const cx<R>* S1, const cx<R>* S2, const cx<R>* V, cx<R>* W, int t) {
   constexpr int s1 = S1N, s1p = S1N | 1;
-  const int s2 = st.s2, P = 4 * st.ng, dimS = st.dimS, s2p = st.s2p;
-  for (int e = t; e < 2 * P; e += kSymPairLanes)
-    if (e % P >= dimS) W[e] = {R(0), R(0)};
+  const int s2 = st.s2, P = 4 * st.ng, s2p = st.s2p;
   if (t < s1) {
     const int a = t;
     const cx<R>* c0 = S1 + a * s1p;  // row a (forward)
@@ -486,42 +464,6 @@ GHX_DEV void sym_apply_pair_rows(const BigRepTables::Sym& st, const cx<R>* S1, c
     }
   }
   wave_sync();
-}
-
-// The forward projection for a half-wave: lane t forms components t and t + 32
-// (d <= 64) from one read of each W entry.
-template <typename T, typename R>
-GHX_DEV void sym_project_pair(const R* __restrict__ Yp, int ng, int d, const cx<R>* W, T* out, int t) {
-  if (t >= d) return;
-  const int l1 = t + kSymPairLanes;
-  const bool two = l1 < d;
-  const R* y0 = Yp + size_t(t) * 4;
-  const R* y1 = Yp + size_t(two ? l1 : t) * 4;  // a valid row when the second component is absent
-  cx<R> a0 = {R(0), R(0)}, a1 = {R(0), R(0)};
-#pragma unroll 4
-  for (int g = 0; g < ng; ++g, y0 += 4 * d, y1 += 4 * d) {
-    R u[4], v[4];
-    if constexpr (sizeof(R) == 4) {
-      const float4 p = *reinterpret_cast<const float4*>(y0), q = *reinterpret_cast<const float4*>(y1);
-      u[0] = p.x; u[1] = p.y; u[2] = p.z; u[3] = p.w;
-      v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
-    } else {
-      const double2 p0 = *reinterpret_cast<const double2*>(y0), p1 = *reinterpret_cast<const double2*>(y0 + 2);
-      const double2 q0 = *reinterpret_cast<const double2*>(y1), q1 = *reinterpret_cast<const double2*>(y1 + 2);
-      u[0] = p0.x; u[1] = p0.y; u[2] = p1.x; u[3] = p1.y;
-      v[0] = q0.x; v[1] = q0.y; v[2] = q1.x; v[3] = q1.y;
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const cx<R> w = W[4 * g + i];
-      a0.r = fma(u[i], w.r, a0.r);
-      a0.i = fma(u[i], w.i, a0.i);
-      a1.r = fma(v[i], w.r, a1.r);
-      a1.i = fma(v[i], w.i, a1.i);
-    }
-  }
-  st_cx(out + 2 * t, cx<T>{T(a0.r), T(a0.i)});
-  if (two) st_cx(out + 2 * l1, cx<T>{T(a1.r), T(a1.i)});
 }
 
 template <typename T, int D, int K1>
@@ -572,8 +514,6 @@ __global__ __launch_bounds__(kSymWave) void sym_link_pair_kernel(BigArgs<T> a, B
   cx<R>* V = L + st.pair_V;
   cx<R>* T1 = L + st.pair_T1;
   cx<R>* W = V;
-  for (int e = t; e < 2 * P; e += NL)  // the projection's padding
-    if (e % P >= dimS) W[e] = {R(0), R(0)};
   // V0 = F^-1 Yc phi(x+mu) for the forward hop of x, V1 = conj(F Yc phi(x)) for
   // the adjoint hop of x+mu
   const R* pre = sizeof(R) == 4 ? reinterpret_cast<const R*>(st.pref) : reinterpret_cast<const R*>(st.pre);
@@ -590,13 +530,9 @@ __global__ __launch_bounds__(kSymWave) void sym_link_pair_kernel(BigArgs<T> a, B
   const cx<R>* Wr = T1;
   if (st.s2 <= kSymRowS2) sym_apply_pair_rows<R, symc::sdim(K1)>(st, C, S2, V, T1, t);  // T1 = V here
   else Wr = sym_apply2<R>(st, C, S2, V, T1, W, t, NL);
-  const R* Yp = sizeof(R) == 4 ? reinterpret_cast<const R*>(st.Ypf) : reinterpret_cast<const R*>(st.Yp);
   if (!act) return;
-  sym_project_pair<T, R>(Yp, st.ng, tb.d, Wr, out + 2 * ((size_t(g) * D + mu) * tb.d), t);
-  if (both) {
-    T* hl = static_cast<T*>(st.hopsL) + 2 * ((size_t(gp) * D + mu) * P);
-    for (int q = t; q < P; q += NL) st_cx(hl + 2 * q, Wr[P + q]);
-  }
+  sym_project<T, R>(st, 0, tb.d, Wr, out + 2 * ((size_t(g) * D + mu) * tb.d), t, NL);
+  if (both) sym_project<T, R>(st, 1, tb.d, Wr + P, static_cast<T*>(st.hopsL) + 2 * ((size_t(gp) * D + mu) * tb.d), t, NL);
 }
 
 template <typename R>
@@ -608,15 +544,15 @@ GHX_DEV R wave_sum(R v) {
 
 template <typename T, int D>
 GHX_DEV const T* hl_of(const BigRepTables& tb, unsigned g) {
-  return static_cast<const T*>(tb.sym.hopsL) + 2 * (size_t(g) * D * (4 * tb.sym.ng));
+  return static_cast<const T*>(tb.sym.hopsL) + 2 * (size_t(g) * D * tb.d);
 }
 
 // The part of a site's finish after its scalar and momentum are loaded: the
-// backward hops projected once, g^a and the forward hops, the on-site term,
+// backward hops summed, g^a and the forward hops, the on-site term,
 // then the kick and drift into the site's staged rows (or its scalar force).
 template <typename T, int N, int D, class HOP>
 GHX_DEV void finish_site_body(const BigArgs<T>& a, const BigRepTables& tb, const RepDesc& rd, const T* __restrict__ hl, HOP&& hop,
-                              const cx<T> ph, const cx<T> pv0, cx<T>* lsum, cx<T> (&psi)[D][kSymWave], cx<T>* phs, T* rph, T* rpi,
+                              const cx<T> ph, const cx<T> pv0, cx<T> (&psi)[D][kSymWave], cx<T>* phs, T* rph, T* rpi,
                               T* io_g, int si, unsigned g, int nmu) {
   using R = T;
   constexpr int NGN = N * N - 1;
@@ -624,9 +560,12 @@ GHX_DEV void finish_site_body(const BigArgs<T>& a, const BigRepTables& tb, const
   const int d = tb.d, l = threadIdx.x;
   const bool on = l < d;
   // loads that do not depend on the backward hops, issued before them
-  cx<R> psf[D];
+  cx<R> psf[D], psb[D];
 #pragma unroll
-  for (int mu = 0; mu < D; ++mu) psf[mu] = mu < nmu ? hop(mu) : cx<R>{R(0), R(0)};
+  for (int mu = 0; mu < D; ++mu) {
+    psf[mu] = mu < nmu ? hop(mu) : cx<R>{R(0), R(0)};
+    psb[mu] = mu < nmu && on ? ld_cx(hl + 2 * (mu * d + l)) : cx<R>{R(0), R(0)};
+  }
   const R k2 = R(2.0 * rd.kappa);
   const BigRepTables::Sym& st = tb.sym;
   const int lpg = st.glpg;
@@ -634,30 +573,11 @@ GHX_DEV void finish_site_body(const BigArgs<T>& a, const BigRepTables& tb, const
   const R* gval = (sizeof(R) == 4 ? reinterpret_cast<const R*>(st.gvalf) : reinterpret_cast<const R*>(st.gval)) + 2 * l;
   phs[l] = ph;
   cx<R> f = {R(0), R(0)};
-  if (nmu) {  // the backward hops: sum the D lifted vectors, project once
-    const int P = 4 * st.ng;
-    for (int q = l; q < P; q += kSymWave) {
-      cx<R> v = ld_cx(hl + 2 * q);
+  if (nmu) {  // the backward hops (carried conjugated): chi = conj(sum_mu hl[mu][l])
+    cx<R> v = psb[0];
 #pragma unroll
-      for (int mu = 1; mu < D; ++mu) v = cadd(v, ld_cx(hl + 2 * (mu * P + q)));
-      lsum[q] = v;
-    }
-    wave_sync();
-    if (on) {  // chi = conj(sum_q Yp[1][q][l] lsum[q]) (the lifted hops are carried conjugated)
-      const R* y = (sizeof(R) == 4 ? reinterpret_cast<const R*>(st.Ypf) : reinterpret_cast<const R*>(st.Yp)) +
-                   (size_t(st.ng) * d + l) * 4;
-      cx<R> acc = {R(0), R(0)};
-#pragma unroll 4
-      for (int gq = 0; gq < st.ng; ++gq, y += 4 * d) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          acc.r = fma(y[i], lsum[4 * gq + i].r, acc.r);
-          acc.i = fma(y[i], lsum[4 * gq + i].i, acc.i);
-        }
-      }
-      f = {k2 * acc.r, -k2 * acc.i};
-    }
-    wave_sync();  // lsum and psi share the wave's LDS slice
+    for (int mu = 1; mu < D; ++mu) v = cadd(v, psb[mu]);
+    f = {k2 * v.r, -k2 * v.i};
     // the D forward hops at once, then g^a = -2 Im sum_lm conj(phi_l) T^a_lm psi_m
     // for every direction over the lanes of generator a (each entry read once)
 #pragma unroll
@@ -756,7 +676,7 @@ template <typename T, int N, int D>
 __global__ __launch_bounds__(kSymWave* kFinishSites) void sym_finish_kernel(BigArgs<T> a, BigRepTables tb, const T* __restrict__ hops) {
   using R = T;
   constexpr int NGN = N * N - 1, S = kFinishSites, NT = kSymWave * kFinishSites;
-  constexpr int kScr = (D > 4 ? D : 4) * kSymWave;  // the lifted backward hops (4 ng <= 4 * 64), then the forward hops
+  constexpr int kScr = D * kSymWave;  // the forward hops
   static_assert(kBlock % S == 0, "a workgroup's sites lie in one replica block");
   __shared__ cx<R> scr_s[S][kScr], phs_s[S][kSymWave];
   __shared__ __align__(16) T io_ph[S * kFinishRow];
@@ -780,7 +700,6 @@ __global__ __launch_bounds__(kSymWave* kFinishSites) void sym_finish_kernel(BigA
   const int si = int(threadIdx.y);
   if (live(si)) {
     const unsigned g = g0 + unsigned(si);
-    cx<R>* lsum = scr_s[si];
     cx<R>(&psi)[D][kSymWave] = *reinterpret_cast<cx<R>(*)[D][kSymWave]>(scr_s[si]);
     cx<R>(&phs)[kSymWave] = phs_s[si];
     T* rph = io_ph + si * kFinishRow;  // this site's staged phi (then phi_out)
@@ -801,7 +720,7 @@ __global__ __launch_bounds__(kSymWave* kFinishSites) void sym_finish_kernel(BigA
       if (l == 0) a.q1[g] = q;
     } else {
       finish_site_body<T, N, D>(a, tb, rd, hl_of<T, D>(tb, g), hop, ph, on ? (stepm ? ld_cx(rpi + 2 * l) : cx<R>{R(0), R(0)}) : cx<R>{R(0), R(0)},
-                                lsum, psi, phs, rph, rpi, io_g, si, g, nmu);
+                                psi, phs, rph, rpi, io_g, si, g, nmu);
     }
   }
   if (a.mode == BIG_ACTION) return;

@@ -3,7 +3,9 @@
 // Yc^T (Sym^k1 U (x) Sym^k2 U) Yc / c against the tensor apply W U^(x n) W^T
 // on generic links) and its device program (run here on the host and checked
 // against the engine's matrices for D(U) and D(U)^dag). Prints one line per
-// irrep: spec ok_build err_build ok_program err_program npass lds_total.
+// irrep: spec ok_build err_build ok_program err_program npass lds_total d
+// ynz s2 nnz (nonzeros per projection row at most, dim Sym^k2, nonzeros of
+// the forward projection).
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -18,7 +20,10 @@ int main(int argc, char** argv) {
     const ghx::RepInfo r = ghx::build_rep(N, a.substr(c + 1));
     const ghx::SymBuild s = ghx::build_sym(r);
     const ghx::SymProgram p = s.ok ? ghx::sym_program(s) : ghx::SymProgram{};
-    std::printf("%s %d %.3e %d %.3e %d %d %d\n", a.c_str(), int(s.ok), s.max_err, int(p.ok), p.max_err, p.npass, p.lds_total, r.d);
+    int nnz = 0;
+    for (int e = 0; e < p.ynz * r.d; ++e) nnz += p.ysv.empty() ? 0 : p.ysv[size_t(e)] != 0.0;
+    std::printf("%s %d %.3e %d %.3e %d %d %d %d %d %d\n", a.c_str(), int(s.ok), s.max_err, int(p.ok), p.max_err, p.npass, p.lds_total, r.d,
+                p.ynz, s.s2, nnz);
     std::fflush(stdout);
   }
   return 0;

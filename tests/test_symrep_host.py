@@ -38,7 +38,7 @@ def run(exe, specs):
         if line.strip():
             f = line.split()
             rows[f[0]] = dict(build=int(f[1]), build_err=float(f[2]), prog=int(f[3]), prog_err=float(f[4]), npass=int(f[5]),
-                              lds=int(f[6]), d=int(f[7]))
+                              lds=int(f[6]), d=int(f[7]), ynz=int(f[8]), s2=int(f[9]), nnz=int(f[10]))
     return rows
 
 
@@ -52,6 +52,19 @@ def test_engine_matches_tensor_apply_and_program_runs(checker):
         # LDS of the link kernel in FP64 (16 B per complex slot) stays far below the 160 KiB of a CU
         assert 16 * r["lds"] < 64 * 1024, (spec, r)
     assert rows["3:6,1"]["d"] == 48 and rows["3:6,2"]["d"] == 60
+
+
+def test_projection_rows_are_sparse(checker):
+    """The irrep's basis vectors are weight vectors, so a row of the projection
+    Yc F / c touches only the monomial pairs (a, b) that split its colour
+    content: at most dim Sym^k2 of them. The program check above runs the
+    device's sparse rows against the dense engine."""
+    rows = run(checker, ENGINE)
+    for spec in ENGINE:
+        r = rows[spec]
+        assert 1 <= r["ynz"] <= r["s2"], (spec, r)
+    # {6,1}: 48 rows of the 84 lifted components, at most 3 entries each
+    assert rows["3:6,1"]["ynz"] == 3 and rows["3:6,1"]["nnz"] <= 3 * 48
 
 
 def test_engine_refuses_what_it_cannot_run(checker):
