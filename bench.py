@@ -241,8 +241,9 @@ def main():
     ap.add_argument("--stream-seg", type=int, default=None, help="force a streaming segment (implies --no-tune)")
     ap.add_argument("--block-order", type=int, default=None, choices=(0, 1, 2, 3, 4, 5, 6),
                     help="force the block walk of the MD-step kernel (implies --no-tune)")
-    ap.add_argument("--sym-pairs", type=int, default=None, choices=(0, 1),
-                    help="large irreps: two links per wave in the symmetric-power engine (1, the default where it applies) or one (0)")
+    ap.add_argument("--sym-pairs", type=int, default=None, choices=(0, 1, 2),
+                    help="large irreps: the symmetric-power engine's link kernel: two links per wave with Sym^k(U) in registers "
+                         "(2, the default where it applies), in LDS (1), or one link per wave (0)")
     ap.add_argument("--no-learned", action="store_true", help="skip the exact vs learned-force HMC comparison")
     ap.add_argument("--only-learned", action="store_true", help="run only the exact vs learned-force comparison")
     ap.add_argument("--learned-meas", type=int, default=200, help="measured trajectories per chain in the comparison")
@@ -309,7 +310,7 @@ def main():
     achieved = V * bpsite / launch_s / 1e9
     kernel = "md_stream_kernel" if tuned.get("stream_seg") else ("md_link_kernel" if tuned.get("link_lanes") else "md_step_kernel")
     if d > 6:  # a large irrep: the matter part runs in the symmetric-power engine before the step kernel
-        link = "sym_link_pair_kernel" if g.lat.sym_pairs() else "sym_link_kernel"
+        link = ("sym_link_kernel", "sym_link_pair_kernel", "sym_link_col_kernel")[g.lat.sym_pairs()]
         kernel = f"sym_lift_kernel + {link} + sym_finish_kernel + " + kernel
         tuned["sym_pairs"] = g.lat.sym_pairs()
     traffic = load_traffic(args.workload, kernel, tuned.get("block_order", 0))

@@ -54,7 +54,7 @@ struct BigRepTables {
     void* hopsL = nullptr;         // backward hops D(U_mu(x))^dag phi(x) of x+mu, conjugated: [Vp][D][d] complex
     int lds_one = 0, lds_S1 = 0, lds_S2 = 0, lds_V = 0, lds_T1 = 0, lds_W = 0, lds_total = 0;  // cx units
     int k1 = 0, k2 = 0, lds_Cb = 0;   // Young rows; the level buffer that alternates with lds_S1
-    int pair = 0;                     // 1: sym_link_pair_kernel (SU(3), k1 <= 6), two LDS regions per workgroup
+    int pair = 0;                     // 1: sym_link_pair_kernel (SU(3), k1 <= 6), two LDS regions per workgroup; 2: sym_link_col_kernel
     int pair_S2 = 0, pair_V = 0, pair_T1 = 0, pair_total = 0;  // its per-link region (C at 0), cx units
     // g^a by lanes: generator a owns lanes [a lpg, (a+1) lpg); entry k of lane t
     // is (l | m << 16, T^a_lm) at [k][t], gK entries per lane (zero padded)

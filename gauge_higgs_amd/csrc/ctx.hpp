@@ -75,7 +75,7 @@ struct ghx_ctx {
   std::vector<void*> dsym;   // symmetric-power engine tables and lifted scalar
   bool sym_valid = false;    // the engine applies to this irrep (host-checked)
   bool sym_enabled = true;   // GHX_TUNE_SYM_ENGINE
-  bool sym_pair_enabled = true;  // GHX_TUNE_SYM_PAIRS
+  int sym_pair_mode = 2;         // GHX_TUNE_SYM_PAIRS: 0 one link per wave, 1 paired in LDS, 2 paired in registers (FP32, k2 <= 1)
   void* G = nullptr;
   double* q1 = nullptr;
   // replica exchange

@@ -248,11 +248,16 @@ int upload_tables(ghx_ctx* ctx) {
 // per-workgroup limit: the engine's LDS program, or the tensor-apply kernel's
 // dimT-entry tensor (dynamic) plus its static arrays. Irreps that fit neither
 // are refused here instead of failing at their first launch.
-bool sym_pair_applies(const ghx_ctx* ctx) {
-  return ctx->sym_valid && ctx->sym_pair_enabled && ctx->N == 3 && ctx->tb.sym.k1 >= 2 && ctx->tb.sym.k1 <= kSymPairMaxK;
+// The engine's link kernel for the chosen mode: 0 one link per wave (LDS
+// program), 1 two links per wave with Sym^k(U) in LDS, 2 two links per wave with
+// the columns of Sym^k(U) in registers (FP32 with a second Young row <= 1; else 1).
+int sym_pair_applies(const ghx_ctx* ctx) {
+  if (!ctx->sym_valid || ctx->sym_pair_mode == 0 || ctx->N != 3 || ctx->tb.sym.k1 < 2 || ctx->tb.sym.k1 > kSymPairMaxK) return 0;
+  return ctx->sym_pair_mode == 2 && ctx->tsize == 4 && ctx->tb.sym.k2 <= 1 ? 2 : 1;
 }
 size_t big_lds_need(const ghx_ctx* ctx, bool engine) {
   const size_t cs = 2 * size_t(ctx->tsize);
+  if (engine && ctx->tb.sym.pair == 2) return 2 * 2 * size_t(ctx->tb.sym.dimS) * cs;  // static: the two lifted vectors per link
   if (engine) return ctx->tb.sym.pair ? 2 * size_t(ctx->tb.sym.pair_total) * cs : size_t(ctx->tb.sym.lds_total) * cs;
   const size_t st = ctx->bops.matter_static_lds ? ctx->bops.matter_static_lds() : size_t(-1);
   return st == size_t(-1) ? st : size_t(ctx->rep.dimT) * cs + st;
@@ -266,7 +271,7 @@ int lds_limit(ghx_ctx* ctx, size_t* lim) {
 }
 int check_big_lds(ghx_ctx* ctx) {
   if (!ctx->big) return GHX_OK;
-  ctx->tb.sym.pair = sym_pair_applies(ctx) ? 1 : 0;
+  ctx->tb.sym.pair = sym_pair_applies(ctx);
   size_t lim = 0;
   int rc = lds_limit(ctx, &lim);
   if (rc) return rc;
@@ -1271,11 +1276,11 @@ int ghx_set_tuning(ghx_ctx* ctx, int key, int value) {
     return GHX_OK;
   }
   if (key == GHX_TUNE_SYM_PAIRS) {
-    if (value < 0 || value > 1) return fail(ctx, GHX_E_INVALID, "paired symmetric-power links must be 0 or 1");
-    const bool was = ctx->sym_pair_enabled;
-    ctx->sym_pair_enabled = value != 0;
+    if (value < 0 || value > 2) return fail(ctx, GHX_E_INVALID, "the symmetric-power link kernel must be 0, 1 or 2");
+    const int was = ctx->sym_pair_mode;
+    ctx->sym_pair_mode = value;
     if (int rc = check_big_lds(ctx)) {
-      ctx->sym_pair_enabled = was;
+      ctx->sym_pair_mode = was;
       check_big_lds(ctx);
       return rc;
     }
@@ -1303,7 +1308,7 @@ int ghx_get_tuning(const ghx_ctx* ctx, int key, int* value) {
   if (key == GHX_TUNE_BLOCK_ORDER) { *value = ctx->block_walk().order; return GHX_OK; }
   if (key == GHX_TUNE_LINK_LANES) { *value = ctx->link_lanes; return GHX_OK; }
   if (key == GHX_TUNE_SYM_ENGINE) { *value = ctx->tb.sym.on; return GHX_OK; }
-  if (key == GHX_TUNE_SYM_PAIRS) { *value = ctx->tb.sym.on && ctx->tb.sym.pair; return GHX_OK; }
+  if (key == GHX_TUNE_SYM_PAIRS) { *value = ctx->tb.sym.on ? ctx->tb.sym.pair : 0; return GHX_OK; }
   if (key == GHX_TUNE_FUSE_INNER) {  // whether a large irrep's inner-scale step runs as one launch
     *value = ctx->fuse_inner && ctx->big && ctx->D == 4 && ctx->link_lanes && !(ctx->nrep == 1 && ctx->stream_seg > 0);
     return GHX_OK;

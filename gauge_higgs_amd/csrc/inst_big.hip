@@ -11,11 +11,31 @@ struct BigLaunch {
     hipLaunchKernelGGL((sym_link_pair_kernel<T, D, K>), dim3(c.Vp, (D + 1) / 2), dim3(kSymWave),
                        2 * size_t(tb.sym.pair_total) * sizeof(cx<T>), c.stream, a, tb, hops, both);
   }
+  // FP32, second Young row <= 1: the columns of Sym^k(U) in registers (sym_link_col_kernel)
+  template <int K>
+  static void col_launch(const LaunchCtx& c, const BigRepTables& tb, const BigArgs<T>& a, T* hops, int both) {
+    if constexpr (sizeof(T) == 4) {
+      if (tb.sym.k2 == 0)
+        hipLaunchKernelGGL((sym_link_col_kernel<D, K, 0>), dim3(c.Vp, (D + 1) / 2), dim3(kSymWave), 0, c.stream, a, tb, hops, both);
+      else
+        hipLaunchKernelGGL((sym_link_col_kernel<D, K, 1>), dim3(c.Vp, (D + 1) / 2), dim3(kSymWave), 0, c.stream, a, tb, hops, both);
+    }
+  }
   static bool link_pair(const LaunchCtx& c, const BigRepTables& tb, const BigArgs<T>& a, T* hops, int both) {
     if constexpr (N != 3) {
       return false;
     } else {
       if (!tb.sym.pair) return false;
+      if (sizeof(T) == 4 && tb.sym.pair == 2 && tb.sym.k2 <= 1) {
+        switch (tb.sym.k1) {
+          case 2: col_launch<2>(c, tb, a, hops, both); return true;
+          case 3: col_launch<3>(c, tb, a, hops, both); return true;
+          case 4: col_launch<4>(c, tb, a, hops, both); return true;
+          case 5: col_launch<5>(c, tb, a, hops, both); return true;
+          case 6: col_launch<6>(c, tb, a, hops, both); return true;
+          default: return false;
+        }
+      }
       switch (tb.sym.k1) {
         case 2: pair_launch<2>(c, tb, a, hops, both); return true;
         case 3: pair_launch<3>(c, tb, a, hops, both); return true;

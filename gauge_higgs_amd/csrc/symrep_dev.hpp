@@ -535,6 +535,145 @@ __global__ __launch_bounds__(kSymWave) void sym_link_pair_kernel(BigArgs<T> a, B
   if (both) sym_project<T, R>(st, 1, tb.d, Wr + P, static_cast<T*>(st.hopsL) + 2 * ((size_t(gp) * D + mu) * tb.d), t, NL);
 }
 
+// ---------------------------------------------------------------------------
+// FP32, SU(3), first Young row 2..6 and second row <= 1: two links per wave,
+// Sym^k(U) never stored. Lane t of a half-wave builds column t of C_k in
+// registers, level by level: column n of level k extends column n - e_j of
+// level k - 1 (as in sym_level), which the lane takes from the lane holding it
+// (ds_bpermute). The adjoint hop is then a column dot, (C1^T V1)[t]. The
+// forward hop needs rows of C1; since
+//   C_k(U)^T = F_k^2 C_k(U^T) F_k^-2,  F_k = diag sqrt(m!),
+// row t of C1 is column t of C_k(U^T) up to those factors, so a second build
+// from U^T gives the forward hop as the same column dot, with V0 lifted by F
+// instead of F^-1 and projected with the adjoint's rows (Yc F^-1 / c). Only
+// the two lifted vectors pass through LDS, and the results over them.
+
+// Column t of C_K(A) (t >= dim Sym^K repeats the last column); lane0: the
+// first lane of this half-wave.
+template <int K>
+GHX_DEV void sym_column(const Mat<float, 3>& A, int t, int lane0, cx<float> (&c)[symc::sdim(K)]) {
+  constexpr int SK = symc::sdim(K), SP = symc::sdim(K - 1);
+  const int n = t < SK ? t : SK - 1;
+  const symc::Mono x = symc::mono(K, n);
+  const int j = x.a[0] > 0 ? 0 : (x.a[1] > 0 ? 1 : 2);
+  const int np = j == 0 ? symc::midx(K - 1, x.a[0] - 1, x.a[1]) : (j == 1 ? symc::midx(K - 1, 0, x.a[1] - 1) : symc::midx(K - 1, 0, 0));
+  cx<float> col[SP];
+  if constexpr (K == 2) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) col[i] = sel3(A.e[i], np);
+  } else {
+    cx<float> p[SP];
+    sym_column<K - 1>(A, t, lane0, p);
+    const int src = (lane0 + np) * 4;
+    static_for<0, SP>([&](auto m) { col[int(m)] = {lane_take(p[int(m)].r, src), lane_take(p[int(m)].i, src)}; });
+  }
+  cx<float> u[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) u[i] = sel3(A.e[i], j);
+  static_for<0, SK>([&](auto mc) {
+    constexpr int m = mc;
+    constexpr int r0 = symc::rowsrc(K, m, 0), r1 = symc::rowsrc(K, m, 1), r2 = symc::rowsrc(K, m, 2);
+    cx<float> acc = {0.f, 0.f};
+    if constexpr (r0 >= 0) cfma(acc, u[0], col[r0]);
+    if constexpr (r1 >= 0) cfma(acc, u[1], col[r1]);
+    if constexpr (r2 >= 0) cfma(acc, u[2], col[r2]);
+    c[m] = acc;
+  });
+}
+
+// W[t][k] for this lane's column c of C1 and the lifted vector V (s1 x s2 in
+// LDS, read as broadcasts), C2 = U applied on the right as C2^T (fwd) or on
+// the left's adjoint form (adjoint), as in sym_apply_pair_rows.
+template <int S1N, int S2N, bool ADJ>
+GHX_DEV void sym_column_apply(const cx<float> (&c)[S1N], const cx<float>* V, const Mat<float, 3>& U, cx<float> (&w)[S2N]) {
+  cx<float> acc[S2N];
+#pragma unroll
+  for (int b = 0; b < S2N; ++b) acc[b] = {0.f, 0.f};
+  static_for<0, S1N>([&](auto ac) {
+    constexpr int a = ac;
+#pragma unroll
+    for (int b = 0; b < S2N; ++b) cfma(acc[b], c[a], V[a * S2N + b]);
+  });
+  if constexpr (S2N == 1) {
+    w[0] = acc[0];
+  } else {
+#pragma unroll
+    for (int k = 0; k < S2N; ++k) {
+      cx<float> z = {0.f, 0.f};
+#pragma unroll
+      for (int k2 = 0; k2 < S2N; ++k2) cfma(z, ADJ ? U.e[k2][k] : U.e[k][k2], acc[k2]);
+      w[k] = z;
+    }
+  }
+}
+
+template <int D, int K1, int K2>
+__global__ __launch_bounds__(kSymWave) void sym_link_col_kernel(BigArgs<float> a, BigRepTables tb, float* out, int both) {
+  using R = float;
+  constexpr int NL = kSymPairLanes, S1N = symc::sdim(K1), S2N = K2 == 0 ? 1 : 3, DS = S1N * S2N;
+  constexpr int kPre = (2 * DS + NL - 1) / NL;
+  static_assert(K2 <= 1 && S1N <= NL, "columns of C_k1 must fit a half-wave");
+  const BigRepTables::Sym& st = tb.sym;
+  __shared__ cx<R> vs[2][2 * DS];  // per half-wave: V0, V1 (then the two hops before projection)
+  const int lane = threadIdx.x, h = lane / NL, t = lane % NL;
+  const unsigned g = blockIdx.x, Vp = a.Vp;
+  const int r = a.blk_rep[g / kBlock];
+  const RepDesc& rd = a.reps[r];
+  const unsigned s = g - rd.offset;
+  if (s >= rd.vol) return;  // whole block exits together
+  const int mu = 2 * int(blockIdx.y) + h;
+  const bool act = mu < D;
+  const int ml = act ? mu : 0;  // an idle half (odd D) repeats link 0 and stores nothing
+  const Nbr<D> nb(rd, s);
+  const unsigned gp = g + nb.upv(rd, ml);
+  const Mat<R, 3> U = load_link<float, R, 3>(a.U, Vp, ml, g);
+  const int d = tb.d, nd = both + 1;
+  // V0 = F Yc phi(x+mu) for the forward hop of x, V1 = conj(F Yc phi(x)) for the
+  // adjoint hop of x+mu
+  const float* lift = static_cast<const float*>(st.lift);
+  const R* pre = st.pref + DS;  // F
+  cx<R>* V = vs[h];
+#pragma unroll
+  for (int c = 0; c < kPre; ++c) {
+    const int e = t + c * NL;
+    if (e < nd * DS) {
+      const int dir = e >= DS, q = e - dir * DS;
+      const cx<R> v = cscale(ld_cx(lift + size_t(dir ? g : gp) * 2 * DS + 2 * q), pre[q]);
+      V[e] = dir ? cconj(v) : v;
+    }
+  }
+  wave_sync();
+  // each half's reads of V precede its writes of the results in the wave's
+  // instruction stream, so the results overwrite V in place
+  if (both) {
+    cx<R> c[S1N], w[S2N];
+    sym_column<K1>(U, t, h * NL, c);
+    sym_column_apply<S1N, S2N, true>(c, V + DS, U, w);
+    if (t < S1N) {
+#pragma unroll
+      for (int k = 0; k < S2N; ++k) V[DS + t * S2N + k] = w[k];
+    }
+  }
+  {
+    Mat<R, 3> UT;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) UT.e[i][j] = U.e[j][i];
+    cx<R> c[S1N], w[S2N];
+    sym_column<K1>(UT, t, h * NL, c);
+    sym_column_apply<S1N, S2N, false>(c, V, U, w);
+    if (t < S1N) {
+#pragma unroll
+      for (int k = 0; k < S2N; ++k) V[t * S2N + k] = w[k];
+    }
+  }
+  wave_sync();
+  if (!act) return;
+  sym_project<float, R>(st, 1, d, V, out + 2 * ((size_t(g) * D + mu) * d), t, NL);
+  if (both) sym_project<float, R>(st, 1, d, V + DS, static_cast<float*>(st.hopsL) + 2 * ((size_t(gp) * D + mu) * d), t, NL);
+}
+
 template <typename R>
 GHX_DEV R wave_sum(R v) {
 #pragma unroll

@@ -224,11 +224,13 @@ class Lattices:
         self._check(self._lib.ghx_get_tuning(self.h, 3, C.byref(v)))
         return v.value
 
-    def set_sym_pairs(self, on):
+    def set_sym_pairs(self, mode):
         """Symmetric-power engine, SU(3) with first Young row <= 6: two links per
-        wave with Sym^k(U) built a column per lane (1), or one link per wave
-        with the per-entry LDS program (0)."""
-        self._check(self._lib.ghx_set_tuning(self.h, 4, int(on)))
+        wave with the columns of Sym^k(U) in registers (2, default; FP32 with a
+        second row <= 1, else as 1), two links per wave with Sym^k(U) built a
+        column per lane into LDS (1), or one link per wave with the per-entry
+        LDS program (0). sym_pairs() reports the kernel in use."""
+        self._check(self._lib.ghx_set_tuning(self.h, 4, int(mode)))
 
     def sym_pairs(self):
         v = C.c_int()

@@ -187,10 +187,13 @@ int ghx_swap_stats(ghx_ctx* ctx, int* npairs, int* pairs, uint64_t* attempts, ui
  * = 1 (default where it applies) applies a large tensor irrep through
  * symmetric powers of the link, D(U) = Yc^T (Sym^k1(U) x Sym^k2(U)) Yc / c, instead of
  * slot by slot on the N^n tensor (get_tuning reports whether it is in use);
- * GHX_TUNE_SYM_PAIRS = 1 (default where it applies: SU(3), first Young row
- * <= 6) runs the engine two links per wave with Sym^k(U) built a column per
- * lane, 0 one link per wave with the per-entry LDS program (get_tuning reports
- * whether the paired kernel is in use); GHX_TUNE_X_LANES = 1 (default where it
+ * GHX_TUNE_SYM_PAIRS selects the engine's link kernel where SU(3) has a first
+ * Young row <= 6: 2 (default) two links per wave with each lane's column of
+ * Sym^k(U) in registers, the forward hop through Sym^k(U^T) (FP32 with a
+ * second row <= 1; elsewhere it runs as 1), 1 two links per wave with
+ * Sym^k(U) built a column per lane into LDS, 0 one link per wave with the
+ * per-entry LDS program (get_tuning reports the kernel in use, 0 where neither
+ * paired kernel applies); GHX_TUNE_X_LANES = 1 (default where it
  * applies: SU(3) in FP32 MD, L0 a power of two dividing 64, volumes and
  * cross-sections multiples of 64 sites) lets the one-site-per-thread and
  * streaming MD steps hand each site's lower staple in plane (0, mu) to its

@@ -88,7 +88,8 @@ def test_reversibility():
 
 
 # the last case sits at the engine's limits: d = 63 (of 64) and dim Sym^7 = 36
-ENGINE_CASES = [("6,1", 3, 4, 3), ("12", 2, 3, 4), ("2,1", 3, 3, 4), ("4,2", 3, 4, 3), ("4,4", 3, 3, 3), ("7,1", 3, 3, 3)]
+ENGINE_CASES = [("6,1", 3, 4, 3), ("12", 2, 3, 4), ("2,1", 3, 3, 4), ("4,2", 3, 4, 3), ("4,4", 3, 3, 3), ("7,1", 3, 3, 3),
+                ("6", 3, 4, 3)]
 
 
 @pytest.mark.parametrize("rep,N,D,L", ENGINE_CASES, ids=[c[0] for c in ENGINE_CASES])
@@ -117,26 +118,38 @@ def test_sym_engine_equals_tensor_apply(rep, N, D, L, prec, tol):
 
 @pytest.mark.parametrize("rep,N,D,L", ENGINE_CASES, ids=[c[0] for c in ENGINE_CASES])
 @pytest.mark.parametrize("prec,tol", [(gha.FP64, 1e-12), (gha.FP32MD, 1e-5)], ids=["fp64", "fp32"])
-def test_paired_link_kernel_equals_program_kernel(rep, N, D, L, prec, tol):
-    """The paired engine kernel (two links per wave, Sym^k(U) built a column per
-    lane from the previous level's column) against the one-link-per-wave LDS
-    program on the same fields: link and scalar forces and the action agree to
-    rounding. It runs exactly where it applies: SU(3) with first Young row <= 6
-    (so not SU(2) spin 6 nor {7,1}); with D = 3 one half of the last wave idles."""
-    k1 = int(rep.split(",")[0])
+def test_paired_link_kernels_equal_program_kernel(rep, N, D, L, prec, tol):
+    """The paired engine kernels (two links per wave) against the
+    one-link-per-wave LDS program on the same fields: link and scalar forces,
+    the action and an MD trajectory agree to rounding. Mode 1 builds Sym^k(U) a
+    column per lane into LDS; mode 2 keeps each lane's column in registers,
+    taking the previous level's column from another lane, and forms the forward
+    hop from the columns of Sym^k(U^T). Each runs exactly where it applies:
+    SU(3) with first Young row <= 6 (so not SU(2) spin 6 nor {7,1}), mode 2 in
+    FP32 with a second row <= 1 (mode 1 otherwise); with D = 3 one half of the
+    last wave idles."""
+    rows = [int(x) for x in rep.split(",")]
+    k1, k2 = rows[0], rows[1] if len(rows) > 1 else 0
+    paired = N == 3 and k1 <= 6
     out = []
-    for pairs in (0, 1):
+    for mode in (0, 1, 2):
         g = gha.GaugeHiggsHMC(L, rep, seed=3, D=D, N=N, precision=prec)
-        g.configure(beta=5.7 if N == 3 else 2.3, kappa=0.3, lam=0.5, nmd=4)
-        g.lat.set_sym_pairs(pairs)
+        g.configure(beta=5.7 if N == 3 else 2.3, kappa=0.3, lam=0.5, nmd=4, reunit_each_traj=False)
+        g.lat.set_sym_pairs(mode)
         assert g.lat.sym_engine() == 1
-        assert g.lat.sym_pairs() == int(pairs == 1 and N == 3 and k1 <= 6)
+        want = 0 if not paired or mode == 0 else (2 if mode == 2 and prec == gha.FP32MD and k2 <= 1 else 1)
+        assert g.lat.sym_pairs() == want
         g.hot(0.8, 12345, 0.4)
         F, Fp = g.forces()
-        out.append((F, Fp, g.observables()["SH"]))
-    a, b = out
-    assert relmax(b[0], a[0]) < tol and relmax(b[1], a[1]) < tol
-    assert abs(b[2] - a[2]) < tol * max(1.0, abs(a[2]))
+        sh = g.observables()["SH"]
+        g.refresh_momenta()
+        g.md_evolve()
+        out.append((F, Fp, sh, g.get("U"), g.get("phi")))
+    a = out[0]
+    for b in out[1:]:
+        assert relmax(b[0], a[0]) < tol and relmax(b[1], a[1]) < tol
+        assert abs(b[2] - a[2]) < tol * max(1.0, abs(a[2]))
+        assert relmax(b[3], a[3]) < 1e2 * tol and relmax(b[4], a[4]) < 1e2 * tol
 
 
 def test_irrep_beyond_one_component_per_lane_is_refused():
