@@ -351,28 +351,100 @@ struct OnsiteArgs {
   int flags;
   const T* Fon;
 };
-// One workgroup's part: the 64 sites of block blk.
+constexpr int kOnsiteK = (64 + kOnsiteGroups - 1) / kOnsiteGroups;  // components per thread (d <= 64)
+
+// A thread's components of its site: l = grp + k kOnsiteGroups, k < kOnsiteK.
 template <typename T>
-GHX_DEV void onsite_step_block(const OnsiteArgs<T>& o, unsigned blk) {
+struct OnsiteRegs {
+  T xr[kOnsiteK], xi[kOnsiteK], pr[kOnsiteK], pm[kOnsiteK];
+};
+
+// One on-site sub-step on a thread's registers: the kick of pi with
+// -2 dV/dphi^* (or the plugged-in potential's planes Fon), the phi drift, and
+// for a frozen scalar the tangent projection and great-circle drift. Products
+// are contracted within expressions only, so every caller rounds alike.
+template <typename T, class SUM>
+GHX_DEV void onsite_substep(OnsiteRegs<T>& v, int d, int grp, bool live, const RepDesc& rd, double ck, double cd, int flags,
+                            const T* Fon, unsigned Vp, unsigned g, SUM&& site_sum) {
+#pragma clang fp contract(on)
   using R = T;
   constexpr int G = kOnsiteGroups;
-  const RepDesc* reps = o.reps;
-  const int* blk_rep = o.blk_rep;
-  const unsigned Vp = o.Vp;
-  const T* phi_in = o.phi_in;
-  T* phi_out = o.phi_out;
-  T* pi = o.pi;
-  const int d = o.d;
-  const double ck = o.ck, cd = o.cd;
-  const int flags = o.flags;
-  const T* Fon = o.Fon;
+  R n2 = R(0);
+#pragma unroll
+  for (int k = 0; k < kOnsiteK; ++k)
+    if (live && grp + k * G < d) n2 += v.xr[k] * v.xr[k] + v.xi[k] * v.xi[k];
+  n2 = site_sum(n2);
+  const R c = (flags & F_NO_ONSITE) ? R(0) : R(-2) * (R(1) + R(2.0 * rd.lambda) * (n2 - R(1)));
+  const bool abs_eps = flags & F_ABS_EPS;
+  const R ek = R(abs_eps ? ck : ck * rd.eps), ed = R(abs_eps ? cd : cd * rd.eps);
+  const bool frozen = flags & F_FROZEN;
+  R rp = R(0);
+#pragma unroll
+  for (int k = 0; k < kOnsiteK; ++k) {
+    const int l = grp + k * G;
+    if (!live || l >= d) continue;
+    R fr = fma(c, v.xr[k], R(0)), fi = fma(c, v.xi[k], R(0));
+    if (Fon) {
+      const bool on = !(flags & F_NO_ONSITE);
+      fr = on ? R(Fon[size_t(2 * l) * Vp + g]) : R(0);
+      fi = on ? R(Fon[size_t(2 * l + 1) * Vp + g]) : R(0);
+    }
+    if (flags & F_KICK) {
+      v.pr[k] = fma(ek, fr, v.pr[k]);
+      v.pm[k] = fma(ek, fi, v.pm[k]);
+    }
+    if (frozen) {
+      rp += v.xr[k] * v.pr[k] + v.xi[k] * v.pm[k];
+      continue;
+    }
+    if (flags & F_DRIFT) {
+      v.xr[k] = fma(ed, v.pr[k], v.xr[k]);
+      v.xi[k] = fma(ed, v.pm[k], v.xi[k]);
+    }
+  }
+  if (!frozen) return;  // uniform over the block
+  // |phi| = 1: tangent projection of the kicked pi, then the great-circle drift
+  // (gauge_higgs_hmc.hpp:61-103)
+  rp = site_sum(rp);
+  R r2 = R(0);
+#pragma unroll
+  for (int k = 0; k < kOnsiteK; ++k) {
+    if (!live || grp + k * G >= d) continue;
+    if (flags & F_KICK) {
+      v.pr[k] -= rp * v.xr[k];
+      v.pm[k] -= rp * v.xi[k];
+    }
+    r2 += v.pr[k] * v.pr[k] + v.pm[k] * v.pm[k];
+  }
+  r2 = site_sum(r2);
+  if (!(flags & F_DRIFT) || !live) return;
+  const R rr = sqrt(r2);
+  const R rmin = sizeof(R) == 8 ? R(1e-300) : R(1e-30);
+  const bool move = rr >= rmin;
+  const R cs = move ? cos(rr * ed) : R(1), sn = move ? sin(rr * ed) : R(0), q = move ? sn / rr : R(0), w = -rr * sn;
+#pragma unroll
+  for (int k = 0; k < kOnsiteK; ++k) {
+    if (grp + k * G >= d) continue;
+    const R fr = v.xr[k], fi = v.xi[k], pr = v.pr[k], pm = v.pm[k];
+    v.xr[k] = move ? cs * fr + q * pr : fr;
+    v.xi[k] = move ? cs * fi + q * pm : fi;
+    v.pr[k] = move ? w * fr + cs * pr : pr;
+    v.pm[k] = move ? w * fi + cs * pm : pm;
+  }
+}
+
+// The 64 sites of block blk: the thread's components in, sub-steps, out.
+template <typename T, class BODY>
+GHX_DEV void onsite_block(const OnsiteArgs<T>& o, unsigned blk, BODY&& body) {
+  using R = T;
+  constexpr int G = kOnsiteGroups;
   __shared__ R red[G][64];
   const int lane = int(threadIdx.x) % 64, grp = int(threadIdx.x) / 64;
-  const unsigned g = blk * 64 + unsigned(lane);
-  const RepDesc& rd = reps[blk_rep[blk * 64 / kBlock]];
+  const unsigned g = blk * 64 + unsigned(lane), Vp = o.Vp;
+  const RepDesc& rd = o.reps[o.blk_rep[blk * 64 / kBlock]];
   const bool live = g - rd.offset < rd.vol;  // the block stays whole for its barriers
-  auto site_sum = [&](R v) {
-    red[grp][lane] = v;
+  auto site_sum = [&](R x) {
+    red[grp][lane] = x;
     __syncthreads();
     R t = red[0][lane];
 #pragma unroll
@@ -380,71 +452,59 @@ GHX_DEV void onsite_step_block(const OnsiteArgs<T>& o, unsigned blk) {
     __syncthreads();
     return t;
   };
-  R n2 = R(0);
-  if (live)
-    for (int l = grp; l < d; l += G) {
-      const R xr = R(phi_in[size_t(2 * l) * Vp + g]), xi = R(phi_in[size_t(2 * l + 1) * Vp + g]);
-      n2 += xr * xr + xi * xi;
-    }
-  n2 = site_sum(n2);
-  const R c = (flags & F_NO_ONSITE) ? R(0) : R(-2) * (R(1) + R(2.0 * rd.lambda) * (n2 - R(1)));
-  const bool abs_eps = flags & F_ABS_EPS;
-  const R ek = R(abs_eps ? ck : ck * rd.eps), ed = R(abs_eps ? cd : cd * rd.eps);
-  const bool frozen = flags & F_FROZEN;
-  R rp = R(0);
-  if (live)
-    for (int l = grp; l < d; l += G) {
-      const R xr = R(phi_in[size_t(2 * l) * Vp + g]), xi = R(phi_in[size_t(2 * l + 1) * Vp + g]);
-      R fr = fma(c, xr, R(0)), fi = fma(c, xi, R(0));
-      if (Fon) {
-        const bool on = !(flags & F_NO_ONSITE);
-        fr = on ? R(Fon[size_t(2 * l) * Vp + g]) : R(0);
-        fi = on ? R(Fon[size_t(2 * l + 1) * Vp + g]) : R(0);
-      }
-      R pr = R(pi[size_t(2 * l) * Vp + g]), pim = R(pi[size_t(2 * l + 1) * Vp + g]);
-      if (flags & F_KICK) {
-        pr = fma(ek, fr, pr);
-        pim = fma(ek, fi, pim);
-        pi[size_t(2 * l) * Vp + g] = T(pr);
-        pi[size_t(2 * l + 1) * Vp + g] = T(pim);
-      }
-      if (frozen) {
-        rp += xr * pr + xi * pim;
-        continue;
-      }
-      if (flags & F_DRIFT) {
-        phi_out[size_t(2 * l) * Vp + g] = T(fma(ed, pr, xr));
-        phi_out[size_t(2 * l + 1) * Vp + g] = T(fma(ed, pim, xi));
-      }
-    }
-  if (!frozen) return;  // uniform over the block
-  // |phi| = 1: tangent projection of the kicked pi, then the great-circle drift
-  // (gauge_higgs_hmc.hpp:61-103), over the site's planes in pi (in place)
-  rp = site_sum(rp);
-  R r2 = R(0);
-  if (live)
-    for (int l = grp; l < d; l += G) {
-      R pr = R(pi[size_t(2 * l) * Vp + g]), pim = R(pi[size_t(2 * l + 1) * Vp + g]);
-      if (flags & F_KICK) {
-        pr -= rp * R(phi_in[size_t(2 * l) * Vp + g]);
-        pim -= rp * R(phi_in[size_t(2 * l + 1) * Vp + g]);
-        pi[size_t(2 * l) * Vp + g] = T(pr);
-        pi[size_t(2 * l + 1) * Vp + g] = T(pim);
-      }
-      r2 += pr * pr + pim * pim;
-    }
-  r2 = site_sum(r2);
-  if (!(flags & F_DRIFT) || !live) return;
-  const R rr = sqrt(r2);
-  const R rmin = sizeof(R) == 8 ? R(1e-300) : R(1e-30);
-  const bool move = rr >= rmin;
-  const R cs = move ? cos(rr * ed) : R(1), sn = move ? sin(rr * ed) : R(0), q = move ? sn / rr : R(0), w = -rr * sn;
-  for (int l = grp; l < d; l += G)
-    for (int k = 0; k < 2; ++k) {
-      const R fv = R(phi_in[size_t(2 * l + k) * Vp + g]), pv = R(pi[size_t(2 * l + k) * Vp + g]);
-      phi_out[size_t(2 * l + k) * Vp + g] = T(move ? cs * fv + q * pv : fv);
-      pi[size_t(2 * l + k) * Vp + g] = T(move ? w * fv + cs * pv : pv);
-    }
+  OnsiteRegs<T> v;
+#pragma unroll
+  for (int k = 0; k < kOnsiteK; ++k) {
+    const int l = grp + k * G;
+    const bool on = live && l < o.d;
+    v.xr[k] = on ? R(o.phi_in[size_t(2 * l) * Vp + g]) : R(0);
+    v.xi[k] = on ? R(o.phi_in[size_t(2 * l + 1) * Vp + g]) : R(0);
+    v.pr[k] = on ? R(o.pi[size_t(2 * l) * Vp + g]) : R(0);
+    v.pm[k] = on ? R(o.pi[size_t(2 * l + 1) * Vp + g]) : R(0);
+  }
+  const bool put = body(v, grp, live, rd, g, site_sum);
+  if (!live || !put) return;
+#pragma unroll
+  for (int k = 0; k < kOnsiteK; ++k) {
+    const int l = grp + k * G;
+    if (l >= o.d) continue;
+    o.pi[size_t(2 * l) * Vp + g] = T(v.pr[k]);
+    o.pi[size_t(2 * l + 1) * Vp + g] = T(v.pm[k]);
+    o.phi_out[size_t(2 * l) * Vp + g] = T(v.xr[k]);
+    o.phi_out[size_t(2 * l + 1) * Vp + g] = T(v.xi[k]);
+  }
+}
+
+// One workgroup's part of one sub-step: the 64 sites of block blk. pi is
+// written back unchanged where the sub-step does not kick, and phi_out gets
+// phi where it does not drift (then the caller keeps phi_in as the scalar).
+template <typename T>
+GHX_DEV void onsite_step_block(const OnsiteArgs<T>& o, unsigned blk) {
+  onsite_block<T>(o, blk, [&](OnsiteRegs<T>& v, int grp, bool live, const RepDesc& rd, unsigned g, auto&& site_sum) {
+    onsite_substep<T>(v, o.d, grp, live, rd, o.ck, o.cd, o.flags, o.Fon, o.Vp, g, site_sum);
+    return true;
+  });
+}
+
+// All on-site sub-steps of one inner-scale block at once (OnsiteSeq: the
+// sub-steps' coefficients and flags), each thread's components held in
+// registers between them: the arithmetic of onsite_step_block sub-step by
+// sub-step, without the round trips through memory. The host uses it where
+// the scalar's inner-scale update does not depend on the links (no surrogate
+// force, no plugged-in potential), so that it can run apart from the gauge
+// sub-steps. phi_out receives the scalar after the last sub-step.
+constexpr int kOnsiteMaxSteps = 48;
+struct OnsiteSeq {
+  int n;
+  int flags[kOnsiteMaxSteps];
+  double ck[kOnsiteMaxSteps], cd[kOnsiteMaxSteps];
+};
+template <typename T>
+__global__ __launch_bounds__(64 * kOnsiteGroups) void onsite_seq_kernel(OnsiteArgs<T> o, OnsiteSeq q) {
+  onsite_block<T>(o, blockIdx.x, [&](OnsiteRegs<T>& v, int grp, bool live, const RepDesc& rd, unsigned g, auto&& site_sum) {
+    for (int s = 0; s < q.n; ++s) onsite_substep<T>(v, o.d, grp, live, rd, q.ck[s], q.cd[s], q.flags[s], nullptr, o.Vp, g, site_sum);
+    return true;
+  });
 }
 template <typename T>
 __global__ __launch_bounds__(64 * kOnsiteGroups) void onsite_step_kernel(OnsiteArgs<T> o) {

@@ -53,6 +53,7 @@ struct ghx_ctx {
   int link_lanes = 0;   // link-parallel MD step (one lane per link) for small lattices
   bool xlane_enabled = true;  // GHX_TUNE_X_LANES
   bool fuse_inner = true;     // GHX_TUNE_FUSE_INNER
+  bool split_inner = true;    // GHX_TUNE_SPLIT_INNER
   // multi-timescale integrator (0 = single scale)
   int n_outer = 0, m_inner = 1, inner_integ = GHX_LEAPFROG;
   // surrogate link force (learned); S planes are filled by surr_fill before a kick

@@ -406,7 +406,65 @@ void launch_step(ghx_ctx* c, double ck, double cd, bool drift) {
 // adjacent K_in(l h) merged (the inner_fast scheme of u1.hpp:300-317): 2m+1
 // fused launches against m+1 for leapfrog, with a far smaller error constant.
 constexpr int kInParts = F_NO_HOP, kOutParts = F_NO_GAUGE | F_NO_ONSITE;
+// The inner sub-steps of a large irrep, (kick, drift, drifts) each, as
+// launch_inner issues them.
+struct InnerStep {
+  double ck, cd;
+  bool drift;
+};
+std::vector<InnerStep> inner_steps(const ghx_ctx* c, double t) {
+  const int m = c->m_inner;
+  const double h = t / m;
+  std::vector<InnerStep> v;
+  if (c->inner_integ == GHX_OMELYAN) {
+    const double lam = kOmelyanLambda;
+    v.push_back({lam * h, 0.5 * h, true});
+    for (int k = 0; k < m; ++k) {
+      v.push_back({(1.0 - 2.0 * lam) * h, 0.5 * h, true});
+      if (k != m - 1) v.push_back({2.0 * lam * h, 0.5 * h, true});
+    }
+    v.push_back({lam * h, 0.0, false});
+  } else {
+    v.push_back({0.5 * h, h, true});
+    for (int k = 1; k < m; ++k) v.push_back({h, h, true});
+    v.push_back({0.5 * h, 0.0, false});
+  }
+  return v;
+}
+// Without a surrogate force or a plugged-in potential, the inner scale of a
+// large irrep moves (U, P) by the gauge force alone and (phi, pi) by the
+// on-site force alone: the two updates do not depend on each other. Then all
+// of a block's on-site sub-steps run in one launch (onsite_seq_kernel, the
+// site's components in registers throughout) and the links take one
+// gauge-only launch per sub-step; the fields are those of the interleaved
+// launches bit for bit.
+bool inner_split_applies(const ghx_ctx* c) {
+  return c->big && c->split_inner && !(c->use_surr && c->surr_fill) && !c->multi && c->rep.d <= 64 &&
+         2 * c->m_inner + 1 <= kOnsiteMaxSteps;
+}
+void launch_inner_split(ghx_ctx* c, double t) {
+  const std::vector<InnerStep> v = inner_steps(c, t);
+  LaunchCtx l = c->lc();
+  OnsiteSeq q{};
+  q.n = int(v.size());
+  int ndrift = 0;
+  for (int s = 0; s < q.n; ++s) {
+    q.ck[s] = v[s].ck;
+    q.cd[s] = v[s].cd;
+    q.flags[s] = F_KICK | (v[s].drift ? F_DRIFT : 0) | kInParts;
+    ndrift += v[s].drift ? 1 : 0;
+  }
+  const int in = c->cur, fin = c->cur ^ (ndrift & 1);
+  c->bops.onsite_seq(l, c->phi[in], c->phi[fin], c->pi, c->rep.d, &q);
+  for (const InnerStep& s : v) {
+    const int a = c->cur, b = c->cur ^ 1;
+    c->bops.gauge_step(l, c->U[a], c->U[b], c->P, c->phi[fin], c->phi[fin], c->pi, s.ck, s.cd,
+                       F_KICK | (s.drift ? F_DRIFT : 0) | kInParts);
+    if (s.drift) c->cur = b;
+  }
+}
 void launch_inner(ghx_ctx* c, double t) {  // t in units of eps
+  if (inner_split_applies(c)) { launch_inner_split(c, t); return; }
   const int m = c->m_inner;
   const double h = t / m;
   if (c->inner_integ == GHX_OMELYAN) {
@@ -1299,6 +1357,12 @@ int ghx_set_tuning(ghx_ctx* ctx, int key, int value) {
     drop_graph(ctx);
     return GHX_OK;
   }
+  if (key == GHX_TUNE_SPLIT_INNER) {
+    if (value < 0 || value > 1) return fail(ctx, GHX_E_INVALID, "split inner scale must be 0 or 1");
+    ctx->split_inner = value != 0;
+    drop_graph(ctx);
+    return GHX_OK;
+  }
   return fail(ctx, GHX_E_INVALID, "unknown tuning key");
 }
 
@@ -1313,6 +1377,7 @@ int ghx_get_tuning(const ghx_ctx* ctx, int key, int* value) {
     *value = ctx->fuse_inner && ctx->big && ctx->D == 4 && ctx->link_lanes && !(ctx->nrep == 1 && ctx->stream_seg > 0);
     return GHX_OK;
   }
+  if (key == GHX_TUNE_SPLIT_INNER) { *value = inner_split_applies(ctx) ? 1 : 0; return GHX_OK; }
   if (key == GHX_TUNE_X_LANES) {  // whether the MD step uses it (not with link lanes or the 4D-tile walks)
     *value = ctx->xlane_enabled && ctx->xlane_ok() && !ctx->link_lanes && ctx->block_walk().order < 3;
     return GHX_OK;

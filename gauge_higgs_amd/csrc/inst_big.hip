@@ -133,6 +133,19 @@ struct BigLaunch {
     onsite(c, phi_in, phi_out, pi, d, ck, cd, flags);
     Launch<T, N, D, RepExt>::md_step(c, U_in, U_out, P, phi_in, phi_out, pi, nullptr, nullptr, ck, cd, flags);
   }
+  static void onsite_seq(const LaunchCtx& c, const void* phi_in, void* phi_out, void* pi, int d, const void* seq) {
+    OnsiteSeq q = *static_cast<const OnsiteSeq*>(seq);
+    if (c.frozen)
+      for (int s = 0; s < q.n; ++s) q.flags[s] |= F_FROZEN;
+    const OnsiteArgs<T> o{c.reps, c.blk_rep, c.Vp, (const T*)phi_in, (T*)phi_out, (T*)pi, d, 0.0, 0.0, 0, nullptr};
+    hipLaunchKernelGGL(onsite_seq_kernel<T>, dim3(c.Vp / 64), dim3(64 * kOnsiteGroups), 0, c.stream, o, q);
+  }
+  // the gauge update of one inner sub-step (the external link force of RepExt
+  // has no scalar components, so the scalar buffers are not touched)
+  static void gauge_step(const LaunchCtx& c, const void* U_in, void* U_out, void* P, const void* phi_in, void* phi_out, void* pi,
+                         double ck, double cd, int flags) {
+    Launch<T, N, D, RepExt>::md_step(c, U_in, U_out, P, phi_in, phi_out, pi, nullptr, nullptr, ck, cd, flags);
+  }
 };
 template <typename T, int N, int D>
 size_t matter_static_lds() {
@@ -149,6 +162,8 @@ void fill(Ops* o, BigOps* b) {
   b->onsite = &BigLaunch<T, N, D>::onsite;
   b->onsite_gauge = &BigLaunch<T, N, D>::onsite_gauge;
   b->matter_static_lds = &matter_static_lds<T, N, D>;
+  b->onsite_seq = &BigLaunch<T, N, D>::onsite_seq;
+  b->gauge_step = &BigLaunch<T, N, D>::gauge_step;
 }
 bool ops_big(int prec, int N, int D, Ops* o, BigOps* b) {
   if (N == 3 && D == 4) { prec ? fill<float, 3, 4>(o, b) : fill<double, 3, 4>(o, b); return true; }

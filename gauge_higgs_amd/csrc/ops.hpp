@@ -66,6 +66,11 @@ struct BigOps {
                        double ck, double cd, int flags);
   // static LDS bytes of the tensor-apply matter kernel (its dynamic LDS holds the dimT-entry tensor)
   size_t (*matter_static_lds)();
+  // the inner scale split: every on-site sub-step of an inner block in one launch
+  // (seq: a const OnsiteSeq*), and the gauge update of one sub-step alone
+  void (*onsite_seq)(const LaunchCtx&, const void* phi_in, void* phi_out, void* pi, int d, const void* seq);
+  void (*gauge_step)(const LaunchCtx&, const void* U_in, void* U_out, void* P, const void* phi_in, void* phi_out, void* pi, double ck,
+                     double cd, int flags);
 };
 
 struct Ops {

@@ -258,6 +258,18 @@ class Lattices:
         self._check(self._lib.ghx_get_tuning(self.h, 6, C.byref(v)))
         return v.value
 
+    def set_split_inner(self, on):
+        """Large irreps under the multi-timescale integrator: every on-site
+        sub-step of an inner block in one launch and the links' gauge sub-steps
+        alone (1, default), where the two are independent (no surrogate force,
+        no plugged-in potential); 0 interleaves them."""
+        self._check(self._lib.ghx_set_tuning(self.h, 7, int(on)))
+
+    def split_inner(self):
+        v = C.c_int()
+        self._check(self._lib.ghx_get_tuning(self.h, 7, C.byref(v)))
+        return v.value
+
     def autotune(self, steps=10):
         """Pick the fastest MD-step launch shape on the live fields (fields restored)."""
         best, ms = C.c_int(), C.c_double()

@@ -204,10 +204,15 @@ int ghx_swap_stats(ghx_ctx* ctx, int* npairs, int* pairs, uint64_t* attempts, ui
  * scalar's on-site update and the links' gauge update, which touch disjoint
  * fields) as one launch where the links run link-parallel in D = 4, with the
  * arithmetic of the two separate launches (get_tuning reports whether it
- * applies). The autotuner times every candidate for `steps` MD steps on the
+ * applies); GHX_TUNE_SPLIT_INNER = 1 (default) runs a large irrep's whole
+ * inner-scale block (the 2 m_inner + 1 or m_inner + 1 sub-steps of one
+ * half step) as one launch for the scalar's on-site sub-steps and one
+ * gauge-only launch per sub-step, where the two updates are independent: no
+ * surrogate force in use, no plugged-in potential (same fields as the
+ * interleaved launches; get_tuning reports whether it applies). The autotuner times every candidate for `steps` MD steps on the
  * live fields, restores the fields, and keeps the fastest. */
 enum { GHX_TUNE_STREAM_SEG = 0, GHX_TUNE_BLOCK_ORDER = 1, GHX_TUNE_LINK_LANES = 2, GHX_TUNE_SYM_ENGINE = 3, GHX_TUNE_SYM_PAIRS = 4,
-       GHX_TUNE_X_LANES = 5, GHX_TUNE_FUSE_INNER = 6 };
+       GHX_TUNE_X_LANES = 5, GHX_TUNE_FUSE_INNER = 6, GHX_TUNE_SPLIT_INNER = 7 };
 int ghx_set_tuning(ghx_ctx* ctx, int key, int value);
 int ghx_get_tuning(const ghx_ctx* ctx, int key, int* value);
 int ghx_autotune(ghx_ctx* ctx, int steps, int* best_seg, double* best_ms_per_step);

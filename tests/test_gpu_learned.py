@@ -392,3 +392,57 @@ def test_fused_inner_step_reported_only_where_it_runs():
     h = gha.GaugeHiggsHMC(4, "2", seed=1, D=4, N=3, precision=gha.FP32MD)
     h.lat.set_link_lanes(1)
     assert h.lat.fuse_inner() == 0  # small irreps fuse the whole step already
+
+
+@pytest.mark.parametrize("prec", [gha.FP64, gha.FP32MD])
+@pytest.mark.parametrize("inner", [gha.LEAPFROG, gha.OMELYAN])
+@pytest.mark.parametrize("lanes", [0, 1])
+def test_split_inner_scale_equals_interleaved(prec, inner, lanes):
+    """A large irrep ({6,1}) under the multi-timescale integrator: all on-site
+    sub-steps of an inner block in one launch (the site's components in
+    registers) plus a gauge-only launch per sub-step give the fields of the
+    interleaved launches bit for bit, against both interleaved forms (one thread
+    per site: two launches per sub-step; link lanes: one fused launch); the
+    same holds for whole trajectories replayed from a captured graph."""
+    out = []
+    for split in (0, 1):
+        g = gha.GaugeHiggsHMC(4, "6,1", seed=1, D=4, N=3, precision=prec).configure(beta=5.7, kappa=0.05, lam=0.5, nmd=4,
+                                                                                    integ=gha.OMELYAN, reunit_each_traj=False)
+        g.hot(0.8, 12345, 0.3)
+        g.lat.set_link_lanes(lanes)
+        g.lat.set_split_inner(split)
+        g.lat.set_nested(3, 2, inner=inner)
+        assert g.lat.split_inner() == split
+        g.refresh_momenta()
+        g.md_evolve()
+        fields = {k: g.get(k) for k in ("U", "P", "phi", "pi")}
+        g.lat.trajectories(2, use_graph=True)
+        out.append((fields, g.lat.counters(), g.get("U"), g.get("phi")))
+    a, b = out
+    for k in a[0]:
+        assert np.array_equal(a[0][k], b[0][k]), k
+    assert a[1] == b[1]
+    assert np.array_equal(a[2], b[2]) and np.array_equal(a[3], b[3])
+
+
+def test_split_inner_scale_not_with_surrogate():
+    """With the surrogate force on, the inner links' kick depends on the scalar
+    at every sub-step, so the inner scale stays interleaved (reported 0) and
+    the switch changes nothing."""
+    out = []
+    for split in (0, 1):
+        g = gha.GaugeHiggsHMC(4, "6,1", seed=1, D=4, N=3, precision=gha.FP32MD).configure(beta=5.7, kappa=0.05, lam=0.5, nmd=4,
+                                                                                          reunit_each_traj=False)
+        g.hot(0.8, 12345, 0.3)
+        g.lat.set_split_inner(split)
+        g.lat.set_nested(2, 3, inner=gha.OMELYAN)
+        g.lat.surrogate_init(seed=2)
+        g.lat.surrogate_enable(True)
+        assert g.lat.split_inner() == 0
+        g.refresh_momenta()
+        g.md_evolve()
+        out.append({k: g.get(k) for k in ("U", "P", "phi", "pi")})
+    for k in out[0]:
+        assert np.array_equal(out[0][k], out[1][k]), k
+    h = gha.GaugeHiggsHMC(4, "2", seed=1, D=4, N=3, precision=gha.FP32MD)
+    assert h.lat.split_inner() == 0  # small irreps update the scalar inside the step kernel
