@@ -519,7 +519,7 @@ __global__ __launch_bounds__(64 * kOnsiteGroups) void onsite_step_kernel(OnsiteA
 template <typename T, int N, int D>
 __global__ __launch_bounds__(kLinkSites * D) void md_link_onsite_kernel(StepArgs<T> a, unsigned nlink, OnsiteArgs<T> o) {
   static_assert(kLinkSites * D == 64 * kOnsiteGroups, "both parts run workgroups of the same size");
-  if (blockIdx.x < nlink) md_link_block<T, N, D, RepExt<T, N>>(a, NoParams{}, xcd_block(blockIdx.x, nlink));
+  if (blockIdx.x < nlink) md_link_block<T, N, D, RepExt<T, N>>(with_store_policy<false>(a), NoParams{}, xcd_block(blockIdx.x, nlink));
   else onsite_step_block<T>(o, blockIdx.x - nlink);
 }
 

@@ -54,6 +54,7 @@ struct ghx_ctx {
   bool xlane_enabled = true;  // GHX_TUNE_X_LANES
   bool fuse_inner = true;     // GHX_TUNE_FUSE_INNER
   bool split_inner = true;    // GHX_TUNE_SPLIT_INNER
+  int store_cached = 0;       // GHX_TUNE_STORE_POLICY
   // multi-timescale integrator (0 = single scale)
   int n_outer = 0, m_inner = 1, inner_integ = GHX_LEAPFROG;
   // surrogate link force (learned); S planes are filled by surr_fill before a kick
@@ -157,6 +158,7 @@ struct ghx_ctx {
     c.link_lanes = link_lanes;
     c.xlane = (xlane_enabled && xlane_ok()) ? 1 : 0;
     c.fuse_inner = fuse_inner ? 1 : 0;
+    c.store_cached = store_cached;
     return c;
   }
 };

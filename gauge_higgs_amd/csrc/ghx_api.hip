@@ -1357,6 +1357,12 @@ int ghx_set_tuning(ghx_ctx* ctx, int key, int value) {
     drop_graph(ctx);
     return GHX_OK;
   }
+  if (key == GHX_TUNE_STORE_POLICY) {
+    if (value < 0 || value > 1) return fail(ctx, GHX_E_INVALID, "store policy must be 0 (streamed) or 1 (cached)");
+    ctx->store_cached = value;
+    drop_graph(ctx);
+    return GHX_OK;
+  }
   if (key == GHX_TUNE_SPLIT_INNER) {
     if (value < 0 || value > 1) return fail(ctx, GHX_E_INVALID, "split inner scale must be 0 or 1");
     ctx->split_inner = value != 0;
@@ -1378,6 +1384,7 @@ int ghx_get_tuning(const ghx_ctx* ctx, int key, int* value) {
     return GHX_OK;
   }
   if (key == GHX_TUNE_SPLIT_INNER) { *value = inner_split_applies(ctx) ? 1 : 0; return GHX_OK; }
+  if (key == GHX_TUNE_STORE_POLICY) { *value = ctx->store_cached; return GHX_OK; }
   if (key == GHX_TUNE_X_LANES) {  // whether the MD step uses it (not with link lanes or the 4D-tile walks)
     *value = ctx->xlane_enabled && ctx->xlane_ok() && !ctx->link_lanes && ctx->block_walk().order < 3;
     return GHX_OK;
@@ -1403,24 +1410,32 @@ int ghx_autotune(ghx_ctx* ctx, int steps, int* best_seg, double* best_ms) {
     HIPCHK(hipMemcpyAsync(ctx->phisave, ctx->phi[0], nphb, hipMemcpyDeviceToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(pis, ctx->pi, nphb, hipMemcpyDeviceToDevice, ctx->stream));
   }
-  // candidates (stream segment, block order or -1 for the link-parallel step);
-  // the walks only for one site per thread
-  std::vector<std::pair<int, int>> cand = {{0, 0}};
+  // candidates (stream segment, block order or -1 for the link-parallel step),
+  // each with streamed and with cached stores; the walks only for one site per
+  // thread
+  std::vector<std::pair<int, int>> shapes = {{0, 0}};
   for (int order : {1, 2}) {
     ctx->block_order = order;
-    if (ctx->block_walk().order == order) cand.push_back({0, order});
+    if (ctx->block_walk().order == order) shapes.push_back({0, order});
   }
-  cand.push_back({0, -1});
+  shapes.push_back({0, -1});
   if (ctx->nrep == 1)
     for (int sg : {1, 2, 4, 8, 16, 32, 64})
-      if (sg <= ctx->hreps[0].L[ctx->D - 1]) cand.push_back({sg, 0});
+      if (sg <= ctx->hreps[0].L[ctx->D - 1]) shapes.push_back({sg, 0});
+  struct Cand {
+    int first, second, store;
+  };
+  std::vector<Cand> cand;
+  for (int st : {0, 1})
+    for (const auto& sh : shapes) cand.push_back({sh.first, sh.second, st});
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  auto use = [&](const std::pair<int, int>& c) {
+  auto use = [&](const Cand& c) {
     ctx->stream_seg = c.first;
     ctx->block_order = c.second < 0 ? 0 : c.second;
     ctx->link_lanes = c.second < 0 ? 1 : 0;
+    ctx->store_cached = c.store;
   };
   auto time_it = [&](int n, double& out) -> int {  // GHX_OK or an error code
     launch_step(ctx, 1.0, 1.0, true);  // the shape's first launch stays out of the timing

@@ -55,7 +55,8 @@ struct RepDesc {
 // F_EXT_ONSITE: the on-site force -2 dV/dphi^* comes from the Fon planes (a
 // plugged-in potential, e.g. the multi-invariant one) instead of the quartic.
 enum StepFlags : int { F_KICK = 1, F_DRIFT = 2, F_FORCE = 4, F_ABS_EPS = 8, F_NO_GAUGE = 16, F_NO_HOP = 32, F_NO_ONSITE = 64,
-                       F_SURR = 128, F_FROZEN = 256, F_EXT_ONSITE = 512 };
+                       F_SURR = 128, F_FROZEN = 256, F_EXT_ONSITE = 512,
+                       F_STORE_CACHED = 1024 };  // the step's stores with the default cache policy instead of streamed
 
 template <typename T>
 struct StepArgs {
@@ -175,26 +176,34 @@ template <typename T, int POL = 0> GHX_DEV T bload(rsrc_t r, unsigned voff, unsi
   if constexpr (sizeof(T) == 4) return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, POL));
   else return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, POL));
 }
-// Stores of the MD step are streamed (non-temporal, cache policy bit NT = 2):
-// nothing written in a step is read again in that step, so they should not
-// evict the neighbour links that the staples re-read through L2.
+// Stores of the MD step are streamed by default (non-temporal, cache policy
+// bit NT = 2): nothing written in a step is read again in that step, so they
+// should not evict the neighbour links that the staples re-read through L2.
+// Where the next step's reads find the written fields in cache (small
+// lattices), the default policy can be faster: F_STORE_CACHED selects it. The
+// step kernels fix it per instance (with_store_policy), so the branch below
+// folds away; the autotuner tries both.
 constexpr int kStoreNT = 2;
-GHX_DEV void bstore(float v, rsrc_t r, unsigned voff, unsigned soff) {
-  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff, kStoreNT);
+GHX_DEV void bstore(float v, rsrc_t r, unsigned voff, unsigned soff, bool nt) {
+  if (nt) __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff, kStoreNT);
+  else __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff, 0);
 }
-GHX_DEV void bstore(double v, rsrc_t r, unsigned voff, unsigned soff) {
+GHX_DEV void bstore(double v, rsrc_t r, unsigned voff, unsigned soff, bool nt) {
   using v2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
-  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2, v), r, (int)voff, (int)soff, kStoreNT);
+  if (nt) __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2, v), r, (int)voff, (int)soff, kStoreNT);
+  else __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2, v), r, (int)voff, (int)soff, 0);
 }
 // Planes of one field: plane c of site g at byte offset (c*Vp + g)*sizeof(T).
 template <typename T>
 struct Planes {
   rsrc_t r;
   unsigned pstride;  // Vp * sizeof(T)
-  GHX_DEV Planes(const T* p, unsigned Vp, unsigned nplanes) : r(make_rsrc(p, nplanes * Vp * unsigned(sizeof(T)))), pstride(Vp * unsigned(sizeof(T))) {}
+  bool nt;           // streamed stores
+  GHX_DEV Planes(const T* p, unsigned Vp, unsigned nplanes, bool stream_stores = true)
+      : r(make_rsrc(p, nplanes * Vp * unsigned(sizeof(T)))), pstride(Vp * unsigned(sizeof(T))), nt(stream_stores) {}
   GHX_DEV T ld(unsigned g, unsigned c) const { return bload<T>(r, g * unsigned(sizeof(T)), c * pstride); }
   template <int POL> GHX_DEV T ldp(unsigned g, unsigned c) const { return bload<T, POL>(r, g * unsigned(sizeof(T)), c * pstride); }
-  GHX_DEV void st(unsigned g, unsigned c, T v) const { bstore(v, r, g * unsigned(sizeof(T)), c * pstride); }
+  GHX_DEV void st(unsigned g, unsigned c, T v) const { bstore(v, r, g * unsigned(sizeof(T)), c * pstride, nt); }
 };
 // FULL = false leaves the last row of a compressed link unset (for a left
 // factor whose product keeps only its first rows).
@@ -421,7 +430,8 @@ GHX_DEV void link_update(const StepArgs<T>& a, const typename REP::template Para
   const R bN = R(rd.beta / N);
   const R k2 = R(2.0 * rd.kappa);
   const R hbN = R(0.5) * bN, hk2 = R(0.5) * k2;  // coefficients of im_tr_gen2 (exact halves)
-  const Planes<T> Uin(a.U_in, Vp, D * NC), Uout(a.U_out, Vp, D * NC), Pp(a.P, Vp, D * NGN);
+  const bool snt = !(a.flags & F_STORE_CACHED);
+  const Planes<T> Uin(a.U_in, Vp, D * NC), Uout(a.U_out, Vp, D * NC, snt), Pp(a.P, Vp, D * NGN, snt);
   const bool do_gauge = !(a.flags & F_NO_GAUGE), do_hop = !(a.flags & F_NO_HOP);
   const Mat<R, N> Ux = bload_link<T, R, N>(Uin, mu, g);
   const unsigned gpm = g + nb.upv(rd, mu);
@@ -637,7 +647,8 @@ GHX_DEV void site_phi_end(const StepArgs<T>& a, const RepDesc& rd, unsigned g, c
   const R ek = R(abs_eps ? a.ck : a.ck * rd.eps);
   const R ed = R(abs_eps ? a.cd : a.cd * rd.eps);
   if (a.flags & F_FORCE) { bstore_vec<T, R, DR>(Planes<T>(a.Fphi_out, Vp, 2 * DR), g, fph); return; }
-  const Planes<T> Pi(a.pi, Vp, 2 * DR);
+  const bool snt = !(a.flags & F_STORE_CACHED);
+  const Planes<T> Pi(a.pi, Vp, 2 * DR, snt);
   cx<R> pv[DR];
 #pragma unroll
   for (int k = 0; k < DR; ++k) {
@@ -657,7 +668,7 @@ GHX_DEV void site_phi_end(const StepArgs<T>& a, const RepDesc& rd, unsigned g, c
 #pragma unroll
       for (int k = 0; k < DR; ++k) { phx[k].r = fma(ed, pv[k].r, phx[k].r); phx[k].i = fma(ed, pv[k].i, phx[k].i); }
     }
-    bstore_vec<T, R, DR>(Planes<T>(a.phi_out, Vp, 2 * DR), g, phx);
+    bstore_vec<T, R, DR>(Planes<T>(a.phi_out, Vp, 2 * DR, snt), g, phx);
   }
   if ((a.flags & F_KICK) || (frozen && (a.flags & F_DRIFT))) bstore_vec<T, R, DR>(Pi, g, pv);
 }
@@ -695,10 +706,19 @@ GHX_DEV void site_update(const StepArgs<T>& a, const typename REP::template Para
   if constexpr (DR > 0) site_phi_end<T, DR, XL>(a, rd, g, phx, fph);
 }
 
+// The store policy of a kernel instance (CACHED: F_STORE_CACHED) as a
+// constant, so that the stores' policy branch folds away.
+template <bool CACHED, typename T>
+GHX_DEV StepArgs<T> with_store_policy(StepArgs<T> a) {
+  a.flags = CACHED ? (a.flags | F_STORE_CACHED) : (a.flags & ~int(F_STORE_CACHED));
+  return a;
+}
+
 // One site per thread; blocks renumbered XCD-aware. XL: launched only in
 // storage-order or XCD-slice walks of lattices whose x-lines fill waves.
-template <typename T, int N, int D, class REP, bool XL = false>
-__global__ __launch_bounds__(kBlock, (md_waves<T, REP>())) void md_step_kernel(StepArgs<T> a, typename REP::template Params<T> rp) {
+template <typename T, int N, int D, class REP, bool XL = false, bool CACHED = false>
+__global__ __launch_bounds__(kBlock, (md_waves<T, REP>())) void md_step_kernel(StepArgs<T> a0, typename REP::template Params<T> rp) {
+  const StepArgs<T> a = with_store_policy<CACHED>(a0);
   if constexpr (!XL) {
     if (a.walk.order >= 3) {  // one lattice in 4D tiles (BlockWalk)
       const BlockWalk& w = a.walk;
@@ -724,9 +744,10 @@ __global__ __launch_bounds__(kBlock, (md_waves<T, REP>())) void md_step_kernel(S
 // by the same block one or two iterations earlier (cache-resident) instead of
 // being re-fetched from HBM. Block b handles cross-section tile (b % ntile),
 // segment (b / ntile); ntile % 8 == 0 keeps all segments of a tile on one XCD.
-template <typename T, int N, int D, class REP, bool XL = false>
-__global__ __launch_bounds__(kBlock, md_waves<T>()) void md_stream_kernel(StepArgs<T> a, typename REP::template Params<T> rp, unsigned ntile,
+template <typename T, int N, int D, class REP, bool XL = false, bool CACHED = false>
+__global__ __launch_bounds__(kBlock, md_waves<T>()) void md_stream_kernel(StepArgs<T> a0, typename REP::template Params<T> rp, unsigned ntile,
                                                                        int seg) {
+  const StepArgs<T> a = with_store_policy<CACHED>(a0);
   const RepDesc& rd = a.reps[0];
   const unsigned V3 = rd.stride[D - 1];
   const unsigned tile = blockIdx.x % ntile, sg = blockIdx.x / ntile;
@@ -785,9 +806,9 @@ GHX_DEV void md_link_block(const StepArgs<T>& a, const typename REP::template Pa
     site_phi_end<T, DR>(a, rd, gsite, phx, fph);
   }
 }
-template <typename T, int N, int D, class REP>
+template <typename T, int N, int D, class REP, bool CACHED = false>
 __global__ __launch_bounds__(kLinkSites * D) void md_link_kernel(StepArgs<T> a, typename REP::template Params<T> rp) {
-  md_link_block<T, N, D, REP>(a, rp, xcd_block(blockIdx.x, gridDim.x));
+  md_link_block<T, N, D, REP>(with_store_policy<CACHED>(a), rp, xcd_block(blockIdx.x, gridDim.x));
 }
 
 // Drift alone (U' = exp(i eps P) U, phi' = phi + eps pi) — for the kick/drift API.

@@ -50,18 +50,32 @@ struct Launch {
                              ParamFill<REP, T>::make(c), ntile, c.stream_seg);
           return;
         }
-      hipLaunchKernelGGL((md_stream_kernel<T, N, D, REP>), dim3(ntile * nseg), dim3(kBlock), 0, c.stream, a, ParamFill<REP, T>::make(c),
-                         ntile, c.stream_seg);
+      // the store policy (cached or streamed stores) is a kernel template
+      // constant; the x-lane kernels always stream
+      if (c.store_cached)
+        hipLaunchKernelGGL((md_stream_kernel<T, N, D, REP, false, true>), dim3(ntile * nseg), dim3(kBlock), 0, c.stream, a,
+                           ParamFill<REP, T>::make(c), ntile, c.stream_seg);
+      else
+        hipLaunchKernelGGL((md_stream_kernel<T, N, D, REP>), dim3(ntile * nseg), dim3(kBlock), 0, c.stream, a, ParamFill<REP, T>::make(c),
+                           ntile, c.stream_seg);
     } else if (c.link_lanes) {
-      hipLaunchKernelGGL((md_link_kernel<T, N, D, REP>), dim3(c.nblk * (kBlock / kLinkSites)), dim3(kLinkSites * D), 0, c.stream, a,
-                         ParamFill<REP, T>::make(c));
+      if (c.store_cached)
+        hipLaunchKernelGGL((md_link_kernel<T, N, D, REP, true>), dim3(c.nblk * (kBlock / kLinkSites)), dim3(kLinkSites * D), 0, c.stream,
+                           a, ParamFill<REP, T>::make(c));
+      else
+        hipLaunchKernelGGL((md_link_kernel<T, N, D, REP>), dim3(c.nblk * (kBlock / kLinkSites)), dim3(kLinkSites * D), 0, c.stream, a,
+                           ParamFill<REP, T>::make(c));
     } else {
       if constexpr (kXLable)
         if (xl) {
           hipLaunchKernelGGL((md_step_kernel<T, N, D, REP, true>), dim3(c.nblk), dim3(kBlock), 0, c.stream, a, ParamFill<REP, T>::make(c));
           return;
         }
-      hipLaunchKernelGGL((md_step_kernel<T, N, D, REP>), dim3(c.nblk), dim3(kBlock), 0, c.stream, a, ParamFill<REP, T>::make(c));
+      if (c.store_cached)
+        hipLaunchKernelGGL((md_step_kernel<T, N, D, REP, false, true>), dim3(c.nblk), dim3(kBlock), 0, c.stream, a,
+                           ParamFill<REP, T>::make(c));
+      else
+        hipLaunchKernelGGL((md_step_kernel<T, N, D, REP>), dim3(c.nblk), dim3(kBlock), 0, c.stream, a, ParamFill<REP, T>::make(c));
     }
   }
   static void drift(const LaunchCtx& c, const void* U_in, void* U_out, const void* P, const void* phi_in, void* phi_out,

@@ -258,6 +258,16 @@ class Lattices:
         self._check(self._lib.ghx_get_tuning(self.h, 6, C.byref(v)))
         return v.value
 
+    def set_store_policy(self, cached):
+        """MD step stores streamed past the caches (0, default) or with the
+        default cache policy (1); the autotuner tries both."""
+        self._check(self._lib.ghx_set_tuning(self.h, 8, int(cached)))
+
+    def store_policy(self):
+        v = C.c_int()
+        self._check(self._lib.ghx_get_tuning(self.h, 8, C.byref(v)))
+        return v.value
+
     def set_split_inner(self, on):
         """Large irreps under the multi-timescale integrator: every on-site
         sub-step of an inner block in one launch and the links' gauge sub-steps

@@ -453,3 +453,25 @@ def test_autotune_restores_fields():
     after = [g.get(k) for k in ("U", "P", "phi", "pi")]
     for a, b in zip(before, after):
         assert np.array_equal(a, b)
+    assert g.lat.store_policy() in (0, 1)
+
+
+@pytest.mark.parametrize("case", [("fund", 2, 6, 0), ("fund", 2, 6, 1), ("2", 3, 8, 0), ("6,1", 3, 4, 1)],
+                         ids=["su2-site", "su2-lanes", "su3-sextet-xl", "su3-61-lanes"])
+def test_store_policy_changes_no_value(case):
+    """The MD step's stores streamed past the caches or stored with the default
+    policy (GHX_TUNE_STORE_POLICY): the fields after MD agree bit for bit."""
+    rep, N, L, lanes = case
+    out = []
+    for pol in (0, 1):
+        g = gha.GaugeHiggsHMC(L, rep, seed=2, D=4, N=N, precision=gha.FP32MD).configure(beta=2.3 if N == 2 else 5.7, kappa=0.1,
+                                                                                       lam=0.5, nmd=4, reunit_each_traj=False)
+        g.hot(0.8, 12345, 0.5)
+        g.lat.set_link_lanes(lanes)
+        g.lat.set_store_policy(pol)
+        assert g.lat.store_policy() == pol
+        g.refresh_momenta()
+        g.md_evolve()
+        out.append([g.get(k) for k in ("U", "P", "phi", "pi")])
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
