@@ -549,9 +549,10 @@ __global__ __launch_bounds__(kSymWave) void sym_link_pair_kernel(BigArgs<T> a, B
 // the two lifted vectors pass through LDS, and the results over them.
 
 // Column t of C_K(A) (t >= dim Sym^K repeats the last column); lane0: the
-// first lane of this half-wave.
-template <int K>
-GHX_DEV void sym_column(const Mat<float, 3>& A, int t, int lane0, cx<float> (&c)[symc::sdim(K)]) {
+// first lane of this half-wave. KEEP: prev receives this lane's column of
+// level K - 1 as well.
+template <int K, bool KEEP = false>
+GHX_DEV void sym_column(const Mat<float, 3>& A, int t, int lane0, cx<float> (&c)[symc::sdim(K)], cx<float>* prev = nullptr) {
   constexpr int SK = symc::sdim(K), SP = symc::sdim(K - 1);
   const int n = t < SK ? t : SK - 1;
   const symc::Mono x = symc::mono(K, n);
@@ -561,11 +562,16 @@ GHX_DEV void sym_column(const Mat<float, 3>& A, int t, int lane0, cx<float> (&c)
   if constexpr (K == 2) {
 #pragma unroll
     for (int i = 0; i < 3; ++i) col[i] = sel3(A.e[i], np);
+    if constexpr (KEEP) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) prev[i] = sel3(A.e[i], t < 3 ? t : 2);
+    }
   } else {
     cx<float> p[SP];
     sym_column<K - 1>(A, t, lane0, p);
     const int src = (lane0 + np) * 4;
     static_for<0, SP>([&](auto m) { col[int(m)] = {lane_take(p[int(m)].r, src), lane_take(p[int(m)].i, src)}; });
+    if constexpr (KEEP) static_for<0, SP>([&](auto m) { prev[int(m)] = p[int(m)]; });
   }
   cx<float> u[3];
 #pragma unroll
@@ -579,6 +585,48 @@ GHX_DEV void sym_column(const Mat<float, 3>& A, int t, int lane0, cx<float> (&c)
     if constexpr (r2 >= 0) cfma(acc, u[2], col[r2]);
     c[m] = acc;
   });
+}
+
+// Column t of C_K(A) for K >= 4 as a polynomial product: column n of C_K
+// holds the coefficients of prod_j l_j^{n_j} (l_j = sum_i A_ij x_i), the
+// product of columns a (level KA = ceil(K/2)) and b = n - a (level KB =
+// floor(K/2)). The lane builds its own columns of levels KA and KB, takes
+// columns a and b from the lanes that hold them, and multiplies: dim Sym^KA x
+// dim Sym^KB complex MACs (100 at K = 6) instead of the 3 dim Sym^(K-1) of
+// the last levels' recursion (63 + 45 + 30 at K = 6), and fewer lane
+// transfers.
+template <int K>
+GHX_DEV void sym_column_prod(const Mat<float, 3>& A, int t, int lane0, cx<float> (&c)[symc::sdim(K)]) {
+  constexpr int KA = (K + 1) / 2, KB = K / 2, SA = symc::sdim(KA), SB = symc::sdim(KB), SK = symc::sdim(K);
+  cx<float> pa[SA], pb[SB];
+  if constexpr (KA == KB) {
+    sym_column<KA>(A, t, lane0, pa);
+    static_for<0, SB>([&](auto m) { pb[int(m)] = pa[int(m)]; });
+  } else {
+    sym_column<KA, true>(A, t, lane0, pa, pb);
+  }
+  const int n = t < SK ? t : SK - 1;
+  const symc::Mono x = symc::mono(K, n);
+  const int a0 = x.a[0] < KA ? x.a[0] : KA;
+  const int a1 = x.a[1] < KA - a0 ? x.a[1] : KA - a0;  // a2 = KA - a0 - a1 <= x.a[2]
+  const int sa = (lane0 + symc::midx(KA, a0, a1)) * 4, sb = (lane0 + symc::midx(KB, x.a[0] - a0, x.a[1] - a1)) * 4;
+  cx<float> ca[SA], cb[SB];
+  static_for<0, SA>([&](auto m) { ca[int(m)] = {lane_take(pa[int(m)].r, sa), lane_take(pa[int(m)].i, sa)}; });
+  static_for<0, SB>([&](auto m) { cb[int(m)] = {lane_take(pb[int(m)].r, sb), lane_take(pb[int(m)].i, sb)}; });
+  static_for<0, SK>([&](auto m) { c[int(m)] = {0.f, 0.f}; });
+  static_for<0, SA>([&](auto mac) {
+    constexpr symc::Mono ma = symc::mono(KA, int(mac));
+    static_for<0, SB>([&](auto mbc) {
+      constexpr symc::Mono mb = symc::mono(KB, int(mbc));
+      constexpr int m = symc::midx(K, ma.a[0] + mb.a[0], ma.a[1] + mb.a[1]);
+      cfma(c[m], ca[int(mac)], cb[int(mbc)]);
+    });
+  });
+}
+template <int K>
+GHX_DEV void sym_column_top(const Mat<float, 3>& A, int t, int lane0, cx<float> (&c)[symc::sdim(K)]) {
+  if constexpr (K >= 4) sym_column_prod<K>(A, t, lane0, c);
+  else sym_column<K>(A, t, lane0, c);
 }
 
 // W[t][k] for this lane's column c of C1 and the lifted vector V (s1 x s2 in
@@ -649,7 +697,7 @@ __global__ __launch_bounds__(kSymWave, 5) void sym_link_col_kernel(BigArgs<float
   // instruction stream, so the results overwrite V in place
   if (both) {
     cx<R> c[S1N], w[S2N];
-    sym_column<K1>(U, t, h * NL, c);
+    sym_column_top<K1>(U, t, h * NL, c);
     sym_column_apply<S1N, S2N, true>(c, V + DS, U, w);
     if (t < S1N) {
 #pragma unroll
@@ -663,7 +711,7 @@ __global__ __launch_bounds__(kSymWave, 5) void sym_link_col_kernel(BigArgs<float
 #pragma unroll
       for (int j = 0; j < 3; ++j) UT.e[i][j] = U.e[j][i];
     cx<R> c[S1N], w[S2N];
-    sym_column<K1>(UT, t, h * NL, c);
+    sym_column_top<K1>(UT, t, h * NL, c);
     sym_column_apply<S1N, S2N, false>(c, V, U, w);
     if (t < S1N) {
 #pragma unroll
