@@ -655,10 +655,11 @@ GHX_DEV void sym_column_apply(const cx<float> (&c)[S1N], const cx<float>* V, con
   }
 }
 
-// Five waves per SIMD (96 VGPRs, a few spilled) run the {6,1} step faster than
-// four without spills (103 VGPRs) or six with more spills (80).
+// Six waves per SIMD (80 VGPRs, 2 spilled) run the {6,1} step fastest: four
+// (101 VGPRs, none spilled) 0.2486 ms, five 0.2376, six 0.2303, seven (12
+// spilled) 0.2338.
 template <int D, int K1, int K2>
-__global__ __launch_bounds__(kSymWave, 5) void sym_link_col_kernel(BigArgs<float> a, BigRepTables tb, float* out, int both) {
+__global__ __launch_bounds__(kSymWave, 6) void sym_link_col_kernel(BigArgs<float> a, BigRepTables tb, float* out, int both) {
   using R = float;
   constexpr int NL = kSymPairLanes, S1N = symc::sdim(K1), S2N = K2 == 0 ? 1 : 3, DS = S1N * S2N;
   constexpr int kPre = (2 * DS + NL - 1) / NL;
