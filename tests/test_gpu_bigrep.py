@@ -88,8 +88,10 @@ def test_reversibility():
 
 
 # the last case sits at the engine's limits: d = 63 (of 64) and dim Sym^7 = 36
+# {6}, {5,1} and {4} take the column kernel's product build of the top level
+# (k = 6 as 3 + 3, k = 5 as 3 + 2 with the lower level kept, k = 4 as 2 + 2)
 ENGINE_CASES = [("6,1", 3, 4, 3), ("12", 2, 3, 4), ("2,1", 3, 3, 4), ("4,2", 3, 4, 3), ("4,4", 3, 3, 3), ("7,1", 3, 3, 3),
-                ("6", 3, 4, 3)]
+                ("6", 3, 4, 3), ("5,1", 3, 3, 3), ("4", 3, 4, 3)]
 
 
 @pytest.mark.parametrize("rep,N,D,L", ENGINE_CASES, ids=[c[0] for c in ENGINE_CASES])
