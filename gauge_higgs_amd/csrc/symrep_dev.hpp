@@ -16,7 +16,7 @@
 namespace ghx {
 
 constexpr int kSymWave = 64;
-constexpr int kFinishSites = 16;  // sites (waves) per workgroup of sym_finish_kernel
+constexpr int kFinishSites = 8;  // sites (waves) per workgroup of sym_finish_kernel (16: +3.5 %, 4: +3.7 % per {6,1} step)
 constexpr int kLiftChunk = 16;  // lifted components per block of sym_lift_kernel
 
 // The link kernel runs one wave per workgroup. LDS instructions of a wave
