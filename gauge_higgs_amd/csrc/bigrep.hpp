@@ -56,12 +56,13 @@ struct BigRepTables {
     int k1 = 0, k2 = 0, lds_Cb = 0;   // Young rows; the level buffer that alternates with lds_S1
     int pair = 0;                     // 1: sym_link_pair_kernel (SU(3), k1 <= 6), two LDS regions per workgroup; 2: sym_link_col_kernel
     int pair_S2 = 0, pair_V = 0, pair_T1 = 0, pair_total = 0;  // its per-link region (C at 0), cx units
-    // g^a by lanes: generator a owns lanes [a lpg, (a+1) lpg); entry k of lane t
-    // is (l | m << 16, T^a_lm) at [k][t], gK entries per lane (zero padded)
+    // g^a by lanes: generator a owns lanes [a lpg, (a+1) lpg); lane j of them
+    // takes rows m = j + rr lpg (rr < grow) of T^a, entry k of row slot rr being
+    // (l, T^a_ml) at [(rr gK + k) 64 + lane] (gK entries per row, zero padded)
     const int* glm = nullptr;
     const double* gval = nullptr;     // complex, FP64 and FP32 copies
     const float* gvalf = nullptr;
-    int gK = 0, glpg = 0;
+    int gK = 0, glpg = 0, grow = 0;
   } sym;
 };
 

@@ -203,31 +203,35 @@ int upload_tables(ghx_ctx* ctx) {
     st.lds_V = sp.lds_V; st.lds_T1 = sp.lds_T1; st.lds_W = sp.lds_W; st.lds_total = sp.lds_total;
     st.k1 = sb.k1; st.k2 = sb.k2; st.lds_Cb = sp.lds_Cb;
     st.pair_S2 = sp.pair_S2; st.pair_V = sp.pair_V; st.pair_T1 = sp.pair_T1; st.pair_total = sp.pair_total;
-    // the nonzeros of T^a_R dealt round-robin to the lanes of generator a
+    // rows of T^a_R for g^a: lane j of generator a (lanes a lpg .. a lpg + lpg - 1)
+    // takes the rows m = j, j + lpg, ... of T^a; entry k of its row slot rr is
+    // (l, T^a_ml) at [(rr nz + k) 64 + lane], rows padded with zeros to nz entries
     const int lpg = ngn <= 4 ? 16 : 8;
-    std::vector<std::vector<int>> lm(64);
-    std::vector<std::vector<std::complex<double>>> val(64);
-    for (int a = 0; a < ngn; ++a) {
-      int i = 0;
-      for (int l = 0; l < r.d; ++l)
-        for (int m = 0; m < r.d; ++m) {
-          const std::complex<double> z = r.T[(size_t(a) * r.d + l) * r.d + m];
-          if (z == 0.0) continue;
-          const int lane = a * lpg + (i++ % lpg);
-          lm[lane].push_back(l | (m << 16));
-          val[lane].push_back(z);
-        }
-    }
+    const int nrow = (r.d + lpg - 1) / lpg;
     size_t K = 1;
-    for (int t = 0; t < 64; ++t) K = std::max(K, lm[t].size());
-    std::vector<int> glm(K * 64, 0);
-    std::vector<double> gval(K * 64 * 2, 0.0);
-    for (int t = 0; t < 64; ++t)
-      for (size_t k = 0; k < lm[t].size(); ++k) {
-        glm[k * 64 + t] = lm[t][k];
-        gval[(k * 64 + t) * 2] = val[t][k].real();
-        gval[(k * 64 + t) * 2 + 1] = val[t][k].imag();
+    for (int a = 0; a < ngn; ++a)
+      for (int m = 0; m < r.d; ++m) {
+        size_t c = 0;
+        for (int l = 0; l < r.d; ++l) c += r.T[(size_t(a) * r.d + m) * r.d + l] != 0.0;
+        K = std::max(K, c);
       }
+    std::vector<int> glm(size_t(nrow) * K * 64, 0);
+    std::vector<double> gval(glm.size() * 2, 0.0);
+    for (int a = 0; a < ngn; ++a)
+      for (int j = 0; j < lpg; ++j)
+        for (int rr = 0; rr < nrow; ++rr) {
+          const int m = j + rr * lpg, lane = a * lpg + j;
+          if (m >= r.d) continue;
+          size_t k = 0;
+          for (int l = 0; l < r.d; ++l) {
+            const std::complex<double> z = r.T[(size_t(a) * r.d + m) * r.d + l];
+            if (z == 0.0) continue;
+            const size_t e = (size_t(rr) * K + k++) * 64 + size_t(lane);
+            glm[e] = l;
+            gval[2 * e] = z.real();
+            gval[2 * e + 1] = z.imag();
+          }
+        }
     const std::vector<float> gvalf(gval.begin(), gval.end());
     const void* gsrc[3] = {glm.data(), gval.data(), gvalf.data()};
     const size_t gbytes[3] = {glm.size() * 4, gval.size() * 8, gvalf.size() * 4};
@@ -238,6 +242,7 @@ int upload_tables(ghx_ctx* ctx) {
       HIPCHK(hipMemcpy(gp[k], gsrc[k], gbytes[k], hipMemcpyHostToDevice));
     }
     st.glm = (const int*)gp[0]; st.gval = (const double*)gp[1]; st.gvalf = (const float*)gp[2];
+    st.grow = nrow;
     st.gK = int(K); st.glpg = lpg;
   }
   ctx->tb.sym.on = (ctx->sym_valid && ctx->sym_enabled) ? 1 : 0;

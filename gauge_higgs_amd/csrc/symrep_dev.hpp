@@ -769,7 +769,8 @@ GHX_DEV void finish_site_body(const BigArgs<T>& a, const BigRepTables& tb, const
     for (int mu = 1; mu < D; ++mu) v = cadd(v, psb[mu]);
     f = {k2 * v.r, -k2 * v.i};
     // the D forward hops at once, then g^a = -2 Im sum_lm conj(phi_l) T^a_lm psi_m
-    // for every direction over the lanes of generator a (each entry read once)
+    // = -2 Im sum_m conj(chi_m) psi_m with chi = T^a phi (T^a hermitian): the
+    // lanes of generator a form chi row by row and dot it with every direction
 #pragma unroll
     for (int mu = 0; mu < D; ++mu) {
       const cx<R> ps = psf[mu];
@@ -781,16 +782,20 @@ GHX_DEV void finish_site_body(const BigArgs<T>& a, const BigRepTables& tb, const
     R gp[D];
 #pragma unroll
     for (int mu = 0; mu < D; ++mu) gp[mu] = R(0);
+    const int nz = st.gK, jl = l % lpg;
+#pragma unroll 2
+    for (int rr = 0; rr < st.grow; ++rr) {
+      const int m = jl + rr * lpg;  // < 64; rows past d are zero in the table and in psi
+      cx<R> chi = {R(0), R(0)};
 #pragma unroll 4
-    for (int k = 0; k < st.gK; ++k) {
-      const int e = glm[k * kSymWave];
-      const cx<R> v = ld_cx(gval + 2 * k * kSymWave);
-      const cx<R> x = phs[e & 0xffff];
+      for (int k = 0; k < nz; ++k) {
+        const int e = (rr * nz + k) * kSymWave;
+        cfma(chi, ld_cx(gval + 2 * e), phs[glm[e]]);
+      }
 #pragma unroll
       for (int mu = 0; mu < D; ++mu) {
-        const cx<R> y = psi[mu][e >> 16];
-        const R zr = x.r * y.r + x.i * y.i, zi = x.r * y.i - x.i * y.r;
-        gp[mu] = fma(v.r, zi, fma(v.i, zr, gp[mu]));
+        const cx<R> y = psi[mu][m];
+        gp[mu] = fma(chi.r, y.i, fma(-chi.i, y.r, gp[mu]));
       }
     }
 #pragma unroll
