@@ -142,7 +142,7 @@ def replica_block(n_therm, n_meas, use_graph=True):
     rs = replicas.kappa_scan()
     rs.lat.autotune(steps=20)  # launch shape (fields restored)
     launch = {"stream_seg": rs.lat.stream_segment(), "block_order": rs.lat.block_order(), "link_lanes": rs.lat.link_lanes(),
-              "store_cached": rs.lat.store_policy()}
+              "store_cached": rs.lat.store_policy(), "link_split": rs.lat.link_split()}
     m = replicas.measure_scan(rs, n_therm=n_therm, n_meas=n_meas, use_graph=use_graph)
     vtot = int(sum(rs.lat.vols))
     steps = 200
@@ -200,7 +200,7 @@ def precision_block():
         g.lat.time_md_steps(5)
         ms = g.lat.time_md_steps(100)
         out[f"md_site_updates_per_s_{name}"] = 16 ** 4 * 100 / (ms / 1e3)
-        out[f"launch_{name}"] = {"block_order": g.lat.block_order(), "link_lanes": g.lat.link_lanes(), "store_cached": g.lat.store_policy(),
+        out[f"launch_{name}"] = {"block_order": g.lat.block_order(), "link_lanes": g.lat.link_lanes(), "store_cached": g.lat.store_policy(), "link_split": g.lat.link_split(),
                                  "stream_seg": g.lat.stream_segment()}
         g.lat.close()
     return out
@@ -227,7 +227,7 @@ def pure_gauge_block():
     out = {"workload": "su2_pure_L8_fp64", "sites": 8 ** 4, "md_site_updates_per_s": 8 ** 4 * steps / (ms / 1e3),
            "ms_per_md_step": ms / steps, "plaquette": float(np.mean(plaq)), "plaquette_err": float(np.std(plaq) / np.sqrt(len(plaq) / 4)),
            "launch": {"block_order": g.lat.block_order(), "link_lanes": g.lat.link_lanes(), "stream_seg": g.lat.stream_segment(),
-                      "store_cached": g.lat.store_policy()}}
+                      "store_cached": g.lat.store_policy(), "link_split": g.lat.link_split()}}
     g.lat.close()
     return out
 
@@ -292,7 +292,7 @@ def main():
         tuned["block_order"] = g.lat.block_order()
     if not args.no_tune:
         seg, tms = g.lat.autotune(steps=50)  # launch-shape autotuner (fields restored)
-        tuned = {"stream_seg": seg, "block_order": g.lat.block_order(), "link_lanes": g.lat.link_lanes(), "store_cached": g.lat.store_policy(),
+        tuned = {"stream_seg": seg, "block_order": g.lat.block_order(), "link_lanes": g.lat.link_lanes(), "store_cached": g.lat.store_policy(), "link_split": g.lat.link_split(),
                  "ms_per_step_at_tune": tms}
     if args.warmup:
         g.lat.time_md_steps(args.warmup)

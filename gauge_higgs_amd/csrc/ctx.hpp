@@ -55,6 +55,7 @@ struct ghx_ctx {
   bool fuse_inner = true;     // GHX_TUNE_FUSE_INNER
   bool split_inner = true;    // GHX_TUNE_SPLIT_INNER
   int store_cached = 0;       // GHX_TUNE_STORE_POLICY
+  bool link_split = true;     // GHX_TUNE_LINK_SPLIT
   // multi-timescale integrator (0 = single scale)
   int n_outer = 0, m_inner = 1, inner_integ = GHX_LEAPFROG;
   // surrogate link force (learned); S planes are filled by surr_fill before a kick
@@ -159,6 +160,7 @@ struct ghx_ctx {
     c.xlane = (xlane_enabled && xlane_ok()) ? 1 : 0;
     c.fuse_inner = fuse_inner ? 1 : 0;
     c.store_cached = store_cached;
+    c.link_split = link_split ? 1 : 0;
     return c;
   }
 };

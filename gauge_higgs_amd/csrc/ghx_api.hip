@@ -1362,6 +1362,12 @@ int ghx_set_tuning(ghx_ctx* ctx, int key, int value) {
     drop_graph(ctx);
     return GHX_OK;
   }
+  if (key == GHX_TUNE_LINK_SPLIT) {
+    if (value < 0 || value > 1) return fail(ctx, GHX_E_INVALID, "link split must be 0 or 1");
+    ctx->link_split = value != 0;
+    drop_graph(ctx);
+    return GHX_OK;
+  }
   if (key == GHX_TUNE_STORE_POLICY) {
     if (value < 0 || value > 1) return fail(ctx, GHX_E_INVALID, "store policy must be 0 (streamed) or 1 (cached)");
     ctx->store_cached = value;
@@ -1390,6 +1396,10 @@ int ghx_get_tuning(const ghx_ctx* ctx, int key, int* value) {
   }
   if (key == GHX_TUNE_SPLIT_INNER) { *value = inner_split_applies(ctx) ? 1 : 0; return GHX_OK; }
   if (key == GHX_TUNE_STORE_POLICY) { *value = ctx->store_cached; return GHX_OK; }
+  if (key == GHX_TUNE_LINK_SPLIT) {  // whether the link-lane step splits the staples: link lanes, no scalar in the kernel, D >= 3
+    *value = ctx->link_split && ctx->link_lanes && (ctx->big || ctx->rep.d == 0) && ctx->D >= 3 && !(ctx->nrep == 1 && ctx->stream_seg > 0);
+    return GHX_OK;
+  }
   if (key == GHX_TUNE_X_LANES) {  // whether the MD step uses it (not with link lanes or the 4D-tile walks)
     *value = ctx->xlane_enabled && ctx->xlane_ok() && !ctx->link_lanes && ctx->block_walk().order < 3;
     return GHX_OK;
@@ -1428,11 +1438,17 @@ int ghx_autotune(ghx_ctx* ctx, int steps, int* best_seg, double* best_ms) {
     for (int sg : {1, 2, 4, 8, 16, 32, 64})
       if (sg <= ctx->hreps[0].L[ctx->D - 1]) shapes.push_back({sg, 0});
   struct Cand {
-    int first, second, store;
+    int first, second, store, split;
   };
+  // the link-lane shape also with and without its staples split over planes
+  // (where the kernel carries no scalar)
+  const bool can_split = (ctx->big || ctx->rep.d == 0) && ctx->D >= 3;
   std::vector<Cand> cand;
   for (int st : {0, 1})
-    for (const auto& sh : shapes) cand.push_back({sh.first, sh.second, st});
+    for (const auto& sh : shapes) {
+      cand.push_back({sh.first, sh.second, st, 0});
+      if (sh.second < 0 && can_split) cand.push_back({sh.first, sh.second, st, 1});
+    }
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
@@ -1441,6 +1457,7 @@ int ghx_autotune(ghx_ctx* ctx, int steps, int* best_seg, double* best_ms) {
     ctx->block_order = c.second < 0 ? 0 : c.second;
     ctx->link_lanes = c.second < 0 ? 1 : 0;
     ctx->store_cached = c.store;
+    ctx->link_split = c.split != 0;
   };
   auto time_it = [&](int n, double& out) -> int {  // GHX_OK or an error code
     launch_step(ctx, 1.0, 1.0, true);  // the shape's first launch stays out of the timing

@@ -811,6 +811,132 @@ __global__ __launch_bounds__(kLinkSites * D) void md_link_kernel(StepArgs<T> a, 
   md_link_block<T, N, D, REP>(with_store_policy<CACHED>(a), rp, xcd_block(blockIdx.x, gridDim.x));
 }
 
+// Link-parallel step with the staples split over planes, for small lattices
+// and no scalar components in the kernel (pure gauge, or the external matter
+// link force of a large irrep): lane p of link (x, mu) forms the forward and
+// backward staples of the p-th plane nu != mu, so a workgroup of kLinkSites
+// sites runs D (D - 1) waves, (D - 1) times the waves of md_link_kernel. The
+// planes' staple sums meet in LDS and the wave of plane 0 finishes every link
+// as link_update does (U Sigma, the force with G and S, the P kick and the
+// drift). Each plane's two staples are summed first and the planes then in
+// order, so the rounding differs from md_link_kernel's.
+template <typename T, int N, int D, class REP, bool CACHED = false>
+__global__ __launch_bounds__(kLinkSites * D * (D - 1)) void md_link3_kernel(StepArgs<T> a0, typename REP::template Params<T> rp) {
+  static_assert(REP::DR == 0, "the plane-split step carries no scalar");
+  using R = T;
+  constexpr int NC = link_reals<T, N>(), NGN = N * N - 1, NP = D - 1;
+  constexpr bool kTopRows = (N == 3 || N == 2) && link_rows<T, N>() == N - 1;
+  constexpr bool kQuat = N == 2 && kTopRows;
+  const StepArgs<T> a = with_store_policy<CACHED>(a0);
+  const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x) / kLinkSites);
+  const int mu = w / NP, p = w % NP, nu = p < mu ? p : p + 1;
+  const int lane = int(threadIdx.x) % kLinkSites;
+  const unsigned g = xcd_block(blockIdx.x, gridDim.x) * kLinkSites + unsigned(lane), Vp = a.Vp;
+  const RepDesc& rd = a.reps[a.blk_rep[g / kBlock]];
+  const unsigned s = g - rd.offset;
+  const bool live = s < rd.vol;
+  const bool do_gauge = !(a.flags & F_NO_GAUGE);
+  __shared__ R part[D][NP > 1 ? NP - 1 : 1][2 * N * N][kLinkSites];
+  const Planes<T> Uin(a.U_in, Vp, D * NC);
+  Mat<R, N> Sig = mzero<R, N>();
+  if (live && do_gauge) {
+    const Nbr<D> nb(rd, s);
+    const unsigned gpm = g + nb.upv(rd, mu);
+    // forward staple U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag
+    Mat<R, N> AB;
+    if constexpr (kTopRows) {
+      AB = mul_nd_top(bload_link<T, R, N, false>(Uin, nu, gpm), bload_link<T, R, N>(Uin, mu, g + nb.upv(rd, nu)));
+      rebuild_last_row<R, N>(AB);
+    } else {
+      AB = mul_nd(bload_link<T, R, N>(Uin, nu, gpm), bload_link<T, R, N>(Uin, mu, g + nb.upv(rd, nu)));
+    }
+    if constexpr (kQuat) mac_nd_top(Sig, AB, bload_link<T, R, N>(Uin, nu, g));
+    else mac_nd(Sig, AB, bload_link<T, R, N>(Uin, nu, g));
+    // backward staple (U_mu(x-nu) U_nu(x+mu-nu))^dag U_nu(x-nu)
+    const unsigned gmn = g + nb.dnv(rd, nu);
+    if constexpr (kTopRows) {
+      Mat<R, N> GF = mul_nn_top(bload_link<T, R, N, false>(Uin, mu, gmn), bload_link<T, R, N>(Uin, nu, gpm + nb.dnv(rd, nu)));
+      rebuild_last_row<R, N>(GF);
+      if constexpr (kQuat) mac_dn_top(Sig, GF, bload_link<T, R, N>(Uin, nu, gmn));
+      else mac_dn(Sig, GF, bload_link<T, R, N>(Uin, nu, gmn));
+    } else {
+      const Mat<R, N> B2 = mul_dd(bload_link<T, R, N>(Uin, nu, gpm + nb.dnv(rd, nu)), bload_link<T, R, N>(Uin, mu, gmn));
+      mac_nn(Sig, B2, bload_link<T, R, N>(Uin, nu, gmn));
+    }
+  }
+  if (p > 0) {
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        part[mu][p - 1][2 * (i * N + j)][lane] = Sig.e[i][j].r;
+        part[mu][p - 1][2 * (i * N + j) + 1][lane] = Sig.e[i][j].i;
+      }
+  }
+  __syncthreads();
+  if (p > 0 || !live) return;
+#pragma unroll
+  for (int q = 0; q < NP - 1; ++q)
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        Sig.e[i][j].r += part[mu][q][2 * (i * N + j)][lane];
+        Sig.e[i][j].i += part[mu][q][2 * (i * N + j) + 1][lane];
+      }
+  // the rest of link_update for a link without scalar terms
+  const bool abs_eps = a.flags & F_ABS_EPS;
+  const R ek = R(abs_eps ? a.ck : a.ck * rd.eps);
+  const R ed = R(abs_eps ? a.cd : a.cd * rd.eps);
+  const R hbN = R(0.5) * R(rd.beta / N);
+  const bool snt = !(a.flags & F_STORE_CACHED);
+  const Planes<T> Uout(a.U_out, Vp, D * NC, snt), Pp(a.P, Vp, D * NGN, snt);
+  const Mat<R, N> Ux = bload_link<T, R, N>(Uin, mu, g);
+  Mat<R, N> X;
+  if constexpr (kQuat) {
+    rebuild_last_row<R, N>(Sig);
+    X = mul_nn_top(Ux, Sig);
+    rebuild_last_row<R, N>(X);
+  } else {
+    X = mul_nn(Ux, Sig);
+  }
+  R f[NGN];
+  im_tr_gen2<R, N>(X, f);
+#pragma unroll
+  for (int c = 0; c < NGN; ++c) f[c] = hbN * f[c];
+  if constexpr (REP::KIND == REP_EXT) {
+    if (!(a.flags & F_NO_HOP)) {
+      const Planes<T> Gp(a.G, Vp, D * NGN);
+#pragma unroll
+      for (int c = 0; c < NGN; ++c) f[c] += R(Gp.template ldp<0>(g, unsigned(mu * NGN + c)));
+    }
+  }
+  if (a.flags & F_SURR) {
+    const Planes<T> Sp(a.S, Vp, D * NGN);
+    const R sg = R(a.surr_sign);
+#pragma unroll
+    for (int c = 0; c < NGN; ++c) f[c] = fma(sg, R(Sp.template ldp<0>(g, unsigned(mu * NGN + c))), f[c]);
+  }
+  if (a.flags & F_FORCE) {
+    const Planes<T> Fp(a.F_out, Vp, D * NGN);
+#pragma unroll
+    for (int c = 0; c < NGN; ++c) Fp.st(g, unsigned(mu * NGN + c), T(f[c]));
+    return;
+  }
+  R pm[NGN];
+#pragma unroll
+  for (int c = 0; c < NGN; ++c) pm[c] = R(Pp.template ldp<0>(g, unsigned(mu * NGN + c)));
+  if (a.flags & F_KICK) {
+#pragma unroll
+    for (int c = 0; c < NGN; ++c) { pm[c] = fma(-ek, f[c], pm[c]); Pp.st(g, unsigned(mu * NGN + c), T(pm[c])); }
+  }
+  if (a.flags & F_DRIFT) {
+    const Mat<R, N> H = alg_to_mat<R, N>(pm, ed);
+    const Mat<R, N> E = expi<R, N>(H);
+    bstore_link<T, R, N>(Uout, mu, g, mul_nn(E, Ux));
+  }
+}
+
 // Drift alone (U' = exp(i eps P) U, phi' = phi + eps pi) — for the kick/drift API.
 template <typename T, int N, int D, int DR>
 __global__ __launch_bounds__(kBlock) void drift_kernel(StepArgs<T> a) {

@@ -59,6 +59,16 @@ struct Launch {
         hipLaunchKernelGGL((md_stream_kernel<T, N, D, REP>), dim3(ntile * nseg), dim3(kBlock), 0, c.stream, a, ParamFill<REP, T>::make(c),
                            ntile, c.stream_seg);
     } else if (c.link_lanes) {
+      if constexpr (REP::DR == 0 && D >= 3) {
+        if (c.link_split) {  // the staples split over planes: (D - 1) times the waves
+          const dim3 grid(c.nblk * (kBlock / kLinkSites)), block(kLinkSites * D * (D - 1));
+          if (c.store_cached)
+            hipLaunchKernelGGL((md_link3_kernel<T, N, D, REP, true>), grid, block, 0, c.stream, a, ParamFill<REP, T>::make(c));
+          else
+            hipLaunchKernelGGL((md_link3_kernel<T, N, D, REP>), grid, block, 0, c.stream, a, ParamFill<REP, T>::make(c));
+          return;
+        }
+      }
       if (c.store_cached)
         hipLaunchKernelGGL((md_link_kernel<T, N, D, REP, true>), dim3(c.nblk * (kBlock / kLinkSites)), dim3(kLinkSites * D), 0, c.stream,
                            a, ParamFill<REP, T>::make(c));

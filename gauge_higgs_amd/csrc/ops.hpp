@@ -44,6 +44,7 @@ struct LaunchCtx {
   int xlane;        // 1: every wave of the MD step holds whole x-lines (x-neighbour lanes share products)
   int fuse_inner;   // 1: a large irrep's inner-scale step runs its on-site and gauge updates in one launch
   int store_cached; // 1: the MD step's stores take the default cache policy (F_STORE_CACHED) instead of streaming
+  int link_split;   // 1: link lanes with no scalar in the kernel split each link's staples over its planes (md_link3_kernel)
   const void* S;    // surrogate link force planes
   double surr_sign;
   int frozen;       // |phi_x| = 1 scalar: adds F_FROZEN to every MD launch

@@ -258,6 +258,17 @@ class Lattices:
         self._check(self._lib.ghx_get_tuning(self.h, 6, C.byref(v)))
         return v.value
 
+    def set_link_split(self, on):
+        """Link-parallel step without scalar components (pure gauge, or the gauge
+        part of a large irrep): each link's staples split over D - 1 lanes, one
+        per plane (1, default), or one lane per link (0)."""
+        self._check(self._lib.ghx_set_tuning(self.h, 9, int(on)))
+
+    def link_split(self):
+        v = C.c_int()
+        self._check(self._lib.ghx_get_tuning(self.h, 9, C.byref(v)))
+        return v.value
+
     def set_store_policy(self, cached):
         """MD step stores streamed past the caches (0, default) or with the
         default cache policy (1); the autotuner tries both."""

@@ -153,7 +153,7 @@ def compare(L=12, rep="6,1", beta=5.7, kappa=0.05, lam=0.5, nmd=16, exact_nmds=(
     out = {"workload": f"su3_{rep.replace(',', '')}_L{L}", "L": L, "sites": L ** 4, "beta": beta, "kappa": kappa, "lambda": lam,
            "start": f"hot sigma_U={warm[0]}, Gaussian phi sigma={warm[1]}; {n_therm} thermalisation trajectories "
                     f"(split {therm_split[0]}x{therm_split[1]}, acceptance {therm['acceptance']:.2f}, plaq {therm['plaq']:.4f}, {therm_s:.1f} s)",
-           "launch": {"link_lanes": g.lat.link_lanes(), "block_order": g.lat.block_order(), "store_cached": g.lat.store_policy()},
+           "launch": {"link_lanes": g.lat.link_lanes(), "block_order": g.lat.block_order(), "store_cached": g.lat.store_policy(), "link_split": g.lat.link_split()},
            "exact": {}, "surrogate_on": {}, "split_no_surrogate": {}}
     g.lat.set_nested(0, 1)
     g.lat.surrogate_enable(False)

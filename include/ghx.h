@@ -212,11 +212,17 @@ int ghx_swap_stats(ghx_ctx* ctx, int* npairs, int* pairs, uint64_t* attempts, ui
  * interleaved launches; get_tuning reports whether it applies);
  * GHX_TUNE_STORE_POLICY = 0 (default) streams the MD step's stores past the
  * caches, 1 stores them with the default cache policy (faster where the next
- * step finds them in cache). The autotuner times every candidate launch shape
- * with both store policies for `steps` MD steps on the
+ * step finds them in cache); GHX_TUNE_LINK_SPLIT = 1 (default) lets the
+ * link-parallel step of a kernel without scalar components (pure gauge, or a
+ * large irrep's gauge part) give each link D - 1 lanes, one per plane of its
+ * staples (get_tuning reports whether it applies; another rounding of the
+ * staple sum). The autotuner times every candidate launch shape
+ * with both store policies (and the link-lane shape with and without the
+ * plane split) for `steps` MD steps on the
  * live fields, restores the fields, and keeps the fastest. */
 enum { GHX_TUNE_STREAM_SEG = 0, GHX_TUNE_BLOCK_ORDER = 1, GHX_TUNE_LINK_LANES = 2, GHX_TUNE_SYM_ENGINE = 3, GHX_TUNE_SYM_PAIRS = 4,
-       GHX_TUNE_X_LANES = 5, GHX_TUNE_FUSE_INNER = 6, GHX_TUNE_SPLIT_INNER = 7, GHX_TUNE_STORE_POLICY = 8 };
+       GHX_TUNE_X_LANES = 5, GHX_TUNE_FUSE_INNER = 6, GHX_TUNE_SPLIT_INNER = 7, GHX_TUNE_STORE_POLICY = 8,
+       GHX_TUNE_LINK_SPLIT = 9 };
 int ghx_set_tuning(ghx_ctx* ctx, int key, int value);
 int ghx_get_tuning(const ghx_ctx* ctx, int key, int* value);
 int ghx_autotune(ghx_ctx* ctx, int steps, int* best_seg, double* best_ms_per_step);

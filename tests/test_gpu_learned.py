@@ -371,6 +371,7 @@ def test_fused_inner_step_equals_two_launches(prec, surr):
                                                                                     reunit_each_traj=False)
         g.hot(0.8, 12345, 0.3)
         g.lat.set_link_lanes(1)
+        g.lat.set_link_split(0)  # the fused launch runs one lane per link; the plane split rounds the staple sum otherwise
         g.lat.set_fuse_inner(fuse)
         g.lat.set_nested(3, 4)
         assert g.lat.fuse_inner() == fuse
@@ -410,6 +411,7 @@ def test_split_inner_scale_equals_interleaved(prec, inner, lanes):
                                                                                     integ=gha.OMELYAN, reunit_each_traj=False)
         g.hot(0.8, 12345, 0.3)
         g.lat.set_link_lanes(lanes)
+        g.lat.set_link_split(0)  # one lane per link on both sides, as in the fused launch
         g.lat.set_split_inner(split)
         g.lat.set_nested(3, 2, inner=inner)
         assert g.lat.split_inner() == split

@@ -475,3 +475,32 @@ def test_store_policy_changes_no_value(case):
         out.append([g.get(k) for k in ("U", "P", "phi", "pi")])
     for a, b in zip(*out):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("case", [("", 2, 4, gha.FP64, 1e-12), ("", 2, 3, gha.FP64, 1e-12), ("", 3, 4, gha.FP32MD, 2e-5),
+                                  ("6,1", 3, 4, gha.FP32MD, 2e-5), ("6,1", 3, 4, gha.FP64, 1e-12)],
+                         ids=["su2-pure-fp64", "su2-pure-d3", "su3-pure-fp32", "su3-61-fp32", "su3-61-fp64"])
+def test_link_split_equals_link_lanes(case):
+    """The link-parallel step with each link's staples split over its D - 1
+    planes (md_link3_kernel; pure gauge, or the gauge part of a large irrep)
+    against one lane per link (md_link_kernel): forces, and the links and
+    momenta after MD, agree to the rounding of the staple sum."""
+    rep, N, D, prec, tol = case
+    out = []
+    for split in (0, 1):
+        g = gha.GaugeHiggsHMC(4, rep, seed=4, D=D, N=N, precision=prec).configure(beta=2.3 if N == 2 else 5.7, kappa=0.05, lam=0.5,
+                                                                                  nmd=4, reunit_each_traj=False)
+        g.hot(0.8, 12345, 0.3)
+        g.lat.set_link_lanes(1)
+        g.lat.set_link_split(split)
+        assert g.lat.link_split() == split
+        F, _ = g.forces()
+        g.refresh_momenta()
+        g.md_evolve()
+        out.append((F, g.get("U"), g.get("P")))
+    a, b = out
+    assert relmax(b[0], a[0]) < tol
+    assert relmax(b[1], a[1]) < 1e2 * tol and relmax(b[2], a[2]) < 1e2 * tol
+    h = gha.GaugeHiggsHMC(4, "fund", seed=4, D=4, N=2)
+    h.lat.set_link_lanes(1)
+    assert h.lat.link_split() == 0  # a scalar in the kernel: one lane per link
