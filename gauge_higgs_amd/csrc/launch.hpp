@@ -60,7 +60,7 @@ struct Launch {
                            ntile, c.stream_seg);
     } else if (c.link_lanes) {
       if constexpr (REP::DR == 0 && D >= 3) {
-        if (c.link_split) {  // the staples split over planes: (D - 1) times the waves
+        if (c.link_split && !(flags & F_NO_GAUGE)) {  // the staples split over planes: (D - 1) times the waves
           const dim3 grid(c.nblk * (kBlock / kLinkSites)), block(kLinkSites * D * (D - 1));
           if (c.store_cached)
             hipLaunchKernelGGL((md_link3_kernel<T, N, D, REP, true>), grid, block, 0, c.stream, a, ParamFill<REP, T>::make(c));
