@@ -443,13 +443,19 @@ def test_link_lanes_with_frozen_multi_nested():
     assert relmax(res[1][0], res[0][0]) < 1e-11 and relmax(res[1][1], res[0][1]) < 1e-11
 
 
-def test_autotune_restores_fields():
-    g = gha.GaugeHiggsHMC(8, "2", seed=2, D=4, N=3, precision=gha.FP32MD).configure(beta=5.7, kappa=0.1, lam=0.5)
+@pytest.mark.parametrize("case", [("2", 8), ("6,1", 4)], ids=["su3-sextet", "su3-61-engine"])
+def test_autotune_restores_fields(case):
+    """The trial steps of every candidate shape (and, for a large irrep, the
+    gauge-only launches that decide the staple split) leave the fields as they
+    were."""
+    rep, L = case
+    g = gha.GaugeHiggsHMC(L, rep, seed=2, D=4, N=3, precision=gha.FP32MD).configure(beta=5.7, kappa=0.1, lam=0.5)
     g.hot(0.8, 12345, 0.5)
     g.refresh_momenta()
     before = [g.get(k) for k in ("U", "P", "phi", "pi")]
     seg, ms = g.lat.autotune(steps=3)
     assert seg in (0, 1, 2, 4, 8) and ms > 0 and g.lat.stream_segment() == seg
+    assert g.lat.link_split() in (0, 1)
     after = [g.get(k) for k in ("U", "P", "phi", "pi")]
     for a, b in zip(before, after):
         assert np.array_equal(a, b)
