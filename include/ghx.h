@@ -219,7 +219,10 @@ int ghx_swap_stats(ghx_ctx* ctx, int* npairs, int* pairs, uint64_t* attempts, ui
  * staple sum). The autotuner times every candidate launch shape
  * with both store policies (and the link-lane shape with and without the
  * plane split) for `steps` MD steps on the
- * live fields, restores the fields, and keeps the fastest. */
+ * live fields, restores the fields, and keeps the fastest; for a large irrep
+ * whose fastest shape is the link-lane one, the plane split is then decided
+ * on 8 x `steps` gauge-only launches of that step with and without it (the
+ * launches a nested chain's inner scale makes), twice each. */
 enum { GHX_TUNE_STREAM_SEG = 0, GHX_TUNE_BLOCK_ORDER = 1, GHX_TUNE_LINK_LANES = 2, GHX_TUNE_SYM_ENGINE = 3, GHX_TUNE_SYM_PAIRS = 4,
        GHX_TUNE_X_LANES = 5, GHX_TUNE_FUSE_INNER = 6, GHX_TUNE_SPLIT_INNER = 7, GHX_TUNE_STORE_POLICY = 8,
        GHX_TUNE_LINK_SPLIT = 9 };
